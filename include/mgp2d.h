@@ -1,0 +1,170 @@
+/*
+ * mgp2d.h -- C ABI of the MI355X (gfx950) semicoarsening-multigrid library.
+ *
+ * Drop-in boundary for libMGPoisson (EdSantilli/MGPoisson2D).  Each entry of
+ * section 2 replaces one Fortran 90 module procedure of the reference; the
+ * Fortran shim in mgpoisson2d_amd/fortran/ (modules Poisson2D, MGPoisson2D,
+ * DeferredPoisson2D, DeferredMGPoisson2D) binds them through iso_c_binding,
+ * so a caller written against the reference recompiles unchanged.
+ *
+ * Array conventions (host arrays, the reference's storage):
+ *   valid cells i = 0..nx-1, j = 0..ny-1, column-major data(i, j), i fastest
+ *   (arrayutils.f90:107-116).  Cell fields without ghosts: nx*ny doubles.
+ *   Fields with one ghost layer (phi, the reference's e/p_tilde/s_tilde):
+ *   (nx+2)*(ny+2) doubles, element (i, j) at (i+1) + (j+1)*(nx+2).
+ *   Metrics (mgp_geo): J nx*ny; Jxx, Jxy (nx+1)*ny; Jyx, Jyy nx*(ny+1).
+ *   BC codes are the reference's integers (arrayutils.f90:27-49).
+ *   Logical arguments are ints (0 = .false.).
+ *
+ * All computation runs on the current HIP device; section-2 calls copy their
+ * host arguments in and results out (PCIe-inclusive).  Section 3 keeps a
+ * problem resident in HBM across solves.  Calls on one mgp_solver are not
+ * thread-safe; distinct solvers may be used from distinct threads.
+ */
+#ifndef MGP2D_H
+#define MGP2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. Types ----------------------------------------------------------- */
+
+typedef enum {
+    MGP_OK = 0,
+    MGP_ERR_ARG = 1,          /* bad shape / argument (reference: print* + stop) */
+    MGP_ERR_UNSUPPORTED = 2,  /* e.g. BCTYPE_CF, non-2x2 prolongation order 2 */
+    MGP_ERR_HIP = 3,          /* HIP runtime failure */
+    MGP_ERR_NOMEM = 4         /* device allocation failed (reference: "Out of memory") */
+} mgp_status;
+
+/* Outcome of an iterative solve. */
+enum { MGP_CONVERGED = 0, MGP_DIVERGED = 1, MGP_MAXITERS = 2 };
+
+/* Relaxation orderings.
+ *   LEX: the reference's lexicographic order (poisson2d.f90:1561-1959,
+ *        MGPoisson2D_DD.f90:288-306), run as t = i + 3j (9-point) or t = i + j
+ *        (5-point) wavefronts in one workgroup; bit-identical to the reference.
+ *   RB:  the DC smoother's red-black branch (MGPoisson2D_DD.f90:262-285).
+ *   MC:  4-colour interior, then the boundary ring in 4 colours, then the
+ *        corners (9-point operator; the multi-workgroup GPU ordering). */
+enum { MGP_ORDER_LEX = 0, MGP_ORDER_RB = 1, MGP_ORDER_MC = 2 };
+
+typedef struct {
+    int nx, ny;
+    double dx, dy;
+    const double *J, *Jxx, *Jxy, *Jyx, *Jyy;
+} mgp_geo;
+
+typedef struct {
+    int type[4];          /* xlo, xhi, ylo, yhi */
+    int mode[4];          /* BCMODE_UNIFORM / NONUNIFORM / LAH */
+    const double *data[4];/* 1 value (UNIFORM, LAH) or ny (x sides) / nx (y sides) */
+} mgp_bc;
+
+/* ---- 2. Drop-in entry points (host arrays) ------------------------------ */
+
+const char *mgp_status_string(int status);
+/* Last HIP error text recorded by the library (per thread). */
+const char *mgp_last_error(void);
+/* Device the library runs on (the current HIP device). */
+int mgp_device_count(void);
+
+/* arrayutils.f90:1277 fill_ghosts(phi, bc, geo, homog, do_neum) */
+int mgp_fill_ghosts(const mgp_geo *geo, const mgp_bc *bc, int homog, int do_neum, double *phi);
+/* poisson2d.f90:886 compute_laplacian(lap, phi, geo, bc, homog, opt_jscale); phi's ghosts are filled */
+int mgp_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, int jscale, double *phi,
+                          double *lap);
+/* poisson2d.f90:1327 compute_residual(res, rhs, phi, geo, bc, homog) */
+int mgp_compute_residual(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *rhs, double *phi,
+                         double *res);
+/* poisson2d.f90:16 compute_invdiags(invdiags, geo, bc, homog) */
+int mgp_compute_invdiags(const mgp_geo *geo, const mgp_bc *bc, int homog, double *invd);
+/* poisson2d.f90:1460 relax_gs(phi, rhs, geo, bc, homog, invdiags, omega, tol<0, maxiters, .false., 0);
+ * ordering MGP_ORDER_LEX (reference order) or MGP_ORDER_MC. */
+int mgp_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega,
+                 int maxiters, const double *rhs, double *phi, int ordering);
+/* MGPoisson2D_DD.f90:45 DeferredPoisson2D::compute_laplacian */
+int mgp_dc_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *lap);
+/* MGPoisson2D_DD.f90:16 DeferredPoisson2D::compute_invdiags */
+int mgp_dc_compute_invdiags(const mgp_geo *geo, double *invd);
+/* MGPoisson2D_DD.f90:153 DeferredPoisson2D::relax_gs; ordering LEX or RB */
+int mgp_dc_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega,
+                    int maxiters, const double *rhs, double *phi, int ordering);
+/* MGPoisson2D.f90:135 restrict(fine, crse); kind 0 cell, 1 x-face, 2 y-face */
+int mgp_restrict(int kind, int fnx, int fny, const double *fine, int cnx, int cny, double *crse);
+/* MGPoisson2D.f90:262 prolong(fine, crse, fine_geo, crse_geo, crse_bc, order), crse_bc
+ * homogeneous with the given types/modes; fine and crse carry one ghost layer. */
+int mgp_prolong(const mgp_geo *fine_geo, const mgp_geo *crse_geo, const int *bc_types, const int *bc_modes,
+                int order, double *fine, double *crse);
+/* MGPoisson2D.f90:38,67 estimate_maxdepth + create_mgschedule; refx/refy hold >= 64 entries */
+int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out,
+                          int *refx, int *refy);
+/* arrayutils.f90:753 pnorm over the valid cells of a ghost-free field */
+int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, double *out);
+/* poisson2d.f90:2253 (kind 0) / MGPoisson2D_DD.f90:339 (kind 1) solve_bicgstab2 */
+int mgp_solve_bicgstab2(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd,
+                        double tol, int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
+                        int ordering);
+/* MGPoisson2D.f90:400 vcycle (kind 0) / MGPoisson2D_DD.f90:589 DeferredMGPoisson2D::vcycle (kind 1):
+ * (phi, rhs, geo, bc, homog, amrrefx, amrrefy, tol, maxiters, maxdepth_user, numcycles,
+ *  smooth_down, smooth_up, smooth_bottom, zerophi, verbosity) -- amrref* are unused by the
+ * reference and omitted.  relres (maxiters+1 doubles, may be NULL) receives the history the
+ * reference prints at verbosity >= 1; *iters the cycles run; *outcome MGP_CONVERGED/DIVERGED/
+ * MAXITERS.  The kind-1 rhs is scaled by J and restored, as in the reference. */
+int mgp_vcycle(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, double tol, int maxiters,
+               int maxdepth, int numcycles, int smooth_down, int smooth_up, int smooth_bottom, int zerophi,
+               double *phi, double *rhs, int ordering, double *relres, int *iters, int *outcome);
+
+/* ---- 3. Resident solver (no counterpart in the reference: keeps the level
+ *         hierarchy and phi/rhs in HBM across solves) ---------------------- */
+
+typedef struct mgp_solver mgp_solver;
+
+enum {
+    MGP_METHOD_VCYCLE = 0,     /* MGPoisson2D::vcycle on device data */
+    MGP_METHOD_DC_VCYCLE = 1,  /* DeferredMGPoisson2D::vcycle on device data */
+    MGP_METHOD_KRYLOV_MG = 2   /* FP64 BiCGStab on the reference operator, preconditioned by one
+                                  semicoarsening (y) x-line-relaxation V-cycle per application */
+};
+
+typedef struct {
+    int method;
+    int ordering;       /* smoother ordering for methods 0/1 */
+    int numcycles;      /* 1 = V, 2 = W */
+    int smooth_down, smooth_up, smooth_bottom;
+    int maxdepth;       /* -1: as deep as the schedule allows */
+    int homog;          /* BCs of the top-level residual */
+} mgp_solver_opts;
+
+void mgp_solver_default_opts(mgp_solver_opts *opts);
+int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts);
+int mgp_solver_destroy(mgp_solver *s);
+/* Upload / download in the host layouts above (phi: one ghost layer). */
+int mgp_solver_set_rhs(mgp_solver *s, const double *rhs);
+int mgp_solver_set_phi(mgp_solver *s, const double *phi);
+int mgp_solver_get_phi(mgp_solver *s, double *phi);
+/* Device copies between resident fields of the same shape (no PCIe). */
+int mgp_solver_copy_rhs_from(mgp_solver *s, const mgp_solver *src);
+/* Solve in place on the resident phi/rhs; relres may be NULL (else maxiters+1 doubles). */
+int mgp_solver_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters,
+                     int *outcome);
+/* Device pointer + pitch (doubles) of the resident fields, padded layout:
+ * element (i, j) at ptr[(j + 1) * pitch + 32 + i], i, j = -1 .. n. which: 0 phi, 1 rhs. */
+int mgp_solver_field(mgp_solver *s, int which, double **ptr, int *pitch);
+/* HIP stream the solver enqueues on (a hipStream_t). */
+void *mgp_solver_stream(mgp_solver *s);
+/* Number of V-cycle applications made by the last solve. */
+int mgp_solver_last_cycles(const mgp_solver *s);
+/* Measurement hook (KRYLOV_MG): times `reps` launches of one finest-level
+ * kernel with HIP events on the solver's stream.  kernel 0: x-line
+ * relaxation of the even interior rows; 1: residual.  *avg_ms is the mean
+ * duration per launch, *bytes the algorithmic bytes one launch moves. */
+int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, double *avg_ms, double *bytes);
+/* hipDeviceSynchronize on the current device. */
+int mgp_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

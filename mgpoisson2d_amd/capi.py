@@ -1,0 +1,302 @@
+"""ctypes binding of include/mgp2d.h (the library's C ABI).
+
+The product path: every function here calls the HIP library
+``mgpoisson2d_amd/libmgp2d.so``.  There is no CPU fallback; importing this
+module raises if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .geometry import BoundaryConditions, Geometry
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmgp2d.so")
+
+MGP_OK = 0
+CONVERGED, DIVERGED, MAXITERS = 0, 1, 2
+ORDER_LEX, ORDER_RB, ORDER_MC = 0, 1, 2
+METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG = 0, 1, 2
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class MgpError(RuntimeError):
+    pass
+
+
+class _Geo(ctypes.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("dx", _d), ("dy", _d),
+                ("J", _dp), ("Jxx", _dp), ("Jxy", _dp), ("Jyx", _dp), ("Jyy", _dp)]
+
+
+class _Bc(ctypes.Structure):
+    _fields_ = [("type", _i * 4), ("mode", _i * 4), ("data", _dp * 4)]
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("method", _i), ("ordering", _i), ("numcycles", _i), ("smooth_down", _i),
+                ("smooth_up", _i), ("smooth_bottom", _i), ("maxdepth", _i), ("homog", _i)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m mgpoisson2d_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.mgp_last_error.restype = ctypes.c_char_p
+    lib.mgp_status_string.restype = ctypes.c_char_p
+    lib.mgp_solver_stream.restype = ctypes.c_void_p
+    return lib
+
+
+lib = _load()
+
+# Every exported symbol of include/mgp2d.h (checked by the CPU test suite).
+EXPORTS = (
+    "mgp_status_string", "mgp_last_error", "mgp_device_count", "mgp_fill_ghosts",
+    "mgp_compute_laplacian", "mgp_compute_residual", "mgp_compute_invdiags", "mgp_relax_gs",
+    "mgp_dc_compute_laplacian", "mgp_dc_compute_invdiags", "mgp_dc_relax_gs", "mgp_restrict",
+    "mgp_prolong", "mgp_create_mgschedule", "mgp_pnorm", "mgp_solve_bicgstab2", "mgp_vcycle",
+    "mgp_solver_default_opts", "mgp_solver_create", "mgp_solver_destroy", "mgp_solver_set_rhs",
+    "mgp_solver_set_phi", "mgp_solver_get_phi", "mgp_solver_copy_rhs_from", "mgp_solver_solve",
+    "mgp_solver_field", "mgp_solver_stream", "mgp_solver_last_cycles", "mgp_solver_time_kernel",
+    "mgp_device_synchronize",
+)
+
+
+def _check(status):
+    if status != MGP_OK:
+        msg = lib.mgp_last_error().decode()
+        raise MgpError(f"{lib.mgp_status_string(status).decode()}: {msg}")
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class _Args:
+    """Keeps numpy buffers alive for the duration of one call."""
+
+    def __init__(self):
+        self.keep = []
+
+    def geo(self, g: Geometry):
+        arrs = [_f64(a) for a in g.arrays()]
+        self.keep += arrs
+        return ctypes.byref(_Geo(g.nx, g.ny, g.dx, g.dy, *[_ptr(a) for a in arrs]))
+
+    def bc(self, b: BoundaryConditions):
+        data = [_f64(d) for d in b.data]
+        self.keep += data
+        s = _Bc()
+        for k in range(4):
+            s.type[k] = int(b.types[k])
+            s.mode[k] = int(b.modes[k])
+            s.data[k] = _ptr(data[k])
+        return ctypes.byref(s)
+
+
+def device_count():
+    return lib.mgp_device_count()
+
+
+def fill_ghosts(geo, bc, homog, do_neum, phi):
+    a = _Args()
+    phi = _f64(phi).copy()
+    _check(lib.mgp_fill_ghosts(a.geo(geo), a.bc(bc), _i(homog), _i(do_neum), _ptr(phi)))
+    return phi
+
+
+def compute_laplacian(geo, bc, homog, phi, jscale=False):
+    a = _Args()
+    phi = _f64(phi).copy()
+    lap = np.zeros((geo.ny, geo.nx))
+    _check(lib.mgp_compute_laplacian(a.geo(geo), a.bc(bc), _i(homog), _i(jscale), _ptr(phi), _ptr(lap)))
+    return lap, phi
+
+
+def compute_residual(geo, bc, homog, rhs, phi):
+    a = _Args()
+    phi = _f64(phi).copy()
+    rhs = _f64(rhs)
+    res = np.zeros((geo.ny, geo.nx))
+    _check(lib.mgp_compute_residual(a.geo(geo), a.bc(bc), _i(homog), _ptr(rhs), _ptr(phi), _ptr(res)))
+    return res, phi
+
+
+def compute_invdiags(geo, bc, homog):
+    a = _Args()
+    out = np.zeros((geo.ny, geo.nx))
+    _check(lib.mgp_compute_invdiags(a.geo(geo), a.bc(bc), _i(homog), _ptr(out)))
+    return out
+
+
+def relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_LEX):
+    a = _Args()
+    phi = _f64(phi).copy()
+    invd, rhs = _f64(invd), _f64(rhs)
+    _check(lib.mgp_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
+                            _ptr(rhs), _ptr(phi), _i(ordering)))
+    return phi
+
+
+def dc_compute_laplacian(geo, bc, homog, phi):
+    a = _Args()
+    phi = _f64(phi).copy()
+    lap = np.zeros((geo.ny, geo.nx))
+    _check(lib.mgp_dc_compute_laplacian(a.geo(geo), a.bc(bc), _i(homog), _ptr(phi), _ptr(lap)))
+    return lap, phi
+
+
+def dc_compute_invdiags(geo):
+    a = _Args()
+    out = np.zeros((geo.ny, geo.nx))
+    _check(lib.mgp_dc_compute_invdiags(a.geo(geo), _ptr(out)))
+    return out
+
+
+def dc_relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_LEX):
+    a = _Args()
+    phi = _f64(phi).copy()
+    invd, rhs = _f64(invd), _f64(rhs)
+    _check(lib.mgp_dc_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
+                               _ptr(rhs), _ptr(phi), _i(ordering)))
+    return phi
+
+
+def restrict(kind, fnx, fny, fine, cnx, cny):
+    shape = {0: (cny, cnx), 1: (cny, cnx + 1), 2: (cny + 1, cnx)}[kind]
+    crse = np.zeros(shape)
+    fine = _f64(fine)
+    _check(lib.mgp_restrict(_i(kind), _i(fnx), _i(fny), _ptr(fine), _i(cnx), _i(cny), _ptr(crse)))
+    return crse
+
+
+def prolong(fgeo, cgeo, bc_types, bc_modes, order, fine, crse):
+    a = _Args()
+    fine = _f64(fine).copy()
+    crse = _f64(crse).copy()
+    t = (_i * 4)(*[int(x) for x in bc_types])
+    m = (_i * 4)(*[int(x) for x in bc_modes])
+    _check(lib.mgp_prolong(a.geo(fgeo), a.geo(cgeo), t, m, _i(order), _ptr(fine), _ptr(crse)))
+    return fine, crse
+
+
+def create_mgschedule(nx, ny, dx, dy, maxdepth=-1):
+    rx = (_i * 64)()
+    ry = (_i * 64)()
+    md = _i(0)
+    _check(lib.mgp_create_mgschedule(_i(nx), _i(ny), _d(dx), _d(dy), _i(maxdepth), ctypes.byref(md), rx, ry))
+    return md.value, list(rx[: md.value]), list(ry[: md.value])
+
+
+def pnorm(nx, ny, dx, dy, arr, p=2):
+    out = _d(0.0)
+    arr = _f64(arr)
+    _check(lib.mgp_pnorm(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(arr), _i(p), ctypes.byref(out)))
+    return out.value
+
+
+def solve_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi,
+                    ordering=ORDER_LEX):
+    a = _Args()
+    phi = _f64(phi).copy()
+    invd, rhs = _f64(invd), _f64(rhs)
+    _check(lib.mgp_solve_bicgstab2(_i(kind), a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(tol),
+                                   _i(maxiters), _i(maxrestarts), _i(zerophi), _ptr(rhs), _ptr(phi),
+                                   _i(ordering)))
+    return phi
+
+
+def vcycle(kind, geo, bc, homog, tol, maxiters, phi, rhs, maxdepth=-1, numcycles=1, smooth_down=2,
+           smooth_up=2, smooth_bottom=2, zerophi=False, ordering=ORDER_LEX):
+    """Reference vcycle (kind 0) / DC vcycle (kind 1) on the GPU.
+    Returns (phi, rhs, relres history, iterations, outcome)."""
+    a = _Args()
+    phi = _f64(phi).copy()
+    rhs = _f64(rhs).copy()
+    relres = np.zeros(maxiters + 1)
+    iters, outcome = _i(0), _i(0)
+    _check(lib.mgp_vcycle(_i(kind), a.geo(geo), a.bc(bc), _i(homog), _d(tol), _i(maxiters), _i(maxdepth),
+                          _i(numcycles), _i(smooth_down), _i(smooth_up), _i(smooth_bottom), _i(zerophi),
+                          _ptr(phi), _ptr(rhs), _i(ordering), _ptr(relres), ctypes.byref(iters),
+                          ctypes.byref(outcome)))
+    return phi, rhs, relres[: iters.value + 1], iters.value, outcome.value
+
+
+class Solver:
+    """A problem resident in HBM (mgp_solver): hierarchy, phi and rhs stay on
+    the device between solves."""
+
+    def __init__(self, geo, bc, method=METHOD_KRYLOV_MG, ordering=ORDER_MC, numcycles=1, smooth_down=2,
+                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True):
+        self.geo = geo
+        opts = SolverOpts(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth,
+                          int(homog))
+        a = _Args()
+        self._h = ctypes.c_void_p()
+        _check(lib.mgp_solver_create(ctypes.byref(self._h), a.geo(geo), a.bc(bc), ctypes.byref(opts)))
+
+    def close(self):
+        if self._h:
+            lib.mgp_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rhs(self, rhs):
+        rhs = _f64(rhs)
+        _check(lib.mgp_solver_set_rhs(self._h, _ptr(rhs)))
+
+    def set_phi(self, phi):
+        phi = _f64(phi)
+        _check(lib.mgp_solver_set_phi(self._h, _ptr(phi)))
+
+    def get_phi(self):
+        phi = np.zeros((self.geo.ny + 2, self.geo.nx + 2))
+        _check(lib.mgp_solver_get_phi(self._h, _ptr(phi)))
+        return phi
+
+    def solve(self, tol=1e-10, maxiters=100, zerophi=True):
+        relres = np.zeros(maxiters + 1)
+        iters, outcome = _i(0), _i(0)
+        _check(lib.mgp_solver_solve(self._h, _d(tol), _i(maxiters), _i(int(zerophi)), _ptr(relres),
+                                    ctypes.byref(iters), ctypes.byref(outcome)))
+        return relres[: iters.value + 1], iters.value, outcome.value
+
+    def solve_fast(self, tol=1e-10, maxiters=100, zerophi=True):
+        """Solve without copying the history out (timed path)."""
+        iters, outcome = _i(0), _i(0)
+        _check(lib.mgp_solver_solve(self._h, _d(tol), _i(maxiters), _i(int(zerophi)), None,
+                                    ctypes.byref(iters), ctypes.byref(outcome)))
+        return iters.value, outcome.value
+
+    def last_cycles(self):
+        return lib.mgp_solver_last_cycles(self._h)
+
+    def stream(self):
+        return lib.mgp_solver_stream(self._h)
+
+    def time_kernel(self, kernel=0, reps=20):
+        """Mean ms per launch and algorithmic bytes per launch of a finest-level kernel."""
+        ms, nbytes = _d(0.0), _d(0.0)
+        _check(lib.mgp_solver_time_kernel(self._h, _i(kernel), _i(reps), ctypes.byref(ms), ctypes.byref(nbytes)))
+        return ms.value, nbytes.value
+
+
+def device_synchronize():
+    _check(lib.mgp_device_synchronize())

@@ -1,0 +1,589 @@
+// mgp_capi.hip -- extern "C" entry points of include/mgp2d.h.
+//
+// Section-2 calls (drop-in replacements of the reference's module
+// procedures) copy host arrays into padded HBM buffers, run the kernels on a
+// per-thread stream and copy results back.  Section-3 calls operate on a
+// resident mgp_solver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mgp2d.h"
+#include "mgp_host.h"
+#include "mgp_kernels.h"
+#include "mgp_krylov.h"
+
+using namespace mgp;
+
+namespace {
+
+hipStream_t thread_stream() {
+    thread_local hipStream_t s = nullptr;
+    if (!s) hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    return s;
+}
+
+template <class F>
+int guarded(F fn) {
+    try {
+        fn();
+        return MGP_OK;
+    } catch (const Failure &f) {
+        return f.status;
+    } catch (...) {
+        g_last_error = "unexpected host exception";
+        return MGP_ERR_HIP;
+    }
+}
+
+void check_geo(const mgp_geo *geo) {
+    if (!geo || geo->nx < 3 || geo->ny < 3 || !(geo->dx > 0) || !(geo->dy > 0) || !geo->J || !geo->Jxx ||
+        !geo->Jxy || !geo->Jyx || !geo->Jyy) {
+        g_last_error = "bad geometry (need nx, ny >= 3, positive spacing, five metric arrays)";
+        throw Failure(MGP_ERR_ARG);
+    }
+}
+
+void check_bc(const mgp_bc *bc) {
+    if (!bc) {
+        g_last_error = "null bc";
+        throw Failure(MGP_ERR_ARG);
+    }
+    for (int k = 0; k < 4; ++k)
+        if (bc->type[k] < BC_NONE || bc->type[k] > BC_EXTRAP2) {
+            g_last_error = "bad BC type";
+            throw Failure(MGP_ERR_ARG);
+        }
+}
+
+void no_lah(const mgp_bc *bc) {
+    for (int k = 0; k < 4; ++k)
+        if (bc->mode[k] == MODE_LAH) {
+            g_last_error = "Poisson2D operators reject BCMODE_LAH (poisson2d.f90:35-39)";
+            throw Failure(MGP_ERR_ARG);
+        }
+}
+
+// One-ghost host field <-> device.
+void up_phi(hipStream_t s, DevBuf &d, const Grid &g, const double *h) { upload_block(s, d.p, g, h, -1, -1, g.nx + 2, g.ny + 2); }
+void down_phi(hipStream_t s, double *h, const DevBuf &d, const Grid &g) { download_block(s, h, d.p, g, -1, -1, g.nx + 2, g.ny + 2); }
+void up_cc(hipStream_t s, DevBuf &d, const Grid &g, const double *h) { upload_block(s, d.p, g, h, 0, 0, g.nx, g.ny); }
+void down_cc(hipStream_t s, double *h, const DevBuf &d, const Grid &g) { download_block(s, h, d.p, g, 0, 0, g.nx, g.ny); }
+
+struct Call {
+    hipStream_t s;
+    DevGeo geo;
+    DevBc bc;
+    Call(const mgp_geo *g, const mgp_bc *b) : s(thread_stream()) {
+        check_geo(g);
+        geo.upload(s, *g);
+        if (b) {
+            check_bc(b);
+            bc.upload(s, *b, g->nx, g->ny);
+        }
+    }
+    DevBuf field() {
+        DevBuf d;
+        d.alloc(level_elems(geo.g.nx, geo.g.ny));
+        return d;
+    }
+    void finish() { hip_check(hipStreamSynchronize(s), "sync"); }
+};
+
+}  // namespace
+
+struct mgp_solver {
+    hipStream_t stream = nullptr;
+    mgp_solver_opts opts;
+    DevGeo geo;
+    DevBc ubc;
+    DevBuf phi, rhs;
+    Workspace W;
+    RefHierarchy ref;
+    KrylovMG kmg;
+    int last_cycles = 0;
+};
+
+extern "C" {
+
+const char *mgp_status_string(int status) {
+    switch (status) {
+    case MGP_OK: return "ok";
+    case MGP_ERR_ARG: return "bad argument";
+    case MGP_ERR_UNSUPPORTED: return "unsupported";
+    case MGP_ERR_HIP: return "HIP runtime error";
+    case MGP_ERR_NOMEM: return "out of device memory";
+    default: return "unknown status";
+    }
+}
+
+const char *mgp_last_error(void) { return g_last_error.c_str(); }
+
+int mgp_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int mgp_fill_ghosts(const mgp_geo *geo, const mgp_bc *bc, int homog, int do_neum, double *phi) {
+    return guarded([&] {
+        Call c(geo, bc);
+        DevBuf f = c.field();
+        up_phi(c.s, f, c.geo.g, phi);
+        launch_fill_ghosts(c.s, f.p, c.geo.g, c.geo.view(), c.bc.bc, homog, do_neum);
+        down_phi(c.s, phi, f, c.geo.g);
+        c.finish();
+    });
+}
+
+int mgp_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, int jscale, double *phi, double *lap) {
+    return guarded([&] {
+        no_lah(bc);
+        Call c(geo, bc);
+        DevBuf f = c.field(), out = c.field();
+        up_phi(c.s, f, c.geo.g, phi);
+        launch_fill_ghosts(c.s, f.p, c.geo.g, c.geo.view(), c.bc.bc, homog, 0);
+        launch_laplacian(c.s, out.p, f.p, nullptr, c.geo.g, c.geo.view(), c.bc.bc, homog, jscale);
+        down_phi(c.s, phi, f, c.geo.g);
+        down_cc(c.s, lap, out, c.geo.g);
+        c.finish();
+    });
+}
+
+int mgp_compute_residual(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *rhs, double *phi,
+                         double *res) {
+    return guarded([&] {
+        no_lah(bc);
+        Call c(geo, bc);
+        DevBuf f = c.field(), r = c.field(), out = c.field();
+        up_phi(c.s, f, c.geo.g, phi);
+        up_cc(c.s, r, c.geo.g, rhs);
+        launch_fill_ghosts(c.s, f.p, c.geo.g, c.geo.view(), c.bc.bc, homog, 0);
+        launch_laplacian(c.s, out.p, f.p, r.p, c.geo.g, c.geo.view(), c.bc.bc, homog, 0);
+        down_phi(c.s, phi, f, c.geo.g);
+        down_cc(c.s, res, out, c.geo.g);
+        c.finish();
+    });
+}
+
+int mgp_compute_invdiags(const mgp_geo *geo, const mgp_bc *bc, int homog, double *invd) {
+    return guarded([&] {
+        no_lah(bc);
+        Call c(geo, bc);
+        DevBuf out = c.field();
+        launch_invdiags(c.s, out.p, c.geo.g, c.geo.view(), c.bc.bc, homog);
+        down_cc(c.s, invd, out, c.geo.g);
+        c.finish();
+    });
+}
+
+int mgp_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega, int maxiters,
+                 const double *rhs, double *phi, int ordering) {
+    return guarded([&] {
+        no_lah(bc);
+        if (maxiters <= 0) return;
+        if (ordering != MGP_ORDER_LEX && ordering != MGP_ORDER_MC) {
+            g_last_error = "relax_gs ordering must be LEX or MC";
+            throw Failure(MGP_ERR_ARG);
+        }
+        Call c(geo, bc);
+        if (ordering == MGP_ORDER_MC)
+            for (int k = 0; k < 4; ++k)
+                if (bc->type[k] == BC_PERIODIC) {
+                    g_last_error = "MC ordering needs non-periodic sides";
+                    throw Failure(MGP_ERR_UNSUPPORTED);
+                }
+        DevBuf f = c.field(), r = c.field(), iv = c.field();
+        up_phi(c.s, f, c.geo.g, phi);
+        up_cc(c.s, r, c.geo.g, rhs);
+        up_cc(c.s, iv, c.geo.g, invd);
+        const Grid &g = c.geo.g;
+        if (ordering == MGP_ORDER_LEX) {
+            launch_gs_lex(c.s, f.p, r.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega, maxiters);
+        } else {
+            for (int k = 0; k < maxiters; ++k) {
+                for (int col = 0; col < 4; ++col)
+                    launch_gs_mc_interior(c.s, f.p, r.p, iv.p, g, c.geo.view(), omega, col & 1, col >> 1);
+                launch_gs_mc_ring(c.s, f.p, r.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega);
+            }
+        }
+        down_phi(c.s, phi, f, g);
+        c.finish();
+    });
+}
+
+int mgp_dc_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *lap) {
+    return guarded([&] {
+        Call c(geo, bc);
+        DevBuf f = c.field(), out = c.field();
+        up_phi(c.s, f, c.geo.g, phi);
+        launch_fill_ghosts(c.s, f.p, c.geo.g, c.geo.view(), c.bc.bc, homog, 1);
+        launch_dc_laplacian(c.s, out.p, f.p, nullptr, c.geo.g, c.geo.view());
+        down_phi(c.s, phi, f, c.geo.g);
+        down_cc(c.s, lap, out, c.geo.g);
+        c.finish();
+    });
+}
+
+int mgp_dc_compute_invdiags(const mgp_geo *geo, double *invd) {
+    return guarded([&] {
+        Call c(geo, nullptr);
+        DevBuf out = c.field();
+        launch_dc_invdiags(c.s, out.p, c.geo.g, c.geo.view());
+        down_cc(c.s, invd, out, c.geo.g);
+        c.finish();
+    });
+}
+
+int mgp_dc_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega, int maxiters,
+                    const double *rhs, double *phi, int ordering) {
+    return guarded([&] {
+        if (maxiters <= 0) return;
+        if (ordering != MGP_ORDER_LEX && ordering != MGP_ORDER_RB) {
+            g_last_error = "DC relax_gs ordering must be LEX or RB";
+            throw Failure(MGP_ERR_ARG);
+        }
+        Call c(geo, bc);
+        DevBuf f = c.field(), r = c.field(), iv = c.field(), mod = c.field();
+        up_phi(c.s, f, c.geo.g, phi);
+        up_cc(c.s, r, c.geo.g, rhs);
+        up_cc(c.s, iv, c.geo.g, invd);
+        const Grid &g = c.geo.g;
+        launch_dc_modrhs(c.s, mod.p, r.p, f.p, g, c.geo.view());
+        if (ordering == MGP_ORDER_RB) {
+            for (int k = 0; k < maxiters; ++k) {
+                launch_fill_ghosts(c.s, f.p, g, c.geo.view(), c.bc.bc, homog, 1);
+                launch_dc_rb(c.s, f.p, mod.p, iv.p, g, c.geo.view(), omega, 0);
+                launch_dc_rb(c.s, f.p, mod.p, iv.p, g, c.geo.view(), omega, 1);
+            }
+        } else {
+            launch_dc_lex(c.s, f.p, mod.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega, maxiters);
+        }
+        down_phi(c.s, phi, f, g);
+        c.finish();
+    });
+}
+
+int mgp_restrict(int kind, int fnx, int fny, const double *fine, int cnx, int cny, double *crse) {
+    return guarded([&] {
+        if (fnx <= 0 || fny <= 0 || cnx <= 0 || cny <= 0 || fnx % cnx || fny % cny || kind < 0 || kind > 2) {
+            g_last_error = "restrict: bad shapes";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid fg = make_grid(fnx, fny, 1.0, 1.0), cg = make_grid(cnx, cny, 1.0, 1.0);
+        const int rx = fnx / cnx, ry = fny / cny;
+        DevBuf f, c;
+        f.alloc(level_elems(fnx, fny));
+        c.alloc(level_elems(cnx, cny));
+        const int fw = fnx + (kind == 1), fh = fny + (kind == 2), cw = cnx + (kind == 1), ch = cny + (kind == 2);
+        upload_block(s, f.p, fg, fine, 0, 0, fw, fh);
+        if (kind == 0) launch_restrict_cc(s, c.p, f.p, cg, fg.pitch, rx, ry, nullptr, nullptr);
+        else if (kind == 1) launch_restrict_xf(s, c.p, f.p, cg, fg.pitch, rx, ry);
+        else launch_restrict_yf(s, c.p, f.p, cg, fg.pitch, rx, ry);
+        download_block(s, crse, c.p, cg, 0, 0, cw, ch);
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int mgp_prolong(const mgp_geo *fine_geo, const mgp_geo *crse_geo, const int *bc_types, const int *bc_modes,
+                int order, double *fine, double *crse) {
+    return guarded([&] {
+        check_geo(fine_geo);
+        check_geo(crse_geo);
+        if (fine_geo->dx * fine_geo->dy > crse_geo->dx * crse_geo->dy) {
+            g_last_error = "prolong: crse and fine geos passed in wrong order";
+            throw Failure(MGP_ERR_ARG);
+        }
+        if (fine_geo->nx != 2 * crse_geo->nx || fine_geo->ny != 2 * crse_geo->ny) {
+            g_last_error = "prolong: only 2x2 refinement is defined (MGPoisson2D.f90:385-388)";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        if (order < 0 || order > 2) {
+            g_last_error = "prolong: order must be 0, 1 or 2";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        HLevel f, c;
+        DevGeo fg, cg;
+        fg.upload(s, *fine_geo);
+        cg.upload(s, *crse_geo);
+        f.g = fg.g;
+        c.g = cg.g;
+        f.geo = fg.view();
+        c.geo = cg.view();
+        f.J.alloc(1);  // unused placeholders
+        mgp_bc hb;
+        const double zero = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            hb.type[k] = bc_types[k];
+            hb.mode[k] = bc_modes[k] == MODE_LAH ? MODE_LAH : MODE_UNIFORM;
+            hb.data[k] = &zero;
+        }
+        DevBc dbc;
+        dbc.upload(s, hb, crse_geo->nx, crse_geo->ny);
+        c.bc = dbc.bc;
+        f.e.alloc(level_elems(f.g.nx, f.g.ny));
+        c.e.alloc(level_elems(c.g.nx, c.g.ny));
+        DevBuf tmp;
+        tmp.alloc(level_elems(c.g.nx, c.g.ny));
+        up_phi(s, f.e, f.g, fine);
+        up_phi(s, c.e, c.g, crse);
+        // prolong_level reads the fine J for the mean removal
+        Workspace W;
+        f.J = std::move(fg.J);
+        prolong_level(W, s, f, c, order, &tmp);
+        down_phi(s, fine, f.e, f.g);
+        down_phi(s, crse, c.e, c.g);
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out, int *refx,
+                          int *refy) {
+    return guarded([&] {
+        std::vector<int> rx, ry;
+        const int d = reference_schedule(nx, ny, dx, dy, maxdepth_in, rx, ry);
+        if (d > 64) {
+            g_last_error = "schedule deeper than 64 levels";
+            throw Failure(MGP_ERR_ARG);
+        }
+        *maxdepth_out = d;
+        for (int k = 0; k < d; ++k) {
+            refx[k] = rx[k];
+            refy[k] = ry[k];
+        }
+    });
+}
+
+int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, double *out) {
+    return guarded([&] {
+        if (p != 0 && p != 2) {
+            g_last_error = "pnorm: p must be 0 or 2";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf d;
+        d.alloc(level_elems(nx, ny));
+        upload_block(s, d.p, g, a, 0, 0, nx, ny);
+        Workspace W;
+        *out = p == 2 ? pnorm2(W, s, d.p, g) : W.reduce(s, d.p, nullptr, nullptr, 1.0, g, 2);
+    });
+}
+
+int mgp_solve_bicgstab2(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double tol,
+                        int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi, int ordering) {
+    return guarded([&] {
+        if (kind == 0) no_lah(bc);
+        Call c(geo, bc);
+        const Grid &g = c.geo.g;
+        std::vector<DevBuf> w(13);
+        for (auto &b : w) b.alloc(level_elems(g.nx, g.ny));
+        up_phi(c.s, w[0], g, phi);
+        up_cc(c.s, w[1], g, rhs);
+        up_cc(c.s, w[2], g, invd);
+        if (zerophi) launch_setval(c.s, w[0].p, 0.0, g, 1);
+        BottomArgs a;
+        a.g = g;
+        a.geo = c.geo.view();
+        a.bc = c.bc.bc;
+        a.phi = w[0].p;
+        a.rhs = w[1].p;
+        a.invd = w[2].p;
+        a.e = w[3].p;
+        a.pt = w[4].p;
+        a.st = w[5].p;
+        a.r = w[6].p;
+        a.rt = w[7].p;
+        a.p = w[8].p;
+        a.t = w[9].p;
+        a.v = w[10].p;
+        a.mod = w[11].p;
+        a.scal = w[12].p;
+        a.dc = kind;
+        a.order = ordering;
+        a.max_iters = maxiters;
+        a.max_restarts = maxrestarts;
+        a.tol = tol;
+        a.homog = homog;
+        launch_bicgstab2(c.s, a);
+        down_phi(c.s, phi, w[0], g);
+        c.finish();
+    });
+}
+
+void mgp_solver_default_opts(mgp_solver_opts *o) {
+    o->method = MGP_METHOD_KRYLOV_MG;
+    o->ordering = MGP_ORDER_MC;
+    o->numcycles = 1;
+    o->smooth_down = 2;
+    o->smooth_up = 2;
+    o->smooth_bottom = 2;
+    o->maxdepth = -1;
+    o->homog = 1;
+}
+
+int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts) {
+    *out = nullptr;
+    mgp_solver *s = new mgp_solver;
+    const int st = guarded([&] {
+        check_geo(geo);
+        check_bc(bc);
+        if (opts) s->opts = *opts;
+        else mgp_solver_default_opts(&s->opts);
+        const mgp_solver_opts &o = s->opts;
+        if (o.method == MGP_METHOD_VCYCLE) no_lah(bc);
+        if (o.method == MGP_METHOD_VCYCLE && o.ordering != MGP_ORDER_LEX && o.ordering != MGP_ORDER_MC) {
+            g_last_error = "VCYCLE ordering must be LEX or MC";
+            throw Failure(MGP_ERR_ARG);
+        }
+        if (o.method == MGP_METHOD_DC_VCYCLE && o.ordering != MGP_ORDER_LEX && o.ordering != MGP_ORDER_RB) {
+            g_last_error = "DC_VCYCLE ordering must be LEX or RB";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+        s->geo.upload(s->stream, *geo);
+        s->ubc.upload(s->stream, *bc, geo->nx, geo->ny);
+        const size_t n = level_elems(geo->nx, geo->ny);
+        s->phi.alloc(n);
+        s->rhs.alloc(n);
+        if (o.method == MGP_METHOD_VCYCLE || o.method == MGP_METHOD_DC_VCYCLE)
+            s->ref.build(s->stream, s->geo, s->ubc.bc, o.method == MGP_METHOD_DC_VCYCLE, o.ordering, o.maxdepth);
+        else if (o.method == MGP_METHOD_KRYLOV_MG)
+            s->kmg.build(s->stream, s->geo, s->ubc.bc, o);
+        else {
+            g_last_error = "unknown method";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+    if (st != MGP_OK) {
+        mgp_solver_destroy(s);
+        return st;
+    }
+    *out = s;
+    return MGP_OK;
+}
+
+int mgp_solver_destroy(mgp_solver *s) {
+    if (!s) return MGP_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    hipStream_t st = s->stream;
+    delete s;
+    if (st) (void)hipStreamDestroy(st);
+    return MGP_OK;
+}
+
+int mgp_solver_set_rhs(mgp_solver *s, const double *rhs) {
+    return guarded([&] {
+        up_cc(s->stream, s->rhs, s->geo.g, rhs);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_set_phi(mgp_solver *s, const double *phi) {
+    return guarded([&] {
+        up_phi(s->stream, s->phi, s->geo.g, phi);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_get_phi(mgp_solver *s, double *phi) {
+    return guarded([&] {
+        down_phi(s->stream, phi, s->phi, s->geo.g);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_copy_rhs_from(mgp_solver *s, const mgp_solver *src) {
+    return guarded([&] {
+        if (src->geo.g.nx != s->geo.g.nx || src->geo.g.ny != s->geo.g.ny) {
+            g_last_error = "copy_rhs_from: shape mismatch";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hip_check(hipMemcpyAsync(s->rhs.p, src->rhs.p, s->rhs.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 s->stream),
+                  "copy");
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters,
+                     int *outcome) {
+    return guarded([&] {
+        const mgp_solver_opts &o = s->opts;
+        int it = 0, oc = MGP_MAXITERS;
+        if (o.method == MGP_METHOD_KRYLOV_MG) {
+            oc = s->kmg.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters,
+                              zerophi, relres, &it);
+            s->last_cycles = s->kmg.last_cycles;
+        } else {
+            oc = s->ref.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters,
+                              o.numcycles, o.smooth_down, o.smooth_up, o.smooth_bottom, zerophi, relres, &it);
+            s->last_cycles = it;
+        }
+        if (iters) *iters = it;
+        if (outcome) *outcome = oc;
+    });
+}
+
+int mgp_solver_field(mgp_solver *s, int which, double **ptr, int *pitch) {
+    if (!s || (which != 0 && which != 1)) return MGP_ERR_ARG;
+    *ptr = which == 0 ? s->phi.p : s->rhs.p;
+    *pitch = s->geo.g.pitch;
+    return MGP_OK;
+}
+
+void *mgp_solver_stream(mgp_solver *s) { return s ? (void *)s->stream : nullptr; }
+
+int mgp_solver_last_cycles(const mgp_solver *s) { return s ? s->last_cycles : 0; }
+
+int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, double *avg_ms, double *bytes) {
+    return guarded([&] {
+        if (s->opts.method != MGP_METHOD_KRYLOV_MG || reps <= 0 || kernel < 0 || kernel > 1) {
+            g_last_error = "time_kernel: KRYLOV_MG solvers, kernel 0 or 1, reps > 0";
+            throw Failure(MGP_ERR_ARG);
+        }
+        *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, bytes);
+    });
+}
+
+int mgp_device_synchronize(void) {
+    return guarded([&] { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
+}
+
+int mgp_vcycle(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, double tol, int maxiters, int maxdepth,
+               int numcycles, int smooth_down, int smooth_up, int smooth_bottom, int zerophi, double *phi,
+               double *rhs, int ordering, double *relres, int *iters, int *outcome) {
+    mgp_solver_opts o;
+    mgp_solver_default_opts(&o);
+    o.method = kind ? MGP_METHOD_DC_VCYCLE : MGP_METHOD_VCYCLE;
+    o.ordering = ordering;
+    o.numcycles = numcycles;
+    o.smooth_down = smooth_down;
+    o.smooth_up = smooth_up;
+    o.smooth_bottom = smooth_bottom;
+    o.maxdepth = maxdepth;
+    o.homog = homog;
+    mgp_solver *s = nullptr;
+    int st = mgp_solver_create(&s, geo, bc, &o);
+    if (st != MGP_OK) return st;
+    st = guarded([&] {
+        up_phi(s->stream, s->phi, s->geo.g, phi);
+        up_cc(s->stream, s->rhs, s->geo.g, rhs);
+    });
+    if (st == MGP_OK) st = mgp_solver_solve(s, tol, maxiters, zerophi, relres, iters, outcome);
+    if (st == MGP_OK)
+        st = guarded([&] {
+            down_phi(s->stream, phi, s->phi, s->geo.g);
+            down_cc(s->stream, rhs, s->rhs, s->geo.g);
+            hip_check(hipStreamSynchronize(s->stream), "sync");
+        });
+    mgp_solver_destroy(s);
+    return st;
+}
+
+}  // extern "C"

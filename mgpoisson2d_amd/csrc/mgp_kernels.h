@@ -1,0 +1,82 @@
+// mgp_kernels.h -- host-side launchers for the kernels in mgp_kernels.hip.
+// Every launcher enqueues on the given stream and returns immediately.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mgp_common.h"
+
+namespace mgp {
+
+void launch_fill_ghosts(hipStream_t s, double *phi, const Grid &g, const Geo &geo, const Bc &bc, int homog,
+                        int do_neum);
+void launch_laplacian(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
+                      const Geo &geo, const Bc &bc, int homog, int jscale);
+void launch_invdiags(hipStream_t s, double *out, const Grid &g, const Geo &geo, const Bc &bc, int homog);
+void launch_dc_invdiags(hipStream_t s, double *out, const Grid &g, const Geo &geo);
+void launch_dc_laplacian(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
+                         const Geo &geo);
+void launch_dc_modrhs(hipStream_t s, double *modrhs, const double *rhs, const double *phi, const Grid &g,
+                      const Geo &geo);
+void launch_gs_mc_interior(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
+                           const Geo &geo, double omega, int ci, int cj);
+void launch_gs_mc_ring(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
+                       const Geo &geo, const Bc &bc, int homog, double omega);
+void launch_gs_lex(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
+                   const Geo &geo, const Bc &bc, int homog, double omega, int sweeps);
+void launch_dc_rb(hipStream_t s, double *phi, const double *modrhs, const double *invd, const Grid &g,
+                  const Geo &geo, double omega, int colour);
+void launch_dc_lex(hipStream_t s, double *phi, const double *modrhs, const double *invd, const Grid &g,
+                   const Geo &geo, const Bc &bc, int homog, double omega, int sweeps);
+void launch_restrict_cc(hipStream_t s, double *crse, const double *fine, const Grid &cg, int fpitch, int rx,
+                        int ry, double *e, const double *invd);
+void launch_restrict_xf(hipStream_t s, double *crse, const double *fine, const Grid &cg, int fpitch, int rx,
+                        int ry);
+void launch_restrict_yf(hipStream_t s, double *crse, const double *fine, const Grid &cg, int fpitch, int rx,
+                        int ry);
+void launch_prolong2x2(hipStream_t s, double *fine, int fpitch, const double *crse, const double *crse_x3,
+                       const Grid &cg, int order);
+void launch_extrap3(hipStream_t s, double *dst, const double *src, const Grid &g);
+void launch_einit(hipStream_t s, double *e, const double *r, const double *invd, const Grid &g);
+void launch_axpy_ghosted(hipStream_t s, double *y, const double *x, double a, const Grid &g);
+void launch_lincomb(hipStream_t s, double *out, double a, const double *y, double b, const double *x,
+                    const Grid &g, int ghosted);
+void launch_setval(hipStream_t s, double *y, double v, const Grid &g, int ghosted);
+// out = a * b / out = a / b over valid cells
+void launch_lincomb_mul(hipStream_t s, double *out, const double *a, const double *b, const Grid &g, int unused);
+void launch_lincomb_div(hipStream_t s, double *out, const double *a, const double *b, const Grid &g, int unused);
+// kind 0: sum w a b; 1: sum w a^2; 2: max |a|; 3: sum w a.  w = vol (or vol*J).
+// Writes the scalar to out (device); partial needs reduce_blocks(g) doubles.
+int reduce_blocks(const Grid &g);
+void launch_reduce(hipStream_t s, double *out, double *partial, const double *a, const double *b,
+                   const double *J, double vol, const Grid &g, int kind);
+
+void launch_seq_mean(hipStream_t s, double *out, const double *fine, const double *J, const Grid &g, double dxprod);
+void launch_sub_mean(hipStream_t s, double *fine, const double *sums, const Grid &g);
+
+struct BottomArgs {
+    Grid g;
+    Geo geo;
+    Bc bc;
+    double *phi, *rhs, *invd;
+    double *e, *pt, *st, *r, *rt, *p, *t, *v, *mod;
+    double *scal;
+    int dc, order, max_iters, max_restarts, homog;
+    double tol;
+};
+void launch_bicgstab2(hipStream_t s, const BottomArgs &a);
+
+// Line relaxation along x (fast solver).
+void launch_line_coef(hipStream_t s, double *A, double *B, double *C, const Grid &g, const Geo &geo, const Bc &bc);
+int xline_threads(int nx);
+size_t xline_lds_bytes(int nx);
+// Solve rows j = row0 + k*rowstep, k < nrows (one workgroup each).
+void launch_xline(hipStream_t s, double *e, const double *f, const double *A, const double *B, const double *C,
+                  const Grid &g, const Geo &geo, const Bc &bc, int row0, int rowstep, int nrows);
+void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
+                      double shi);
+void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, double b, const double *y, double c,
+                     const double *z, const Grid &g);
+void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sweeps, int homog);
+
+}  // namespace mgp

@@ -1,0 +1,31 @@
+// mgp_krylov.h -- the resident fast solver: FP64 BiCGStab on the reference
+// operator, preconditioned by one semicoarsening (y) V-cycle with x-line
+// zebra relaxation.  No counterpart in the reference (its own cycle drivers
+// stall or diverge on strongly stretched mapped grids); the operator it
+// solves is the reference's, bit for bit.
+#pragma once
+
+#include "mgp_host.h"
+
+namespace mgp {
+
+struct KrylovMG {
+    struct Impl;
+    Impl *impl;
+    int last_cycles = 0;
+    bool built = false;
+    KrylovMG();
+    ~KrylovMG();
+    KrylovMG(const KrylovMG &) = delete;
+    KrylovMG &operator=(const KrylovMG &) = delete;
+    void build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o);
+    int solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
+              double *rhs, double tol, int maxiters, int zerophi, double *relres, int *iters);
+    // Times `reps` launches of one fine-level kernel with HIP events on stream s.
+    // kernel 0: x-line relaxation of the even interior rows; 1: residual
+    // (9-point operator).  Returns the mean ms per launch; *bytes receives the
+    // algorithmic bytes one launch moves (each array element it needs, once).
+    double time_kernel(hipStream_t s, int kernel, int reps, double *bytes);
+};
+
+}  // namespace mgp

@@ -1,0 +1,273 @@
+// mgp_krylov.hip -- the resident fast solver (see mgp_krylov.h).
+//
+// Outer iteration: right-preconditioned BiCGStab in FP64 on A = the
+// reference's 9-point operator with the caller's BCs (the same kernels the
+// drop-in path uses, bit-identical to poisson2d.f90 compute_laplacian).
+// Convergence is declared on the true residual ||rhs - A phi|| relative to
+// the initial one, i.e. the relres of MGPoisson2D.f90:614-615.
+//
+// Preconditioner: one V-cycle on a hierarchy coarsened in y only (1x2, the
+// reference's restrict supports any ratio, MGPoisson2D.f90:145-158), each
+// level re-discretised from restricted metrics with the reference stencil,
+// x-line zebra relaxation (even rows, odd rows, then the two boundary rows,
+// whose extrapolated cross terms reach two rows in), 1x2 averaging
+// restriction and linear-in-y prolongation (new: the reference has no
+// semicoarsened prolongation, MGPoisson2D.f90:385-388).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "mgp_kernels.h"
+#include "mgp_krylov.h"
+
+namespace mgp {
+
+namespace {
+
+constexpr int MIN_COARSE_NY = 4;
+constexpr int BOTTOM_SWEEPS = 8;
+
+}  // namespace
+
+struct KLevel {
+    Grid g;
+    Geo geo;
+    Bc bc;
+    DevBuf J, Jxx, Jxy, Jyx, Jyy, e, r, w, A, B, C;
+};
+
+struct KrylovMG::Impl {
+    std::vector<KLevel> L;
+    DevBuf r, rhat, p, v, s, t, phat, shat;
+    int sd = 2, su = 2;
+};
+
+KrylovMG::KrylovMG() : impl(nullptr) {}
+KrylovMG::~KrylovMG() { delete impl; }
+
+static double side_sign(const Bc &bc, int side) { return bc.type[side] == BC_DIRI ? -1.0 : 1.0; }
+
+void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o) {
+    if (top.g.ny < 4 || top.g.nx < 4) {
+        g_last_error = "KRYLOV_MG needs nx, ny >= 4";
+        throw Failure(MGP_ERR_ARG);
+    }
+    for (int k = 0; k < 4; ++k)
+        if (user_bc.type[k] != BC_DIRI && user_bc.type[k] != BC_NEUM) {
+            g_last_error = "KRYLOV_MG supports Dirichlet and Neumann sides";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+    if (xline_lds_bytes(top.g.nx) > 160 * 1024) {
+        g_last_error = "KRYLOV_MG: row too long for one workgroup's LDS (nx <= 4096 in FP64)";
+        throw Failure(MGP_ERR_UNSUPPORTED);
+    }
+    delete impl;
+    impl = new Impl;
+    impl->sd = o.smooth_down > 0 ? o.smooth_down : 1;
+    impl->su = o.smooth_up > 0 ? o.smooth_up : 1;
+    std::vector<KLevel> &L = impl->L;
+    int ny = top.g.ny;
+    double dy = top.g.dy;
+    const int maxlev = o.maxdepth < 0 ? 64 : o.maxdepth + 1;
+    while (true) {
+        L.emplace_back();
+        KLevel &l = L.back();
+        l.g = make_grid(top.g.nx, ny, top.g.dx, dy);
+        const size_t n = level_elems(l.g.nx, l.g.ny);
+        for (DevBuf *b : {&l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy, &l.e, &l.r, &l.w, &l.A, &l.B, &l.C}) b->alloc(n);
+        l.bc = mg_bc(user_bc, MODE_UNIFORM);
+        const size_t bytes = n * sizeof(double);
+        if (L.size() == 1) {
+            hip_check(hipMemcpyAsync(l.J.p, top.J.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+            hip_check(hipMemcpyAsync(l.Jxx.p, top.Jxx.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+            hip_check(hipMemcpyAsync(l.Jxy.p, top.Jxy.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+            hip_check(hipMemcpyAsync(l.Jyx.p, top.Jyx.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+            hip_check(hipMemcpyAsync(l.Jyy.p, top.Jyy.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+        } else {
+            const KLevel &f = L[L.size() - 2];
+            launch_restrict_cc(s, l.J.p, f.J.p, l.g, f.g.pitch, 1, 2, nullptr, nullptr);
+            launch_restrict_xf(s, l.Jxx.p, f.Jxx.p, l.g, f.g.pitch, 1, 2);
+            launch_restrict_xf(s, l.Jxy.p, f.Jxy.p, l.g, f.g.pitch, 1, 2);
+            launch_restrict_yf(s, l.Jyx.p, f.Jyx.p, l.g, f.g.pitch, 1, 2);
+            launch_restrict_yf(s, l.Jyy.p, f.Jyy.p, l.g, f.g.pitch, 1, 2);
+        }
+        l.geo = Geo{l.J.p, l.Jxx.p, l.Jxy.p, l.Jyx.p, l.Jyy.p};
+        launch_line_coef(s, l.A.p, l.B.p, l.C.p, l.g, l.geo, l.bc);
+        if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || (int)L.size() >= maxlev) break;
+        ny /= 2;
+        dy *= 2.0;
+    }
+    const size_t n0 = level_elems(top.g.nx, top.g.ny);
+    for (DevBuf *b : {&impl->r, &impl->rhat, &impl->p, &impl->v, &impl->s, &impl->t, &impl->phat, &impl->shat})
+        b->alloc(n0);
+    built = true;
+}
+
+// One x-line zebra sweep on level l for  A_l e = f.
+static void zebra_sweep(hipStream_t s, KLevel &l, double *e, const double *f) {
+    const Grid &g = l.g;
+    const int ny = g.ny;
+    launch_fill_ghosts(s, e, g, l.geo, l.bc, 1, 0);
+    // interior rows 1 .. ny-2: even ones, then odd ones
+    const int n_even = ny >= 4 ? (ny - 4) / 2 + 1 : 0;  // rows 2, 4, ..., <= ny-2
+    const int n_odd = ny >= 3 ? (ny - 3) / 2 + 1 : 0;   // rows 1, 3, ..., <= ny-2
+    launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 2, 2, n_even);
+    launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 1, 2, n_odd);
+    launch_fill_ghosts(s, e, g, l.geo, l.bc, 1, 0);
+    launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 0, ny - 1, 2);
+}
+
+static void vcycle(hipStream_t s, KrylovMG::Impl &K, size_t d, double *e, const double *f) {
+    KLevel &l = K.L[d];
+    launch_setval(s, e, 0.0, l.g, 1);
+    if (d + 1 == K.L.size()) {
+        for (int k = 0; k < BOTTOM_SWEEPS; ++k) zebra_sweep(s, l, e, f);
+        return;
+    }
+    KLevel &c = K.L[d + 1];
+    for (int k = 0; k < K.sd; ++k) zebra_sweep(s, l, e, f);
+    launch_fill_ghosts(s, e, l.g, l.geo, l.bc, 1, 0);
+    launch_laplacian(s, l.w.p, e, f, l.g, l.geo, l.bc, 1, 0);
+    launch_restrict_cc(s, c.r.p, l.w.p, c.g, l.g.pitch, 1, 2, nullptr, nullptr);
+    vcycle(s, K, d + 1, c.e.p, c.r.p);
+    launch_prolong_y(s, e, c.e.p, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
+    for (int k = 0; k < K.su; ++k) zebra_sweep(s, l, e, f);
+}
+
+// out = A_h v (homogeneous user BCs); v's ghosts are refilled.
+static void apply_h(hipStream_t s, const DevGeo &top, const Bc &ubc, double *v, double *out) {
+    launch_fill_ghosts(s, v, top.g, top.view(), ubc, 1, 0);
+    launch_laplacian(s, out, v, nullptr, top.g, top.view(), ubc, 1, 0);
+}
+
+int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
+                    double *rhs, double tol, int maxiters, int zerophi, double *relres, int *iters) {
+    if (!built) {
+        g_last_error = "solver not built";
+        throw Failure(MGP_ERR_ARG);
+    }
+    Impl &K = *impl;
+    const Grid &g = top.g;
+    const Geo geo = top.view();
+    const double vol = g.dx * g.dy;
+    auto dot = [&](const double *a, const double *b) { return W.reduce(s, a, b, nullptr, vol, g, 0); };
+    auto true_residual = [&]() {
+        launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
+        launch_laplacian(s, K.r.p, phi, rhs, g, geo, user_bc, homog, 0);
+        return pnorm2(W, s, K.r.p, g);
+    };
+    if (zerophi) launch_setval(s, phi, 0.0, g, 1);
+    const double r0 = true_residual();
+    std::vector<double> hist;
+    hist.push_back(r0 == 0.0 ? 0.0 : 1.0);
+    last_cycles = 0;
+    int outcome = MGP_MAXITERS, it = 0;
+    if (r0 == 0.0) {
+        outcome = MGP_CONVERGED;
+    } else {
+        bool restart = true;
+        double rho = 1.0, alpha = 1.0, omega = 1.0;
+        for (it = 1; it <= maxiters; ++it) {
+            if (restart) {
+                hip_check(hipMemcpyAsync(K.rhat.p, K.r.p, K.r.n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+                launch_setval(s, K.p.p, 0.0, g, 0);
+                launch_setval(s, K.v.p, 0.0, g, 0);
+                rho = alpha = omega = 1.0;
+                restart = false;
+            }
+            const double rho_new = dot(K.rhat.p, K.r.p);
+            const double beta = (rho_new / rho) * (alpha / omega);
+            rho = rho_new;
+            // p = r + beta (p - omega v)
+            launch_axpbypcz(s, K.p.p, beta, K.p.p, -beta * omega, K.v.p, 1.0, K.r.p, g);
+            vcycle(s, K, 0, K.phat.p, K.p.p);
+            apply_h(s, top, user_bc, K.phat.p, K.v.p);
+            alpha = rho / dot(K.rhat.p, K.v.p);
+            launch_axpbypcz(s, K.s.p, 1.0, K.r.p, -alpha, K.v.p, 0.0, nullptr, g);
+            launch_axpbypcz(s, phi, 1.0, phi, alpha, K.phat.p, 0.0, nullptr, g);
+            last_cycles += 1;
+            const double snorm = pnorm2(W, s, K.s.p, g);
+            if (snorm / r0 <= tol) {
+                const double tr = true_residual() / r0;
+                hist.push_back(tr);
+                if (tr <= tol) {
+                    outcome = MGP_CONVERGED;
+                    break;
+                }
+                restart = true;
+                continue;
+            }
+            vcycle(s, K, 0, K.shat.p, K.s.p);
+            apply_h(s, top, user_bc, K.shat.p, K.t.p);
+            const double tt = dot(K.t.p, K.t.p);
+            omega = tt > 0.0 ? dot(K.t.p, K.s.p) / tt : 0.0;
+            launch_axpbypcz(s, phi, 1.0, phi, omega, K.shat.p, 0.0, nullptr, g);
+            launch_axpbypcz(s, K.r.p, 1.0, K.s.p, -omega, K.t.p, 0.0, nullptr, g);
+            last_cycles += 1;
+            const double rn = pnorm2(W, s, K.r.p, g) / r0;
+            if (rn <= tol || omega == 0.0 || !std::isfinite(rn)) {
+                const double tr = true_residual() / r0;
+                hist.push_back(tr);
+                if (tr <= tol) {
+                    outcome = MGP_CONVERGED;
+                    break;
+                }
+                if (!std::isfinite(tr)) {
+                    outcome = MGP_DIVERGED;
+                    break;
+                }
+                restart = true;
+                continue;
+            }
+            hist.push_back(rn);
+        }
+        if (it > maxiters) it = maxiters;
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (relres)
+        for (size_t k = 0; k < hist.size() && (int)k <= maxiters; ++k) relres[k] = hist[k];
+    if (iters) *iters = (int)hist.size() - 1 < maxiters ? (int)hist.size() - 1 : maxiters;
+    return outcome;
+}
+
+double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, double *bytes) {
+    if (!built) {
+        g_last_error = "solver not built";
+        throw Failure(MGP_ERR_ARG);
+    }
+    KLevel &l = impl->L[0];
+    const Grid &g = l.g;
+    const double nx = g.nx;
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&b), "event");
+    const int n_even = g.ny >= 4 ? (g.ny - 4) / 2 + 1 : 0;
+    launch_fill_ghosts(s, l.e.p, g, l.geo, l.bc, 1, 0);
+    if (kernel == 0) {
+        // rows 2, 4, ..: reads e on rows 1 .. 2*n_even+1, writes e on the updated
+        // rows; f, Jxx, Jxy, A, B, C on the updated rows; Jyx, Jyy on face rows
+        // 2 .. 2*n_even+1.
+        const double R = n_even;
+        *bytes = nx * 8.0 * ((2.0 * R + 1.0) + R + R + 2.0 * R + 3.0 * R) + nx * 16.0 * (2.0 * R);
+    } else {
+        // r = f - L e over all cells: e (with its one-cell halo), f, r, J-free metrics
+        *bytes = nx * (double)g.ny * 8.0 * 3.0 + 16.0 * (nx + 1) * g.ny + 16.0 * nx * (g.ny + 1);
+    }
+    auto launch = [&]() {
+        if (kernel == 0) launch_xline(s, l.e.p, l.r.p, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 2, 2, n_even);
+        else launch_laplacian(s, l.w.p, l.e.p, l.r.p, g, l.geo, l.bc, 1, 0);
+    };
+    launch();  // warm
+    hip_check(hipEventRecord(a, s), "event");
+    for (int k = 0; k < reps; ++k) launch();
+    hip_check(hipEventRecord(b, s), "event");
+    hip_check(hipEventSynchronize(b), "event");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, a, b), "event");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
+}
+
+}  // namespace mgp

@@ -1,0 +1,201 @@
+"""ctypes front end for the two CPU checkers used by the parity tests.
+
+* ``oracle/liboracle.so``            the C restatement (``mgo_*`` entry points)
+* ``oracle/_ref/parity/libmgref.so`` the unmodified reference Fortran plus a
+  bind(C) bridge (``ref_*``), built from /root/reference by oracle/Makefile;
+  ``oracle/_ref/rb/libmgref_rb.so`` is the same with the DC smoother's
+  red-black switch on.
+
+Both expose the same argument order, so one ``Checker`` drives either.  Arrays
+use the numpy layouts documented in mgpoisson2d_amd/geometry.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "parity", "libmgref.so")
+REF_RB_SO = os.path.join(ROOT, "oracle", "_ref", "rb", "libmgref_rb.so")
+REF_FAST_SO = os.path.join(ROOT, "oracle", "_ref", "fast", "libmgref_fast.so")
+
+LEX, RB, MC = 0, 1, 2
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return _p(a.ctypes.data)
+
+
+def _iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return _p(a.ctypes.data)
+
+
+class Checker:
+    """One CPU implementation of the reference API (oracle or reference)."""
+
+    def __init__(self, path, prefix):
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        self.is_oracle = prefix == "mgo_"
+        self._keep = []
+
+    def _fn(self, name, restype=None):
+        f = getattr(self.lib, self.prefix + name)
+        f.restype = restype
+        return f
+
+    @staticmethod
+    def _geo_args(geo):
+        return [_i(geo.nx), _i(geo.ny), _d(geo.dx), _d(geo.dy)] + [_ptr(a) for a in geo.arrays()]
+
+    def _bc_args(self, bc):
+        bct = np.ascontiguousarray(bc.types, dtype=np.int32)
+        bcm = np.ascontiguousarray(bc.modes, dtype=np.int32)
+        data = [np.ascontiguousarray(d, dtype=np.float64) for d in bc.data]
+        self._keep = [bct, bcm] + data
+        return [_iptr(bct), _iptr(bcm)] + [_ptr(d) for d in data]
+
+    # -- operators -----------------------------------------------------------
+    def fill_ghosts(self, geo, bc, homog, do_neum, phi):
+        phi = phi.copy()
+        self._fn("fill_ghosts")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _i(do_neum), _ptr(phi))
+        return phi
+
+    def extrapolate_ghosts(self, nx, ny, order, phi):
+        phi = phi.copy()
+        self._fn("extrapolate_ghosts")(_i(nx), _i(ny), _i(order), _ptr(phi))
+        return phi
+
+    def laplacian(self, geo, bc, homog, phi, jscale=False):
+        phi = phi.copy()
+        lap = np.zeros((geo.ny, geo.nx))
+        self._fn("laplacian")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _i(jscale), _ptr(phi), _ptr(lap))
+        return lap, phi
+
+    def residual(self, geo, bc, homog, rhs, phi):
+        phi = phi.copy()
+        res = np.zeros((geo.ny, geo.nx))
+        rhs = np.ascontiguousarray(rhs)
+        self._fn("residual")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(rhs), _ptr(phi), _ptr(res))
+        return res, phi
+
+    def invdiags(self, geo, bc, homog):
+        out = np.zeros((geo.ny, geo.nx))
+        self._fn("invdiags")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(out))
+        return out
+
+    def relax_gs(self, geo, bc, homog, invd, omega, iters, rhs, phi, order=LEX):
+        phi = phi.copy()
+        args = [*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(np.ascontiguousarray(invd)),
+                _d(omega), _i(iters), _ptr(np.ascontiguousarray(rhs)), _ptr(phi)]
+        if self.is_oracle:
+            args.append(_i(order))
+        elif order != LEX:
+            raise ValueError("the reference relax_gs is lexicographic")
+        self._fn("relax_gs")(*args)
+        return phi
+
+    def dc_laplacian(self, geo, bc, homog, phi):
+        phi = phi.copy()
+        lap = np.zeros((geo.ny, geo.nx))
+        self._fn("dc_laplacian")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(phi), _ptr(lap))
+        return lap, phi
+
+    def dc_invdiags(self, geo):
+        out = np.zeros((geo.ny, geo.nx))
+        self._fn("dc_invdiags")(*self._geo_args(geo), _ptr(out))
+        return out
+
+    def dc_relax_gs(self, geo, bc, homog, invd, omega, iters, rhs, phi, order=LEX):
+        phi = phi.copy()
+        args = [*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(np.ascontiguousarray(invd)),
+                _d(omega), _i(iters), _ptr(np.ascontiguousarray(rhs)), _ptr(phi)]
+        if self.is_oracle:
+            args.append(_i(order))
+        self._fn("dc_relax_gs")(*args)
+        return phi
+
+    def restrict(self, kind, fnx, fny, fine, cnx, cny):
+        shape = {0: (cny, cnx), 1: (cny, cnx + 1), 2: (cny + 1, cnx)}[kind]
+        crse = np.zeros(shape)
+        fine = np.ascontiguousarray(fine)
+        self._fn("restrict")(_i(kind), _i(fnx), _i(fny), _ptr(fine), _i(cnx), _i(cny), _ptr(crse))
+        return crse
+
+    def prolong(self, fgeo, cgeo, bct, bcm, order, fine, crse):
+        fine = fine.copy()
+        crse = crse.copy()
+        t = np.ascontiguousarray(bct, dtype=np.int32)
+        m = np.ascontiguousarray(bcm, dtype=np.int32)
+        self._fn("prolong")(_i(fgeo.nx), _i(fgeo.ny), _d(fgeo.dx), _d(fgeo.dy), *[_ptr(a) for a in fgeo.arrays()],
+                            _i(cgeo.nx), _i(cgeo.ny), _d(cgeo.dx), _d(cgeo.dy), *[_ptr(a) for a in cgeo.arrays()],
+                            _iptr(t), _iptr(m), _i(order), _ptr(fine), _ptr(crse))
+        return fine, crse
+
+    def schedule(self, nx, ny, dx, dy, maxdepth=-1):
+        rx = np.zeros(64, dtype=np.int32)
+        ry = np.zeros(64, dtype=np.int32)
+        md = _i(0)
+        self._fn("schedule")(_i(nx), _i(ny), _d(dx), _d(dy), _i(maxdepth), ctypes.byref(md), _iptr(rx), _iptr(ry))
+        return md.value, rx[: md.value].tolist(), ry[: md.value].tolist()
+
+    def pnorm(self, nx, ny, dx, dy, a, p):
+        return self._fn("pnorm", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(np.ascontiguousarray(a)), _i(p))
+
+    def bicgstab2(self, kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi, order=LEX):
+        phi = phi.copy()
+        args = [_i(kind), *self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(np.ascontiguousarray(invd)),
+                _d(tol), _i(maxiters), _i(maxrestarts), _i(zerophi), _ptr(np.ascontiguousarray(rhs)), _ptr(phi)]
+        if self.is_oracle:
+            args.append(_i(order))
+        self._fn("bicgstab2")(*args)
+        return phi
+
+    def vcycle(self, kind, geo, bc, homog, tol, maxiters, phi, rhs, maxdepth=-1, numcycles=1,
+               sdown=2, sup=2, sbottom=2, zerophi=False, order=LEX):
+        """Returns (phi, rhs_after, relres history or None, iterations or None)."""
+        phi = phi.copy()
+        rhs = np.ascontiguousarray(rhs).copy()
+        args = [_i(kind), *self._geo_args(geo), *self._bc_args(bc), _i(homog), _d(tol), _i(maxiters),
+                _i(maxdepth), _i(numcycles), _i(sdown), _i(sup), _i(sbottom), _i(zerophi), _ptr(phi), _ptr(rhs)]
+        if self.is_oracle:
+            relres = np.zeros(maxiters + 1)
+            iters = _i(0)
+            args += [_i(order), _ptr(relres), ctypes.byref(iters)]
+            status = self._fn("vcycle", _i)(*args)
+            return phi, rhs, relres[: iters.value + 1], iters.value, status
+        self._fn("vcycle")(*args)
+        return phi, rhs, None, None, None
+
+
+_cache = {}
+
+
+def oracle():
+    if "o" not in _cache:
+        _cache["o"] = Checker(ORACLE_SO, "mgo_")
+    return _cache["o"]
+
+
+def reference(rb=False):
+    key = "rrb" if rb else "r"
+    if key not in _cache:
+        path = REF_RB_SO if rb else REF_SO
+        _cache[key] = Checker(path, "ref_")
+    return _cache[key]
+
+
+def reference_available(rb=False):
+    return os.path.exists(REF_RB_SO if rb else REF_SO)

@@ -1,0 +1,140 @@
+"""GPU (HIP library, through the C ABI) against the CPU checkers.
+
+Kernel-level results must equal the oracle bit for bit (all code is built
+with -ffp-contract=off and keeps the reference's operand order); where the
+reference build is present (oracle/_ref) it is compared too.  Cycle-level
+results use parallel compensated reductions for the norms, so relres
+histories agree to 1e-12 relative and solutions to 1e-12 of their scale.
+"""
+import numpy as np
+import pytest
+
+import refcall as rc
+from cases import CYCLE_CASES, OPERATOR_CASES, coarse_geo, setup
+
+pytestmark = pytest.mark.gpu
+
+mg = pytest.importorskip("mgpoisson2d_amd.capi")
+
+
+def checkers():
+    out = [rc.oracle()]
+    if rc.reference_available():
+        out.append(rc.reference())
+    return out
+
+
+def same(a, b):
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+@pytest.mark.parametrize("homog", [1, 0])
+def test_operator_fill_invdiags(case, homog):
+    geo, bc, phi, rhs = setup(*case, homog=homog, seed=1)
+    lap, phig = mg.compute_laplacian(geo, bc, homog, phi)
+    res, _ = mg.compute_residual(geo, bc, homog, rhs, phi)
+    inv = mg.compute_invdiags(geo, bc, homog)
+    fn = mg.fill_ghosts(geo, bc, homog, 1, phi)
+    for c in checkers():
+        l2, p2 = c.laplacian(geo, bc, homog, phi)
+        same(lap, l2)
+        same(phig, p2)
+        same(res, c.residual(geo, bc, homog, rhs, phi)[0])
+        same(inv, c.invdiags(geo, bc, homog))
+        same(fn, c.fill_ghosts(geo, bc, homog, 1, phi))
+
+
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+def test_relax_gs_lex_and_mc(case):
+    geo, bc, phi, rhs = setup(*case, seed=2)
+    inv = mg.compute_invdiags(geo, bc, 1)
+    lex = mg.relax_gs(geo, bc, 1, inv, 1.0, 3, rhs, phi, mg.ORDER_LEX)
+    for c in checkers():
+        same(lex, c.relax_gs(geo, bc, 1, inv, 1.0, 3, rhs, phi))
+    mc = mg.relax_gs(geo, bc, 1, inv, 2.0 / 3.0, 3, rhs, phi, mg.ORDER_MC)
+    same(mc, rc.oracle().relax_gs(geo, bc, 1, inv, 2.0 / 3.0, 3, rhs, phi, rc.MC))
+
+
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+def test_dc_operator_and_smoothers(case):
+    from mgpoisson2d_amd import geometry as G
+    geo, bc, phi, rhs = setup(*case, seed=3)
+    lah = G.BoundaryConditions.uniform(case[4], mode=G.BCMODE_LAH)
+    lap, p1 = mg.dc_compute_laplacian(geo, lah, 1, phi)
+    inv = mg.dc_compute_invdiags(geo)
+    lex = mg.dc_relax_gs(geo, lah, 1, inv, 0.8, 3, rhs, phi, mg.ORDER_LEX)
+    rb = mg.dc_relax_gs(geo, lah, 1, inv, 0.8, 3, rhs, phi, mg.ORDER_RB)
+    o = rc.oracle()
+    l2, p2 = o.dc_laplacian(geo, lah, 1, phi)
+    same(lap, l2)
+    same(p1, p2)
+    same(inv, o.dc_invdiags(geo))
+    same(lex, o.dc_relax_gs(geo, lah, 1, inv, 0.8, 3, rhs, phi, rc.LEX))
+    same(rb, o.dc_relax_gs(geo, lah, 1, inv, 0.8, 3, rhs, phi, rc.RB))
+    if rc.reference_available(rb=True):
+        same(lex, rc.reference().dc_relax_gs(geo, lah, 1, inv, 0.8, 3, rhs, phi))
+        same(rb, rc.reference(rb=True).dc_relax_gs(geo, lah, 1, inv, 0.8, 3, rhs, phi))
+
+
+@pytest.mark.parametrize("case", OPERATOR_CASES[:4])
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_restrict_prolong(case, mode, order):
+    geo, bc, phi, rhs = setup(*case, seed=4)
+    o = rc.oracle()
+    cgeo = coarse_geo(mg.restrict, geo)
+    cgeo_o = coarse_geo(o.restrict, geo)
+    for a, b in zip(cgeo.arrays(), cgeo_o.arrays()):
+        same(a, b)
+    rng = np.random.default_rng(5)
+    fine = rng.standard_normal((geo.ny + 2, geo.nx + 2))
+    crse = rng.standard_normal((cgeo.ny + 2, cgeo.nx + 2))
+    f1, c1 = mg.prolong(geo, cgeo, case[4], (mode,) * 4, order, fine, crse)
+    f2, c2 = o.prolong(geo, cgeo, case[4], (mode,) * 4, order, fine, crse)
+    if all(t == 0 for t in case[4]):
+        # mean removal: parallel compensated sum vs the reference's sequential sum
+        np.testing.assert_allclose(f1, f2, rtol=0, atol=1e-13 * np.max(np.abs(f2[1:-1, 1:-1])))
+    else:
+        same(f1, f2)
+    same(c1, c2)
+
+
+def test_schedule():
+    o = rc.oracle()
+    for nx, ny in [(64, 64), (256, 64), (8192, 2048), (96, 40), (2048, 2048), (12, 7)]:
+        assert mg.create_mgschedule(nx, ny, 1.0 / nx, 1.0 / ny) == o.schedule(nx, ny, 1.0 / nx, 1.0 / ny)
+
+
+@pytest.mark.parametrize("case", CYCLE_CASES)
+@pytest.mark.parametrize("kind", [0, 1])
+def test_bicgstab2_bottom(case, kind):
+    from mgpoisson2d_amd import geometry as G
+    nx = ny = 8
+    geo, bc, phi, rhs = setup(nx, ny, *case[2:], seed=6)
+    if kind == 1:
+        bc = G.BoundaryConditions.uniform(case[4], mode=G.BCMODE_LAH)
+        inv = mg.dc_compute_invdiags(geo)
+    else:
+        inv = mg.compute_invdiags(geo, bc, 1)
+    p1 = mg.solve_bicgstab2(kind, geo, bc, 1, inv, 1e-6, 80, 5, 1, rhs, phi)
+    p2 = rc.oracle().bicgstab2(kind, geo, bc, 1, inv, 1e-6, 80, 5, 1, rhs, phi)
+    same(p1, p2)
+
+
+@pytest.mark.parametrize("case", CYCLE_CASES)
+@pytest.mark.parametrize("kind,ordering", [(0, 0), (0, 2), (1, 0), (1, 1)])
+def test_vcycle_history(case, kind, ordering):
+    geo, bc, phi, rhs = setup(*case, seed=7)
+    maxit = 6
+    o = rc.oracle()
+    g = mg.vcycle(kind, geo, bc, 1, 0.0, maxit, phi, rhs, zerophi=True, ordering=ordering)
+    c = o.vcycle(kind, geo, bc, 1, 0.0, maxit, phi, rhs, zerophi=True, order=ordering)
+    assert g[3] == c[3] and g[4] == c[4]
+    np.testing.assert_allclose(g[2], c[2], rtol=1e-12, atol=0)
+    scale = np.max(np.abs(c[0][1:-1, 1:-1]))
+    np.testing.assert_allclose(g[0][1:-1, 1:-1], c[0][1:-1, 1:-1], rtol=0, atol=1e-12 * scale)
+    np.testing.assert_array_equal(g[1], c[1])
+    if ordering == 0 and rc.reference_available():
+        r = rc.reference().vcycle(kind, geo, bc, 1, 0.0, maxit, phi, rhs, zerophi=True)
+        np.testing.assert_allclose(g[0][1:-1, 1:-1], r[0][1:-1, 1:-1], rtol=0, atol=1e-12 * scale)

@@ -1,0 +1,105 @@
+"""The resident solvers on the GPU.
+
+* KRYLOV_MG (FP64 BiCGStab + semicoarsening line-relaxation V-cycles) must
+  reach the requested relative residual of the reference operator, checked
+  independently with the oracle's operator (bit-identical to the reference),
+  and its solution must equal a sparse direct solve of that operator to
+  1e-10 relative (closed check: scipy spsolve on the probed matrix).
+* The reference drivers on resident data (VCYCLE, DC_VCYCLE) reproduce the
+  oracle's relres history.
+* Alone-vs-again determinism: solving twice gives identical bits.
+"""
+import numpy as np
+import pytest
+
+import refcall as rc
+from cases import setup
+
+pytestmark = pytest.mark.gpu
+
+mg = pytest.importorskip("mgpoisson2d_amd.capi")
+
+
+def probed_matrix(geo, bc):
+    """Sparse matrix of the reference operator (homogeneous BCs) by probing the
+    oracle with 25 interleaved impulse patterns (the stencil reaches 2 cells)."""
+    import scipy.sparse as sp
+    o = rc.oracle()
+    nx, ny = geo.nx, geo.ny
+    rows, cols, vals = [], [], []
+    for ci in range(5):
+        for cj in range(5):
+            phi = np.zeros((ny + 2, nx + 2))
+            phi[1 + cj:ny + 1:5, 1 + ci:nx + 1:5] = 1.0
+            lap, _ = o.laplacian(geo, bc, 1, phi)
+            J, I = np.nonzero(lap)
+            d = (I - ci) % 5
+            d = np.where(d > 2, d - 5, d)
+            e = (J - cj) % 5
+            e = np.where(e > 2, e - 5, e)
+            si, sj = I - d, J - e
+            ok = (si >= 0) & (si < nx) & (sj >= 0) & (sj < ny)
+            rows.append(J[ok] * nx + I[ok])
+            cols.append(sj[ok] * nx + si[ok])
+            vals.append(lap[J[ok], I[ok]])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nx * ny,) * 2)
+
+
+def relres_oracle(geo, bc, rhs, phi):
+    o = rc.oracle()
+    r, _ = o.residual(geo, bc, 1, rhs, phi)
+    r0, _ = o.residual(geo, bc, 1, rhs, np.zeros_like(phi))
+    return np.sqrt(np.sum(r * r)) / np.sqrt(np.sum(r0 * r0))
+
+
+@pytest.mark.parametrize("case", [
+    (64, 64, 0.0, 0.0, (1, 1, 1, 1)),
+    (128, 128, 0.10, 5.0, (1, 1, 0, 0)),
+    (96, 64, 0.12, 3.0, (1, 1, 0, 0)),
+    (64, 128, 0.05, 2.0, (1, 0, 1, 0)),
+])
+def test_krylov_mg_reaches_tolerance(case):
+    geo, bc, _, rhs = setup(*case, seed=11)
+    s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
+    s.set_rhs(rhs)
+    hist, iters, outcome = s.solve(1e-10, 200, zerophi=True)
+    assert outcome == mg.CONVERGED, (hist, iters)
+    phi = s.get_phi()
+    assert relres_oracle(geo, bc, rhs, phi) <= 1.5e-10
+    # determinism: a second solve is bit-identical
+    s.solve(1e-10, 200, zerophi=True)
+    np.testing.assert_array_equal(s.get_phi()[1:-1, 1:-1], phi[1:-1, 1:-1])
+
+
+def test_krylov_mg_matches_direct_solve():
+    import scipy.sparse.linalg as spl
+    geo, bc, _, rhs = setup(48, 48, 0.1, 4.0, (1, 1, 0, 0), seed=12)
+    A = probed_matrix(geo, bc)
+    exact = spl.spsolve(A.tocsc(), rhs.ravel()).reshape(geo.ny, geo.nx)
+    s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
+    s.set_rhs(rhs)
+    _, _, outcome = s.solve(1e-12, 300, zerophi=True)
+    assert outcome == mg.CONVERGED
+    phi = s.get_phi()[1:-1, 1:-1]
+    np.testing.assert_allclose(phi, exact, rtol=0, atol=1e-10 * np.max(np.abs(exact)))
+
+
+def test_krylov_mg_all_neumann_compatible():
+    geo, bc, _, rhs = setup(64, 64, 0.12, 3.0, (0, 0, 0, 0), seed=13)
+    s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
+    s.set_rhs(rhs)
+    hist, iters, outcome = s.solve(1e-10, 300, zerophi=True)
+    assert outcome == mg.CONVERGED, (hist, iters)
+    assert relres_oracle(geo, bc, rhs, s.get_phi()) <= 1.5e-10
+
+
+@pytest.mark.parametrize("method,ordering", [(0, 0), (0, 2), (1, 0), (1, 1)])
+def test_resident_reference_drivers(method, ordering):
+    geo, bc, _, rhs = setup(64, 64, 0.0, 0.0, (1, 1, 1, 1), seed=14)
+    s = mg.Solver(geo, bc, method=method, ordering=ordering)
+    s.set_rhs(rhs)
+    hist, iters, outcome = s.solve(1e-10, 30, zerophi=True)
+    _, _, h2, it2, oc2 = rc.oracle().vcycle(method, geo, bc, 1, 1e-10, 30, np.zeros((66, 66)), rhs,
+                                            zerophi=True, order=ordering)
+    assert (iters, outcome) == (it2, oc2)
+    np.testing.assert_allclose(hist, h2, rtol=1e-12, atol=0)
