@@ -135,6 +135,7 @@ typedef struct {
     int smooth_down, smooth_up, smooth_bottom;
     int maxdepth;       /* -1: as deep as the schedule allows */
     int homog;          /* BCs of the top-level residual */
+    int use_graph;      /* KRYLOV_MG: replay each iteration as a captured hipGraph */
 } mgp_solver_opts;
 
 void mgp_solver_default_opts(mgp_solver_opts *opts);

@@ -41,7 +41,7 @@ class _Bc(ctypes.Structure):
 
 class SolverOpts(ctypes.Structure):
     _fields_ = [("method", _i), ("ordering", _i), ("numcycles", _i), ("smooth_down", _i),
-                ("smooth_up", _i), ("smooth_bottom", _i), ("maxdepth", _i), ("homog", _i)]
+                ("smooth_up", _i), ("smooth_bottom", _i), ("maxdepth", _i), ("homog", _i), ("use_graph", _i)]
 
 
 def _load():
@@ -239,10 +239,10 @@ class Solver:
     the device between solves."""
 
     def __init__(self, geo, bc, method=METHOD_KRYLOV_MG, ordering=ORDER_MC, numcycles=1, smooth_down=2,
-                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True):
+                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True, use_graph=True):
         self.geo = geo
         opts = SolverOpts(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth,
-                          int(homog))
+                          int(homog), int(use_graph))
         a = _Args()
         self._h = ctypes.c_void_p()
         _check(lib.mgp_solver_create(ctypes.byref(self._h), a.geo(geo), a.bc(bc), ctypes.byref(opts)))

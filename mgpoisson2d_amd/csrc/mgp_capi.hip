@@ -424,6 +424,7 @@ void mgp_solver_default_opts(mgp_solver_opts *o) {
     o->smooth_bottom = 2;
     o->maxdepth = -1;
     o->homog = 1;
+    o->use_graph = 1;
 }
 
 int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts) {

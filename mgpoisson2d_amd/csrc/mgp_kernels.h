@@ -54,6 +54,13 @@ void launch_reduce(hipStream_t s, double *out, double *partial, const double *a,
 void launch_seq_mean(hipStream_t s, double *out, const double *fine, const double *J, const Grid &g, double dxprod);
 void launch_sub_mean(hipStream_t s, double *fine, const double *sums, const Grid &g);
 
+// BiCGStab device scalars
+enum { SC_RHO = 0, SC_RHO_NEW = 1, SC_ALPHA = 2, SC_OMEGA = 3, SC_BETA = 4, SC_RV = 5, SC_TS = 6, SC_TT = 7,
+       SC_RN = 8, SC_COUNT = 16 };
+void launch_bicg_scalars(hipStream_t s, double *sc, int op);
+void launch_bicg_update(hipStream_t s, double *out, const double *x, const double *y, const double *sc,
+                        const Grid &g, int op);
+
 struct BottomArgs {
     Grid g;
     Geo geo;

@@ -1012,6 +1012,49 @@ __global__ void __launch_bounds__(256) k_axpbypcz(double *out, double a, const d
     out[q] = v;
 }
 
+// ---------------------------------------------------------------------------
+// BiCGStab scalars kept in device memory (so an iteration needs no host
+// round trip and can be replayed as a graph).  Layout of sc[]: see SC_* in
+// mgp_kernels.h.  Reductions deliver sum(vol * a * b).
+// ---------------------------------------------------------------------------
+__global__ void k_bicg_scalars(double *sc, int op) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (op == 0) {          // beta = (rho_new/rho)(alpha/omega); rho = rho_new
+        sc[SC_BETA] = (sc[SC_RHO_NEW] / sc[SC_RHO]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
+        sc[SC_RHO] = sc[SC_RHO_NEW];
+    } else if (op == 1) {   // alpha = rho / (rhat . v)
+        sc[SC_ALPHA] = sc[SC_RHO] / sc[SC_RV];
+    } else if (op == 2) {   // omega = (t . s) / (t . t)
+        sc[SC_OMEGA] = sc[SC_TT] > 0.0 ? sc[SC_TS] / sc[SC_TT] : 0.0;
+    } else {                // restart
+        sc[SC_RHO] = 1.0;
+        sc[SC_ALPHA] = 1.0;
+        sc[SC_OMEGA] = 1.0;
+        sc[SC_BETA] = 0.0;
+    }
+}
+
+// Vector updates with device scalars (valid cells):
+//  0: p = r + beta (p - omega v)     (out=p, x=r, y=v)
+//  1: s = r - alpha v                (out=s, x=r, y=v)
+//  2: phi = phi + alpha phat         (out=phi, x=phat)
+//  3: phi = phi + omega shat         (out=phi, x=shat)
+//  4: r = s - omega t                (out=r, x=s, y=t)
+__global__ void __launch_bounds__(256) k_bicg_update(double *out, const double *x, const double *y,
+                                                     const double *sc, Grid g, int op) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= g.nx || j >= g.ny) return;
+    const size_t c = ix(g.pitch, i, j);
+    switch (op) {
+    case 0: out[c] = x[c] + sc[SC_BETA] * (out[c] - sc[SC_OMEGA] * y[c]); break;
+    case 1: out[c] = x[c] - sc[SC_ALPHA] * y[c]; break;
+    case 2: out[c] = out[c] + sc[SC_ALPHA] * x[c]; break;
+    case 3: out[c] = out[c] + sc[SC_OMEGA] * x[c]; break;
+    default: out[c] = x[c] - sc[SC_OMEGA] * y[c]; break;
+    }
+}
+
 }  // namespace mgp
 
 // ===========================================================================
@@ -1118,6 +1161,11 @@ void launch_reduce(hipStream_t s, double *out, double *partial, const double *a,
     const int nb = reduce_blocks(g);
     k_reduce_partial<<<nb, 256, 0, s>>>(partial, a, b, J, vol, g, kind);
     k_reduce_final<<<1, 64, 0, s>>>(out, partial, nb, kind);
+}
+void launch_bicg_scalars(hipStream_t s, double *sc, int op) { k_bicg_scalars<<<1, 64, 0, s>>>(sc, op); }
+void launch_bicg_update(hipStream_t s, double *out, const double *x, const double *y, const double *sc,
+                        const Grid &g, int op) {
+    k_bicg_update<<<tiles(g.nx, g.ny), 256, 0, s>>>(out, x, y, sc, g, op);
 }
 void launch_seq_mean(hipStream_t s, double *out, const double *fine, const double *J, const Grid &g, double dxprod) {
     k_seq_mean<<<1, 64, 0, s>>>(out, fine, J, g, dxprod);

@@ -39,8 +39,15 @@ struct KLevel {
 
 struct KrylovMG::Impl {
     std::vector<KLevel> L;
-    DevBuf r, rhat, p, v, s, t, phat, shat;
+    DevBuf r, rhat, p, v, s, t, phat, shat, scal, partial;
+    double *host = nullptr;  // pinned copy of the device scalars
     int sd = 2, su = 2;
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;
+    ~Impl() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (host) (void)hipHostFree(host);
+    }
 };
 
 KrylovMG::KrylovMG() : impl(nullptr) {}
@@ -101,6 +108,10 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
     const size_t n0 = level_elems(top.g.nx, top.g.ny);
     for (DevBuf *b : {&impl->r, &impl->rhat, &impl->p, &impl->v, &impl->s, &impl->t, &impl->phat, &impl->shat})
         b->alloc(n0);
+    impl->scal.alloc(SC_COUNT);
+    impl->partial.alloc((size_t)reduce_blocks(top.g) + 8);
+    hip_check(hipHostMalloc((void **)&impl->host, SC_COUNT * sizeof(double)), "hipHostMalloc");
+    impl->use_graph = o.use_graph != 0;
     built = true;
 }
 
@@ -151,61 +162,74 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     const Grid &g = top.g;
     const Geo geo = top.view();
     const double vol = g.dx * g.dy;
-    auto dot = [&](const double *a, const double *b) { return W.reduce(s, a, b, nullptr, vol, g, 0); };
+    double *sc = K.scal.p;
+    double *partial = K.partial.p;
     auto true_residual = [&]() {
         launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
         launch_laplacian(s, K.r.p, phi, rhs, g, geo, user_bc, homog, 0);
         return pnorm2(W, s, K.r.p, g);
     };
+    // One BiCGStab iteration with every scalar on the device (right
+    // preconditioning, M = one V-cycle).  Ends with ||r||^2 * vol in sc[SC_RN].
+    auto body = [&]() {
+        launch_reduce(s, sc + SC_RHO_NEW, partial, K.rhat.p, K.r.p, nullptr, vol, g, 0);
+        launch_bicg_scalars(s, sc, 0);
+        launch_bicg_update(s, K.p.p, K.r.p, K.v.p, sc, g, 0);
+        vcycle(s, K, 0, K.phat.p, K.p.p);
+        apply_h(s, top, user_bc, K.phat.p, K.v.p);
+        launch_reduce(s, sc + SC_RV, partial, K.rhat.p, K.v.p, nullptr, vol, g, 0);
+        launch_bicg_scalars(s, sc, 1);
+        launch_bicg_update(s, K.s.p, K.r.p, K.v.p, sc, g, 1);
+        launch_bicg_update(s, phi, K.phat.p, nullptr, sc, g, 2);
+        vcycle(s, K, 0, K.shat.p, K.s.p);
+        apply_h(s, top, user_bc, K.shat.p, K.t.p);
+        launch_reduce(s, sc + SC_TS, partial, K.t.p, K.s.p, nullptr, vol, g, 0);
+        launch_reduce(s, sc + SC_TT, partial, K.t.p, K.t.p, nullptr, vol, g, 0);
+        launch_bicg_scalars(s, sc, 2);
+        launch_bicg_update(s, phi, K.shat.p, nullptr, sc, g, 3);
+        launch_bicg_update(s, K.r.p, K.s.p, K.t.p, sc, g, 4);
+        launch_reduce(s, sc + SC_RN, partial, K.r.p, K.r.p, nullptr, vol, g, 0);
+    };
+    if (K.use_graph && !K.graph) {
+        hipGraph_t graph = nullptr;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(s, &graph), "capture end");
+        hip_check(hipGraphInstantiate(&K.graph, graph, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(graph);
+    }
+
     if (zerophi) launch_setval(s, phi, 0.0, g, 1);
     const double r0 = true_residual();
     std::vector<double> hist;
     hist.push_back(r0 == 0.0 ? 0.0 : 1.0);
     last_cycles = 0;
-    int outcome = MGP_MAXITERS, it = 0;
+    int outcome = MGP_MAXITERS;
     if (r0 == 0.0) {
         outcome = MGP_CONVERGED;
     } else {
         bool restart = true;
-        double rho = 1.0, alpha = 1.0, omega = 1.0;
-        for (it = 1; it <= maxiters; ++it) {
+        for (int it = 1; it <= maxiters; ++it) {
             if (restart) {
                 hip_check(hipMemcpyAsync(K.rhat.p, K.r.p, K.r.n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
                 launch_setval(s, K.p.p, 0.0, g, 0);
                 launch_setval(s, K.v.p, 0.0, g, 0);
-                rho = alpha = omega = 1.0;
+                launch_bicg_scalars(s, sc, 3);
                 restart = false;
             }
-            const double rho_new = dot(K.rhat.p, K.r.p);
-            const double beta = (rho_new / rho) * (alpha / omega);
-            rho = rho_new;
-            // p = r + beta (p - omega v)
-            launch_axpbypcz(s, K.p.p, beta, K.p.p, -beta * omega, K.v.p, 1.0, K.r.p, g);
-            vcycle(s, K, 0, K.phat.p, K.p.p);
-            apply_h(s, top, user_bc, K.phat.p, K.v.p);
-            alpha = rho / dot(K.rhat.p, K.v.p);
-            launch_axpbypcz(s, K.s.p, 1.0, K.r.p, -alpha, K.v.p, 0.0, nullptr, g);
-            launch_axpbypcz(s, phi, 1.0, phi, alpha, K.phat.p, 0.0, nullptr, g);
-            last_cycles += 1;
-            const double snorm = pnorm2(W, s, K.s.p, g);
-            if (snorm / r0 <= tol) {
-                const double tr = true_residual() / r0;
-                hist.push_back(tr);
-                if (tr <= tol) {
-                    outcome = MGP_CONVERGED;
-                    break;
-                }
-                restart = true;
-                continue;
-            }
-            vcycle(s, K, 0, K.shat.p, K.s.p);
-            apply_h(s, top, user_bc, K.shat.p, K.t.p);
-            const double tt = dot(K.t.p, K.t.p);
-            omega = tt > 0.0 ? dot(K.t.p, K.s.p) / tt : 0.0;
-            launch_axpbypcz(s, phi, 1.0, phi, omega, K.shat.p, 0.0, nullptr, g);
-            launch_axpbypcz(s, K.r.p, 1.0, K.s.p, -omega, K.t.p, 0.0, nullptr, g);
-            last_cycles += 1;
-            const double rn = pnorm2(W, s, K.r.p, g) / r0;
+            if (K.graph) hip_check(hipGraphLaunch(K.graph, s), "graph launch");
+            else body();
+            last_cycles += 2;
+            hip_check(hipMemcpyAsync(K.host, sc, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s), "scalars");
+            hip_check(hipStreamSynchronize(s), "sync");
+            const double rn = std::sqrt(K.host[SC_RN]) / r0;
+            const double omega = K.host[SC_OMEGA];
             if (rn <= tol || omega == 0.0 || !std::isfinite(rn)) {
                 const double tr = true_residual() / r0;
                 hist.push_back(tr);
@@ -217,12 +241,11 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
                     outcome = MGP_DIVERGED;
                     break;
                 }
-                restart = true;
+                restart = true;  // recursive residual drifted: restart from the true one
                 continue;
             }
             hist.push_back(rn);
         }
-        if (it > maxiters) it = maxiters;
     }
     hip_check(hipStreamSynchronize(s), "sync");
     if (relres)

@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every entry point include/mgp2d.h
+declares; host-only entry points answer without a GPU."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "mgp2d.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mgp_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from mgpoisson2d_amd import capi
+    names = declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(capi.lib, n)]
+    assert not missing, missing
+    assert set(capi.EXPORTS) == set(names)
+
+
+def test_host_only_entry_points():
+    from mgpoisson2d_amd import capi
+    assert capi.lib.mgp_status_string(0) == b"ok"
+    # the schedule is host code (MGPoisson2D.f90:38-129)
+    assert capi.create_mgschedule(64, 64, 1 / 64, 1 / 64) == (4, [2, 2, 2, 2], [2, 2, 2, 2])
+    md, rx, ry = capi.create_mgschedule(8192, 2048, 1 / 8192, 1 / 2048)
+    assert (rx[:3], ry[:3]) == ([2, 2, 2], [1, 1, 2])
+
+
+def test_bad_arguments_rejected_without_device_work():
+    import numpy as np
+    import pytest
+    from mgpoisson2d_amd import capi, geometry as G
+    geo = G.mapped_geometry(2, 2)  # too small: the reference stencils need 3 cells
+    with pytest.raises(capi.MgpError):
+        capi.compute_invdiags(geo, G.BoundaryConditions.uniform((1, 1, 1, 1)), 1)
+    with pytest.raises(capi.MgpError):
+        capi.restrict(0, 10, 10, np.zeros((10, 10)), 3, 5)

@@ -103,3 +103,21 @@ def test_resident_reference_drivers(method, ordering):
                                             zerophi=True, order=ordering)
     assert (iters, outcome) == (it2, oc2)
     np.testing.assert_allclose(hist, h2, rtol=1e-12, atol=0)
+
+
+def test_graph_replay_matches_eager():
+    """The captured-graph iteration must give the bits of the eager one,
+    replayed several times with eager work and syncs in between."""
+    geo, bc, _, rhs = setup(128, 96, 0.1, 4.0, (1, 1, 0, 0), seed=15)
+    eager = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG, use_graph=False)
+    graph = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG, use_graph=True)
+    eager.set_rhs(rhs)
+    graph.set_rhs(rhs)
+    h1, i1, o1 = eager.solve(1e-10, 200, zerophi=True)
+    ref = eager.get_phi()
+    for _ in range(3):
+        h2, i2, o2 = graph.solve(1e-10, 200, zerophi=True)
+        assert (i1, o1) == (i2, o2)
+        np.testing.assert_array_equal(h1, h2)
+        np.testing.assert_array_equal(graph.get_phi()[1:-1, 1:-1], ref[1:-1, 1:-1])
+        eager.solve(1e-10, 5, zerophi=True)  # eager work between replays
