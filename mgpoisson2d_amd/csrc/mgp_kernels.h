@@ -13,6 +13,10 @@ void launch_fill_ghosts(hipStream_t s, double *phi, const Grid &g, const Geo &ge
 void launch_laplacian(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
                       const Geo &geo, const Bc &bc, int homog, int jscale);
 void launch_invdiags(hipStream_t s, double *out, const Grid &g, const Geo &geo, const Bc &bc, int homog);
+// out = rhs - L[phi] (or L[phi] if rhs is null) on a homogeneous level; Dirichlet
+// ghosts are synthesised in the kernel, so no fill is needed first.
+void launch_residual_synth(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
+                           const Geo &geo, const Bc &bc);
 void launch_dc_invdiags(hipStream_t s, double *out, const Grid &g, const Geo &geo);
 void launch_dc_laplacian(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
                          const Geo &geo);

@@ -201,6 +201,19 @@ __global__ void __launch_bounds__(256) k_laplacian(double *out, CFld phi, const 
     out[c] = rhs ? rhs[c] - v : v;
 }
 
+// out = rhs - L[phi] on a homogeneous MG level without a prior ghost fill.
+__global__ void __launch_bounds__(256) k_residual_synth(double *out, const double *phi, const double *rhs, Grid g,
+                                                        Geo geo, Bc bc) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= g.nx || j >= g.ny) return;
+    const size_t c = ix(g.pitch, i, j);
+    const bool interior = i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+    const double v = interior ? cell_lap_interior(CFld{phi, g.pitch}, g, geo, i, j)
+                              : cell_lap(synth(phi, g, bc), g, geo, bc, 1, i, j);
+    out[c] = rhs ? rhs[c] - v : v;
+}
+
 __global__ void __launch_bounds__(256) k_invdiags(double *out, Grid g, Geo geo, Bc bc, int homog) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -895,10 +908,11 @@ __global__ void __launch_bounds__(256) k_xline(double *e, const double *f, const
     double *ra = lds + 4 * nx, *rb = ra + n, *rc = rb + n, *rd = rc + n;
     const int tid = threadIdx.x;
     CFld ef{e, P};
+    const SynthGet eg = synth(e, g, bc);
     for (int i = tid; i < nx; i += blockDim.x) {
         const size_t c = ix(P, i, j);
         const bool interior = i > 0 && i < nx - 1 && j > 0 && j < g.ny - 1;
-        const double lap = interior ? cell_lap_interior(ef, g, geo, i, j) : cell_lap(ef, g, geo, bc, 1, i, j);
+        const double lap = interior ? cell_lap_interior(ef, g, geo, i, j) : cell_lap(eg, g, geo, bc, 1, i, j);
         sd[i] = f[c] - lap;
         sa[i] = A[c];
         sb[i] = B[c];
@@ -1071,6 +1085,10 @@ void launch_fill_ghosts(hipStream_t s, double *phi, const Grid &g, const Geo &ge
 void launch_laplacian(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
                       const Geo &geo, const Bc &bc, int homog, int jscale) {
     k_laplacian<<<tiles(g.nx, g.ny), 256, 0, s>>>(out, CFld{phi, g.pitch}, rhs, g, geo, bc, homog, jscale);
+}
+void launch_residual_synth(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
+                           const Geo &geo, const Bc &bc) {
+    k_residual_synth<<<tiles(g.nx, g.ny), 256, 0, s>>>(out, phi, rhs, g, geo, bc);
 }
 void launch_invdiags(hipStream_t s, double *out, const Grid &g, const Geo &geo, const Bc &bc, int homog) {
     k_invdiags<<<tiles(g.nx, g.ny), 256, 0, s>>>(out, g, geo, bc, homog);

@@ -119,13 +119,13 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
 static void zebra_sweep(hipStream_t s, KLevel &l, double *e, const double *f) {
     const Grid &g = l.g;
     const int ny = g.ny;
-    launch_fill_ghosts(s, e, g, l.geo, l.bc, 1, 0);
+    // Dirichlet ghosts are synthesised inside the line kernel from the values
+    // each row sees, which equals a fill before each phase
     // interior rows 1 .. ny-2: even ones, then odd ones
     const int n_even = ny >= 4 ? (ny - 4) / 2 + 1 : 0;  // rows 2, 4, ..., <= ny-2
     const int n_odd = ny >= 3 ? (ny - 3) / 2 + 1 : 0;   // rows 1, 3, ..., <= ny-2
     launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 2, 2, n_even);
     launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 1, 2, n_odd);
-    launch_fill_ghosts(s, e, g, l.geo, l.bc, 1, 0);
     launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 0, ny - 1, 2);
 }
 
@@ -138,8 +138,7 @@ static void vcycle(hipStream_t s, KrylovMG::Impl &K, size_t d, double *e, const 
     }
     KLevel &c = K.L[d + 1];
     for (int k = 0; k < K.sd; ++k) zebra_sweep(s, l, e, f);
-    launch_fill_ghosts(s, e, l.g, l.geo, l.bc, 1, 0);
-    launch_laplacian(s, l.w.p, e, f, l.g, l.geo, l.bc, 1, 0);
+    launch_residual_synth(s, l.w.p, e, f, l.g, l.geo, l.bc);
     launch_restrict_cc(s, c.r.p, l.w.p, c.g, l.g.pitch, 1, 2, nullptr, nullptr);
     vcycle(s, K, d + 1, c.e.p, c.r.p);
     launch_prolong_y(s, e, c.e.p, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
@@ -148,8 +147,7 @@ static void vcycle(hipStream_t s, KrylovMG::Impl &K, size_t d, double *e, const 
 
 // out = A_h v (homogeneous user BCs); v's ghosts are refilled.
 static void apply_h(hipStream_t s, const DevGeo &top, const Bc &ubc, double *v, double *out) {
-    launch_fill_ghosts(s, v, top.g, top.view(), ubc, 1, 0);
-    launch_laplacian(s, out, v, nullptr, top.g, top.view(), ubc, 1, 0);
+    launch_residual_synth(s, out, v, nullptr, top.g, top.view(), ubc);
 }
 
 int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
@@ -273,13 +271,20 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, double *bytes)
         // 2 .. 2*n_even+1.
         const double R = n_even;
         *bytes = nx * 8.0 * ((2.0 * R + 1.0) + R + R + 2.0 * R + 3.0 * R) + nx * 16.0 * (2.0 * R);
-    } else {
+    } else if (kernel == 1) {
         // r = f - L e over all cells: e (with its one-cell halo), f, r, J-free metrics
         *bytes = nx * (double)g.ny * 8.0 * 3.0 + 16.0 * (nx + 1) * g.ny + 16.0 * nx * (g.ny + 1);
+    } else if (kernel == 2) {
+        // plain copy w = e over the valid cells: the bandwidth reference for the two above
+        *bytes = nx * (double)g.ny * 16.0;
+    } else {
+        g_last_error = "time_kernel: kernel must be 0 (x-line, even rows), 1 (residual) or 2 (copy)";
+        throw Failure(MGP_ERR_ARG);
     }
     auto launch = [&]() {
         if (kernel == 0) launch_xline(s, l.e.p, l.r.p, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 2, 2, n_even);
-        else launch_laplacian(s, l.w.p, l.e.p, l.r.p, g, l.geo, l.bc, 1, 0);
+        else if (kernel == 1) launch_laplacian(s, l.w.p, l.e.p, l.r.p, g, l.geo, l.bc, 1, 0);
+        else launch_lincomb(s, l.w.p, 1.0, l.e.p, 0.0, nullptr, g, 0);
     };
     launch();  // warm
     hip_check(hipEventRecord(a, s), "event");

@@ -34,6 +34,49 @@ struct CFld {
     __device__ __forceinline__ double operator()(int i, int j) const { return p[ix(pitch, i, j)]; }
 };
 
+// Field accessor that synthesises the homogeneous Dirichlet ghosts
+// (order 3, or LAH negation) from the interior values at the time of the read,
+// and returns the stored value for every other ghost.  Equivalent to
+// fill_ghosts(homog, do_neum = .false.) immediately before the read, which is
+// what every MG-level stencil of the fast solver is preceded by.
+struct SynthGet {
+    const double *p;
+    int pitch, nx, ny;
+    int t[4], m[4];
+    __device__ __forceinline__ double raw(int i, int j) const { return p[ix(pitch, i, j)]; }
+    __device__ __forceinline__ double dg(int side, double p0, double p1, double p2) const {
+        if (m[side] == MODE_LAH) return -p0;
+        return -3.0 * p0 + p1 - (1.0 / 5.0) * p2;
+    }
+    __device__ __forceinline__ double operator()(int i, int j) const {
+        const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
+        if (xin == yin) return raw(i, j);  // valid cell, or a corner (never read)
+        if (!xin) {
+            const int side = i < 0 ? XLO : XHI;
+            if (t[side] != BC_DIRI) return raw(i, j);
+            return i < 0 ? dg(side, raw(0, j), raw(1, j), raw(2, j))
+                         : dg(side, raw(nx - 1, j), raw(nx - 2, j), raw(nx - 3, j));
+        }
+        const int side = j < 0 ? YLO : YHI;
+        if (t[side] != BC_DIRI) return raw(i, j);
+        return j < 0 ? dg(side, raw(i, 0), raw(i, 1), raw(i, 2))
+                     : dg(side, raw(i, ny - 1), raw(i, ny - 2), raw(i, ny - 3));
+    }
+};
+
+__device__ __forceinline__ SynthGet synth(const double *p, const Grid &g, const Bc &bc) {
+    SynthGet s;
+    s.p = p;
+    s.pitch = g.pitch;
+    s.nx = g.nx;
+    s.ny = g.ny;
+    for (int k = 0; k < 4; ++k) {
+        s.t[k] = bc.type[k];
+        s.m[k] = bc.mode[k];
+    }
+    return s;
+}
+
 // Unit impulse at (ci, cj), zero everywhere else, ghosts included: what
 // compute_invdiags evaluates the stencil on (poisson2d.f90:29-33).
 struct Impulse {

@@ -1,0 +1,1145 @@
+! ------------------------------------------------------------------------------
+! Fortran 90 drop-in layer for libMGPoisson (EdSantilli/MGPoisson2D).
+!
+! Modules precision, ArrayUtils, Poisson2D, MGPoisson2D, DeferredPoisson2D and
+! DeferredMGPoisson2D keep the reference's names, derived types (component
+! names, kinds, allocatable ranks, bounds) and procedure signatures, so a
+! caller written against the reference recompiles unchanged against this
+! layer.  The hot-path procedures marshal their arguments through
+! iso_c_binding into the C ABI of include/mgp2d.h, which runs them on the
+! MI355X; grid bookkeeping (define_*, setval_*, norms of host arrays) stays in
+! Fortran.  Errors are reported the reference's way: print* then stop.
+! ------------------------------------------------------------------------------
+
+! precision.f90:4-53 -- kinds and named constants
+module precision
+    integer, parameter  :: dp = kind(0.d0)
+    real(dp), parameter :: zero = 0.0_dp, one = 1.0_dp, two = 2.0_dp, three = 3.0_dp
+    real(dp), parameter :: four = 4.0_dp, five = 5.0_dp, six = 6.0_dp, seven = 7.0_dp
+    real(dp), parameter :: eight = 8.0_dp, nine = 9.0_dp, ten = 10.0_dp, eleven = 11.0_dp
+    real(dp), parameter :: twelve = 12.0_dp, thirteen = 13.0_dp, fourteen = 14.0_dp
+    real(dp), parameter :: fifteen = 15.0_dp, sixteen = 16.0_dp, seventeen = 17.0_dp
+    real(dp), parameter :: eighteen = 18.0_dp, nineteen = 19.0_dp, twenty = 20.0_dp
+    real(dp), parameter :: half = one / two, third = one / three, fourth = one / four
+    real(dp), parameter :: fifth = one / five, sixth = one / six, seventh = one / seven
+    real(dp), parameter :: eighth = one / eight, ninth = one / nine, tenth = one / ten
+    real(dp), parameter :: eleventh = one / eleven, twelfth = one / twelve
+    real(dp), parameter :: thirteenth = one / thirteen, fourteenth = one / fourteen
+    real(dp), parameter :: fifteenth = one / fifteen, sixteenth = one / sixteen
+    real(dp), parameter :: seventeenth = one / seventeen, eighteenth = one / eighteen
+    real(dp), parameter :: nineteenth = one / nineteen, twentieth = one / twenty
+    real(dp), parameter :: threehalves = three / two, threefourths = three / four
+    real(dp), parameter :: pi = four * atan(one)
+    real(dp), parameter :: bogus_val = 1.2345E300_dp
+end module precision
+
+
+! arrayutils.f90 -- grid containers and helpers
+module ArrayUtils
+    use precision
+    use iso_c_binding
+    implicit none
+
+    type box
+        integer  :: ilo, ihi
+        integer  :: jlo, jhi
+        integer  :: nx, ny
+        real(dp) :: dx, dy
+    end type box
+
+    type(box), parameter :: empty_box = box(0, -1, 0, -1, 0, 0, zero, zero)
+
+    integer, parameter :: BCTYPE_UNDEFINED = -2
+    integer, parameter :: BCTYPE_NONE = -1
+    integer, parameter :: BCTYPE_NEUM = 0
+    integer, parameter :: BCTYPE_DIRI = 1
+    integer, parameter :: BCTYPE_PERIODIC = 2
+    integer, parameter :: BCTYPE_CF = 3
+    integer, parameter :: BCTYPE_EXTRAP0 = 4
+    integer, parameter :: BCTYPE_EXTRAP1 = 5
+    integer, parameter :: BCTYPE_EXTRAP2 = 6
+
+    integer, parameter :: BCMODE_UNIFORM = 0
+    integer, parameter :: BCMODE_NONUNIFORM = 1
+    integer, parameter :: BCMODE_LAH = 2
+
+    ! Smoother orderings of the GPU layer (optional trailing argument of
+    ! relax_gs / vcycle; ORDER_LEX reproduces the reference bit for bit).
+    integer, parameter :: ORDER_LEX = 0
+    integer, parameter :: ORDER_RB = 1
+    integer, parameter :: ORDER_MC = 2
+
+    type bdry_data
+        type(box) :: valid
+        integer :: type_xlo, type_xhi
+        integer :: type_ylo, type_yhi
+        integer :: mode_xlo, mode_xhi
+        integer :: mode_ylo, mode_yhi
+        real(dp), dimension(:), allocatable :: data_xlo, data_xhi
+        real(dp), dimension(:), allocatable :: data_ylo, data_yhi
+    end type bdry_data
+
+    integer, parameter :: BD_NODE = 0
+    integer, parameter :: BD_CELL = 1
+
+    type box_data
+        type(box) :: bx
+        type(box) :: valid
+        integer   :: ngx, ngy
+        integer   :: offi, offj
+        real(dp)  :: offx, offy
+        real(dp)  :: L, H
+        real(dp), dimension(:,:), allocatable :: data
+    end type box_data
+
+    integer, parameter :: SIDE_LO = 0
+    integer, parameter :: SIDE_HI = 1
+
+    type geo_data
+        type(box_data) :: J
+        type(box_data) :: Jgup_xx
+        type(box_data) :: Jgup_xy
+        type(box_data) :: Jgup_yx
+        type(box_data) :: Jgup_yy
+        real(dp)       :: dx, dy
+    end type geo_data
+
+    interface define_box_data
+        module procedure define_box_data_itemize
+        module procedure define_box_data_dup
+    end interface
+
+    ! C ABI (include/mgp2d.h) ---------------------------------------------------
+    type, bind(C) :: mgp_geo_c
+        integer(c_int) :: nx, ny
+        real(c_double) :: dx, dy
+        type(c_ptr)    :: J, Jxx, Jxy, Jyx, Jyy
+    end type mgp_geo_c
+
+    type, bind(C) :: mgp_bc_c
+        integer(c_int) :: btype(4)
+        integer(c_int) :: bmode(4)
+        type(c_ptr)    :: bdata(4)
+    end type mgp_bc_c
+
+    ! Contiguous copies of a geo_data's valid faces (kept alive for a call).
+    type geo_pack
+        real(dp), allocatable :: J(:,:), Jxx(:,:), Jxy(:,:), Jyx(:,:), Jyy(:,:)
+    end type geo_pack
+
+    type bc_pack
+        real(dp), allocatable :: d1(:), d2(:), d3(:), d4(:)
+    end type bc_pack
+
+    interface
+        integer(c_int) function mgp_fill_ghosts_c(geo, bc, homog, do_neum, phi) bind(C, name="mgp_fill_ghosts")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog, do_neum
+            real(c_double)              :: phi(*)
+        end function
+        integer(c_int) function mgp_pnorm_c(nx, ny, dx, dy, a, p, out) bind(C, name="mgp_pnorm")
+            import :: c_int, c_double
+            integer(c_int), value      :: nx, ny, p
+            real(c_double), value      :: dx, dy
+            real(c_double), intent(in) :: a(*)
+            real(c_double)             :: out
+        end function
+        type(c_ptr) function mgp_last_error_c() bind(C, name="mgp_last_error")
+            import :: c_ptr
+        end function
+    end interface
+
+contains
+
+    ! Reports a failed C-ABI call the way the reference reports errors.
+    subroutine mgp_check (status, where)
+        integer(c_int), intent(in)   :: status
+        character(len=*), intent(in) :: where
+        character(kind=c_char), pointer :: msg(:)
+        character(len=256) :: text
+        integer :: k
+        if (status .eq. 0) return
+        text = ''
+        call c_f_pointer (mgp_last_error_c(), msg, [256])
+        do k = 1, 256
+            if (msg(k) .eq. c_null_char) exit
+            text(k:k) = msg(k)
+        enddo
+        print*, where, ': ', trim(text)
+        stop
+    end subroutine mgp_check
+
+    pure subroutine define_box (bx, ilo, ihi, jlo, jhi, dx, dy)
+        type(box), intent(out) :: bx
+        integer, intent(in)    :: ilo, ihi, jlo, jhi
+        real(dp), intent(in)   :: dx, dy
+        bx%ilo = ilo
+        bx%ihi = ihi
+        bx%jlo = jlo
+        bx%jhi = jhi
+        bx%nx = ihi - ilo + 1
+        bx%ny = jhi - jlo + 1
+        bx%dx = dx
+        bx%dy = dy
+    end subroutine define_box
+
+    pure subroutine coarsen_box (bx, refx, refy)
+        type(box), intent(inout) :: bx
+        integer, intent(in)      :: refx, refy
+        bx%ilo = bx%ilo / refx
+        bx%ihi = bx%ihi / refx
+        bx%nx = bx%ihi - bx%ilo + 1
+        bx%dx = bx%dx * refx
+        bx%jlo = bx%jlo / refy
+        bx%jhi = bx%jhi / refy
+        bx%ny = bx%jhi - bx%jlo + 1
+        bx%dy = bx%dy * refy
+    end subroutine coarsen_box
+
+    pure subroutine grow_box (bx, growx, growy)
+        type(box), intent(inout) :: bx
+        integer, intent(in)      :: growx, growy
+        bx%ilo = bx%ilo - growx
+        bx%ihi = bx%ihi + growx
+        bx%jlo = bx%jlo - growy
+        bx%jhi = bx%jhi + growy
+    end subroutine grow_box
+
+    function bdry_box (src, off, dir, side) result (bx)
+        type(box), intent(in) :: src
+        integer, intent(in)   :: off, dir, side
+        type(box)             :: bx
+        if (side .ne. SIDE_LO .and. side .ne. SIDE_HI) then
+            print*, 'bdry_box: Bad side.'
+            stop
+        endif
+        if (dir .eq. 1) then
+            if (side .eq. SIDE_LO) then
+                call define_box (bx, src%ilo, src%ilo, src%jlo, src%jhi, src%dx, src%dy)
+            else
+                call define_box (bx, src%ihi+off, src%ihi+off, src%jlo, src%jhi, src%dx, src%dy)
+            endif
+        else if (dir .eq. 2) then
+            if (side .eq. SIDE_LO) then
+                call define_box (bx, src%ilo, src%ihi, src%jlo, src%jlo, src%dx, src%dy)
+            else
+                call define_box (bx, src%ilo, src%ihi, src%jhi+off, src%jhi+off, src%dx, src%dy)
+            endif
+        else
+            print*, 'bdry_box: Bad dir.'
+            stop
+        endif
+    end function bdry_box
+
+    subroutine define_bdry_data (bcd, valid, type_xlo, type_xhi, type_ylo, type_yhi, &
+                                 mode_xlo, mode_xhi, mode_ylo, mode_yhi)
+        type(bdry_data), intent(out) :: bcd
+        type(box), intent(in)        :: valid
+        integer, intent(in)          :: type_xlo, type_xhi, type_ylo, type_yhi
+        integer, intent(in)          :: mode_xlo, mode_xhi, mode_ylo, mode_yhi
+        bcd%valid = valid
+        bcd%type_xlo = type_xlo
+        bcd%type_xhi = type_xhi
+        bcd%type_ylo = type_ylo
+        bcd%type_yhi = type_yhi
+        bcd%mode_xlo = mode_xlo
+        bcd%mode_xhi = mode_xhi
+        bcd%mode_ylo = mode_ylo
+        bcd%mode_yhi = mode_yhi
+        call alloc_side (bcd%data_xlo, mode_xlo, valid%jlo, valid%jhi)
+        call alloc_side (bcd%data_xhi, mode_xhi, valid%jlo, valid%jhi)
+        call alloc_side (bcd%data_ylo, mode_ylo, valid%ilo, valid%ihi)
+        call alloc_side (bcd%data_yhi, mode_yhi, valid%ilo, valid%ihi)
+    end subroutine define_bdry_data
+
+    subroutine alloc_side (d, mode, lo, hi)
+        real(dp), allocatable, intent(inout) :: d(:)
+        integer, intent(in) :: mode, lo, hi
+        integer :: ierr
+        select case (mode)
+            case (BCMODE_UNIFORM, BCMODE_LAH)
+                allocate (d(1:1), stat=ierr)
+            case (BCMODE_NONUNIFORM)
+                allocate (d(lo:hi), stat=ierr)
+            case default
+                print*, 'define_bdry_data: Bad BCMODE'
+                stop
+        end select
+        if (ierr .ne. 0) then
+            print*, 'define_bdry_data: Out of memory'
+            stop
+        endif
+    end subroutine alloc_side
+
+    pure subroutine undefine_bdry_data (bcd)
+        type(bdry_data), intent(inout) :: bcd
+        bcd%valid = empty_box
+        bcd%type_xlo = BCTYPE_UNDEFINED
+        bcd%type_xhi = BCTYPE_UNDEFINED
+        bcd%type_ylo = BCTYPE_UNDEFINED
+        bcd%type_yhi = BCTYPE_UNDEFINED
+        if (allocated(bcd%data_xlo)) deallocate(bcd%data_xlo)
+        if (allocated(bcd%data_xhi)) deallocate(bcd%data_xhi)
+        if (allocated(bcd%data_ylo)) deallocate(bcd%data_ylo)
+        if (allocated(bcd%data_yhi)) deallocate(bcd%data_yhi)
+    end subroutine undefine_bdry_data
+
+    subroutine define_box_data_itemize (bd, cc_valid, ngx, ngy, offi, offj)
+        type(box_data), intent(out) :: bd
+        type(box), intent(in)       :: cc_valid
+        integer, intent(in)         :: ngx, ngy, offi, offj
+        type(box) :: bx, valid
+        integer   :: ierr
+        valid = cc_valid
+        bd%offx = half
+        bd%offy = half
+        if (offi .eq. BD_NODE) then
+            valid%ihi = valid%ihi + 1
+            valid%nx = valid%nx + 1
+            bd%offx = zero
+        else if (offi .ne. BD_CELL) then
+            print*, 'define_box_data: Bad offi'
+        endif
+        if (offj .eq. BD_NODE) then
+            valid%jhi = valid%jhi + 1
+            valid%ny = valid%ny + 1
+            bd%offy = zero
+        else if (offj .ne. BD_CELL) then
+            print*, 'define_box_data: Bad offj'
+        endif
+        bx = valid
+        bx%ilo = bx%ilo - ngx
+        bx%ihi = bx%ihi + ngx
+        bx%jlo = bx%jlo - ngy
+        bx%jhi = bx%jhi + ngy
+        bd%valid = valid
+        bd%bx = bx
+        bd%ngx = ngx
+        bd%ngy = ngy
+        bd%offi = offi
+        bd%offj = offj
+        bd%L = cc_valid%nx * cc_valid%dx
+        bd%H = cc_valid%ny * cc_valid%dy
+        allocate (bd%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi), stat=ierr)
+        if (ierr .ne. 0) then
+            print*, 'define_box_data: Out of memory'
+            stop
+        endif
+        bd%data = zero
+    end subroutine define_box_data_itemize
+
+    subroutine define_box_data_dup (dest, src)
+        type(box_data), intent(inout) :: dest
+        type(box_data), intent(in)    :: src
+        type(box) :: cc_valid
+        cc_valid = src%valid
+        if (src%offi .eq. BD_NODE) then
+            cc_valid%ihi = cc_valid%ihi - 1
+            cc_valid%nx = cc_valid%nx - 1
+        endif
+        if (src%offj .eq. BD_NODE) then
+            cc_valid%jhi = cc_valid%jhi - 1
+            cc_valid%ny = cc_valid%ny - 1
+        endif
+        call define_box_data_itemize (dest, cc_valid, src%ngx, src%ngy, src%offi, src%offj)
+    end subroutine define_box_data_dup
+
+    pure subroutine undefine_box_data (bd)
+        type(box_data), intent(inout) :: bd
+        bd%valid = empty_box
+        bd%bx = empty_box
+        bd%ngx = 0
+        bd%ngy = 0
+        bd%offx = bogus_val
+        bd%offy = bogus_val
+        bd%offi = 0
+        bd%offj = 0
+        bd%L = bogus_val
+        bd%H = bogus_val
+        if (allocated(bd%data)) deallocate(bd%data)
+    end subroutine undefine_box_data
+
+    pure function stagger (min) result (mout)
+        integer, intent(in) :: min
+        integer             :: mout
+        mout = 1 - min
+    end function stagger
+
+    pure subroutine setval_valid (bd, val)
+        type(box_data), intent(inout) :: bd
+        real(dp), intent(in)          :: val
+        bd%data(bd%valid%ilo:bd%valid%ihi, bd%valid%jlo:bd%valid%jhi) = val
+    end subroutine setval_valid
+
+    function inner_prod (bd1, bd2) result (ip)
+        type(box_data), intent(in) :: bd1, bd2
+        real(dp) :: ip, vol
+        integer  :: ilo, ihi, jlo, jhi, j
+        ilo = max(bd1%valid%ilo, bd2%valid%ilo)
+        ihi = min(bd1%valid%ihi, bd2%valid%ihi)
+        jlo = max(bd1%valid%jlo, bd2%valid%jlo)
+        jhi = min(bd1%valid%jhi, bd2%valid%jhi)
+        if ((ilo .gt. ihi) .or. (jlo .gt. jhi)) then
+            print*, 'ERROR: inner_prod_box_data: given box_data do not overlap.'
+            stop
+        endif
+        vol = bd1%valid%dx * bd1%valid%dy
+        ip = zero
+        do j = jlo, jhi
+            ip = ip + vol * dot_product (bd1%data(ilo:ihi,j), bd2%data(ilo:ihi,j))
+        enddo
+    end function inner_prod
+
+    ! 2-norm and max norm run on the GPU (mgp_pnorm); other p on the host.
+    function pnorm (bd, bx, p) result (pn)
+        type(box_data), intent(in) :: bd
+        type(box), intent(in)      :: bx
+        integer, intent(in)        :: p
+        real(dp)                   :: pn
+        real(dp), allocatable      :: a(:,:)
+        real(c_double)             :: out
+        integer :: i, j
+        if (p .eq. 0 .or. p .eq. 2) then
+            allocate (a(bx%ilo:bx%ihi, bx%jlo:bx%jhi))
+            a = bd%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi)
+            call mgp_check (mgp_pnorm_c(bx%nx, bx%ny, bd%bx%dx, bd%bx%dy, a, p, out), 'pnorm')
+            pn = out
+            return
+        endif
+        pn = zero
+        do j = bx%jlo, bx%jhi
+            do i = bx%ilo, bx%ihi
+                pn = pn + bd%bx%dx * bd%bx%dy * abs(bd%data(i,j))**p
+            enddo
+        enddo
+        pn = pn**(one/p)
+    end function pnorm
+
+    function integrate2d (bd, bx, geo, opt_jscale) result (sum)
+        type(box_data), intent(in) :: bd
+        type(box), intent(in)      :: bx
+        type(geo_data), intent(in) :: geo
+        logical, optional          :: opt_jscale
+        real(dp) :: sum
+        logical  :: jscale
+        integer  :: i, j
+        jscale = .true.
+        if (present(opt_jscale)) jscale = opt_jscale
+        sum = zero
+        do j = bx%jlo, bx%jhi
+            do i = bx%ilo, bx%ihi
+                if (jscale) then
+                    sum = sum + geo%dx * geo%dy * geo%J%data(i,j) * bd%data(i,j)
+                else
+                    sum = sum + geo%dx * geo%dy * bd%data(i,j)
+                endif
+            enddo
+        enddo
+    end function integrate2d
+
+    ! arrayutils.f90:1497 (order 0..3), host side
+    subroutine extrapolate_ghosts (phi, order)
+        type(box_data), intent(inout) :: phi
+        integer, intent(in)           :: order
+        integer :: ilo, ihi, jlo, jhi
+        ilo = phi%valid%ilo
+        ihi = phi%valid%ihi
+        jlo = phi%valid%jlo
+        jhi = phi%valid%jhi
+        select case (order)
+            case (0)
+                phi%data(ilo-1,:) = phi%data(ilo,:)
+                phi%data(ihi+1,:) = phi%data(ihi,:)
+                phi%data(:,jlo-1) = phi%data(:,jlo)
+                phi%data(:,jhi+1) = phi%data(:,jhi)
+            case (1)
+                phi%data(ilo-1,:) = two*phi%data(ilo,:) - phi%data(ilo+1,:)
+                phi%data(ihi+1,:) = two*phi%data(ihi,:) - phi%data(ihi-1,:)
+                phi%data(:,jlo-1) = two*phi%data(:,jlo) - phi%data(:,jlo+1)
+                phi%data(:,jhi+1) = two*phi%data(:,jhi) - phi%data(:,jhi-1)
+            case (2)
+                phi%data(ilo-1,:) = three*(phi%data(ilo,:) - phi%data(ilo+1,:)) + phi%data(ilo+2,:)
+                phi%data(ihi+1,:) = three*(phi%data(ihi,:) - phi%data(ihi-1,:)) + phi%data(ihi-2,:)
+                phi%data(:,jlo-1) = three*(phi%data(:,jlo) - phi%data(:,jlo+1)) + phi%data(:,jlo+2)
+                phi%data(:,jhi+1) = three*(phi%data(:,jhi) - phi%data(:,jhi-1)) + phi%data(:,jhi-2)
+            case (3)
+                phi%data(ilo-1,:) = four*(phi%data(ilo,:) + phi%data(ilo+2,:)) - six*phi%data(ilo+1,:) &
+                                  - phi%data(ilo+3,:)
+                phi%data(ihi+1,:) = four*(phi%data(ihi,:) + phi%data(ihi-2,:)) - six*phi%data(ihi-1,:) &
+                                  - phi%data(ihi-3,:)
+                phi%data(:,jlo-1) = four*(phi%data(:,jlo) + phi%data(:,jlo+2)) - six*phi%data(:,jlo+1) &
+                                  - phi%data(:,jlo+3)
+                phi%data(:,jhi+1) = four*(phi%data(:,jhi) + phi%data(:,jhi-2)) - six*phi%data(:,jhi-1) &
+                                  - phi%data(:,jhi-3)
+            case default
+                print*, 'extrapolate_ghosts: order = ', order, ' is not supported.'
+                stop
+        end select
+    end subroutine extrapolate_ghosts
+
+    ! ---- marshalling helpers -------------------------------------------------
+    subroutine pack_geo (gp, gc, geo, valid)
+        type(geo_pack), target, intent(inout) :: gp
+        type(mgp_geo_c), intent(out)          :: gc
+        type(geo_data), intent(in)            :: geo
+        type(box), intent(in)                 :: valid
+        integer :: ilo, ihi, jlo, jhi
+        ilo = valid%ilo
+        ihi = valid%ihi
+        jlo = valid%jlo
+        jhi = valid%jhi
+        allocate (gp%J(ilo:ihi, jlo:jhi), gp%Jxx(ilo:ihi+1, jlo:jhi), gp%Jxy(ilo:ihi+1, jlo:jhi))
+        allocate (gp%Jyx(ilo:ihi, jlo:jhi+1), gp%Jyy(ilo:ihi, jlo:jhi+1))
+        gp%J = geo%J%data(ilo:ihi, jlo:jhi)
+        gp%Jxx = geo%Jgup_xx%data(ilo:ihi+1, jlo:jhi)
+        gp%Jxy = geo%Jgup_xy%data(ilo:ihi+1, jlo:jhi)
+        gp%Jyx = geo%Jgup_yx%data(ilo:ihi, jlo:jhi+1)
+        gp%Jyy = geo%Jgup_yy%data(ilo:ihi, jlo:jhi+1)
+        gc%nx = valid%nx
+        gc%ny = valid%ny
+        gc%dx = geo%dx
+        gc%dy = geo%dy
+        gc%J = c_loc(gp%J)
+        gc%Jxx = c_loc(gp%Jxx)
+        gc%Jxy = c_loc(gp%Jxy)
+        gc%Jyx = c_loc(gp%Jyx)
+        gc%Jyy = c_loc(gp%Jyy)
+    end subroutine pack_geo
+
+    subroutine pack_side (dst, src)
+        real(dp), allocatable, target, intent(inout) :: dst(:)
+        real(dp), allocatable, intent(in)            :: src(:)
+        if (allocated(src)) then
+            allocate (dst(size(src)))
+            dst = src
+        else
+            allocate (dst(1))
+            dst = zero
+        endif
+    end subroutine pack_side
+
+    subroutine pack_bc (bp, bcc, bc)
+        type(bc_pack), target, intent(inout) :: bp
+        type(mgp_bc_c), intent(out)          :: bcc
+        type(bdry_data), intent(in)          :: bc
+        bcc%btype = [bc%type_xlo, bc%type_xhi, bc%type_ylo, bc%type_yhi]
+        bcc%bmode = [bc%mode_xlo, bc%mode_xhi, bc%mode_ylo, bc%mode_yhi]
+        call pack_side (bp%d1, bc%data_xlo)
+        call pack_side (bp%d2, bc%data_xhi)
+        call pack_side (bp%d3, bc%data_ylo)
+        call pack_side (bp%d4, bc%data_yhi)
+        bcc%bdata(1) = c_loc(bp%d1)
+        bcc%bdata(2) = c_loc(bp%d2)
+        bcc%bdata(3) = c_loc(bp%d3)
+        bcc%bdata(4) = c_loc(bp%d4)
+    end subroutine pack_bc
+
+    ! one-ghost copy of a cell field's valid region and ghost ring
+    subroutine pack_phi (a, f)
+        real(dp), allocatable, intent(out) :: a(:,:)
+        type(box_data), intent(in)         :: f
+        if (f%ngx .ne. 1 .or. f%ngy .ne. 1) then
+            print*, 'Only one ghost layer is supported for this field'
+            stop
+        endif
+        allocate (a(f%valid%ilo-1:f%valid%ihi+1, f%valid%jlo-1:f%valid%jhi+1))
+        a = f%data(f%valid%ilo-1:f%valid%ihi+1, f%valid%jlo-1:f%valid%jhi+1)
+    end subroutine pack_phi
+
+    subroutine pack_cc (a, f)
+        real(dp), allocatable, intent(out) :: a(:,:)
+        type(box_data), intent(in)         :: f
+        allocate (a(f%valid%ilo:f%valid%ihi, f%valid%jlo:f%valid%jhi))
+        a = f%data(f%valid%ilo:f%valid%ihi, f%valid%jlo:f%valid%jhi)
+    end subroutine pack_cc
+
+    ! arrayutils.f90:1277 fill_ghosts (runs on the GPU)
+    subroutine fill_ghosts (phi, bc, geo, homog, do_neum_opt)
+        type(box_data), intent(inout) :: phi
+        type(bdry_data), intent(in)   :: bc
+        type(geo_data), intent(in)    :: geo
+        logical, intent(in)           :: homog
+        logical, intent(in), optional :: do_neum_opt
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:)
+        logical :: do_neum
+        do_neum = .true.
+        if (present(do_neum_opt)) do_neum = do_neum_opt
+        call pack_geo (gp, gc, geo, phi%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_phi (a, phi)
+        call mgp_check (mgp_fill_ghosts_c(gc, bcc, merge(1, 0, homog), merge(1, 0, do_neum), a), 'fill_ghosts')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+    end subroutine fill_ghosts
+
+end module ArrayUtils
+
+
+! poisson2d.f90 -- operator, residual, inverse diagonal, GS, BiCGStab2 (GPU)
+module Poisson2D
+    use ArrayUtils
+    implicit none
+
+    interface
+        integer(c_int) function c_laplacian(geo, bc, homog, jscale, phi, lap) bind(C, name="mgp_compute_laplacian")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog, jscale
+            real(c_double)              :: phi(*), lap(*)
+        end function
+        integer(c_int) function c_residual(geo, bc, homog, rhs, phi, res) bind(C, name="mgp_compute_residual")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog
+            real(c_double), intent(in)  :: rhs(*)
+            real(c_double)              :: phi(*), res(*)
+        end function
+        integer(c_int) function c_invdiags(geo, bc, homog, invd) bind(C, name="mgp_compute_invdiags")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog
+            real(c_double)              :: invd(*)
+        end function
+        integer(c_int) function c_relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering) &
+                                           bind(C, name="mgp_relax_gs")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog, maxiters, ordering
+            real(c_double), value       :: omega
+            real(c_double), intent(in)  :: invd(*), rhs(*)
+            real(c_double)              :: phi(*)
+        end function
+        integer(c_int) function c_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, &
+                                            rhs, phi, ordering) bind(C, name="mgp_solve_bicgstab2")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            integer(c_int), value       :: kind, homog, maxiters, maxrestarts, zerophi, ordering
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            real(c_double), value       :: tol
+            real(c_double), intent(in)  :: invd(*), rhs(*)
+            real(c_double)              :: phi(*)
+        end function
+    end interface
+
+contains
+
+    ! poisson2d.f90:16
+    subroutine compute_invdiags (invdiags, geo, bc, homog)
+        type(box_data), intent(inout) :: invdiags
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:)
+        call pack_geo (gp, gc, geo, invdiags%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (a, invdiags)
+        call mgp_check (c_invdiags(gc, bcc, merge(1, 0, homog), a), 'Poisson2d::compute_invdiags')
+        invdiags%data(invdiags%valid%ilo:invdiags%valid%ihi, invdiags%valid%jlo:invdiags%valid%jhi) = a
+    end subroutine compute_invdiags
+
+    ! poisson2d.f90:886
+    subroutine compute_laplacian (lap, phi, geo, bc, homog, opt_jscale)
+        type(box_data), intent(inout) :: lap
+        type(box_data), intent(inout) :: phi
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        logical, intent(in), optional :: opt_jscale
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), l(:,:)
+        integer :: js
+        js = 0
+        if (present(opt_jscale)) js = merge(1, 0, opt_jscale)
+        call pack_geo (gp, gc, geo, lap%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_phi (a, phi)
+        call pack_cc (l, lap)
+        call mgp_check (c_laplacian(gc, bcc, merge(1, 0, homog), js, a, l), 'Poisson2d::compute_laplacian')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+        lap%data(lap%valid%ilo:lap%valid%ihi, lap%valid%jlo:lap%valid%jhi) = l
+    end subroutine compute_laplacian
+
+    ! poisson2d.f90:1327
+    subroutine compute_residual (res, rhs, phi, geo, bc, homog, opt_jscale)
+        type(box_data), intent(inout) :: res, phi
+        type(box_data), intent(in)    :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        logical, intent(in), optional :: opt_jscale
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), r(:,:), f(:,:)
+        logical :: jscale
+        integer :: ilo, ihi, jlo, jhi
+        jscale = .false.
+        if (present(opt_jscale)) jscale = opt_jscale
+        ilo = rhs%valid%ilo
+        ihi = rhs%valid%ihi
+        jlo = rhs%valid%jlo
+        jhi = rhs%valid%jhi
+        if (jscale) then
+            call compute_laplacian (res, phi, geo, bc, homog, .true.)
+            res%data(ilo:ihi,jlo:jhi) = rhs%data(ilo:ihi,jlo:jhi) - res%data(ilo:ihi,jlo:jhi)
+            return
+        endif
+        call pack_geo (gp, gc, geo, res%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_phi (a, phi)
+        call pack_cc (f, rhs)
+        call pack_cc (r, res)
+        call mgp_check (c_residual(gc, bcc, merge(1, 0, homog), f, a, r), 'Poisson2d::compute_residual')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+        res%data(ilo:ihi,jlo:jhi) = r
+    end subroutine compute_residual
+
+    ! poisson2d.f90:1460 -- lexicographic order as in the reference.  A
+    ! positive tol monitors the residual after each sweep on the host side.
+    subroutine relax_gs (phi, rhs, geo, bc, homog, invdiags, omega, tol, maxiters, zerophi, verbosity, ordering)
+        type(box_data), intent(inout)   :: phi
+        type(box_data), intent(in)      :: rhs
+        type(geo_data), intent(in)      :: geo
+        type(bdry_data), intent(in)     :: bc
+        logical, intent(in)             :: homog
+        type(box_data), intent(in)      :: invdiags
+        real(dp), intent(in)            :: omega, tol
+        integer, intent(in)             :: maxiters
+        logical, intent(in)             :: zerophi
+        integer, intent(in)             :: verbosity
+        integer, intent(in), optional   :: ordering
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), f(:,:), iv(:,:)
+        type(box_data) :: r
+        real(dp) :: rscale, rel
+        integer :: iter, ord
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (maxiters .eq. 0) return
+        if (zerophi) phi%data = zero
+        call pack_geo (gp, gc, geo, rhs%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (f, rhs)
+        call pack_cc (iv, invdiags)
+        if (tol .le. zero) then
+            call pack_phi (a, phi)
+            call mgp_check (c_relax_gs(gc, bcc, merge(1, 0, homog), iv, omega, maxiters, f, a, ord), &
+                            'Poisson2d::relax_gs')
+            phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+            return
+        endif
+        call define_box_data (r, rhs)
+        call compute_residual (r, rhs, phi, geo, bc, homog)
+        rscale = pnorm (r, r%valid, 2)
+        if (rscale .eq. zero) rscale = one
+        do iter = 1, maxiters
+            call pack_phi (a, phi)
+            call mgp_check (c_relax_gs(gc, bcc, merge(1, 0, homog), iv, omega, 1, f, a, ord), 'Poisson2d::relax_gs')
+            phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+            call compute_residual (r, rhs, phi, geo, bc, homog)
+            rel = pnorm (r, r%valid, 2) / rscale
+            if (verbosity .ge. 3) print*, 'iter ', iter, ': rel |res| = ', rel
+            if (rel .le. tol) exit
+        enddo
+        call undefine_box_data (r)
+    end subroutine relax_gs
+
+    ! poisson2d.f90:2253
+    subroutine solve_bicgstab2 (phi, rhs, geo, bc, homog, invdiags, tol, max_iters, max_restarts, &
+                                zerophi, verbosity)
+        type(box_data), intent(inout) :: phi
+        type(box_data), intent(in)    :: rhs
+        type(box_data), intent(in)    :: invdiags
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        real(dp), intent(in)          :: tol
+        integer, intent(in)           :: max_iters, max_restarts
+        logical, intent(in)           :: zerophi
+        integer, intent(in)           :: verbosity
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), f(:,:), iv(:,:)
+        if (verbosity .lt. -100) print*, verbosity
+        call pack_geo (gp, gc, geo, rhs%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (f, rhs)
+        call pack_cc (iv, invdiags)
+        call pack_phi (a, phi)
+        call mgp_check (c_bicgstab2(0, gc, bcc, merge(1, 0, homog), iv, tol, max_iters, max_restarts, &
+                                    merge(1, 0, zerophi), f, a, 0), 'Poisson2d::solve_bicgstab2')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+    end subroutine solve_bicgstab2
+
+end module Poisson2D
+
+
+! MGPoisson2D.f90 -- schedule, transfers, V-cycle (GPU)
+module MGPoisson2D
+    use Poisson2D
+    implicit none
+
+    interface
+        integer(c_int) function c_schedule(nx, ny, dx, dy, md_in, md_out, refx, refy) &
+                                           bind(C, name="mgp_create_mgschedule")
+            import :: c_int, c_double
+            integer(c_int), value :: nx, ny, md_in
+            real(c_double), value :: dx, dy
+            integer(c_int)        :: md_out, refx(*), refy(*)
+        end function
+        integer(c_int) function c_restrict(kind, fnx, fny, fine, cnx, cny, crse) bind(C, name="mgp_restrict")
+            import :: c_int, c_double
+            integer(c_int), value      :: kind, fnx, fny, cnx, cny
+            real(c_double), intent(in) :: fine(*)
+            real(c_double)             :: crse(*)
+        end function
+        integer(c_int) function c_prolong(fgeo, cgeo, btypes, bmodes, order, fine, crse) bind(C, name="mgp_prolong")
+            import :: mgp_geo_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: fgeo, cgeo
+            integer(c_int), intent(in)  :: btypes(4), bmodes(4)
+            integer(c_int), value       :: order
+            real(c_double)              :: fine(*), crse(*)
+        end function
+        integer(c_int) function c_vcycle(kind, geo, bc, homog, tol, maxiters, maxdepth, numcycles, sdown, sup, &
+                                         sbottom, zerophi, phi, rhs, ordering, relres, iters, outcome) &
+                                         bind(C, name="mgp_vcycle")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            integer(c_int), value       :: kind, homog, maxiters, maxdepth, numcycles, sdown, sup, sbottom
+            integer(c_int), value       :: zerophi, ordering
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            real(c_double), value       :: tol
+            real(c_double)              :: phi(*), rhs(*), relres(*)
+            integer(c_int)              :: iters, outcome
+        end function
+    end interface
+
+contains
+
+    ! MGPoisson2D.f90:38 -- this overshoots, exactly like the reference
+    subroutine estimate_maxdepth (maxdepth, valid)
+        integer, intent(inout) :: maxdepth
+        type(box), intent(in)  :: valid
+        integer :: mgnx, mgny, d
+        mgnx = valid%nx
+        mgny = valid%ny
+        d = 0
+        do while (d .lt. maxdepth)
+            if (mod(mgnx,2) .eq. 0) then
+                mgnx = mgnx / 2
+            else if (mod(mgny,2) .eq. 0) then
+                mgny = mgny / 2
+            else
+                exit
+            endif
+            d = d + 1
+        enddo
+        maxdepth = min(maxdepth, d+1)
+    end subroutine estimate_maxdepth
+
+    ! MGPoisson2D.f90:67
+    subroutine create_mgschedule (refx, refy, maxdepth, valid)
+        integer, intent(inout)                        :: maxdepth
+        integer, intent(out), dimension(0:maxdepth-1) :: refx, refy
+        type(box), intent(in)                         :: valid
+        integer(c_int) :: rx(64), ry(64), md
+        refx = 0
+        refy = 0
+        call mgp_check (c_schedule(valid%nx, valid%ny, valid%dx, valid%dy, maxdepth, md, rx, ry), &
+                        'create_mgschedule')
+        maxdepth = md
+        if (md .gt. 0) then
+            refx(0:md-1) = rx(1:md)
+            refy(0:md-1) = ry(1:md)
+        endif
+    end subroutine create_mgschedule
+
+    ! MGPoisson2D.f90:135
+    subroutine restrict (fine, crse)
+        type(box_data), intent(in)    :: fine
+        type(box_data), intent(inout) :: crse
+        real(dp), allocatable :: f(:,:), c(:,:)
+        integer :: kind, fnx, fny, cnx, cny
+        kind = 0
+        if (fine%offi .eq. BD_NODE .and. fine%offj .eq. BD_CELL) kind = 1
+        if (fine%offi .eq. BD_CELL .and. fine%offj .eq. BD_NODE) kind = 2
+        if (fine%offi .eq. BD_NODE .and. fine%offj .eq. BD_NODE) then
+            print*, 'restrict: nodal data is not supported'
+            stop
+        endif
+        fnx = fine%valid%nx
+        fny = fine%valid%ny
+        cnx = crse%valid%nx
+        cny = crse%valid%ny
+        if (kind .eq. 1) then
+            fnx = fnx - 1
+            cnx = cnx - 1
+        else if (kind .eq. 2) then
+            fny = fny - 1
+            cny = cny - 1
+        endif
+        call pack_cc (f, fine)
+        call pack_cc (c, crse)
+        call mgp_check (c_restrict(kind, fnx, fny, f, cnx, cny, c), 'restrict')
+        crse%data = zero
+        crse%data(crse%valid%ilo:crse%valid%ihi, crse%valid%jlo:crse%valid%jhi) = c
+    end subroutine restrict
+
+    ! MGPoisson2D.f90:262
+    subroutine prolong (fine, crse, fine_geo, crse_geo, crse_bc, order)
+        type(box_data), intent(inout) :: fine
+        type(box_data), intent(inout) :: crse
+        type(geo_data), intent(in)    :: fine_geo
+        type(geo_data), intent(in)    :: crse_geo
+        type(bdry_data), intent(in)   :: crse_bc
+        integer, intent(in)           :: order
+        type(geo_pack) :: fp, cp
+        type(mgp_geo_c) :: fg, cg
+        real(dp), allocatable :: f(:,:), c(:,:)
+        integer(c_int) :: t(4), m(4)
+        call pack_geo (fp, fg, fine_geo, fine%valid)
+        call pack_geo (cp, cg, crse_geo, crse%valid)
+        t = [crse_bc%type_xlo, crse_bc%type_xhi, crse_bc%type_ylo, crse_bc%type_yhi]
+        m = [crse_bc%mode_xlo, crse_bc%mode_xhi, crse_bc%mode_ylo, crse_bc%mode_yhi]
+        call pack_phi (f, fine)
+        call pack_phi (c, crse)
+        call mgp_check (c_prolong(fg, cg, t, m, order, f, c), 'prolong')
+        fine%data(fine%valid%ilo-1:fine%valid%ihi+1, fine%valid%jlo-1:fine%valid%jhi+1) = f
+        crse%data(crse%valid%ilo-1:crse%valid%ihi+1, crse%valid%jlo-1:crse%valid%jhi+1) = c
+    end subroutine prolong
+
+    ! Shared body of MGPoisson2D::vcycle (kind 0) and DeferredMGPoisson2D::vcycle (kind 1).
+    subroutine dropin_vcycle (kind, phi, rhs, geo, bc, homog, tol, maxiters, maxdepth_user, numcycles, &
+                              smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ordering)
+        integer, intent(in)           :: kind
+        type(box_data), intent(inout) :: phi, rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog, zerophi
+        real(dp), intent(in)          :: tol
+        integer, intent(in)           :: maxiters, maxdepth_user, numcycles
+        integer, intent(in)           :: smooth_down, smooth_up, smooth_bottom, verbosity, ordering
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), f(:,:), relres(:)
+        integer(c_int) :: iters, outcome
+        integer :: k
+        call pack_geo (gp, gc, geo, rhs%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_phi (a, phi)
+        call pack_cc (f, rhs)
+        allocate (relres(0:max(maxiters, 0)))
+        call mgp_check (c_vcycle(kind, gc, bcc, merge(1, 0, homog), tol, maxiters, maxdepth_user, numcycles, &
+                                 smooth_down, smooth_up, smooth_bottom, merge(1, 0, zerophi), a, f, ordering, relres, &
+                                 iters, outcome), 'vcycle')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+        rhs%data(rhs%valid%ilo:rhs%valid%ihi, rhs%valid%jlo:rhs%valid%jhi) = f
+        if (verbosity .ge. 1) then
+            do k = 0, iters
+                print*, 'V-Cycle iter ', k, ': rel |res| = ', relres(k)
+            enddo
+            if (outcome .eq. 0) print*, 'Converged.'
+            if (outcome .eq. 1) print*, 'Diverging.'
+        endif
+    end subroutine dropin_vcycle
+
+    ! MGPoisson2D.f90:400
+    subroutine vcycle (phi, rhs, geo, bc, homog, amrrefx, amrrefy, tol, maxiters, maxdepth_user, numcycles, &
+                       smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ordering)
+        type(box_data), intent(inout) :: phi
+        type(box_data), intent(inout) :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        integer, intent(in)           :: amrrefx, amrrefy
+        real(dp), intent(in)          :: tol
+        integer, intent(in)           :: maxiters, maxdepth_user, numcycles
+        integer, intent(in)           :: smooth_down, smooth_up, smooth_bottom
+        logical, intent(in)           :: zerophi
+        integer, intent(in)           :: verbosity
+        integer, intent(in), optional :: ordering
+        integer :: ord
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (amrrefx .lt. -1000000 .or. amrrefy .lt. -1000000) print*, 'unused'
+        call dropin_vcycle (0, phi, rhs, geo, bc, homog, tol, maxiters, maxdepth_user, numcycles, &
+                            smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ord)
+    end subroutine vcycle
+
+end module MGPoisson2D
+
+
+! MGPoisson2D_DD.f90:3 -- low-order deferred-correction operator (GPU)
+module DeferredPoisson2D
+    use ArrayUtils
+    implicit none
+
+    interface
+        integer(c_int) function c_dc_laplacian(geo, bc, homog, phi, lap) bind(C, name="mgp_dc_compute_laplacian")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog
+            real(c_double)              :: phi(*), lap(*)
+        end function
+        integer(c_int) function c_dc_invdiags(geo, invd) bind(C, name="mgp_dc_compute_invdiags")
+            import :: mgp_geo_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            real(c_double)              :: invd(*)
+        end function
+        integer(c_int) function c_dc_relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering) &
+                                              bind(C, name="mgp_dc_relax_gs")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog, maxiters, ordering
+            real(c_double), value       :: omega
+            real(c_double), intent(in)  :: invd(*), rhs(*)
+            real(c_double)              :: phi(*)
+        end function
+    end interface
+
+contains
+
+    ! MGPoisson2D_DD.f90:16
+    subroutine compute_invdiags (invdiags, geo)
+        type(box_data), intent(inout) :: invdiags
+        type(geo_data), intent(in)    :: geo
+        type(geo_pack) :: gp
+        type(mgp_geo_c) :: gc
+        real(dp), allocatable :: a(:,:)
+        call pack_geo (gp, gc, geo, invdiags%valid)
+        call pack_cc (a, invdiags)
+        call mgp_check (c_dc_invdiags(gc, a), 'DeferredPoisson2D::compute_invdiags')
+        invdiags%data(invdiags%valid%ilo:invdiags%valid%ihi, invdiags%valid%jlo:invdiags%valid%jhi) = a
+    end subroutine compute_invdiags
+
+    ! MGPoisson2D_DD.f90:45
+    subroutine compute_laplacian (lap, phi, geo, bc, homog)
+        type(box_data), intent(inout) :: lap, phi
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), l(:,:)
+        call pack_geo (gp, gc, geo, lap%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_phi (a, phi)
+        call pack_cc (l, lap)
+        call mgp_check (c_dc_laplacian(gc, bcc, merge(1, 0, homog), a, l), 'DeferredPoisson2D::compute_laplacian')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+        lap%data(lap%valid%ilo:lap%valid%ihi, lap%valid%jlo:lap%valid%jhi) = l
+    end subroutine compute_laplacian
+
+    ! MGPoisson2D_DD.f90:127
+    subroutine compute_residual (res, rhs, phi, geo, bc, homog)
+        type(box_data), intent(inout) :: res, phi
+        type(box_data), intent(in)    :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        integer :: ilo, ihi, jlo, jhi
+        call compute_laplacian (res, phi, geo, bc, homog)
+        ilo = rhs%valid%ilo
+        ihi = rhs%valid%ihi
+        jlo = rhs%valid%jlo
+        jhi = rhs%valid%jhi
+        res%data(ilo:ihi,jlo:jhi) = rhs%data(ilo:ihi,jlo:jhi) - res%data(ilo:ihi,jlo:jhi)
+    end subroutine compute_residual
+
+    ! MGPoisson2D_DD.f90:153 (lexicographic, as compiled in the reference)
+    subroutine relax_gs (phi, rhs, geo, bc, homog, invdiags, omega, tol, maxiters, zerophi, verbosity, ordering)
+        type(box_data), intent(inout) :: phi
+        type(box_data), intent(in)    :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        type(box_data), intent(in)    :: invdiags
+        real(dp), intent(in)          :: omega, tol
+        integer, intent(in)           :: maxiters
+        logical, intent(in)           :: zerophi
+        integer, intent(in)           :: verbosity
+        integer, intent(in), optional :: ordering
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), f(:,:), iv(:,:)
+        integer :: ord
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (maxiters .eq. 0) return
+        if (tol .gt. zero .and. verbosity .ge. 3) print*, 'relax_gs: residual monitoring is not implemented'
+        if (zerophi) phi%data = zero
+        call pack_geo (gp, gc, geo, rhs%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (f, rhs)
+        call pack_cc (iv, invdiags)
+        call pack_phi (a, phi)
+        call mgp_check (c_dc_relax_gs(gc, bcc, merge(1, 0, homog), iv, omega, maxiters, f, a, ord), &
+                        'DeferredPoisson2D::relax_gs')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+    end subroutine relax_gs
+
+end module DeferredPoisson2D
+
+
+! MGPoisson2D_DD.f90:574 -- deferred-correction V-cycle (GPU)
+module DeferredMGPoisson2D
+    use ArrayUtils
+    use MGPoisson2D, only: estimate_maxdepth, create_mgschedule, restrict, prolong, dropin_vcycle
+    implicit none
+
+contains
+
+    ! MGPoisson2D_DD.f90:589
+    subroutine vcycle (phi, rhs, geo, bc, homog, amrrefx, amrrefy, tol, maxiters, maxdepth_user, numcycles, &
+                       smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ordering)
+        type(box_data), intent(inout) :: phi
+        type(box_data), intent(inout) :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        integer, intent(in)           :: amrrefx, amrrefy
+        real(dp), intent(in)          :: tol
+        integer, intent(in)           :: maxiters, maxdepth_user, numcycles
+        integer, intent(in)           :: smooth_down, smooth_up, smooth_bottom
+        logical, intent(in)           :: zerophi
+        integer, intent(in)           :: verbosity
+        integer, intent(in), optional :: ordering
+        integer :: ord
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (amrrefx .lt. -1000000 .or. amrrefy .lt. -1000000) print*, 'unused'
+        call dropin_vcycle (1, phi, rhs, geo, bc, homog, tol, maxiters, maxdepth_user, numcycles, &
+                            smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ord)
+    end subroutine vcycle
+
+end module DeferredMGPoisson2D
