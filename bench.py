@@ -89,6 +89,13 @@ def main():
     ap.add_argument("--n", type=int, default=NX)
     ap.add_argument("--tol", type=float, default=TOL)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="set up the solver and time only the x-line, residual and copy kernels "
+                         "(for hardware-counter passes)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_xline_2048.json"),
+                    help="measured HBM bytes per x-line launch (from FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
@@ -96,9 +103,17 @@ def main():
 
     from mgpoisson2d_amd import capi
     geo, bc, rhs = config2_problem(args.n, args.n)
-    solver = capi.Solver(geo, bc, method=capi.METHOD_KRYLOV_MG, smooth_down=2, smooth_up=2)
+    solver = capi.Solver(geo, bc, method=capi.METHOD_KRYLOV_MG, smooth_down=2, smooth_up=2,
+                         use_graph=not args.no_graph)
     solver.set_rhs(rhs)
     print(f"# setup done: {args.n}x{args.n}", file=sys.stderr, flush=True)
+    if args.kernels_only:
+        for k, name in ((0, "x-line even rows"), (1, "residual"), (2, "copy")):
+            ms, nbytes = solver.time_kernel(k, reps=5)
+            print(f"# {name}: {ms:.4f} ms/launch, {nbytes / 1e6:.1f} MB algorithmic, "
+                  f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+        solver.close()
+        return
 
     for _ in range(args.warmup):
         iters, outcome = solver.solve_fast(args.tol, 400, zerophi=True)
@@ -126,6 +141,11 @@ def main():
     k_ms, k_bytes = solver.time_kernel(0, reps=20)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     r_ms, r_bytes = solver.time_kernel(1, reps=20)
+
+    traffic = None
+    if args.n == NX and os.path.exists(args.traffic):
+        with open(args.traffic) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -161,7 +181,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": k_ms,
             "alg_bytes_per_launch": k_bytes,
         },

@@ -84,6 +84,12 @@ int mgp_compute_invdiags(const mgp_geo *geo, const mgp_bc *bc, int homog, double
  * ordering MGP_ORDER_LEX (reference order) or MGP_ORDER_MC. */
 int mgp_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega,
                  int maxiters, const double *rhs, double *phi, int ordering);
+/* x-line zebra relaxation, the smoother of MGP_METHOD_KRYLOV_MG (no counterpart in the
+ * reference): each sweep solves, row by row, T_j delta = rhs - L phi and adds delta, where L is
+ * the reference operator with homogeneous BCs and T_j its tridiagonal part along row j; even
+ * interior rows, then odd ones, then rows 0 and ny-1.  Dirichlet/Neumann sides, nx, ny >= 4;
+ * phi's ghost cells are neither read nor written. */
+int mgp_xline_relax(const mgp_geo *geo, const mgp_bc *bc, const double *rhs, double *phi, int sweeps);
 /* MGPoisson2D_DD.f90:45 DeferredPoisson2D::compute_laplacian */
 int mgp_dc_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *lap);
 /* MGPoisson2D_DD.f90:16 DeferredPoisson2D::compute_invdiags */
@@ -159,7 +165,8 @@ void *mgp_solver_stream(mgp_solver *s);
 int mgp_solver_last_cycles(const mgp_solver *s);
 /* Measurement hook (KRYLOV_MG): times `reps` launches of one finest-level
  * kernel with HIP events on the solver's stream.  kernel 0: x-line
- * relaxation of the even interior rows; 1: residual.  *avg_ms is the mean
+ * relaxation of the even interior rows; 1: residual; 2: plain copy of one
+ * finest-level field (bandwidth reference).  *avg_ms is the mean
  * duration per launch, *bytes the algorithmic bytes one launch moves. */
 int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, double *avg_ms, double *bytes);
 /* hipDeviceSynchronize on the current device. */

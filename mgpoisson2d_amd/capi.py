@@ -65,7 +65,7 @@ EXPORTS = (
     "mgp_solver_default_opts", "mgp_solver_create", "mgp_solver_destroy", "mgp_solver_set_rhs",
     "mgp_solver_set_phi", "mgp_solver_get_phi", "mgp_solver_copy_rhs_from", "mgp_solver_solve",
     "mgp_solver_field", "mgp_solver_stream", "mgp_solver_last_cycles", "mgp_solver_time_kernel",
-    "mgp_device_synchronize",
+    "mgp_device_synchronize", "mgp_xline_relax",
 )
 
 
@@ -147,6 +147,15 @@ def relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_LEX
     invd, rhs = _f64(invd), _f64(rhs)
     _check(lib.mgp_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
                             _ptr(rhs), _ptr(phi), _i(ordering)))
+    return phi
+
+
+def xline_relax(geo, bc, rhs, phi, sweeps):
+    """x-line zebra sweeps of the homogeneous operator (the fast solver's smoother)."""
+    a = _Args()
+    phi = _f64(phi).copy()
+    rhs = _f64(rhs)
+    _check(lib.mgp_xline_relax(a.geo(geo), a.bc(bc), _ptr(rhs), _ptr(phi), _i(sweeps)))
     return phi
 
 

@@ -214,6 +214,32 @@ int mgp_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *
     });
 }
 
+int mgp_xline_relax(const mgp_geo *geo, const mgp_bc *bc, const double *rhs, double *phi, int sweeps) {
+    return guarded([&] {
+        if (sweeps <= 0) return;
+        Call c(geo, bc);
+        const Grid &g = c.geo.g;
+        for (int k = 0; k < 4; ++k)
+            if (bc->type[k] != BC_DIRI && bc->type[k] != BC_NEUM) {
+                g_last_error = "xline_relax supports Dirichlet and Neumann sides";
+                throw Failure(MGP_ERR_UNSUPPORTED);
+            }
+        if (g.nx < 4 || g.ny < 4 || !xline_supported(g.nx)) {
+            g_last_error = "xline_relax needs 4 <= nx <= 4111 and ny >= 4";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const Bc hb = mg_bc(c.bc.bc, MODE_UNIFORM);
+        DevBuf e = c.field(), f = c.field(), K;
+        K.alloc((size_t)ring_table_elems(g));
+        up_phi(c.s, e, g, phi);
+        up_cc(c.s, f, g, rhs);
+        launch_ring_coef(c.s, K.p, g, c.geo.view(), hb);
+        for (int k = 0; k < sweeps; ++k) xline_zebra(c.s, g, c.geo.view(), K.p, e.p, f.p);
+        down_phi(c.s, phi, e, g);
+        c.finish();
+    });
+}
+
 int mgp_dc_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *lap) {
     return guarded([&] {
         Call c(geo, bc);
@@ -544,8 +570,8 @@ int mgp_solver_last_cycles(const mgp_solver *s) { return s ? s->last_cycles : 0;
 
 int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, double *avg_ms, double *bytes) {
     return guarded([&] {
-        if (s->opts.method != MGP_METHOD_KRYLOV_MG || reps <= 0 || kernel < 0 || kernel > 1) {
-            g_last_error = "time_kernel: KRYLOV_MG solvers, kernel 0 or 1, reps > 0";
+        if (s->opts.method != MGP_METHOD_KRYLOV_MG || reps <= 0 || kernel < 0 || kernel > 2) {
+            g_last_error = "time_kernel: KRYLOV_MG solvers, kernel 0, 1 or 2, reps > 0";
             throw Failure(MGP_ERR_ARG);
         }
         *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, bytes);

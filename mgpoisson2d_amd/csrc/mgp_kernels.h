@@ -77,13 +77,19 @@ struct BottomArgs {
 };
 void launch_bicgstab2(hipStream_t s, const BottomArgs &a);
 
-// Line relaxation along x (fast solver).
-void launch_line_coef(hipStream_t s, double *A, double *B, double *C, const Grid &g, const Geo &geo, const Bc &bc);
-int xline_threads(int nx);
+// Line relaxation along x (fast solver, mgp_xline.hip).
+// Ring-cell stencil table of the homogeneous operator: ring_table_elems(g) doubles.
+int ring_table_elems(const Grid &g);
+void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, const Bc &bc);
+// rc = 1x2 (y) average of the fine residual f - L e (table at ring cells); ec = 0.
+void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
+                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo);
+bool xline_supported(int nx);
 size_t xline_lds_bytes(int nx);
-// Solve rows j = row0 + k*rowstep, k < nrows (one workgroup each).
-void launch_xline(hipStream_t s, double *e, const double *f, const double *A, const double *B, const double *C,
-                  const Grid &g, const Geo &geo, const Bc &bc, int row0, int rowstep, int nrows);
+// Relax rows j = row0 + k*rowstep, k < nrows (one workgroup each; the rows of
+// one launch must not be within two rows of each other).
+void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
+                  int row0, int rowstep, int nrows);
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
                       double shi);
 void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, double b, const double *y, double c,

@@ -554,49 +554,64 @@ struct Acc {
     __device__ double val() const { return s + c; }
 };
 
+// Fixed-order tree combine of per-thread (s, c) pairs in LDS; thread 0 ends
+// with the block's compensated sum (or max for kind 2).
+__device__ __forceinline__ double block_combine(double *ss, double *sc, Acc acc, double mx, int kind) {
+    const int t = threadIdx.x;
+    ss[t] = kind == 2 ? mx : acc.s;
+    sc[t] = kind == 2 ? 0.0 : acc.c;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            if (kind == 2) {
+                ss[t] = fmax(ss[t], ss[t + off]);
+            } else {
+                Acc u{ss[t], sc[t]};
+                u.add(ss[t + off]);
+                u.c += sc[t + off];
+                ss[t] = u.s;
+                sc[t] = u.c;
+            }
+        }
+        __syncthreads();
+    }
+    return kind == 2 ? ss[0] : ss[0] + sc[0];
+}
+
 // kind 0: sum w*a*b, kind 1: sum w*a*a, kind 2: max |a|, kind 3: sum w*a  (w = vol or vol*J)
+// Block b takes rows b, b + gridDim.x, ...; threads stride along the row.
 __global__ void __launch_bounds__(256) k_reduce_partial(double *partial, const double *a, const double *b,
                                                         const double *J, double vol, Grid g, int kind) {
     __shared__ double ss[256], sc[256];
     Acc acc{0.0, 0.0};
     double mx = 0.0;
-    const long n = (long)g.nx * g.ny;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(k % g.nx), j = (int)(k / g.nx);
-        const size_t c = ix(g.pitch, i, j);
-        const double w = J ? vol * J[c] : vol;
-        const double x = a[c];
-        if (kind == 0) acc.add(w * (x * b[c]));
-        else if (kind == 1) acc.add(w * (x * x));
-        else if (kind == 2) mx = fmax(mx, fabs(x));
-        else acc.add(w * x);
-    }
-    ss[threadIdx.x] = kind == 2 ? mx : acc.s;
-    sc[threadIdx.x] = kind == 2 ? 0.0 : acc.c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Acc t{0.0, 0.0};
-        double m = 0.0;
-        for (int k = 0; k < (int)blockDim.x; ++k) {
-            if (kind == 2) m = fmax(m, ss[k]);
-            else {
-                t.add(ss[k]);
-                t.add(sc[k]);
-            }
+    for (int j = blockIdx.x; j < g.ny; j += gridDim.x) {
+        const size_t row = ix(g.pitch, 0, j);
+#pragma unroll 4
+        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+            const size_t c = row + i;
+            const double w = J ? vol * J[c] : vol;
+            const double x = a[c];
+            if (kind == 0) acc.add(w * (x * b[c]));
+            else if (kind == 1) acc.add(w * (x * x));
+            else if (kind == 2) mx = fmax(mx, fabs(x));
+            else acc.add(w * x);
         }
-        partial[blockIdx.x] = kind == 2 ? m : t.val();
     }
+    const double v = block_combine(ss, sc, acc, mx, kind);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
-__global__ void __launch_bounds__(64) k_reduce_final(double *out, const double *partial, int n, int kind) {
-    if (threadIdx.x != 0) return;
-    Acc t{0.0, 0.0};
-    double m = 0.0;
-    for (int k = 0; k < n; ++k) {
-        if (kind == 2) m = fmax(m, partial[k]);
-        else t.add(partial[k]);
+__global__ void __launch_bounds__(256) k_reduce_final(double *out, const double *partial, int n, int kind) {
+    __shared__ double ss[256], sc[256];
+    Acc acc{0.0, 0.0};
+    double mx = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        if (kind == 2) mx = fmax(mx, partial[k]);
+        else acc.add(partial[k]);
     }
-    *out = kind == 2 ? m : t.val();
+    const double v = block_combine(ss, sc, acc, mx, kind);
+    if (threadIdx.x == 0) *out = v;
 }
 
 // Sequential J-weighted sums in the reference's loop order (one thread):
@@ -841,163 +856,6 @@ __global__ void __launch_bounds__(256) k_relax_block(Bottom B, double omega, int
 
 namespace mgp {
 
-// ===========================================================================
-// Line relaxation along x (rows) for the resident fast solver.
-//
-// The tridiagonal part of the reference operator in a row -- the coefficients
-// of phi(i-1,j), phi(i,j), phi(i+1,j) in L(i,j), Dirichlet ghosts eliminated
-// -- is probed once per level.  A sweep solves T delta = f - L e for every
-// row of one colour (one workgroup per row) and adds delta to e.
-// ===========================================================================
-
-// Impulse whose ghosts follow the homogeneous fill of the MG levels
-// (Dirichlet order 3 / LAH negate; Neumann and others contribute nothing).
-struct ImpulseFilled {
-    int ci, cj;
-    Grid g;
-    int t[4], m[4];
-    __device__ double v(int i, int j) const { return (i == ci && j == cj) ? 1.0 : 0.0; }
-    __device__ double ghost(int side, double p0, double p1, double p2) const {
-        if (t[side] != BC_DIRI) return 0.0;
-        if (m[side] == MODE_LAH) return -p0;
-        return -3.0 * p0 + p1 - (1.0 / 5.0) * p2;
-    }
-    __device__ double operator()(int i, int j) const {
-        const int nx = g.nx, ny = g.ny;
-        const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
-        if (xin && yin) return v(i, j);
-        if (!xin && !yin) return 0.0;
-        if (!xin) return i < 0 ? ghost(XLO, v(0, j), v(1, j), v(2, j))
-                               : ghost(XHI, v(nx - 1, j), v(nx - 2, j), v(nx - 3, j));
-        return j < 0 ? ghost(YLO, v(i, 0), v(i, 1), v(i, 2)) : ghost(YHI, v(i, ny - 1), v(i, ny - 2), v(i, ny - 3));
-    }
-};
-
-__global__ void __launch_bounds__(256) k_line_coef(double *A, double *B, double *C, Grid g, Geo geo, Bc bc) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= g.nx || j >= g.ny) return;
-    ImpulseFilled imp;
-    imp.g = g;
-    for (int k = 0; k < 4; ++k) {
-        imp.t[k] = bc.type[k];
-        imp.m[k] = bc.mode[k];
-    }
-    imp.cj = j;
-    const size_t c = ix(g.pitch, i, j);
-    imp.ci = i - 1;
-    A[c] = i > 0 ? cell_lap(imp, g, geo, bc, 1, i, j) : 0.0;
-    imp.ci = i;
-    B[c] = cell_lap(imp, g, geo, bc, 1, i, j);
-    imp.ci = i + 1;
-    C[c] = i < g.nx - 1 ? cell_lap(imp, g, geo, bc, 1, i, j) : 0.0;
-}
-
-// One workgroup per row j = row0 + blockIdx.x * rowstep.  nt threads (a power
-// of two, nt <= blockDim, each owning >= 2 unknowns) run the partition
-// method: per-chunk elimination towards the chunk ends, PCR on the 2*nt
-// reduced unknowns, back substitution.  LDS: 4*nx + 8*nt doubles.
-__global__ void __launch_bounds__(256) k_xline(double *e, const double *f, const double *A, const double *B,
-                                               const double *C, Grid g, Geo geo, Bc bc, int row0, int rowstep,
-                                               int nt) {
-    extern __shared__ double lds[];
-    const int nx = g.nx, P = g.pitch;
-    const int j = row0 + (int)blockIdx.x * rowstep;
-    double *sa = lds, *sb = lds + nx, *sc = lds + 2 * nx, *sd = lds + 3 * nx;
-    const int n = 2 * nt;
-    double *ra = lds + 4 * nx, *rb = ra + n, *rc = rb + n, *rd = rc + n;
-    const int tid = threadIdx.x;
-    CFld ef{e, P};
-    const SynthGet eg = synth(e, g, bc);
-    for (int i = tid; i < nx; i += blockDim.x) {
-        const size_t c = ix(P, i, j);
-        const bool interior = i > 0 && i < nx - 1 && j > 0 && j < g.ny - 1;
-        const double lap = interior ? cell_lap_interior(ef, g, geo, i, j) : cell_lap(eg, g, geo, bc, 1, i, j);
-        sd[i] = f[c] - lap;
-        sa[i] = A[c];
-        sb[i] = B[c];
-        sc[i] = C[c];
-    }
-    __syncthreads();
-    int s0 = 0, s1 = 0;
-    if (tid < nt) {
-        s0 = (int)((long)tid * nx / nt);
-        s1 = (int)((long)(tid + 1) * nx / nt) - 1;  // inclusive chunk end
-        // upward pass (registers): equation for x_s in terms of x_{s-1}, x_s, x_e
-        double bu = sb[s1 - 1], cu = sc[s1 - 1], du = sd[s1 - 1];
-        for (int k = s1 - 2; k >= s0; --k) {
-            const double r = sc[k] / bu;
-            bu = sb[k] - r * sa[k + 1];
-            cu = -r * cu;
-            du = sd[k] - r * du;
-        }
-        ra[2 * tid] = sa[s0];
-        rb[2 * tid] = bu;
-        rc[2 * tid] = cu;
-        rd[2 * tid] = du;
-        // downward pass, in place: a'_k x_s + b'_k x_k + c_k x_{k+1} = d'_k
-        for (int k = s0 + 2; k <= s1; ++k) {
-            const double r = sa[k] / sb[k - 1];
-            sa[k] = -r * sa[k - 1];
-            sb[k] = sb[k] - r * sc[k - 1];
-            sd[k] = sd[k] - r * sd[k - 1];
-        }
-        ra[2 * tid + 1] = sa[s1];
-        rb[2 * tid + 1] = sb[s1];
-        rc[2 * tid + 1] = sc[s1];
-        rd[2 * tid + 1] = sd[s1];
-    }
-    __syncthreads();
-    // PCR on the reduced system (n = 2*nt unknowns, ordering s_0, e_0, s_1, ...)
-    for (int st = 1; st < n; st *= 2) {
-        double na[2], nb[2], nc[2], nd[2];
-        int cnt = 0;
-        for (int q = tid; q < n; q += blockDim.x, ++cnt) {
-            const double a = ra[q], b = rb[q], c = rc[q], d = rd[q];
-            double k1 = 0.0, k2 = 0.0, am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
-            if (q - st >= 0) {
-                k1 = a / rb[q - st];
-                am = ra[q - st];
-                cm = rc[q - st];
-                dm = rd[q - st];
-            }
-            if (q + st < n) {
-                k2 = c / rb[q + st];
-                ap = ra[q + st];
-                cp = rc[q + st];
-                dp = rd[q + st];
-            }
-            na[cnt] = -am * k1;
-            nc[cnt] = -cp * k2;
-            nb[cnt] = b - cm * k1 - ap * k2;
-            nd[cnt] = d - dm * k1 - dp * k2;
-        }
-        __syncthreads();
-        cnt = 0;
-        for (int q = tid; q < n; q += blockDim.x, ++cnt) {
-            ra[q] = na[cnt];
-            rb[q] = nb[cnt];
-            rc[q] = nc[cnt];
-            rd[q] = nd[cnt];
-        }
-        __syncthreads();
-    }
-    // back substitution within the chunk
-    if (tid < nt) {
-        const double xs = rd[2 * tid] / rb[2 * tid], xe = rd[2 * tid + 1] / rb[2 * tid + 1];
-        sd[s1] = xe;
-        double xnext = xe;
-        for (int k = s1 - 1; k > s0; --k) {
-            const double x = (sd[k] - sa[k] * xs - sc[k] * xnext) / sb[k];
-            sd[k] = x;
-            xnext = x;
-        }
-        sd[s0] = xs;
-    }
-    __syncthreads();
-    for (int i = tid; i < nx; i += blockDim.x) e[ix(P, i, j)] += sd[i];
-}
-
 // y-direction prolongation for 1x2 semicoarsening: linear in y, coarse
 // ghosts reflected (Neumann, sign +1) or negated (Dirichlet, sign -1).
 __global__ void __launch_bounds__(256) k_prolong_y(double *ef, const double *ec, Grid gf, Grid gc, double slo,
@@ -1176,9 +1034,9 @@ int reduce_blocks(const Grid &g) {
 }
 void launch_reduce(hipStream_t s, double *out, double *partial, const double *a, const double *b,
                    const double *J, double vol, const Grid &g, int kind) {
-    const int nb = reduce_blocks(g);
+    const int nb = reduce_blocks(g) < g.ny ? reduce_blocks(g) : g.ny;
     k_reduce_partial<<<nb, 256, 0, s>>>(partial, a, b, J, vol, g, kind);
-    k_reduce_final<<<1, 64, 0, s>>>(out, partial, nb, kind);
+    k_reduce_final<<<1, 256, 0, s>>>(out, partial, nb, kind);
 }
 void launch_bicg_scalars(hipStream_t s, double *sc, int op) { k_bicg_scalars<<<1, 64, 0, s>>>(sc, op); }
 void launch_bicg_update(hipStream_t s, double *out, const double *x, const double *y, const double *sc,
@@ -1199,26 +1057,6 @@ void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sw
 }  // namespace mgp
 
 namespace mgp {
-void launch_line_coef(hipStream_t s, double *A, double *B, double *C, const Grid &g, const Geo &geo, const Bc &bc) {
-    k_line_coef<<<tiles(g.nx, g.ny), 256, 0, s>>>(A, B, C, g, geo, bc);
-}
-int xline_threads(int nx) {
-    int nt = 1;
-    while (nt * 2 <= 256 && nt * 2 * 2 <= nx) nt *= 2;
-    return nt;
-}
-size_t xline_lds_bytes(int nx) { return (size_t)(4 * nx + 8 * xline_threads(nx)) * sizeof(double); }
-void launch_xline(hipStream_t s, double *e, const double *f, const double *A, const double *B, const double *C,
-                  const Grid &g, const Geo &geo, const Bc &bc, int row0, int rowstep, int nrows) {
-    if (nrows <= 0) return;
-    const int nt = xline_threads(g.nx);
-    k_xline<<<nrows, 256, xline_lds_bytes(g.nx), s>>>(e, f, A, B, C, g, geo, bc, row0, rowstep, nt);
-    const hipError_t err = hipGetLastError();
-    if (err != hipSuccess) {
-        g_last_error = std::string("k_xline launch: ") + hipGetErrorString(err);
-        throw Failure(MGP_ERR_HIP);
-    }
-}
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
                       double shi) {
     k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, slo, shi);

@@ -23,9 +23,14 @@ struct KrylovMG {
               double *rhs, double tol, int maxiters, int zerophi, double *relres, int *iters);
     // Times `reps` launches of one fine-level kernel with HIP events on stream s.
     // kernel 0: x-line relaxation of the even interior rows; 1: residual
-    // (9-point operator).  Returns the mean ms per launch; *bytes receives the
+    // (9-point operator); 2: plain copy of one field.  Returns the mean ms per launch; *bytes receives the
     // algorithmic bytes one launch moves (each array element it needs, once).
     double time_kernel(hipStream_t s, int kernel, int reps, double *bytes);
 };
+
+// One x-line zebra sweep of the homogeneous operator on one level: even
+// interior rows, odd interior rows, then rows 0 and ny-1.  K is the level's
+// ring stencil table (launch_ring_coef).  e's ghost cells are not read.
+void xline_zebra(hipStream_t s, const Grid &g, const Geo &geo, const double *K, double *e, const double *f);
 
 }  // namespace mgp

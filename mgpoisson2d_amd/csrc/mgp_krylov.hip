@@ -34,7 +34,7 @@ struct KLevel {
     Grid g;
     Geo geo;
     Bc bc;
-    DevBuf J, Jxx, Jxy, Jyx, Jyy, e, r, w, A, B, C;
+    DevBuf J, Jxx, Jxy, Jyx, Jyy, e, r, w, K;
 };
 
 struct KrylovMG::Impl {
@@ -65,8 +65,8 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
             g_last_error = "KRYLOV_MG supports Dirichlet and Neumann sides";
             throw Failure(MGP_ERR_UNSUPPORTED);
         }
-    if (xline_lds_bytes(top.g.nx) > 160 * 1024) {
-        g_last_error = "KRYLOV_MG: row too long for one workgroup's LDS (nx <= 4096 in FP64)";
+    if (!xline_supported(top.g.nx)) {
+        g_last_error = "KRYLOV_MG: row too long for one workgroup's line solve (nx <= 4111 in FP64)";
         throw Failure(MGP_ERR_UNSUPPORTED);
     }
     delete impl;
@@ -82,7 +82,8 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         KLevel &l = L.back();
         l.g = make_grid(top.g.nx, ny, top.g.dx, dy);
         const size_t n = level_elems(l.g.nx, l.g.ny);
-        for (DevBuf *b : {&l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy, &l.e, &l.r, &l.w, &l.A, &l.B, &l.C}) b->alloc(n);
+        for (DevBuf *b : {&l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy, &l.e, &l.r, &l.w}) b->alloc(n);
+        l.K.alloc((size_t)ring_table_elems(l.g));
         l.bc = mg_bc(user_bc, MODE_UNIFORM);
         const size_t bytes = n * sizeof(double);
         if (L.size() == 1) {
@@ -100,7 +101,7 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
             launch_restrict_yf(s, l.Jyy.p, f.Jyy.p, l.g, f.g.pitch, 1, 2);
         }
         l.geo = Geo{l.J.p, l.Jxx.p, l.Jxy.p, l.Jyx.p, l.Jyy.p};
-        launch_line_coef(s, l.A.p, l.B.p, l.C.p, l.g, l.geo, l.bc);
+        launch_ring_coef(s, l.K.p, l.g, l.geo, l.bc);
         if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || (int)L.size() >= maxlev) break;
         ny /= 2;
         dy *= 2.0;
@@ -116,30 +117,33 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
 }
 
 // One x-line zebra sweep on level l for  A_l e = f.
-static void zebra_sweep(hipStream_t s, KLevel &l, double *e, const double *f) {
-    const Grid &g = l.g;
+void xline_zebra(hipStream_t s, const Grid &g, const Geo &geo, const double *K, double *e, const double *f) {
     const int ny = g.ny;
-    // Dirichlet ghosts are synthesised inside the line kernel from the values
-    // each row sees, which equals a fill before each phase
+    // The line kernel never reads ghosts (ring cells use the level's stencil
+    // table, which folds the Dirichlet ghost rule in).
     // interior rows 1 .. ny-2: even ones, then odd ones
     const int n_even = ny >= 4 ? (ny - 4) / 2 + 1 : 0;  // rows 2, 4, ..., <= ny-2
     const int n_odd = ny >= 3 ? (ny - 3) / 2 + 1 : 0;   // rows 1, 3, ..., <= ny-2
-    launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 2, 2, n_even);
-    launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 1, 2, n_odd);
-    launch_xline(s, e, f, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 0, ny - 1, 2);
+    launch_xline(s, e, f, K, g, geo, 2, 2, n_even);
+    launch_xline(s, e, f, K, g, geo, 1, 2, n_odd);
+    launch_xline(s, e, f, K, g, geo, 0, ny - 1, 2);
 }
 
+static void zebra_sweep(hipStream_t s, KLevel &l, double *e, const double *f) {
+    xline_zebra(s, l.g, l.geo, l.K.p, e, f);
+}
+
+// e must be zero on entry for d > 0 (the restriction of level d-1 clears it).
 static void vcycle(hipStream_t s, KrylovMG::Impl &K, size_t d, double *e, const double *f) {
     KLevel &l = K.L[d];
-    launch_setval(s, e, 0.0, l.g, 1);
+    if (d == 0) launch_setval(s, e, 0.0, l.g, 1);
     if (d + 1 == K.L.size()) {
         for (int k = 0; k < BOTTOM_SWEEPS; ++k) zebra_sweep(s, l, e, f);
         return;
     }
     KLevel &c = K.L[d + 1];
     for (int k = 0; k < K.sd; ++k) zebra_sweep(s, l, e, f);
-    launch_residual_synth(s, l.w.p, e, f, l.g, l.geo, l.bc);
-    launch_restrict_cc(s, c.r.p, l.w.p, c.g, l.g.pitch, 1, 2, nullptr, nullptr);
+    launch_residual_restrict_y(s, c.r.p, c.e.p, e, f, l.K.p, l.g, c.g, l.geo);
     vcycle(s, K, d + 1, c.e.p, c.r.p);
     launch_prolong_y(s, e, c.e.p, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
     for (int k = 0; k < K.su; ++k) zebra_sweep(s, l, e, f);
@@ -266,11 +270,11 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, double *bytes)
     const int n_even = g.ny >= 4 ? (g.ny - 4) / 2 + 1 : 0;
     launch_fill_ghosts(s, l.e.p, g, l.geo, l.bc, 1, 0);
     if (kernel == 0) {
-        // rows 2, 4, ..: reads e on rows 1 .. 2*n_even+1, writes e on the updated
-        // rows; f, Jxx, Jxy, A, B, C on the updated rows; Jyx, Jyy on face rows
-        // 2 .. 2*n_even+1.
+        // rows 2, 4, ..: reads e on rows 1 .. 2*n_even+1 and writes it on the
+        // updated rows; f, Jxx, Jxy on the updated rows; Jyx, Jyy on face rows
+        // 2 .. 2*n_even+1; the two ring cells of each row read 9 table entries.
         const double R = n_even;
-        *bytes = nx * 8.0 * ((2.0 * R + 1.0) + R + R + 2.0 * R + 3.0 * R) + nx * 16.0 * (2.0 * R);
+        *bytes = nx * 8.0 * ((2.0 * R + 1.0) + R + R + R + 2.0 * R + 2.0 * R + R) + R * 2.0 * 9.0 * 8.0;
     } else if (kernel == 1) {
         // r = f - L e over all cells: e (with its one-cell halo), f, r, J-free metrics
         *bytes = nx * (double)g.ny * 8.0 * 3.0 + 16.0 * (nx + 1) * g.ny + 16.0 * nx * (g.ny + 1);
@@ -282,7 +286,7 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, double *bytes)
         throw Failure(MGP_ERR_ARG);
     }
     auto launch = [&]() {
-        if (kernel == 0) launch_xline(s, l.e.p, l.r.p, l.A.p, l.B.p, l.C.p, g, l.geo, l.bc, 2, 2, n_even);
+        if (kernel == 0) launch_xline(s, l.e.p, l.r.p, l.K.p, g, l.geo, 2, 2, n_even);
         else if (kernel == 1) launch_laplacian(s, l.w.p, l.e.p, l.r.p, g, l.geo, l.bc, 1, 0);
         else launch_lincomb(s, l.w.p, 1.0, l.e.p, 0.0, nullptr, g, 0);
     };

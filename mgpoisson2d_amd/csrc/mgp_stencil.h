@@ -44,23 +44,20 @@ struct SynthGet {
     int pitch, nx, ny;
     int t[4], m[4];
     __device__ __forceinline__ double raw(int i, int j) const { return p[ix(pitch, i, j)]; }
-    __device__ __forceinline__ double dg(int side, double p0, double p1, double p2) const {
-        if (m[side] == MODE_LAH) return -p0;
+    // Side indices are compile-time constants below (a runtime index into
+    // t[] / m[] would place them in scratch memory).
+    __device__ __forceinline__ static double dg(int mode, double p0, double p1, double p2) {
+        if (mode == MODE_LAH) return -p0;
         return -3.0 * p0 + p1 - (1.0 / 5.0) * p2;
     }
     __device__ __forceinline__ double operator()(int i, int j) const {
         const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
         if (xin == yin) return raw(i, j);  // valid cell, or a corner (never read)
-        if (!xin) {
-            const int side = i < 0 ? XLO : XHI;
-            if (t[side] != BC_DIRI) return raw(i, j);
-            return i < 0 ? dg(side, raw(0, j), raw(1, j), raw(2, j))
-                         : dg(side, raw(nx - 1, j), raw(nx - 2, j), raw(nx - 3, j));
-        }
-        const int side = j < 0 ? YLO : YHI;
-        if (t[side] != BC_DIRI) return raw(i, j);
-        return j < 0 ? dg(side, raw(i, 0), raw(i, 1), raw(i, 2))
-                     : dg(side, raw(i, ny - 1), raw(i, ny - 2), raw(i, ny - 3));
+        if (i < 0) return t[XLO] != BC_DIRI ? raw(i, j) : dg(m[XLO], raw(0, j), raw(1, j), raw(2, j));
+        if (i >= nx)
+            return t[XHI] != BC_DIRI ? raw(i, j) : dg(m[XHI], raw(nx - 1, j), raw(nx - 2, j), raw(nx - 3, j));
+        if (j < 0) return t[YLO] != BC_DIRI ? raw(i, j) : dg(m[YLO], raw(i, 0), raw(i, 1), raw(i, 2));
+        return t[YHI] != BC_DIRI ? raw(i, j) : dg(m[YHI], raw(i, ny - 1), raw(i, ny - 2), raw(i, ny - 3));
     }
 };
 
@@ -108,6 +105,40 @@ __device__ __forceinline__ void ring_of(const Grid &g, const Bc &bc, int i, int 
 // neighbour outside two sides is the mean of its x- and y-extrapolations.
 // Normal fluxes use the stored (ghost) values; a Neumann side's face flux is
 // replaced by its BC data.
+// Stencil neighbours of a ring cell on the outer side (OX, OY) (each -1, 0
+// or +1, not both 0): values outside a non-periodic side are extrapolated
+// 3(a - b) + c along the grid line, the corner one is the mean of its x and
+// y extrapolations (poisson2d.f90:170-1430).  The side is a template
+// argument so every e[][] index is a compile-time constant (registers, not
+// scratch memory).
+template <int OX, int OY, class Get>
+__device__ __forceinline__ void ring_neighbours(const Get &get, int i, int j, double (&e)[3][3]) {
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+        for (int di = -1; di <= 1; ++di)
+            e[dj + 1][di + 1] = ((OX && di == OX) || (OY && dj == OY)) ? 0.0 : get(i + di, j + dj);
+    if (OX) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+            if (OY && dj == OY) continue;
+            e[dj + 1][OX + 1] = 3.0 * (e[dj + 1][1] - e[dj + 1][1 - OX]) + get(i - 2 * OX, j + dj);
+        }
+    }
+    if (OY) {
+#pragma unroll
+        for (int di = -1; di <= 1; ++di) {
+            if (OX && di == OX) continue;
+            e[OY + 1][di + 1] = 3.0 * (e[1][di + 1] - e[1 - OY][di + 1]) + get(i + di, j - 2 * OY);
+        }
+    }
+    if (OX && OY) {
+        const double X = 3.0 * (e[OY + 1][1] - e[OY + 1][1 - OX]) + get(i - 2 * OX, j + OY);
+        const double Y = 3.0 * (e[1][OX + 1] - e[1 - OY][OX + 1]) + get(i + OX, j - 2 * OY);
+        e[OY + 1][OX + 1] = 0.5 * (X + Y);
+    }
+}
+
 template <class Get>
 __device__ __forceinline__ double cell_lap(const Get &get, const Grid &g, const Geo &geo,
                                            const Bc &bc, int homog, int i, int j) {
@@ -129,29 +160,15 @@ __device__ __forceinline__ double cell_lap(const Get &get, const Grid &g, const 
         esw = get(i - 1, j - 1);
     } else {
         double e[3][3];  // e[dj+1][di+1]
-#pragma unroll
-        for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll
-            for (int di = -1; di <= 1; ++di)
-                e[dj + 1][di + 1] = ((ox && di == ox) || (oy && dj == oy)) ? 0.0 : get(i + di, j + dj);
-        if (ox) {
-#pragma unroll
-            for (int dj = -1; dj <= 1; ++dj) {
-                if (oy && dj == oy) continue;
-                e[dj + 1][ox + 1] = 3.0 * (e[dj + 1][1] - e[dj + 1][1 - ox]) + get(i - 2 * ox, j + dj);
-            }
-        }
-        if (oy) {
-#pragma unroll
-            for (int di = -1; di <= 1; ++di) {
-                if (ox && di == ox) continue;
-                e[oy + 1][di + 1] = 3.0 * (e[1][di + 1] - e[1 - oy][di + 1]) + get(i + di, j - 2 * oy);
-            }
-        }
-        if (ox && oy) {
-            const double X = 3.0 * (e[oy + 1][1] - e[oy + 1][1 - ox]) + get(i - 2 * ox, j + oy);
-            const double Y = 3.0 * (e[1][ox + 1] - e[1 - oy][ox + 1]) + get(i + ox, j - 2 * oy);
-            e[oy + 1][ox + 1] = 0.5 * (X + Y);
+        switch ((ox + 1) * 3 + (oy + 1)) {
+            case 0: ring_neighbours<-1, -1>(get, i, j, e); break;
+            case 1: ring_neighbours<-1, 0>(get, i, j, e); break;
+            case 2: ring_neighbours<-1, 1>(get, i, j, e); break;
+            case 3: ring_neighbours<0, -1>(get, i, j, e); break;
+            case 5: ring_neighbours<0, 1>(get, i, j, e); break;
+            case 6: ring_neighbours<1, -1>(get, i, j, e); break;
+            case 7: ring_neighbours<1, 0>(get, i, j, e); break;
+            default: ring_neighbours<1, 1>(get, i, j, e); break;
         }
         ee = e[1][2];
         ew = e[1][0];

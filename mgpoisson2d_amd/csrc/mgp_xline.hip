@@ -1,0 +1,443 @@
+// mgp_xline.hip -- x-line relaxation of the resident fast solver on gfx950.
+//
+// A sweep of one row colour solves, for every row j of the colour,
+//     T_j delta = f - L e     (row j),   e(:, j) += delta
+// where L is the reference 9-point operator with homogeneous BCs (ghosts
+// eliminated) and T_j its tridiagonal part along the row: the coefficients
+// of e(i-1,j), e(i,j), e(i+1,j) in (L e)(i,j).
+//
+// Interior cells evaluate L and T from the face metrics already loaded for
+// the residual (cell_lap_interior's stencil, differentiated by hand).  Ring
+// cells (i = 0, nx-1 or j = 0, ny-1), whose stencils carry the boundary
+// extrapolations and the Dirichlet ghost rule, use an explicit coefficient
+// table probed once per level with unit impulses (k_ring_coef).
+//
+// One workgroup (256 threads) per row:
+//  1. coalesced pass: each thread evaluates d = f - L e and the row
+//     coefficients for cells i = tid + 256 u (all loads issued up front);
+//  2. the values go through LDS into contiguous chunks of C or C+1 cells
+//     per thread (padded stride C+1: no bank conflicts);
+//  3. partition method in registers: each chunk is eliminated towards its
+//     two ends, leaving a tridiagonal system of 2 nt unknowns;
+//  4. parallel cyclic reduction of that system in LDS (log2(2 nt) steps);
+//  5. back substitution in registers, transpose back, coalesced e update.
+// Rows are mapped to workgroups so that the rows an XCD processes are
+// contiguous (workgroups are dealt to the 8 XCDs round robin): neighbouring
+// rows share their halo rows through that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "mgp_host.h"
+#include "mgp_kernels.h"
+#include "mgp_stencil.h"
+
+namespace mgp {
+
+namespace {
+
+constexpr int XL_THREADS = 256;
+constexpr int RING_PLANES = 15;
+
+// Impulse whose ghosts follow the homogeneous fill of the MG levels
+// (Dirichlet order 3 / LAH negate; Neumann sides contribute no ghost).
+struct ImpulseFilled {
+    int ci, cj;
+    Grid g;
+    int t[4], m[4];
+    __device__ double v(int i, int j) const { return (i == ci && j == cj) ? 1.0 : 0.0; }
+    __device__ static double dg(int type, int mode, double p0, double p1, double p2) {
+        if (type != BC_DIRI) return 0.0;
+        if (mode == MODE_LAH) return -p0;
+        return -3.0 * p0 + p1 - (1.0 / 5.0) * p2;
+    }
+    __device__ double operator()(int i, int j) const {
+        const int nx = g.nx, ny = g.ny;
+        const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
+        if (xin && yin) return v(i, j);
+        if (!xin && !yin) return 0.0;
+        if (i < 0) return dg(t[XLO], m[XLO], v(0, j), v(1, j), v(2, j));
+        if (i >= nx) return dg(t[XHI], m[XHI], v(nx - 1, j), v(nx - 2, j), v(nx - 3, j));
+        if (j < 0) return dg(t[YLO], m[YLO], v(i, 0), v(i, 1), v(i, 2));
+        return dg(t[YHI], m[YHI], v(i, ny - 1), v(i, ny - 2), v(i, ny - 3));
+    }
+};
+
+__host__ __device__ inline int ring_count(int nx, int ny) { return 2 * nx + 2 * (ny - 2); }
+
+// Ring cell r -> (i, j) and the offset of table plane q.
+//  rows    (r < 2 nx):  j = 0 (r < nx) or ny-1;  plane q = 5 k + (di + 2),
+//                       offset (di, k s), s = +1 on row 0, -1 on row ny-1
+//  columns (r >= 2 nx): i = 0 or nx-1, j = 1 .. ny-2;  plane q = 3 k + (dj + 1),
+//                       q < 9, offset (k s, dj), s = +1 on column 0, -1 on nx-1
+__device__ inline void ring_plane(int nx, int ny, int r, int q, int &i, int &j, int &di, int &dj, bool &used) {
+    if (r < 2 * nx) {
+        const bool top = r >= nx;
+        i = top ? r - nx : r;
+        j = top ? ny - 1 : 0;
+        di = q % 5 - 2;
+        dj = (q / 5) * (top ? -1 : 1);
+        used = true;
+    } else {
+        const int rr = r - 2 * nx;
+        const bool right = rr >= ny - 2;
+        i = right ? nx - 1 : 0;
+        j = (right ? rr - (ny - 2) : rr) + 1;
+        di = (q / 3) * (right ? -1 : 1);
+        dj = q % 3 - 1;
+        used = q < 9;
+    }
+}
+
+// (L e)(i, j) at a ring cell from the table (valid neighbours only).
+__device__ __forceinline__ double ring_lap(const double *e, const double *K, int nring, const Grid &g, int i,
+                                           int j) {
+    const int nx = g.nx, ny = g.ny, P = g.pitch;
+    double lap = 0.0;
+    if (j == 0 || j == ny - 1) {
+        const int s = j == 0 ? 1 : -1;
+        const int r = (j == 0 ? 0 : nx) + i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int di = -2; di <= 2; ++di) {
+                const int ci = i + di;
+                const double ev = (ci >= 0 && ci < nx) ? e[ix(P, ci, j + k * s)] : 0.0;
+                lap += K[(size_t)(5 * k + di + 2) * nring + r] * ev;
+            }
+    } else {
+        const bool right = i == nx - 1;
+        const int s = right ? -1 : 1;
+        const int r = 2 * nx + (right ? ny - 2 : 0) + (j - 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int dj = -1; dj <= 1; ++dj)
+                lap += K[(size_t)(3 * k + dj + 1) * nring + r] * e[ix(P, i + k * s, j + dj)];
+    }
+    return lap;
+}
+
+// K[q * nring + r]: coefficient of e(i+di, j+dj) in the homogeneous L at ring
+// cell r (zero where the neighbour is not a valid cell).
+__global__ void __launch_bounds__(256) k_ring_coef(double *K, Grid g, Geo geo, Bc bc) {
+    const int nring = ring_count(g.nx, g.ny);
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= RING_PLANES * nring) return;
+    const int q = id / nring, r = id % nring;
+    int i, j, di, dj;
+    bool used;
+    ring_plane(g.nx, g.ny, r, q, i, j, di, dj, used);
+    const int ci = i + di, cj = j + dj;
+    double v = 0.0;
+    if (used && ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny) {
+        ImpulseFilled imp;
+        imp.g = g;
+        for (int k = 0; k < 4; ++k) {
+            imp.t[k] = bc.type[k];
+            imp.m[k] = bc.mode[k];
+        }
+        imp.ci = ci;
+        imp.cj = cj;
+        v = cell_lap(imp, g, geo, bc, 1, i, j);
+    }
+    K[id] = v;
+}
+
+// Position of cell i in the chunk layout: the first rem chunks hold C+1
+// cells, the others C; chunk t starts at LDS slot t*(C+1).
+template <int C>
+__device__ __forceinline__ int chunk_pos(int i, int rem) {
+    const int big = rem * (C + 1);
+    if (i < big) return i;
+    const int k = i - big;
+    return big + (k / C) * (C + 1) + k % C;
+}
+
+template <int C>
+__global__ void __launch_bounds__(XL_THREADS) k_xline(double *__restrict__ e, const double *__restrict__ f,
+                                                      const double *__restrict__ K, Grid g, Geo geo, int row0,
+                                                      int rowstep, int nt) {
+    constexpr int CP = C + 1;  // LDS stride of a chunk, and the longest chunk
+    constexpr int U = C + 1;   // cells per thread in the coalesced passes
+    extern __shared__ double lds[];
+    const int nx = g.nx, ny = g.ny, P = g.pitch, tid = threadIdx.x;
+    const int rem = nx - nt * C;
+    const int nring = ring_count(nx, ny);
+    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, fourth = 0.25;
+    const double idx2 = idx * idx, idy2 = idy * idy, ixy4 = fourth * idx * idy;
+
+    // XCD-aware row of this workgroup: XCD x (= blockIdx % 8) takes a
+    // contiguous block of q8 or q8+1 rows.
+    const int nb = gridDim.x, b = blockIdx.x, x = b & 7;
+    const int q8 = nb >> 3, r8 = nb & 7;
+    const int slot = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    const int j = row0 + slot * rowstep;
+    const bool ring_row = j == 0 || j == ny - 1;
+
+    // ---- 1. coalesced residual and row coefficients, stored straight into
+    //         the chunk layout: LDS = [d | b | a | c], nt*(C+1) doubles each
+    const int NT1 = nt * CP;
+    double *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
+    if (!ring_row) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = tid + XL_THREADS * u;
+            if (i >= nx) break;
+            const size_t c = ix(P, i, j);
+            const double *q = e + c;
+            // interior stencil; for i = 0, nx-1 the reads stay inside the
+            // padded level and the values are replaced below
+            const double p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
+            const double ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
+            const double Jxx_e = geo.Jxx[c + 1], Jxx_w = geo.Jxx[c];
+            const double Jxy_e = geo.Jxy[c + 1], Jxy_w = geo.Jxy[c];
+            const double Jyy_n = geo.Jyy[c + P], Jyy_s = geo.Jyy[c];
+            const double Jyx_n = geo.Jyx[c + P], Jyx_s = geo.Jyx[c];
+            const double xfe = Jxx_e * (pe - p) * idx + Jxy_e * fourth * (ene - ese + pn - ps) * idy;
+            const double xfw = Jxx_w * (p - pw) * idx + Jxy_w * fourth * (pn - ps + enw - esw) * idy;
+            const double yfn = Jyy_n * (pn - p) * idy + Jyx_n * fourth * (ene - enw + pe - pw) * idx;
+            const double yfs = Jyy_s * (p - ps) * idy + Jyx_s * fourth * (pe - pw + ese - esw) * idx;
+            const int o = chunk_pos<C>(i, rem);
+            Ld[o] = f[c] - ((xfe - xfw) * idx + (yfn - yfs) * idy);
+            La[o] = Jxx_w * idx2 - (Jyx_n - Jyx_s) * ixy4;
+            Lb[o] = -(Jxx_e + Jxx_w) * idx2 - (Jyy_n + Jyy_s) * idy2;
+            Lc[o] = Jxx_e * idx2 + (Jyx_n - Jyx_s) * ixy4;
+        }
+        // the two ring cells of the row (i = 0, nx-1): explicit stencil table
+        for (int side = 0; side < 2; ++side) {
+            const int iend = side ? nx - 1 : 0;
+            if (iend % XL_THREADS != tid) continue;
+            const int r = 2 * nx + (side ? ny - 2 : 0) + (j - 1);
+            // coefficient of e(i-1) / e(i+1): table offset (s, 0) = plane 4
+            const double kin = K[(size_t)4 * nring + r];
+            const int o = chunk_pos<C>(iend, rem);
+            Ld[o] = f[ix(P, iend, j)] - ring_lap(e, K, nring, g, iend, j);
+            La[o] = side ? kin : 0.0;
+            Lb[o] = K[(size_t)1 * nring + r];
+            Lc[o] = side ? 0.0 : kin;
+        }
+    } else {
+        // boundary row: every cell from the table
+        const int rbase = j == 0 ? 0 : nx;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = tid + XL_THREADS * u;
+            if (i >= nx) break;
+            const int r = rbase + i;
+            const int o = chunk_pos<C>(i, rem);
+            Ld[o] = f[ix(P, i, j)] - ring_lap(e, K, nring, g, i, j);
+            La[o] = K[(size_t)1 * nring + r];
+            Lb[o] = K[(size_t)2 * nring + r];
+            Lc[o] = K[(size_t)3 * nring + r];
+        }
+    }
+    __syncthreads();
+
+    // ---- 2. each thread takes its chunk into registers ------------------
+    double ca[CP], cb[CP], cc[CP], cd[CP];
+    const int len = tid < nt ? C + (tid < rem ? 1 : 0) : 0;
+#pragma unroll
+    for (int k = 0; k < CP; ++k)
+        if (k < len) {
+            cd[k] = Ld[tid * CP + k];
+            cb[k] = Lb[tid * CP + k];
+            ca[k] = La[tid * CP + k];
+            cc[k] = Lc[tid * CP + k];
+        }
+    __syncthreads();
+
+    // ---- 3. partition elimination in registers -------------------------
+    // start row:  a_0 x_{prev end} + bu x_s + cu x_e = du
+    // end row:    a'_L x_s + b'_L x_e + c_L x_{next start} = d'_L
+    double ea = 0.0, eb = 1.0, ec = 0.0, ed = 0.0, oa = 0.0, ob = 1.0, oc = 0.0, od = 0.0;
+    if (tid < nt) {
+        const bool longc = len == CP;
+        double bu = longc ? cb[C - 1] : cb[C - 2];
+        double cu = longc ? cc[C - 1] : cc[C - 2];
+        double du = longc ? cd[C - 1] : cd[C - 2];
+#pragma unroll
+        for (int k = C - 2; k >= 0; --k)
+            if (k <= len - 3) {
+                const double r = cc[k] / bu;
+                bu = cb[k] - r * ca[k + 1];
+                cu = -r * cu;
+                du = cd[k] - r * du;
+            }
+        ea = ca[0];
+        eb = bu;
+        ec = cu;
+        ed = du;
+#pragma unroll
+        for (int k = 2; k <= C; ++k)
+            if (k <= len - 1) {
+                const double r = ca[k] / cb[k - 1];
+                ca[k] = -r * ca[k - 1];
+                cb[k] = cb[k] - r * cc[k - 1];
+                cd[k] = cd[k] - r * cd[k - 1];
+            }
+        oa = longc ? ca[C] : ca[C - 1];
+        ob = longc ? cb[C] : cb[C - 1];
+        oc = longc ? cc[C] : cc[C - 1];
+        od = longc ? cd[C] : cd[C - 1];
+    }
+
+    // ---- 4. PCR on the 2 nt reduced unknowns (even: chunk starts) --------
+    // LDS: [Ea Eb Ec Ed Oa Ob Oc Od], nt doubles each; entry q lives in
+    // half (q & 1) at index q >> 1.
+    const int n = 2 * nt;
+    for (int st = 1; st < n; st *= 2) {
+        if (tid < nt) {
+            lds[tid] = ea;
+            lds[nt + tid] = eb;
+            lds[2 * nt + tid] = ec;
+            lds[3 * nt + tid] = ed;
+            lds[4 * nt + tid] = oa;
+            lds[5 * nt + tid] = ob;
+            lds[6 * nt + tid] = oc;
+            lds[7 * nt + tid] = od;
+        }
+        __syncthreads();
+        if (tid < nt) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int qq = 2 * tid + h;
+                const double a = h ? oa : ea, bb = h ? ob : eb, c = h ? oc : ec, d = h ? od : ed;
+                double k1 = 0.0, k2 = 0.0, am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
+                const int qm = qq - st, qp = qq + st;
+                if (qm >= 0) {
+                    const int o = (qm & 1) * 4 * nt + (qm >> 1);
+                    k1 = a / lds[o + nt];
+                    am = lds[o];
+                    cm = lds[o + 2 * nt];
+                    dm = lds[o + 3 * nt];
+                }
+                if (qp < n) {
+                    const int o = (qp & 1) * 4 * nt + (qp >> 1);
+                    k2 = c / lds[o + nt];
+                    ap = lds[o];
+                    cp = lds[o + 2 * nt];
+                    dp = lds[o + 3 * nt];
+                }
+                const double na = -am * k1, nc = -cp * k2;
+                const double nbb = bb - cm * k1 - ap * k2, nd = d - dm * k1 - dp * k2;
+                if (h) {
+                    oa = na;
+                    ob = nbb;
+                    oc = nc;
+                    od = nd;
+                } else {
+                    ea = na;
+                    eb = nbb;
+                    ec = nc;
+                    ed = nd;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 5. back substitution, transpose back, update e -----------------
+    if (tid < nt) {
+        const double xs = ed / eb, xe = od / ob;
+        const bool longc = len == CP;
+        double xnext = xe;
+#pragma unroll
+        for (int k = C - 1; k >= 1; --k)
+            if (k <= len - 2) {
+                const double xk = (cd[k] - ca[k] * xs - cc[k] * xnext) / cb[k];
+                lds[tid * CP + k] = xk;
+                xnext = xk;
+            }
+        lds[tid * CP] = xs;
+        lds[tid * CP + (longc ? C : C - 1)] = xe;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = tid + XL_THREADS * u;
+        if (i < nx) e[ix(P, i, j)] += lds[chunk_pos<C>(i, rem)];
+    }
+}
+
+// Preconditioner residual restricted 1x2 in y, with the coarse correction
+// cleared:  rc(i, J) = (r(i, 2J) + r(i, 2J+1)) / 2,  r = f - L e  (interior
+// stencil from the metrics, ring cells from the table),  ec(i, J) = 0.
+__global__ void __launch_bounds__(256) k_residual_restrict_y(double *rc, double *ec, const double *e,
+                                                             const double *f, const double *K, Grid gf, Grid gc,
+                                                             Geo geo) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= gc.nx || J >= gc.ny) return;
+    const int nring = ring_count(gf.nx, gf.ny);
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int j = 2 * J + b;
+        const bool ring = i == 0 || i == gf.nx - 1 || j == 0 || j == gf.ny - 1;
+        const double lap =
+            ring ? ring_lap(e, K, nring, gf, i, j) : cell_lap_interior(CFld{e, gf.pitch}, gf, geo, i, j);
+        s = s + 0.5 * (f[ix(gf.pitch, i, j)] - lap);
+    }
+    const size_t c = ix(gc.pitch, i, J);
+    rc[c] = s;
+    ec[c] = 0.0;
+}
+
+// Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
+// floor(nx / C) <= 256 chunks.  nx >= C*C then holds, so the remainder is
+// spread one cell per chunk (chunks of C or C+1 cells).
+int xline_chunk(int nx) {
+    for (int C = 2; C <= 16; C *= 2)
+        if (nx / C <= XL_THREADS) return C;
+    return 0;
+}
+
+}  // namespace
+
+int ring_table_elems(const Grid &g) { return RING_PLANES * ring_count(g.nx, g.ny); }
+
+void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, const Bc &bc) {
+    const int n = ring_table_elems(g);
+    k_ring_coef<<<(n + 255) / 256, 256, 0, s>>>(K, g, geo, bc);
+}
+
+size_t xline_lds_bytes(int nx) {
+    const int C = xline_chunk(nx);
+    if (C == 0) return (size_t)-1;
+    const size_t nt = (size_t)(nx / C);
+    const size_t tr = 4 * nt * (C + 1), pcr = 8 * nt;
+    return (tr > pcr ? tr : pcr) * sizeof(double);
+}
+
+void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
+                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo) {
+    const dim3 grid((unsigned)((gc.nx + 63) / 64), (unsigned)((gc.ny + 3) / 4));
+    k_residual_restrict_y<<<grid, 256, 0, s>>>(rc, ec, e, f, K, gf, gc, geo);
+}
+
+bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
+
+void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
+                  int row0, int rowstep, int nrows) {
+    if (nrows <= 0) return;
+    const int C = xline_chunk(g.nx);
+    const int nt = g.nx / C;
+    const size_t lds = xline_lds_bytes(g.nx);
+    switch (C) {
+        case 2: k_xline<2><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
+        case 4: k_xline<4><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
+        case 8: k_xline<8><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
+        case 16: k_xline<16><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
+        default:
+            g_last_error = "k_xline: row length not supported";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+    }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        g_last_error = std::string("k_xline launch: ") + hipGetErrorString(err);
+        throw Failure(MGP_ERR_HIP);
+    }
+}
+
+}  // namespace mgp

@@ -121,3 +121,50 @@ def test_graph_replay_matches_eager():
         np.testing.assert_array_equal(h1, h2)
         np.testing.assert_array_equal(graph.get_phi()[1:-1, 1:-1], ref[1:-1, 1:-1])
         eager.solve(1e-10, 5, zerophi=True)  # eager work between replays
+
+
+def zebra_reference(geo, bc, rhs, phi, sweeps):
+    """x-line zebra relaxation with scipy on the probed operator: per row j,
+    T_j delta = (f - A e)_j with T_j the tridiagonal part of A's (j, j) block,
+    rows in the order even interior, odd interior, then 0 and ny-1 (rows of
+    one colour do not couple, so a sequential pass equals the parallel one)."""
+    from scipy.linalg import solve_banded
+    A = probed_matrix(geo, bc).tocsr()
+    nx, ny = geo.nx, geo.ny
+    e = phi[1:-1, 1:-1].copy().ravel()
+    f = rhs.ravel()
+    order = list(range(2, ny - 1, 2)) + list(range(1, ny - 1, 2)) + [0, ny - 1]
+    blocks = {}
+    for j in order:
+        blk = A[j * nx:(j + 1) * nx, j * nx:(j + 1) * nx].toarray()
+        ab = np.zeros((3, nx))
+        ab[0, 1:] = np.diag(blk, 1)
+        ab[1, :] = np.diag(blk)
+        ab[2, :-1] = np.diag(blk, -1)
+        blocks[j] = ab
+    for _ in range(sweeps):
+        for j in order:
+            r = (f - A @ e)[j * nx:(j + 1) * nx]
+            e[j * nx:(j + 1) * nx] += solve_banded((1, 1), blocks[j], r)
+    return e.reshape(ny, nx)
+
+
+@pytest.mark.parametrize("case", [
+    (16, 12, 0.0, 0.0, (1, 1, 1, 1)),
+    (16, 16, 0.10, 3.0, (1, 1, 0, 0)),
+    (20, 12, 0.12, 2.0, (0, 0, 0, 0)),
+    (12, 16, 0.08, 1.0, (0, 1, 1, 0)),
+    (33, 17, 0.05, 4.0, (1, 0, 0, 1)),
+    (257, 6, 0.10, 3.0, (1, 1, 0, 0)),    # one thread owns both row ends
+    (515, 5, 0.10, 3.0, (0, 1, 1, 0)),    # chunks of 4 and 5 cells
+    (1100, 4, 0.10, 3.0, (1, 1, 0, 0)),   # chunks of 8 and 9 cells
+    (2100, 4, 0.10, 3.0, (1, 0, 1, 0)),   # chunks of 16 and 17 cells
+])
+def test_xline_relax_matches_dense(case):
+    geo, bc, phi, rhs = setup(*case, seed=21)
+    got = mg.xline_relax(geo, bc, rhs, phi, 2)
+    want = zebra_reference(geo, bc, rhs, phi, 2)
+    scale = np.max(np.abs(want))
+    np.testing.assert_allclose(got[1:-1, 1:-1], want, rtol=0, atol=1e-10 * scale)
+    # ghost cells are left as they were
+    np.testing.assert_array_equal(got[0, 1:-1], phi[0, 1:-1])
