@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--n", type=int, default=NX)
     ap.add_argument("--tol", type=float, default=TOL)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sd", type=int, default=2, help="pre-smoothing sweeps per level")
+    ap.add_argument("--su", type=int, default=2, help="post-smoothing sweeps per level")
+    ap.add_argument("--sb", type=int, default=2, help="sweeps on the coarsest level")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--kernels-only", action="store_true",
@@ -103,15 +106,16 @@ def main():
 
     from mgpoisson2d_amd import capi
     geo, bc, rhs = config2_problem(args.n, args.n)
-    solver = capi.Solver(geo, bc, method=capi.METHOD_KRYLOV_MG, smooth_down=2, smooth_up=2,
-                         use_graph=not args.no_graph)
+    solver = capi.Solver(geo, bc, method=capi.METHOD_KRYLOV_MG, smooth_down=args.sd, smooth_up=args.su,
+                         smooth_bottom=args.sb, use_graph=not args.no_graph)
     solver.set_rhs(rhs)
     print(f"# setup done: {args.n}x{args.n}", file=sys.stderr, flush=True)
     if args.kernels_only:
-        for k, name in ((0, "x-line even rows"), (1, "residual"), (2, "copy")):
-            ms, nbytes = solver.time_kernel(k, reps=5)
-            print(f"# {name}: {ms:.4f} ms/launch, {nbytes / 1e6:.1f} MB algorithmic, "
-                  f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+        for cold in (True, False):
+            for k, name in ((0, "x-line even rows"), (1, "residual"), (2, "copy")):
+                ms, nbytes = solver.time_kernel(k, reps=5, cold=cold)
+                print(f"# {'cold' if cold else 'warm'} {name}: {ms:.4f} ms/launch, {nbytes / 1e6:.1f} MB "
+                      f"algorithmic, {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
         solver.close()
         return
 
@@ -138,9 +142,12 @@ def main():
     relres, _, _ = solver.solve(args.tol, 400, zerophi=True)
 
     # dominant kernel: finest-level x-line relaxation (even rows), HIP events
-    k_ms, k_bytes = solver.time_kernel(0, reps=20)
+    # around each launch, caches flushed before it (as inside a solve)
+    k_ms, k_bytes = solver.time_kernel(0, reps=20, cold=True)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
-    r_ms, r_bytes = solver.time_kernel(1, reps=20)
+    kw_ms, _ = solver.time_kernel(0, reps=20, cold=False)
+    r_ms, r_bytes = solver.time_kernel(1, reps=20, cold=True)
+    c_ms, c_bytes = solver.time_kernel(2, reps=20, cold=True)
 
     traffic = None
     if args.n == NX and os.path.exists(args.traffic):
@@ -168,11 +175,13 @@ def main():
             "workload": f"FP64 single solve, {args.n}x{args.n} warped/stretched mapped grid "
                         f"(A=0.1, beta=5), Dirichlet xi / Neumann eta, to {args.tol:g} relative residual",
             "tol": args.tol,
-            "method": "FP64 BiCGStab + V(2,2) y-semicoarsening x-line multigrid preconditioner",
+            "method": f"FP64 BiCGStab + V({args.sd},{args.su}) y-semicoarsening x-line multigrid preconditioner "
+                      f"({args.sb} sweeps on the coarsest level)",
             "v_cycles_per_solve": cycles_per_solve,
             "final_relres": float(relres[-1]),
             "residual_kernel_ms": r_ms,
             "residual_kernel_GBps": r_bytes / (r_ms * 1e-3) / 1e9,
+            "copy_kernel_GBps": c_bytes / (c_ms * 1e-3) / 1e9,
         },
         "roofline": {
             "bound": "hbm",
@@ -183,6 +192,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "avg_launch_ms": k_ms,
+            "timing": "HIP events around each launch after a 512 MiB cache flush",
+            "warm_avg_launch_ms": kw_ms,
             "alg_bytes_per_launch": k_bytes,
         },
         "cpu_baseline": cpu,

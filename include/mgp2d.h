@@ -112,7 +112,8 @@ int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, doub
 int mgp_solve_bicgstab2(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd,
                         double tol, int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
                         int ordering);
-/* MGPoisson2D.f90:400 vcycle (kind 0) / MGPoisson2D_DD.f90:589 DeferredMGPoisson2D::vcycle (kind 1):
+/* MGPoisson2D.f90:400 vcycle (kind 0) / MGPoisson2D_DD.f90:589 DeferredMGPoisson2D::vcycle (kind 1) /
+ * MGPoisson2D.f90:860 fmg (kind 2, same arguments; order-2 prolongation inside, 2x2 schedules only):
  * (phi, rhs, geo, bc, homog, amrrefx, amrrefy, tol, maxiters, maxdepth_user, numcycles,
  *  smooth_down, smooth_up, smooth_bottom, zerophi, verbosity) -- amrref* are unused by the
  * reference and omitted.  relres (maxiters+1 doubles, may be NULL) receives the history the
@@ -130,8 +131,9 @@ typedef struct mgp_solver mgp_solver;
 enum {
     MGP_METHOD_VCYCLE = 0,     /* MGPoisson2D::vcycle on device data */
     MGP_METHOD_DC_VCYCLE = 1,  /* DeferredMGPoisson2D::vcycle on device data */
-    MGP_METHOD_KRYLOV_MG = 2   /* FP64 BiCGStab on the reference operator, preconditioned by one
+    MGP_METHOD_KRYLOV_MG = 2,  /* FP64 BiCGStab on the reference operator, preconditioned by one
                                   semicoarsening (y) x-line-relaxation V-cycle per application */
+    MGP_METHOD_FMG = 3         /* MGPoisson2D::fmg on device data */
 };
 
 typedef struct {
@@ -166,9 +168,12 @@ int mgp_solver_last_cycles(const mgp_solver *s);
 /* Measurement hook (KRYLOV_MG): times `reps` launches of one finest-level
  * kernel with HIP events on the solver's stream.  kernel 0: x-line
  * relaxation of the even interior rows; 1: residual; 2: plain copy of one
- * finest-level field (bandwidth reference).  *avg_ms is the mean
- * duration per launch, *bytes the algorithmic bytes one launch moves. */
-int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, double *avg_ms, double *bytes);
+ * finest-level field (bandwidth reference).  cold != 0: each launch is timed
+ * alone after 512 MiB of writes have flushed the L2s and the MALL (as the
+ * kernels run inside a solve); cold == 0: back-to-back launches on
+ * cache-resident operands.  *avg_ms is the mean duration per launch, *bytes
+ * the algorithmic bytes one launch moves. */
+int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double *avg_ms, double *bytes);
 /* hipDeviceSynchronize on the current device. */
 int mgp_device_synchronize(void);
 

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmgp2d.so
 MGP_OK = 0
 CONVERGED, DIVERGED, MAXITERS = 0, 1, 2
 ORDER_LEX, ORDER_RB, ORDER_MC = 0, 1, 2
-METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG = 0, 1, 2
+METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG, METHOD_FMG = 0, 1, 2, 3
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -229,7 +229,7 @@ def solve_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zero
 
 def vcycle(kind, geo, bc, homog, tol, maxiters, phi, rhs, maxdepth=-1, numcycles=1, smooth_down=2,
            smooth_up=2, smooth_bottom=2, zerophi=False, ordering=ORDER_LEX):
-    """Reference vcycle (kind 0) / DC vcycle (kind 1) on the GPU.
+    """Reference vcycle (kind 0) / DC vcycle (kind 1) / fmg (kind 2) on the GPU.
     Returns (phi, rhs, relres history, iterations, outcome)."""
     a = _Args()
     phi = _f64(phi).copy()
@@ -300,10 +300,11 @@ class Solver:
     def stream(self):
         return lib.mgp_solver_stream(self._h)
 
-    def time_kernel(self, kernel=0, reps=20):
-        """Mean ms per launch and algorithmic bytes per launch of a finest-level kernel."""
+    def time_kernel(self, kernel=0, reps=20, cold=True):
+        """Mean ms per launch and algorithmic bytes per launch of a finest-level
+        kernel; cold: each launch after a cache flush (as inside a solve)."""
         ms, nbytes = _d(0.0), _d(0.0)
-        _check(lib.mgp_solver_time_kernel(self._h, _i(kernel), _i(reps), ctypes.byref(ms), ctypes.byref(nbytes)))
+        _check(lib.mgp_solver_time_kernel(self._h, _i(kernel), _i(reps), _i(int(cold)), ctypes.byref(ms), ctypes.byref(nbytes)))
         return ms.value, nbytes.value
 
 

@@ -462,9 +462,10 @@ int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, co
         if (opts) s->opts = *opts;
         else mgp_solver_default_opts(&s->opts);
         const mgp_solver_opts &o = s->opts;
-        if (o.method == MGP_METHOD_VCYCLE) no_lah(bc);
-        if (o.method == MGP_METHOD_VCYCLE && o.ordering != MGP_ORDER_LEX && o.ordering != MGP_ORDER_MC) {
-            g_last_error = "VCYCLE ordering must be LEX or MC";
+        const bool plain = o.method == MGP_METHOD_VCYCLE || o.method == MGP_METHOD_FMG;
+        if (plain) no_lah(bc);
+        if (plain && o.ordering != MGP_ORDER_LEX && o.ordering != MGP_ORDER_MC) {
+            g_last_error = "VCYCLE / FMG ordering must be LEX or MC";
             throw Failure(MGP_ERR_ARG);
         }
         if (o.method == MGP_METHOD_DC_VCYCLE && o.ordering != MGP_ORDER_LEX && o.ordering != MGP_ORDER_RB) {
@@ -477,9 +478,10 @@ int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, co
         const size_t n = level_elems(geo->nx, geo->ny);
         s->phi.alloc(n);
         s->rhs.alloc(n);
-        if (o.method == MGP_METHOD_VCYCLE || o.method == MGP_METHOD_DC_VCYCLE)
+        if (plain || o.method == MGP_METHOD_DC_VCYCLE) {
             s->ref.build(s->stream, s->geo, s->ubc.bc, o.method == MGP_METHOD_DC_VCYCLE, o.ordering, o.maxdepth);
-        else if (o.method == MGP_METHOD_KRYLOV_MG)
+            s->ref.fmg = o.method == MGP_METHOD_FMG;
+        } else if (o.method == MGP_METHOD_KRYLOV_MG)
             s->kmg.build(s->stream, s->geo, s->ubc.bc, o);
         else {
             g_last_error = "unknown method";
@@ -568,13 +570,13 @@ void *mgp_solver_stream(mgp_solver *s) { return s ? (void *)s->stream : nullptr;
 
 int mgp_solver_last_cycles(const mgp_solver *s) { return s ? s->last_cycles : 0; }
 
-int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, double *avg_ms, double *bytes) {
+int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double *avg_ms, double *bytes) {
     return guarded([&] {
         if (s->opts.method != MGP_METHOD_KRYLOV_MG || reps <= 0 || kernel < 0 || kernel > 2) {
             g_last_error = "time_kernel: KRYLOV_MG solvers, kernel 0, 1 or 2, reps > 0";
             throw Failure(MGP_ERR_ARG);
         }
-        *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, bytes);
+        *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, cold, bytes);
     });
 }
 
@@ -587,7 +589,11 @@ int mgp_vcycle(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, double
                double *rhs, int ordering, double *relres, int *iters, int *outcome) {
     mgp_solver_opts o;
     mgp_solver_default_opts(&o);
-    o.method = kind ? MGP_METHOD_DC_VCYCLE : MGP_METHOD_VCYCLE;
+    if (kind < 0 || kind > 2) {
+        g_last_error = "vcycle: kind must be 0 (MGPoisson2D::vcycle), 1 (DeferredMGPoisson2D::vcycle) or 2 (fmg)";
+        return MGP_ERR_ARG;
+    }
+    o.method = kind == 0 ? MGP_METHOD_VCYCLE : (kind == 1 ? MGP_METHOD_DC_VCYCLE : MGP_METHOD_FMG);
     o.ordering = ordering;
     o.numcycles = numcycles;
     o.smooth_down = smooth_down;

@@ -89,6 +89,7 @@ void prolong_level(Workspace &W, hipStream_t s, HLevel &f, HLevel &c, int order,
 // The reference's level hierarchy (isotropic schedule) and cycle drivers.
 struct RefHierarchy {
     int dc = 0, ordering = 0, maxdepth = 0;
+    int fmg = 0;  // outer loop of MGPoisson2D::fmg (MGPoisson2D.f90:860) instead of vcycle
     std::vector<int> refx, refy;
     std::vector<HLevel> levels;
     DevBuf bt_pt, bt_st, bt_r, bt_rt, bt_p, bt_t, bt_v, bt_e, bt_mod, bt_scal;
@@ -97,6 +98,7 @@ struct RefHierarchy {
     void relax(hipStream_t s, int d, double *phi, const double *rhs, int sweeps, double omega);
     void bottom(hipStream_t s, int d);
     void vcycle_noinit(Workspace &W, hipStream_t s, int d, int numcycles, int sd, int su, int sb);
+    void fmg_noinit(Workspace &W, hipStream_t s, int d, int numcycles, int sd, int su, int sb);
     int solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
               double *rhs, double tol, int maxiters, int numcycles, int sd, int su, int sb, int zerophi,
               double *relres, int *iters_out);

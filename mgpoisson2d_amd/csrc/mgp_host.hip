@@ -391,6 +391,26 @@ void RefHierarchy::vcycle_noinit(Workspace &W, hipStream_t s, int d, int numcycl
     relax(s, d, l.e.p, l.r.p, su, omega);
 }
 
+// MGPoisson2D::fmg_noinit (MGPoisson2D.f90:1127-1189): e(d) = 0; restrict r(d)
+// to r(d+1), solve there recursively and prolong with order 2; then numcycles
+// V-cycles at depth d from that initial guess.
+void RefHierarchy::fmg_noinit(Workspace &W, hipStream_t s, int d, int numcycles, int sd, int su, int sb) {
+    HLevel &l = levels[d];
+    launch_setval(s, l.e.p, 0.0, l.g, 1);
+    if (d < maxdepth) {
+        HLevel &c = levels[d + 1];
+        launch_restrict_cc(s, c.r.p, l.r.p, c.g, l.g.pitch, refx[d], refy[d], nullptr, nullptr);
+        fmg_noinit(W, s, d + 1, numcycles, sd, su, sb);
+        if (refx[d] != 2 || refy[d] != 2) {
+            g_last_error = "prolong: only 2x2 coarsening is defined by the reference (MGPoisson2D.f90:385-388)";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        // the coarse work array is free here and sized for the order-3 ghost extrapolation
+        prolong_level(W, s, l, c, 2, &c.w);
+    }
+    for (int k = 0; k < numcycles; ++k) vcycle_noinit(W, s, d, numcycles, sd, su, sb);
+}
+
 int RefHierarchy::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog,
                         double *phi, double *rhs, double tol, int maxiters, int numcycles, int sd, int su,
                         int sb, int zerophi, double *relres, int *iters_out) {
@@ -412,8 +432,12 @@ int RefHierarchy::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc
     }
     int outcome = MGP_MAXITERS, it;
     for (it = 1; it <= maxiters; ++it) {
-        if (!dc) launch_einit(s, l0.e.p, l0.r.p, l0.invd.p, l0.g);
-        vcycle_noinit(W, s, 0, numcycles, sd, su, sb);
+        if (fmg) {
+            fmg_noinit(W, s, 0, numcycles, sd, su, sb);  // MGPoisson2D.f90:1083-1088
+        } else {
+            if (!dc) launch_einit(s, l0.e.p, l0.r.p, l0.invd.p, l0.g);
+            vcycle_noinit(W, s, 0, numcycles, sd, su, sb);
+        }
         launch_axpy_ghosted(s, phi, l0.e.p, 1.0, g);
         launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
         launch_laplacian(s, l0.r.p, phi, rhs, g, geo, user_bc, homog, 0);

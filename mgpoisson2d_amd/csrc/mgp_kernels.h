@@ -86,10 +86,12 @@ void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const dou
                                 const double *K, const Grid &gf, const Grid &gc, const Geo &geo);
 bool xline_supported(int nx);
 size_t xline_lds_bytes(int nx);
-// Relax rows j = row0 + k*rowstep, k < nrows (one workgroup each; the rows of
-// one launch must not be within two rows of each other).
+// Relax rows j = row0 + k*rowstep, k < nrows (one workgroup each; no row of a
+// launch may read another: interior rows at least two apart, boundary rows
+// three).  The first workgroup then also relaxes row extra0, the last one row
+// extra1 (< 0: none), each after its own row.
 void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
-                  int row0, int rowstep, int nrows);
+                  int row0, int rowstep, int nrows, int extra0, int extra1);
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
                       double shi);
 void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, double b, const double *y, double c,

@@ -25,7 +25,8 @@ struct KrylovMG {
     // kernel 0: x-line relaxation of the even interior rows; 1: residual
     // (9-point operator); 2: plain copy of one field.  Returns the mean ms per launch; *bytes receives the
     // algorithmic bytes one launch moves (each array element it needs, once).
-    double time_kernel(hipStream_t s, int kernel, int reps, double *bytes);
+    // cold != 0: each launch timed alone after the caches have been flushed.
+    double time_kernel(hipStream_t s, int kernel, int reps, int cold, double *bytes);
 };
 
 // One x-line zebra sweep of the homogeneous operator on one level: even

@@ -37,7 +37,7 @@ namespace mgp {
 namespace {
 
 constexpr int XL_THREADS = 256;
-constexpr int RING_PLANES = 15;
+constexpr int RING_PLANES = 9;
 
 // Impulse whose ghosts follow the homogeneous fill of the MG levels
 // (Dirichlet order 3 / LAH negate; Neumann sides contribute no ghost).
@@ -63,85 +63,89 @@ struct ImpulseFilled {
     }
 };
 
+// Device-memory pointers typed as global: the compiler then knows they
+// cannot alias LDS and may issue loads ahead of LDS stores (with generic
+// pointers every LDS store would wait for the loads before it).
+typedef const __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble *gptr(const double *p) { return (gdouble *)p; }
+
 __host__ __device__ inline int ring_count(int nx, int ny) { return 2 * nx + 2 * (ny - 2); }
 
-// Ring cell r -> (i, j) and the offset of table plane q.
-//  rows    (r < 2 nx):  j = 0 (r < nx) or ny-1;  plane q = 5 k + (di + 2),
-//                       offset (di, k s), s = +1 on row 0, -1 on row ny-1
-//  columns (r >= 2 nx): i = 0 or nx-1, j = 1 .. ny-2;  plane q = 3 k + (dj + 1),
-//                       q < 9, offset (k s, dj), s = +1 on column 0, -1 on nx-1
-__device__ inline void ring_plane(int nx, int ny, int r, int q, int &i, int &j, int &di, int &dj, bool &used) {
+// Every ring-cell stencil lies in a 3x3 window of valid cells: the window
+// reaches inward (two cells) across each boundary side the cell touches and
+// one cell either way along it.  Window origin (w0, v0) of cell (i, j):
+//   w0 = 0 on column 0, -2 on column nx-1, else -1   (likewise v0 for rows).
+// Ring cell index r: rows j = 0 (r = i) and ny-1 (r = nx + i), then columns
+// i = 0 (r = 2 nx + j - 1) and nx-1 (r = 2 nx + ny - 2 + j - 1), 1 <= j <= ny-2.
+// Table plane q = 3 a + b holds the coefficient of e(i + w0 + b, j + v0 + a).
+__device__ __forceinline__ int window_origin(int i, int n) { return i == 0 ? 0 : (i == n - 1 ? -2 : -1); }
+
+__device__ inline void ring_cell(int nx, int ny, int r, int &i, int &j) {
     if (r < 2 * nx) {
         const bool top = r >= nx;
         i = top ? r - nx : r;
         j = top ? ny - 1 : 0;
-        di = q % 5 - 2;
-        dj = (q / 5) * (top ? -1 : 1);
-        used = true;
     } else {
         const int rr = r - 2 * nx;
         const bool right = rr >= ny - 2;
         i = right ? nx - 1 : 0;
         j = (right ? rr - (ny - 2) : rr) + 1;
-        di = (q / 3) * (right ? -1 : 1);
-        dj = q % 3 - 1;
-        used = q < 9;
     }
 }
 
-// (L e)(i, j) at a ring cell from the table (valid neighbours only).
-__device__ __forceinline__ double ring_lap(const double *e, const double *K, int nring, const Grid &g, int i,
-                                           int j) {
-    const int nx = g.nx, ny = g.ny, P = g.pitch;
-    double lap = 0.0;
-    if (j == 0 || j == ny - 1) {
-        const int s = j == 0 ? 1 : -1;
-        const int r = (j == 0 ? 0 : nx) + i;
+__device__ __forceinline__ int ring_index(int nx, int ny, int i, int j) {
+    if (j == 0) return i;
+    if (j == ny - 1) return nx + i;
+    return 2 * nx + (i == 0 ? 0 : ny - 2) + (j - 1);
+}
+
+// (L e)(i, j) at a ring cell from the table.  All nine table entries and
+// field values are loaded before any is used (one memory round trip).
+__device__ __forceinline__ double ring_lap(gdouble *e, gdouble *K, int nring, const Grid &g, int i, int j) {
+    const int w0 = window_origin(i, g.nx), v0 = window_origin(j, g.ny);
+    const int r = ring_index(g.nx, g.ny, i, j);
+    double kv[9], ev[9];
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int di = -2; di <= 2; ++di) {
-                const int ci = i + di;
-                const double ev = (ci >= 0 && ci < nx) ? e[ix(P, ci, j + k * s)] : 0.0;
-                lap += K[(size_t)(5 * k + di + 2) * nring + r] * ev;
-            }
-    } else {
-        const bool right = i == nx - 1;
-        const int s = right ? -1 : 1;
-        const int r = 2 * nx + (right ? ny - 2 : 0) + (j - 1);
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int dj = -1; dj <= 1; ++dj)
-                lap += K[(size_t)(3 * k + dj + 1) * nring + r] * e[ix(P, i + k * s, j + dj)];
+    for (int q = 0; q < 9; ++q) {
+        kv[q] = K[(size_t)q * nring + r];
+        ev[q] = e[ix(g.pitch, i + w0 + q % 3, j + v0 + q / 3)];
     }
+    double lap = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) lap += kv[q] * ev[q];
     return lap;
 }
 
-// K[q * nring + r]: coefficient of e(i+di, j+dj) in the homogeneous L at ring
-// cell r (zero where the neighbour is not a valid cell).
+// Coefficients of e(i-1, j), e(i, j), e(i+1, j) at a ring cell (0 where the
+// neighbour is outside the window, i.e. not a valid cell).
+__device__ __forceinline__ void ring_row_coef(gdouble *K, int nring, const Grid &g, int i, int j, double &a,
+                                              double &b, double &c) {
+    const int w0 = window_origin(i, g.nx), v0 = window_origin(j, g.ny);
+    const int r = ring_index(g.nx, g.ny, i, j);
+    const int base = 3 * (-v0);  // window row of j itself
+    a = w0 <= -1 ? K[(size_t)(base + (-1 - w0)) * nring + r] : 0.0;
+    b = K[(size_t)(base + (0 - w0)) * nring + r];
+    c = w0 + 2 >= 1 ? K[(size_t)(base + (1 - w0)) * nring + r] : 0.0;
+}
+
+// K[q * nring + r]: coefficient of e(i + w0 + q%3, j + v0 + q/3) in the
+// homogeneous L at ring cell r (impulse response with the ghost rule).
 __global__ void __launch_bounds__(256) k_ring_coef(double *K, Grid g, Geo geo, Bc bc) {
     const int nring = ring_count(g.nx, g.ny);
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= RING_PLANES * nring) return;
     const int q = id / nring, r = id % nring;
-    int i, j, di, dj;
-    bool used;
-    ring_plane(g.nx, g.ny, r, q, i, j, di, dj, used);
-    const int ci = i + di, cj = j + dj;
-    double v = 0.0;
-    if (used && ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny) {
-        ImpulseFilled imp;
-        imp.g = g;
-        for (int k = 0; k < 4; ++k) {
-            imp.t[k] = bc.type[k];
-            imp.m[k] = bc.mode[k];
-        }
-        imp.ci = ci;
-        imp.cj = cj;
-        v = cell_lap(imp, g, geo, bc, 1, i, j);
+    int i, j;
+    ring_cell(g.nx, g.ny, r, i, j);
+    ImpulseFilled imp;
+    imp.g = g;
+    for (int k = 0; k < 4; ++k) {
+        imp.t[k] = bc.type[k];
+        imp.m[k] = bc.mode[k];
     }
-    K[id] = v;
+    imp.ci = i + window_origin(i, g.nx) + q % 3;
+    imp.cj = j + window_origin(j, g.ny) + q / 3;
+    K[id] = cell_lap(imp, g, geo, bc, 1, i, j);
 }
 
 // Position of cell i in the chunk layout: the first rem chunks hold C+1
@@ -154,89 +158,129 @@ __device__ __forceinline__ int chunk_pos(int i, int rem) {
     return big + (k / C) * (C + 1) + k % C;
 }
 
-template <int C>
-__global__ void __launch_bounds__(XL_THREADS) k_xline(double *__restrict__ e, const double *__restrict__ f,
-                                                      const double *__restrict__ K, Grid g, Geo geo, int row0,
-                                                      int rowstep, int nt) {
-    constexpr int CP = C + 1;  // LDS stride of a chunk, and the longest chunk
-    constexpr int U = C + 1;   // cells per thread in the coalesced passes
-    extern __shared__ double lds[];
-    const int nx = g.nx, ny = g.ny, P = g.pitch, tid = threadIdx.x;
-    const int rem = nx - nt * C;
-    const int nring = ring_count(nx, ny);
-    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, fourth = 0.25;
-    const double idx2 = idx * idx, idy2 = idy * idy, ixy4 = fourth * idx * idy;
-
-    // XCD-aware row of this workgroup: XCD x (= blockIdx % 8) takes a
-    // contiguous block of q8 or q8+1 rows.
+// Row j of the workgroup's launch.  XCD-aware: XCD x (= blockIdx % 8) takes a
+// contiguous block of q8 or q8+1 slots.
+__device__ __forceinline__ int launch_slot() {
     const int nb = gridDim.x, b = blockIdx.x, x = b & 7;
     const int q8 = nb >> 3, r8 = nb & 7;
-    const int slot = x * q8 + (x < r8 ? x : r8) + (b >> 3);
-    const int j = row0 + slot * rowstep;
-    const bool ring_row = j == 0 || j == ny - 1;
+    return x * q8 + (x < r8 ? x : r8) + (b >> 3);
+}
 
-    // ---- 1. coalesced residual and row coefficients, stored straight into
-    //         the chunk layout: LDS = [d | b | a | c], nt*(C+1) doubles each
-    const int NT1 = nt * CP;
-    double *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
-    if (!ring_row) {
+struct RowCtx {
+    int nx, ny, P, nt, rem, nring;
+    double idx, idy, idx2, idy2, ixy4;
+};
+
+__device__ __forceinline__ RowCtx row_ctx(const Grid &g, int nt, int C) {
+    RowCtx r;
+    r.nx = g.nx;
+    r.ny = g.ny;
+    r.P = g.pitch;
+    r.nt = nt;
+    r.rem = g.nx - nt * C;
+    r.nring = ring_count(g.nx, g.ny);
+    r.idx = 1.0 / g.dx;
+    r.idy = 1.0 / g.dy;
+    r.idx2 = r.idx * r.idx;
+    r.idy2 = r.idy * r.idy;
+    r.ixy4 = 0.25 * r.idx * r.idy;
+    return r;
+}
+
+// Coalesced pass over row j: for cells i = tid + 256 u, the residual
+// d = f - L e and the row coefficients a, b, c, written into the chunk layout
+// (Ld, La, Lb, Lc).  Interior cells use the metric stencil, ring cells the
+// table.  Lanes past the row end evaluate the last cell and store into the
+// chunk layout's last slot, which no chunk uses (the last chunk has C cells):
+// the loop stays branch-free.
+template <int C>
+__device__ __forceinline__ void row_pass(const RowCtx &X, const double *e_, const double *f_, const double *K_,
+                                         const Grid &g, const Geo &geo_, int j, double *Ld, double *La, double *Lb,
+                                         double *Lc) {
+    constexpr int U = C + 1;
+    gdouble *e = gptr(e_), *f = gptr(f_), *K = gptr(K_);
+    gdouble *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
+    const int tid = threadIdx.x, nx = X.nx, ny = X.ny, P = X.P;
+    const double fourth = 0.25;
+    const int dump = X.nt * (C + 1) - 1;
+    if (j != 0 && j != ny - 1) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int i = tid + XL_THREADS * u;
-            if (i >= nx) break;
+            const int iraw = tid + XL_THREADS * u;
+            const int i = iraw < nx ? iraw : nx - 1;
             const size_t c = ix(P, i, j);
-            const double *q = e + c;
             // interior stencil; for i = 0, nx-1 the reads stay inside the
             // padded level and the values are replaced below
+            gdouble *q = e + c;
             const double p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
             const double ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
-            const double Jxx_e = geo.Jxx[c + 1], Jxx_w = geo.Jxx[c];
-            const double Jxy_e = geo.Jxy[c + 1], Jxy_w = geo.Jxy[c];
-            const double Jyy_n = geo.Jyy[c + P], Jyy_s = geo.Jyy[c];
-            const double Jyx_n = geo.Jyx[c + P], Jyx_s = geo.Jyx[c];
-            const double xfe = Jxx_e * (pe - p) * idx + Jxy_e * fourth * (ene - ese + pn - ps) * idy;
-            const double xfw = Jxx_w * (p - pw) * idx + Jxy_w * fourth * (pn - ps + enw - esw) * idy;
-            const double yfn = Jyy_n * (pn - p) * idy + Jyx_n * fourth * (ene - enw + pe - pw) * idx;
-            const double yfs = Jyy_s * (p - ps) * idy + Jyx_s * fourth * (pe - pw + ese - esw) * idx;
-            const int o = chunk_pos<C>(i, rem);
-            Ld[o] = f[c] - ((xfe - xfw) * idx + (yfn - yfs) * idy);
-            La[o] = Jxx_w * idx2 - (Jyx_n - Jyx_s) * ixy4;
-            Lb[o] = -(Jxx_e + Jxx_w) * idx2 - (Jyy_n + Jyy_s) * idy2;
-            Lc[o] = Jxx_e * idx2 + (Jyx_n - Jyx_s) * ixy4;
+            const double Jxx_e = Jxx[c + 1], Jxx_w = Jxx[c], Jxy_e = Jxy[c + 1], Jxy_w = Jxy[c];
+            const double Jyy_n = Jyy[c + P], Jyy_s = Jyy[c], Jyx_n = Jyx[c + P], Jyx_s = Jyx[c];
+            const double xfe = Jxx_e * (pe - p) * X.idx + Jxy_e * fourth * (ene - ese + pn - ps) * X.idy;
+            const double xfw = Jxx_w * (p - pw) * X.idx + Jxy_w * fourth * (pn - ps + enw - esw) * X.idy;
+            const double yfn = Jyy_n * (pn - p) * X.idy + Jyx_n * fourth * (ene - enw + pe - pw) * X.idx;
+            const double yfs = Jyy_s * (p - ps) * X.idy + Jyx_s * fourth * (pe - pw + ese - esw) * X.idx;
+            const int o = iraw < nx ? chunk_pos<C>(i, X.rem) : dump;
+            Ld[o] = f[c] - ((xfe - xfw) * X.idx + (yfn - yfs) * X.idy);
+            La[o] = Jxx_w * X.idx2 - (Jyx_n - Jyx_s) * X.ixy4;
+            Lb[o] = -(Jxx_e + Jxx_w) * X.idx2 - (Jyy_n + Jyy_s) * X.idy2;
+            Lc[o] = Jxx_e * X.idx2 + (Jyx_n - Jyx_s) * X.ixy4;
         }
         // the two ring cells of the row (i = 0, nx-1): explicit stencil table
         for (int side = 0; side < 2; ++side) {
             const int iend = side ? nx - 1 : 0;
             if (iend % XL_THREADS != tid) continue;
-            const int r = 2 * nx + (side ? ny - 2 : 0) + (j - 1);
-            // coefficient of e(i-1) / e(i+1): table offset (s, 0) = plane 4
-            const double kin = K[(size_t)4 * nring + r];
-            const int o = chunk_pos<C>(iend, rem);
-            Ld[o] = f[ix(P, iend, j)] - ring_lap(e, K, nring, g, iend, j);
-            La[o] = side ? kin : 0.0;
-            Lb[o] = K[(size_t)1 * nring + r];
-            Lc[o] = side ? 0.0 : kin;
+            const int o = chunk_pos<C>(iend, X.rem);
+            Ld[o] = f[ix(P, iend, j)] - ring_lap(e, K, X.nring, g, iend, j);
+            ring_row_coef(K, X.nring, g, iend, j, La[o], Lb[o], Lc[o]);
         }
     } else {
         // boundary row: every cell from the table
-        const int rbase = j == 0 ? 0 : nx;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int i = tid + XL_THREADS * u;
-            if (i >= nx) break;
-            const int r = rbase + i;
-            const int o = chunk_pos<C>(i, rem);
-            Ld[o] = f[ix(P, i, j)] - ring_lap(e, K, nring, g, i, j);
-            La[o] = K[(size_t)1 * nring + r];
-            Lb[o] = K[(size_t)2 * nring + r];
-            Lc[o] = K[(size_t)3 * nring + r];
+            const int iraw = tid + XL_THREADS * u;
+            const int i = iraw < nx ? iraw : nx - 1;
+            const double fv = f[ix(P, i, j)];
+            const double lap = ring_lap(e, K, X.nring, g, i, j);
+            double a, b, c;
+            ring_row_coef(K, X.nring, g, i, j, a, b, c);
+            const int o = iraw < nx ? chunk_pos<C>(i, X.rem) : dump;
+            Ld[o] = fv - lap;
+            La[o] = a;
+            Lb[o] = b;
+            Lc[o] = c;
         }
     }
+}
+
+// Coalesced update e(:, j) += x, x in the chunk layout (e re-read).
+template <int C>
+__device__ __forceinline__ void row_update_add(const RowCtx &X, double *e, int j, const double *Lx) {
+#pragma unroll
+    for (int u = 0; u < C + 1; ++u) {
+        const int i = threadIdx.x + XL_THREADS * u;
+        if (i < X.nx) e[ix(X.P, i, j)] += Lx[chunk_pos<C>(i, X.rem)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// On-the-fly solve of one row (fine levels, bandwidth-bound): coefficients
+// from the metrics, elimination, PCR and back substitution per call.
+// LDS: 4 nt (C+1) doubles.
+// ---------------------------------------------------------------------------
+template <int C>
+__device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double *__restrict__ f,
+                          const double *__restrict__ K, const Grid &g, const Geo &geo, int j, double *lds) {
+    constexpr int CP = C + 1;
+    const int tid = threadIdx.x, nt = X.nt;
+    const int NT1 = nt * CP;
+    double *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
+    row_pass<C>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
     __syncthreads();
 
-    // ---- 2. each thread takes its chunk into registers ------------------
+    // each thread takes its chunk into registers
     double ca[CP], cb[CP], cc[CP], cd[CP];
-    const int len = tid < nt ? C + (tid < rem ? 1 : 0) : 0;
+    const int len = tid < nt ? C + (tid < X.rem ? 1 : 0) : 0;
 #pragma unroll
     for (int k = 0; k < CP; ++k)
         if (k < len) {
@@ -247,7 +291,7 @@ __global__ void __launch_bounds__(XL_THREADS) k_xline(double *__restrict__ e, co
         }
     __syncthreads();
 
-    // ---- 3. partition elimination in registers -------------------------
+    // partition elimination in registers
     // start row:  a_0 x_{prev end} + bu x_s + cu x_e = du
     // end row:    a'_L x_s + b'_L x_e + c_L x_{next start} = d'_L
     double ea = 0.0, eb = 1.0, ec = 0.0, ed = 0.0, oa = 0.0, ob = 1.0, oc = 0.0, od = 0.0;
@@ -282,9 +326,9 @@ __global__ void __launch_bounds__(XL_THREADS) k_xline(double *__restrict__ e, co
         od = longc ? cd[C] : cd[C - 1];
     }
 
-    // ---- 4. PCR on the 2 nt reduced unknowns (even: chunk starts) --------
-    // LDS: [Ea Eb Ec Ed Oa Ob Oc Od], nt doubles each; entry q lives in
-    // half (q & 1) at index q >> 1.
+    // PCR on the 2 nt reduced unknowns (even: chunk starts).  LDS:
+    // [Ea Eb Ec Ed Oa Ob Oc Od], nt doubles each; entry q lives in half
+    // (q & 1) at index q >> 1.
     const int n = 2 * nt;
     for (int st = 1; st < n; st *= 2) {
         if (tid < nt) {
@@ -337,7 +381,7 @@ __global__ void __launch_bounds__(XL_THREADS) k_xline(double *__restrict__ e, co
         __syncthreads();
     }
 
-    // ---- 5. back substitution, transpose back, update e -----------------
+    // back substitution, then the coalesced update
     if (tid < nt) {
         const double xs = ed / eb, xe = od / ob;
         const bool longc = len == CP;
@@ -353,11 +397,49 @@ __global__ void __launch_bounds__(XL_THREADS) k_xline(double *__restrict__ e, co
         lds[tid * CP + (longc ? C : C - 1)] = xe;
     }
     __syncthreads();
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int i = tid + XL_THREADS * u;
-        if (i < nx) e[ix(P, i, j)] += lds[chunk_pos<C>(i, rem)];
+    row_update_add<C>(X, e, j, lds);
+}
+
+// Rows j = row0 + slot*rowstep; the workgroup of slot 0 then also relaxes
+// row extra0 and the last slot row extra1 (< 0: none), each after its own
+// row.  The boundary rows ride on the odd-row launch this way: row 0 after
+// row 1 and row ny-1 after the last odd row, the order of a separate third
+// phase, without its launch.
+// Rows j = row0 + slot*rowstep; the workgroup of slot 0 then also relaxes row
+// extra0 and the last slot row extra1 (< 0: none), each after its own row.
+template <int C>
+__device__ __forceinline__ void xline_rows(double *__restrict__ e, const double *__restrict__ f,
+                                           const double *__restrict__ K, const Grid &g, const Geo &geo, int row0,
+                                           int rowstep, int nt, int extra0, int extra1) {
+    extern __shared__ double lds[];
+    const RowCtx X = row_ctx(g, nt, C);
+    const int slot = launch_slot();
+    const int ex0 = slot == 0 ? extra0 : -1, ex1 = slot == (int)gridDim.x - 1 ? extra1 : -1;
+#pragma unroll 1
+    for (int r = 0; r < 3; ++r) {
+        const int jr = r == 0 ? row0 + slot * rowstep : (r == 1 ? ex0 : ex1);
+        if (jr < 0) continue;  // uniform over the workgroup
+        if (r) __syncthreads();
+        xline_row<C>(X, e, f, K, g, geo, jr, lds);
     }
+}
+
+// Rows up to 2055 cells: two waves per SIMD (two workgroups per CU, as the
+// 73.7 KB of LDS allow): the fine levels are bandwidth-bound and need the
+// second workgroup to cover the elimination and PCR phases of the first.
+template <int C>
+__global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_xline(double *__restrict__ e, const double *__restrict__ f, const double *__restrict__ K, Grid g, Geo geo,
+        int row0, int rowstep, int nt, int extra0, int extra1) {
+    xline_rows<C>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+}
+
+// Longer rows (C = 16): one workgroup per CU (139 KB of LDS), all registers.
+template <int C>
+__global__ void __launch_bounds__(XL_THREADS) k_xline_long(double *__restrict__ e, const double *__restrict__ f,
+                                                           const double *__restrict__ K, Grid g, Geo geo, int row0,
+                                                           int rowstep, int nt, int extra0, int extra1) {
+    xline_rows<C>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
 }
 
 // Preconditioner residual restricted 1x2 in y, with the coarse correction
@@ -376,7 +458,7 @@ __global__ void __launch_bounds__(256) k_residual_restrict_y(double *rc, double 
         const int j = 2 * J + b;
         const bool ring = i == 0 || i == gf.nx - 1 || j == 0 || j == gf.ny - 1;
         const double lap =
-            ring ? ring_lap(e, K, nring, gf, i, j) : cell_lap_interior(CFld{e, gf.pitch}, gf, geo, i, j);
+            ring ? ring_lap(gptr(e), gptr(K), nring, gf, i, j) : cell_lap_interior(CFld{e, gf.pitch}, gf, geo, i, j);
         s = s + 0.5 * (f[ix(gf.pitch, i, j)] - lap);
     }
     const size_t c = ix(gc.pitch, i, J);
@@ -410,28 +492,29 @@ size_t xline_lds_bytes(int nx) {
     return (tr > pcr ? tr : pcr) * sizeof(double);
 }
 
+bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
+
 void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
                                 const double *K, const Grid &gf, const Grid &gc, const Geo &geo) {
     const dim3 grid((unsigned)((gc.nx + 63) / 64), (unsigned)((gc.ny + 3) / 4));
     k_residual_restrict_y<<<grid, 256, 0, s>>>(rc, ec, e, f, K, gf, gc, geo);
 }
 
-bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
-
 void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
-                  int row0, int rowstep, int nrows) {
+                  int row0, int rowstep, int nrows, int extra0, int extra1) {
     if (nrows <= 0) return;
     const int C = xline_chunk(g.nx);
+    if (C == 0) {
+        g_last_error = "k_xline: row length not supported";
+        throw Failure(MGP_ERR_UNSUPPORTED);
+    }
     const int nt = g.nx / C;
     const size_t lds = xline_lds_bytes(g.nx);
     switch (C) {
-        case 2: k_xline<2><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
-        case 4: k_xline<4><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
-        case 8: k_xline<8><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
-        case 16: k_xline<16><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt); break;
-        default:
-            g_last_error = "k_xline: row length not supported";
-            throw Failure(MGP_ERR_UNSUPPORTED);
+        case 2: k_xline<2><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
+        case 4: k_xline<4><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
+        case 8: k_xline<8><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
+        default: k_xline_long<16><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {

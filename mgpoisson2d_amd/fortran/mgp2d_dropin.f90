@@ -930,7 +930,8 @@ contains
         crse%data(crse%valid%ilo-1:crse%valid%ihi+1, crse%valid%jlo-1:crse%valid%jhi+1) = c
     end subroutine prolong
 
-    ! Shared body of MGPoisson2D::vcycle (kind 0) and DeferredMGPoisson2D::vcycle (kind 1).
+    ! Shared body of MGPoisson2D::vcycle (kind 0), DeferredMGPoisson2D::vcycle (kind 1)
+    ! and MGPoisson2D::fmg (kind 2).
     subroutine dropin_vcycle (kind, phi, rhs, geo, bc, homog, tol, maxiters, maxdepth_user, numcycles, &
                               smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ordering)
         integer, intent(in)           :: kind
@@ -960,7 +961,11 @@ contains
         rhs%data(rhs%valid%ilo:rhs%valid%ihi, rhs%valid%jlo:rhs%valid%jhi) = f
         if (verbosity .ge. 1) then
             do k = 0, iters
-                print*, 'V-Cycle iter ', k, ': rel |res| = ', relres(k)
+                if (kind .eq. 2) then
+                    print*, 'FMG-Cycle iter ', k, ': rel |res| = ', relres(k)
+                else
+                    print*, 'V-Cycle iter ', k, ': rel |res| = ', relres(k)
+                endif
             enddo
             if (outcome .eq. 0) print*, 'Converged.'
             if (outcome .eq. 1) print*, 'Diverging.'
@@ -989,6 +994,30 @@ contains
         call dropin_vcycle (0, phi, rhs, geo, bc, homog, tol, maxiters, maxdepth_user, numcycles, &
                             smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ord)
     end subroutine vcycle
+
+    ! MGPoisson2D.f90:860 -- full multigrid; rhs is not rescaled (the reference's
+    ! J scaling is commented out there)
+    subroutine fmg (phi, rhs, geo, bc, homog, amrrefx, amrrefy, tol, maxiters, maxdepth_user, numcycles, &
+                    smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ordering)
+        type(box_data), intent(inout) :: phi
+        type(box_data), intent(inout) :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        integer, intent(in)           :: amrrefx, amrrefy
+        real(dp), intent(in)          :: tol
+        integer, intent(in)           :: maxiters, maxdepth_user, numcycles
+        integer, intent(in)           :: smooth_down, smooth_up, smooth_bottom
+        logical, intent(in)           :: zerophi
+        integer, intent(in)           :: verbosity
+        integer, intent(in), optional :: ordering
+        integer :: ord
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (amrrefx .lt. -1000000 .or. amrrefy .lt. -1000000) print*, 'unused'
+        call dropin_vcycle (2, phi, rhs, geo, bc, homog, tol, maxiters, maxdepth_user, numcycles, &
+                            smooth_down, smooth_up, smooth_bottom, zerophi, verbosity, ord)
+    end subroutine fmg
 
 end module MGPoisson2D
 

@@ -995,10 +995,32 @@ static void vcycle_noinit(const Cyc *c, int d)
     op_relax(c->dc, g, &l->bc, 1, l->invd, relax_omega, c->su, l->r, l->e, rorder);
 }
 
-static int vcycle(int dc, const Geo *g0, const Bc *bc, int homog, double tol, int maxiters,
+/* MGPoisson2D::fmg_noinit (MGPoisson2D.f90:1127-1189): e(d) = 0; restrict r(d)
+ * to r(d+1), solve there recursively, prolong with order 2; then numcycles
+ * V-cycles at depth d from that initial guess. */
+static void fmg_noinit(const Cyc *c, int d)
+{
+    Lvl *l = &c->L[d];
+    const Geo *g = &l->g;
+    memset(l->e, 0, (size_t)(g->nx + 2) * (g->ny + 2) * sizeof(double));
+    if (d < c->maxdepth) {
+        Lvl *lc = &c->L[d + 1];
+        restrict_field(0, g->nx, g->ny, l->r, lc->g.nx, lc->g.ny, lc->r);
+        fmg_noinit(c, d + 1);
+        prolong(g, &lc->g, &lc->bc, 2, l->e, lc->e);
+    }
+    for (int k = 0; k < c->numcycles; ++k) vcycle_noinit(c, d);
+}
+
+/* kind 0: MGPoisson2D::vcycle (MGPoisson2D.f90:400-670); 1: DeferredMGPoisson2D::vcycle
+ * (MGPoisson2D_DD.f90:589-862); 2: MGPoisson2D::fmg (MGPoisson2D.f90:860-1122), whose outer
+ * loop is the plain one with fmg_noinit in place of e-init + vcycle_noinit and no rhs scaling
+ * (the J scaling is commented out there, :1071-1072). */
+static int vcycle(int kind, const Geo *g0, const Bc *bc, int homog, double tol, int maxiters,
                   int maxdepth_user, int numcycles, int sd, int su, int sb, int zerophi,
                   double *phi, double *rhs, int order, double *relres, int *iters)
 {
+    const int dc = kind == 1, fmg = kind == 2;
     const Geo *g = g0;
     int md = maxdepth_user < 0 ? 1000 : maxdepth_user;
     int *refx = calloc((size_t)md + 1, sizeof(int)), *refy = calloc((size_t)md + 1, sizeof(int));
@@ -1072,8 +1094,12 @@ static int vcycle(int dc, const Geo *g0, const Bc *bc, int homog, double tol, in
     Cyc cyc = {dc, maxdepth, numcycles, sd, su, sb, order, refx, refy, L};
     int status = 2, it;
     for (it = 1; it <= maxiters; ++it) {
-        if (!dc) einit(g, l0->e, l0->r, l0->invd); /* MGPoisson2D.f90:600 */
-        vcycle_noinit(&cyc, 0);
+        if (fmg) {
+            fmg_noinit(&cyc, 0); /* MGPoisson2D.f90:1083-1088 */
+        } else {
+            if (!dc) einit(g, l0->e, l0->r, l0->invd); /* MGPoisson2D.f90:600 */
+            vcycle_noinit(&cyc, 0);
+        }
         for (size_t k = 0; k < n1; ++k) phi[k] = phi[k] + l0->e[k];
         residual(g, bc, homog, rhs, phi, l0->r);
         relres[it] = pnorm(g->nx, g->ny, g->dx, g->dy, l0->r, 2) / rscale;

@@ -68,7 +68,8 @@ double mgo_pnorm(int nx, int ny, double dx, double dy, const double *a, int p);
 void mgo_bicgstab2(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol,
                    int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
                    int order);
-/* kind 0: MGPoisson2D::vcycle, kind 1: DeferredMGPoisson2D::vcycle.
+/* kind 0: MGPoisson2D::vcycle, kind 1: DeferredMGPoisson2D::vcycle,
+ * kind 2: MGPoisson2D::fmg (MGPoisson2D.f90:860).
  * relres[0..maxiters] receives the relative-residual history, *iters the
  * number of cycles run (the reference prints this history at verbosity >= 1,
  * MGPoisson2D.f90:590-594,616-618).  Returns 0 converged, 1 diverged (last

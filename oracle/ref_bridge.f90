@@ -500,14 +500,15 @@ contains
     end subroutine ref_schedule
 
     ! --------------------------------------------------------------------------
-    ! MGPoisson2D::vcycle (MGPoisson2D.f90:400) when kind = 0, or
-    ! DeferredMGPoisson2D::vcycle (MGPoisson2D_DD.f90:589) when kind = 1.
+    ! MGPoisson2D::vcycle (MGPoisson2D.f90:400) when kind = 0,
+    ! DeferredMGPoisson2D::vcycle (MGPoisson2D_DD.f90:589) when kind = 1, or
+    ! MGPoisson2D::fmg (MGPoisson2D.f90:860) when kind = 2.
     ! --------------------------------------------------------------------------
     subroutine ref_vcycle (kind, nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy, bct, bcm, &
                            bd_xlo, bd_xhi, bd_ylo, bd_yhi, homog, tol, maxiters, maxdepth, &
                            numcycles, sdown, sup, sbottom, zerophi, phi, rhs) &
                            bind(C, name="ref_vcycle")
-        use MGPoisson2D, only: plain_vcycle => vcycle
+        use MGPoisson2D, only: plain_vcycle => vcycle, fmg
         use DeferredMGPoisson2D, only: dc_vcycle => vcycle
         integer(c_int), value       :: kind, nx, ny, homog, maxiters, maxdepth, numcycles
         integer(c_int), value       :: sdown, sup, sbottom, zerophi
@@ -531,6 +532,9 @@ contains
         if (kind .eq. 0) then
             call plain_vcycle (fphi, frhs, geo, bc, homog .ne. 0, 0, 0, tol, maxiters, maxdepth, &
                                numcycles, sdown, sup, sbottom, zerophi .ne. 0, 0)
+        else if (kind .eq. 2) then
+            call fmg (fphi, frhs, geo, bc, homog .ne. 0, 0, 0, tol, maxiters, maxdepth, &
+                      numcycles, sdown, sup, sbottom, zerophi .ne. 0, 0)
         else
             call dc_vcycle (fphi, frhs, geo, bc, homog .ne. 0, 0, 0, tol, maxiters, maxdepth, &
                             numcycles, sdown, sup, sbottom, zerophi .ne. 0, 0)

@@ -8,7 +8,7 @@
 program dropin_caller
     use ArrayUtils
     use Poisson2D
-    use MGPoisson2D, only: restrict, plain_vcycle => vcycle
+    use MGPoisson2D, only: restrict, plain_vcycle => vcycle, fmg
     use DeferredMGPoisson2D, only: dc_vcycle => vcycle
     implicit none
 
@@ -70,6 +70,11 @@ program dropin_caller
 
     phi%data = phi0%data
     call plain_vcycle (phi, rhs, geo, bc, .true., 0, 0, 1.0d-30, 1, -1, 1, 2, 2, 2, .false., 0)
+    write (u) phi%data(valid%ilo:valid%ihi, valid%jlo:valid%jhi)
+
+    ! one full-multigrid cycle, kept (a tolerance it always meets)
+    phi%data = phi0%data
+    call fmg (phi, rhs, geo, bc, .true., 0, 0, 1.0d30, 1, -1, 1, 2, 2, 2, .false., 0)
     write (u) phi%data(valid%ilo:valid%ihi, valid%jlo:valid%jhi)
     close (u)
 

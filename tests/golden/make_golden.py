@@ -65,7 +65,7 @@ def main():
             put(p + name, a)
         put(p + "shape", [geo.nx, geo.ny, geo.dx, geo.dy, *case[4]])
         put(p + "rhs", rhs)
-        for kind in (0, 1):
+        for kind in (0, 1, 2):  # MGPoisson2D::vcycle, DeferredMGPoisson2D::vcycle, MGPoisson2D::fmg
             out = ref.vcycle(kind, geo, bc, 1, 0.0, 4, np.zeros_like(phi), rhs, zerophi=True)
             put(p + f"vcycle{kind}_phi4", out[0])
     np.savez_compressed(OUT, **data)

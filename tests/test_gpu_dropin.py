@@ -4,8 +4,8 @@ reference's own modules and against the drop-in layer over the HIP library;
 both executables run and their outputs agree.
 
 Operators, inverse diagonals, Gauss-Seidel sweeps and restriction must match
-bit for bit; the V-cycle results to 1e-12 (their relres norms are summed in
-parallel on the GPU)."""
+bit for bit; the V-cycle and FMG results to 1e-12 (their relres norms are
+summed in parallel on the GPU)."""
 import os
 import subprocess
 
@@ -26,7 +26,7 @@ def run(exe, out):
 
 def split(a):
     n2, c2 = N * N, (N // 2) ** 2
-    sizes = [n2, n2, n2, c2, n2, n2, n2]
+    sizes = [n2, n2, n2, c2, n2, n2, n2, n2]
     parts, k = [], 0
     for s in sizes:
         parts.append(a[k:k + s])
@@ -42,6 +42,6 @@ def test_fortran_caller_dropin_matches_reference(tmp_path):
     r = split(run("caller_ref", str(tmp_path / "ref.bin")))
     for k in range(4):  # laplacian, invdiags, 3 GS sweeps, restrict
         np.testing.assert_array_equal(g[k], r[k])
-    for k in (4, 6):    # DC vcycle x4, plain vcycle x1
+    for k in (4, 6, 7):  # DC vcycle x4, plain vcycle x1, fmg x1
         np.testing.assert_allclose(g[k], r[k], rtol=0, atol=1e-12 * np.max(np.abs(r[k])))
     np.testing.assert_array_equal(g[5], r[5])  # rhs restored by the DC driver

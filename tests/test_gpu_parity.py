@@ -123,7 +123,7 @@ def test_bicgstab2_bottom(case, kind):
 
 
 @pytest.mark.parametrize("case", CYCLE_CASES)
-@pytest.mark.parametrize("kind,ordering", [(0, 0), (0, 2), (1, 0), (1, 1)])
+@pytest.mark.parametrize("kind,ordering", [(0, 0), (0, 2), (1, 0), (1, 1), (2, 0), (2, 2)])
 def test_vcycle_history(case, kind, ordering):
     geo, bc, phi, rhs = setup(*case, seed=7)
     maxit = 6

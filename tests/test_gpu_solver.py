@@ -93,13 +93,15 @@ def test_krylov_mg_all_neumann_compatible():
     assert relres_oracle(geo, bc, rhs, s.get_phi()) <= 1.5e-10
 
 
-@pytest.mark.parametrize("method,ordering", [(0, 0), (0, 2), (1, 0), (1, 1)])
-def test_resident_reference_drivers(method, ordering):
-    geo, bc, _, rhs = setup(64, 64, 0.0, 0.0, (1, 1, 1, 1), seed=14)
+# (solver method, oracle cycle kind, ordering)
+@pytest.mark.parametrize("method,kind,ordering", [(0, 0, 0), (0, 0, 2), (1, 1, 0), (1, 1, 1), (3, 2, 0), (3, 2, 2)])
+@pytest.mark.parametrize("types", [(1, 1, 1, 1), (0, 0, 0, 0)])
+def test_resident_reference_drivers(method, kind, ordering, types):
+    geo, bc, _, rhs = setup(64, 64, 0.0, 0.0, types, seed=14)
     s = mg.Solver(geo, bc, method=method, ordering=ordering)
     s.set_rhs(rhs)
     hist, iters, outcome = s.solve(1e-10, 30, zerophi=True)
-    _, _, h2, it2, oc2 = rc.oracle().vcycle(method, geo, bc, 1, 1e-10, 30, np.zeros((66, 66)), rhs,
+    _, _, h2, it2, oc2 = rc.oracle().vcycle(kind, geo, bc, 1, 1e-10, 30, np.zeros((66, 66)), rhs,
                                             zerophi=True, order=ordering)
     assert (iters, outcome) == (it2, oc2)
     np.testing.assert_allclose(hist, h2, rtol=1e-12, atol=0)

@@ -69,7 +69,7 @@ def test_oracle_matches_golden_operators(golden, k, homog):
 
 
 @pytest.mark.parametrize("k", range(len(CYCLE_CASES)))
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_oracle_matches_golden_vcycles(golden, k, kind):
     p = f"cyc{k}_"
     geo = geo_from(golden, p)
@@ -116,7 +116,7 @@ def test_oracle_vs_reference_transfers_and_bottom(case, seed):
 
 @needs_ref
 @pytest.mark.parametrize("case", CYCLE_CASES)
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 @pytest.mark.parametrize("cycles", [1, 5])
 def test_oracle_vs_reference_vcycle(case, kind, cycles):
     geo, bc, phi, rhs = setup(*case, seed=33)
@@ -129,7 +129,7 @@ def test_oracle_vs_reference_vcycle(case, kind, cycles):
 @needs_ref
 def test_oracle_vs_reference_wcycle_and_depth():
     geo, bc, phi, rhs = setup(64, 64, 0.08, 1.0, (1, 1, 0, 0), seed=34)
-    for kind in (0, 1):
+    for kind in (0, 1, 2):
         for numcycles, maxdepth in ((2, -1), (1, 2)):
             a = rc.oracle().vcycle(kind, geo, bc, 1, 0.0, 3, phi, rhs, zerophi=True, numcycles=numcycles,
                                    maxdepth=maxdepth)
