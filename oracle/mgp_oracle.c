@@ -1126,6 +1126,248 @@ static int vcycle(int kind, const Geo *g0, const Bc *bc, int homog, double tol, 
 }
 
 /* ---------------------------------------------------------------------------
+ * Flux tools of the projection workflow: Poisson2D::compute_pd
+ * (poisson2d.f90:404-795), compute_grad (:809-850), compute_div (:860-879)
+ * and ArrayUtils::fill_boundary_fluxes (arrayutils.f90:1571-1622).
+ * ------------------------------------------------------------------------- */
+
+/* compute_pd with pd nodal in x (an x-face field, (nx+1) x ny).
+ * dir 1: d/dxi across the face, (phi(i) - phi(i-1))/dx, ghosts included (:446-449).
+ * dir 2: d/deta at the face, the compute_laplacian cross-term stencil with
+ * its boundary closures (:619-793, simple_bdry_stencil = .false.). */
+static void pd_xface(const Geo *g, const double *phi, int dir, double *pd)
+{
+    const int nx = g->nx, ny = g->ny, ilo = 0, ihi = nx - 1, jlo = 0, jhi = ny - 1;
+    if (dir == 1) {
+        const double scale = 1.0 / g->dx;
+        for (int j = jlo; j <= jhi; ++j)
+            for (int i = ilo; i <= ihi + 1; ++i) XF(pd, i, j) = scale * (PH(phi, i, j) - PH(phi, i - 1, j));
+        return;
+    }
+    const double scale = 0.25 / g->dy;
+    /* away from the x boundaries: interior rows, then the two y-boundary rows */
+    for (int j = jlo + 1; j <= jhi - 1; ++j)
+        for (int i = ilo + 1; i <= ihi; ++i)
+            XF(pd, i, j) = scale * (PH(phi, i, j + 1) - PH(phi, i, j - 1) + PH(phi, i - 1, j + 1) -
+                                    PH(phi, i - 1, j - 1));
+    for (int i = ilo + 1; i <= ihi; ++i) {
+        XF(pd, i, jlo) = -(scale * (3.0 * PH(phi, i, jlo) - 4.0 * PH(phi, i, jlo + 1) + PH(phi, i, jlo + 2) +
+                                    3.0 * PH(phi, i - 1, jlo) - 4.0 * PH(phi, i - 1, jlo + 1) +
+                                    PH(phi, i - 1, jlo + 2)));
+        XF(pd, i, jhi) = scale * (3.0 * PH(phi, i, jhi) - 4.0 * PH(phi, i, jhi - 1) + PH(phi, i, jhi - 2) +
+                                  3.0 * PH(phi, i - 1, jhi) - 4.0 * PH(phi, i - 1, jhi - 1) +
+                                  PH(phi, i - 1, jhi - 2));
+    }
+    /* x boundaries: the west face of cell ilo, the east face of cell ihi */
+    double p, pn, pw, ee, ew, en, es, ene, enw, ese, esw;
+    int i = ilo, j;
+    for (j = jlo; j <= jhi; ++j) {
+        p = PH(phi, i, j);
+        ee = PH(phi, i + 1, j);
+        ew = 3.0 * (p - ee) + PH(phi, i + 2, j);
+        if (j == jlo) {
+            en = PH(phi, i, j + 1);
+            es = 3.0 * (p - en) + PH(phi, i, j + 2);
+            ene = PH(phi, i + 1, j + 1);
+            enw = 3.0 * (en - ene) + PH(phi, i + 2, j + 1);
+            ese = 3.0 * (ee - ene) + PH(phi, i + 1, j + 2);
+            esw = 0.5 * ((3.0 * (es - ese) + PH(phi, i + 2, j - 1)) + (3.0 * (ew - enw) + PH(phi, i - 1, j + 2)));
+        } else if (j == jhi) {
+            es = PH(phi, i, j - 1);
+            en = 3.0 * (p - es) + PH(phi, i, j - 2);
+            ese = PH(phi, i + 1, j - 1);
+            ene = 3.0 * (ee - ese) + PH(phi, i + 1, j - 2);
+            esw = 3.0 * (es - ese) + PH(phi, i + 2, j - 1);
+            enw = 0.5 * ((3.0 * (ew - esw) + PH(phi, i - 1, j - 2)) + (3.0 * (en - ene) + PH(phi, i + 2, j + 1)));
+        } else {
+            en = PH(phi, i, j + 1);
+            es = PH(phi, i, j - 1);
+            ene = PH(phi, i + 1, j + 1);
+            ese = PH(phi, i + 1, j - 1);
+            enw = 3.0 * (en - ene) + PH(phi, i + 2, j + 1);
+            esw = 3.0 * (es - ese) + PH(phi, i + 2, j - 1);
+        }
+        XF(pd, i, j) = scale * (en - es + enw - esw);
+    }
+    i = ihi;
+    for (j = jlo; j <= jhi; ++j) {
+        p = PH(phi, i, j);
+        ew = PH(phi, i - 1, j);
+        ee = 3.0 * (p - ew) + PH(phi, i - 2, j);
+        if (j == jlo) {
+            pn = PH(phi, i, j + 1);
+            pw = PH(phi, i - 1, j);
+            en = pn;
+            es = 3.0 * (p - en) + PH(phi, i, j + 2);
+            enw = PH(phi, i - 1, j + 1);
+            ene = 3.0 * (pn - enw) + PH(phi, i - 2, j + 1);
+            esw = 3.0 * (pw - enw) + PH(phi, i - 1, j + 2);
+            ese = 0.5 * ((3.0 * (es - esw) + PH(phi, i - 2, j - 1)) + (3.0 * (ee - ene) + PH(phi, i + 1, j + 2)));
+        } else if (j == jhi) {
+            es = PH(phi, i, j - 1);
+            en = 3.0 * (p - es) + PH(phi, i, j - 2);
+            esw = PH(phi, i - 1, j - 1);
+            enw = 3.0 * (ew - esw) + PH(phi, i - 1, j - 2);
+            ese = 3.0 * (es - esw) + PH(phi, i - 2, j - 1);
+            ene = 0.5 * ((3.0 * (en - enw) + PH(phi, i - 2, j + 1)) + (3.0 * (ee - ese) + PH(phi, i + 1, j - 2)));
+        } else {
+            en = PH(phi, i, j + 1);
+            es = PH(phi, i, j - 1);
+            enw = PH(phi, i - 1, j + 1);
+            esw = PH(phi, i - 1, j - 1);
+            ene = 3.0 * (en - enw) + PH(phi, i - 2, j + 1);
+            ese = 3.0 * (es - esw) + PH(phi, i - 2, j - 1);
+        }
+        XF(pd, i + 1, j) = scale * (ene - ese + en - es);
+    }
+}
+
+/* compute_pd with pd nodal in y (a y-face field, nx x (ny+1)).
+ * dir 2: (phi(j) - phi(j-1))/dy (:610-613).  dir 1: d/dxi at the face
+ * (:450-605), the transpose of pd_xface's dir 2. */
+static void pd_yface(const Geo *g, const double *phi, int dir, double *pd)
+{
+    const int nx = g->nx, ny = g->ny, ilo = 0, ihi = nx - 1, jlo = 0, jhi = ny - 1;
+    if (dir == 2) {
+        const double scale = 1.0 / g->dy;
+        for (int j = jlo; j <= jhi + 1; ++j)
+            for (int i = ilo; i <= ihi; ++i) YF(pd, i, j) = scale * (PH(phi, i, j) - PH(phi, i, j - 1));
+        return;
+    }
+    const double scale = 0.25 / g->dx;
+    for (int j = jlo + 1; j <= jhi; ++j)
+        for (int i = ilo + 1; i <= ihi - 1; ++i)
+            YF(pd, i, j) = scale * (PH(phi, i + 1, j) - PH(phi, i - 1, j) + PH(phi, i + 1, j - 1) -
+                                    PH(phi, i - 1, j - 1));
+    for (int j = jlo + 1; j <= jhi; ++j) {
+        YF(pd, ilo, j) = -(scale * (3.0 * PH(phi, ilo, j) - 4.0 * PH(phi, ilo + 1, j) + PH(phi, ilo + 2, j) +
+                                    3.0 * PH(phi, ilo, j - 1) - 4.0 * PH(phi, ilo + 1, j - 1) +
+                                    PH(phi, ilo + 2, j - 1)));
+        YF(pd, ihi, j) = scale * (3.0 * PH(phi, ihi, j) - 4.0 * PH(phi, ihi - 1, j) + PH(phi, ihi - 2, j) +
+                                  3.0 * PH(phi, ihi, j - 1) - 4.0 * PH(phi, ihi - 1, j - 1) +
+                                  PH(phi, ihi - 2, j - 1));
+    }
+    /* y boundaries: the south face of row jlo, the north face of row jhi */
+    double p, pn, pw, ee, ew, en, es, ene, enw, ese, esw;
+    int i, j = jlo;
+    for (i = ilo; i <= ihi; ++i) {
+        p = PH(phi, i, j);
+        en = PH(phi, i, j + 1);
+        es = 3.0 * (p - en) + PH(phi, i, j + 2);
+        if (i == ilo) {
+            ee = PH(phi, i + 1, j);
+            ew = 3.0 * (p - ee) + PH(phi, i + 2, j);
+            ene = PH(phi, i + 1, j + 1);
+            enw = 3.0 * (en - ene) + PH(phi, i + 2, j + 1);
+            ese = 3.0 * (ee - ene) + PH(phi, i + 1, j + 2);
+            esw = 0.5 * ((3.0 * (es - ese) + PH(phi, i + 2, j - 1)) + (3.0 * (ew - enw) + PH(phi, i - 1, j + 2)));
+        } else if (i == ihi) {
+            pn = PH(phi, i, j + 1);
+            pw = PH(phi, i - 1, j);
+            ew = pw;
+            ee = 3.0 * (p - ew) + PH(phi, i - 2, j);
+            enw = PH(phi, i - 1, j + 1);
+            ene = 3.0 * (pn - enw) + PH(phi, i - 2, j + 1);
+            esw = 3.0 * (pw - enw) + PH(phi, i - 1, j + 2);
+            ese = 0.5 * ((3.0 * (es - esw) + PH(phi, i - 2, j - 1)) + (3.0 * (ee - ene) + PH(phi, i + 1, j + 2)));
+        } else {
+            ee = PH(phi, i + 1, j);
+            ew = PH(phi, i - 1, j);
+            ene = PH(phi, i + 1, j + 1);
+            enw = PH(phi, i - 1, j + 1);
+            ese = 3.0 * (ee - ene) + PH(phi, i + 1, j + 2);
+            esw = 3.0 * (ew - enw) + PH(phi, i - 1, j + 2);
+        }
+        YF(pd, i, j) = scale * (ee - ew + ese - esw);
+    }
+    j = jhi;
+    for (i = ilo; i <= ihi; ++i) {
+        p = PH(phi, i, j);
+        es = PH(phi, i, j - 1);
+        en = 3.0 * (p - es) + PH(phi, i, j - 2);
+        if (i == ilo) {
+            ee = PH(phi, i + 1, j);
+            ew = 3.0 * (p - ee) + PH(phi, i + 2, j);
+            ese = PH(phi, i + 1, j - 1);
+            ene = 3.0 * (ee - ese) + PH(phi, i + 1, j - 2);
+            esw = 3.0 * (es - ese) + PH(phi, i + 2, j - 1);
+            enw = 0.5 * ((3.0 * (ew - esw) + PH(phi, i - 1, j - 2)) + (3.0 * (en - ene) + PH(phi, i + 2, j + 1)));
+        } else if (i == ihi) {
+            ew = PH(phi, i - 1, j);
+            ee = 3.0 * (p - ew) + PH(phi, i - 2, j);
+            esw = PH(phi, i - 1, j - 1);
+            enw = 3.0 * (ew - esw) + PH(phi, i - 1, j - 2);
+            ese = 3.0 * (es - esw) + PH(phi, i - 2, j - 1);
+            ene = 0.5 * ((3.0 * (en - enw) + PH(phi, i - 2, j + 1)) + (3.0 * (ee - ese) + PH(phi, i + 1, j - 2)));
+        } else {
+            ee = PH(phi, i + 1, j);
+            ew = PH(phi, i - 1, j);
+            ese = PH(phi, i + 1, j - 1);
+            esw = PH(phi, i - 1, j - 1);
+            ene = 3.0 * (ee - ese) + PH(phi, i + 1, j - 2);
+            enw = 3.0 * (ew - esw) + PH(phi, i - 1, j - 2);
+        }
+        YF(pd, i, j + 1) = scale * (ene - enw + ee - ew);
+    }
+}
+
+/* fill_boundary_fluxes (arrayutils.f90:1571-1622): Neumann faces take the BC
+ * data (homogeneous: zero; LAH: zero). */
+static void boundary_fluxes(const Geo *g, const Bc *bc, int homog, double *xflux, double *yflux)
+{
+    const int nx = g->nx, ny = g->ny;
+    for (int side = 0; side < 4; ++side) {
+        if (bc->type[side] != MGO_BC_NEUM) continue;
+        const int n = side < 2 ? ny : nx;
+        for (int k = 0; k < n; ++k) {
+            double v = 0.0;
+            if (!homog) {
+                if (bc->mode[side] == MGO_MODE_UNIFORM) v = bc->d[side][0];
+                else if (bc->mode[side] == MGO_MODE_NONUNIFORM) v = bc->d[side][k];
+            }
+            if (side == XLO) XF(xflux, 0, k) = v;
+            else if (side == XHI) XF(xflux, nx, k) = v;
+            else if (side == YLO) YF(yflux, k, 0) = v;
+            else YF(yflux, k, ny) = v;
+        }
+    }
+}
+
+/* compute_grad (poisson2d.f90:809-850): fills phi's ghosts (do_neum off), then
+ * J grad phi on the faces, then the Neumann boundary fluxes. */
+static void compute_grad(const Geo *g, const Bc *bc, int homog, double *phi, double *xflux, double *yflux)
+{
+    const int nx = g->nx, ny = g->ny;
+    double *xwk = malloc((size_t)(nx + 1) * ny * sizeof(double));
+    double *ywk = malloc((size_t)nx * (ny + 1) * sizeof(double));
+    if (!xwk || !ywk) die("out of memory");
+    fill_ghosts(g, bc, homog, 0, phi);
+    pd_xface(g, phi, 1, xflux);
+    pd_xface(g, phi, 2, xwk);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i)
+            XF(xflux, i, j) = XF(g->Jxx, i, j) * XF(xflux, i, j) + XF(g->Jxy, i, j) * XF(xwk, i, j);
+    pd_yface(g, phi, 1, ywk);
+    pd_yface(g, phi, 2, yflux);
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            YF(yflux, i, j) = YF(g->Jyx, i, j) * YF(ywk, i, j) + YF(g->Jyy, i, j) * YF(yflux, i, j);
+    boundary_fluxes(g, bc, homog, xflux, yflux);
+    free(xwk);
+    free(ywk);
+}
+
+/* compute_div (poisson2d.f90:860-879): J div F. */
+static void compute_div(const Geo *g, const double *xflux, const double *yflux, double *div)
+{
+    const double invdx = 1.0 / g->dx, invdy = 1.0 / g->dy;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i)
+            C0(div, i, j) = invdx * (XF(xflux, i + 1, j) - XF(xflux, i, j)) +
+                            invdy * (YF(yflux, i, j + 1) - YF(yflux, i, j));
+}
+
+/* ---------------------------------------------------------------------------
  * Exported entry points (argument order of oracle/ref_bridge.f90).
  * ------------------------------------------------------------------------- */
 #define MKG Geo G = mkgeo(nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy)
@@ -1240,4 +1482,24 @@ int mgo_vcycle(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double tol, int m
     MKG; MKB;
     return vcycle(kind, &G, &B, homog, tol, maxiters, maxdepth, numcycles, sdown, sup, sbottom,
                   zerophi, phi, rhs, order, relres, iters);
+}
+
+void mgo_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd)
+{
+    Geo G = mkgeo(nx, ny, dx, dy, 0, 0, 0, 0, 0);
+    if (nodedir == 1) pd_xface(&G, phi, dir, pd);
+    else pd_yface(&G, phi, dir, pd);
+}
+
+void mgo_compute_grad(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double *xflux, double *yflux)
+{
+    MKG; MKB;
+    compute_grad(&G, &B, homog, phi, xflux, yflux);
+}
+
+void mgo_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
+                     double *div)
+{
+    Geo G = mkgeo(nx, ny, dx, dy, 0, 0, 0, 0, 0);
+    compute_div(&G, xflux, yflux, div);
 }

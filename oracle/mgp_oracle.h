@@ -78,6 +78,15 @@ int mgo_vcycle(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double tol, int m
                int maxdepth, int numcycles, int sdown, int sup, int sbottom, int zerophi,
                double *phi, double *rhs, int order, double *relres, int *iters);
 
+/* Flux tools (poisson2d.f90:404-879).  compute_pd: pd nodal in direction
+ * nodedir (1: x-face field (nx+1) x ny, 2: y-face field nx x (ny+1)), derivative
+ * along dir.  compute_grad fills phi's ghosts, then J grad phi with Neumann
+ * boundary fluxes.  compute_div: J div F into an nx x ny field. */
+void mgo_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd);
+void mgo_compute_grad(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double *xflux, double *yflux);
+void mgo_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
+                     double *div);
+
 #ifdef __cplusplus
 }
 #endif

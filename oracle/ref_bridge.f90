@@ -151,6 +151,108 @@ contains
     end subroutine ref_laplacian
 
     ! --------------------------------------------------------------------------
+    ! Poisson2D::compute_pd (poisson2d.f90:404): pd nodal in x (nodedir 1,
+    ! pd(0:nx,0:ny-1)) or in y (nodedir 2, pd(0:nx-1,0:ny)); derivative along dir.
+    ! --------------------------------------------------------------------------
+    subroutine ref_compute_pd (nx, ny, dx, dy, phi, dir, nodedir, pd) bind(C, name="ref_compute_pd")
+        use Poisson2D, only: compute_pd
+        integer(c_int), value       :: nx, ny, dir, nodedir
+        real(c_double), value       :: dx, dy
+        real(c_double), intent(in)  :: phi(-1:nx,-1:ny)
+        real(c_double), intent(out) :: pd(*)
+
+        type(box) :: valid
+        type(box_data) :: fphi, fpd
+
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_cc1 (fphi, valid, phi)
+        if (nodedir .eq. 1) then
+            call define_box_data (fpd, valid, 0, 0, BD_NODE, BD_CELL)
+        else
+            call define_box_data (fpd, valid, 0, 0, BD_CELL, BD_NODE)
+        endif
+        fpd%data = zero
+        call compute_pd (fpd, fphi, dir)
+        pd(1:size(fpd%data)) = reshape(fpd%data, [size(fpd%data)])
+        call undefine_box_data (fphi)
+        call undefine_box_data (fpd)
+    end subroutine ref_compute_pd
+
+    ! --------------------------------------------------------------------------
+    ! Poisson2D::compute_grad (poisson2d.f90:809): fills phi's ghosts, returns
+    ! J grad phi on x-faces xflux(0:nx,0:ny-1) and y-faces yflux(0:nx-1,0:ny).
+    ! --------------------------------------------------------------------------
+    subroutine ref_compute_grad (nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy, bct, bcm, &
+                                 bd_xlo, bd_xhi, bd_ylo, bd_yhi, homog, phi, xflux, yflux) &
+                                 bind(C, name="ref_compute_grad")
+        use Poisson2D, only: compute_grad
+        integer(c_int), value       :: nx, ny, homog
+        real(c_double), value       :: dx, dy
+        real(c_double), intent(in)  :: J(0:nx-1,0:ny-1), Jxx(0:nx,0:ny-1), Jxy(0:nx,0:ny-1)
+        real(c_double), intent(in)  :: Jyx(0:nx-1,0:ny), Jyy(0:nx-1,0:ny)
+        integer(c_int), intent(in)  :: bct(4), bcm(4)
+        type(c_ptr), value          :: bd_xlo, bd_xhi, bd_ylo, bd_yhi
+        real(c_double), intent(inout) :: phi(-1:nx,-1:ny)
+        real(c_double), intent(out) :: xflux(0:nx,0:ny-1), yflux(0:nx-1,0:ny)
+
+        type(box) :: valid
+        type(geo_data) :: geo
+        type(bdry_data) :: bc
+        type(box_data) :: fphi, fx, fy, xwk, ywk
+
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_geo (geo, valid, J, Jxx, Jxy, Jyx, Jyy)
+        call mk_bc (bc, valid, bct, bcm, bd_xlo, bd_xhi, bd_ylo, bd_yhi)
+        call mk_cc1 (fphi, valid, phi)
+        call define_box_data (fx, valid, 0, 0, BD_NODE, BD_CELL)
+        call define_box_data (fy, valid, 0, 0, BD_CELL, BD_NODE)
+        call define_box_data (xwk, fx)
+        call define_box_data (ywk, fy)
+        fx%data = zero
+        fy%data = zero
+        xwk%data = zero
+        ywk%data = zero
+        call compute_grad (fx, fy, fphi, geo, bc, homog .ne. 0, xwk, ywk)
+        phi = fphi%data
+        xflux = fx%data
+        yflux = fy%data
+        call undefine_box_data (fphi)
+        call undefine_box_data (fx)
+        call undefine_box_data (fy)
+        call undefine_box_data (xwk)
+        call undefine_box_data (ywk)
+        call undefine_bdry_data (bc)
+        call free_geo (geo)
+    end subroutine ref_compute_grad
+
+    ! --------------------------------------------------------------------------
+    ! Poisson2D::compute_div (poisson2d.f90:860)
+    ! --------------------------------------------------------------------------
+    subroutine ref_compute_div (nx, ny, dx, dy, xflux, yflux, div) bind(C, name="ref_compute_div")
+        use Poisson2D, only: compute_div
+        integer(c_int), value       :: nx, ny
+        real(c_double), value       :: dx, dy
+        real(c_double), intent(in)  :: xflux(0:nx,0:ny-1), yflux(0:nx-1,0:ny)
+        real(c_double), intent(out) :: div(0:nx-1,0:ny-1)
+
+        type(box) :: valid
+        type(box_data) :: fx, fy, fd
+
+        call mk_valid (valid, nx, ny, dx, dy)
+        call define_box_data (fx, valid, 0, 0, BD_NODE, BD_CELL)
+        call define_box_data (fy, valid, 0, 0, BD_CELL, BD_NODE)
+        call define_box_data (fd, valid, 0, 0, BD_CELL, BD_CELL)
+        fx%data = xflux
+        fy%data = yflux
+        fd%data = zero
+        call compute_div (fd, fx, fy)
+        div = fd%data
+        call undefine_box_data (fx)
+        call undefine_box_data (fy)
+        call undefine_box_data (fd)
+    end subroutine ref_compute_div
+
+    ! --------------------------------------------------------------------------
     ! Poisson2D::compute_residual (poisson2d.f90:1327)
     ! --------------------------------------------------------------------------
     subroutine ref_residual (nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy, bct, bcm, &

@@ -58,6 +58,14 @@ def main():
             put(p + "dc_invd", dinv)
             put(p + "dc_lex3", ref.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi))
             put(p + "dc_rb3", ref_rb.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi))
+            # flux tools (poisson2d.f90:404-879): J grad phi on the faces, J div of it
+            _, xf, yf = ref.compute_grad(geo, bc, homog, phi)
+            put(p + "grad_x", xf)
+            put(p + "grad_y", yf)
+            put(p + "div", ref.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, xf, yf))
+            for nodedir in (1, 2):
+                for d in (1, 2):
+                    put(p + f"pd{nodedir}{d}", ref.compute_pd(geo.nx, geo.ny, geo.dx, geo.dy, phi, d, nodedir))
     for k, case in enumerate(CYCLE_CASES):
         geo, bc, phi, rhs = setup(*case, seed=200 + k)
         p = f"cyc{k}_"

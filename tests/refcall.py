@@ -91,6 +91,29 @@ class Checker:
         self._fn("residual")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(rhs), _ptr(phi), _ptr(res))
         return res, phi
 
+    # -- flux tools ------------------------------------------------------------
+    def compute_pd(self, nx, ny, dx, dy, phi, direction, nodedir):
+        """pd nodal in x (nodedir 1: shape (ny, nx+1)) or y (2: (ny+1, nx))."""
+        pd = np.zeros((ny, nx + 1) if nodedir == 1 else (ny + 1, nx))
+        phi = np.ascontiguousarray(phi)
+        self._fn("compute_pd")(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(phi), _i(direction), _i(nodedir), _ptr(pd))
+        return pd
+
+    def compute_grad(self, geo, bc, homog, phi):
+        """Returns (phi with ghosts filled, xflux (ny, nx+1), yflux (ny+1, nx))."""
+        phi = phi.copy()
+        xf = np.zeros((geo.ny, geo.nx + 1))
+        yf = np.zeros((geo.ny + 1, geo.nx))
+        self._fn("compute_grad")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(phi), _ptr(xf),
+                                 _ptr(yf))
+        return phi, xf, yf
+
+    def compute_div(self, nx, ny, dx, dy, xflux, yflux):
+        div = np.zeros((ny, nx))
+        xflux, yflux = np.ascontiguousarray(xflux), np.ascontiguousarray(yflux)
+        self._fn("compute_div")(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(xflux), _ptr(yflux), _ptr(div))
+        return div
+
     def invdiags(self, geo, bc, homog):
         out = np.zeros((geo.ny, geo.nx))
         self._fn("invdiags")(*self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(out))

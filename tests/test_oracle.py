@@ -66,6 +66,14 @@ def test_oracle_matches_golden_operators(golden, k, homog):
     np.testing.assert_array_equal(dinv, golden[p + "dc_invd"])
     np.testing.assert_array_equal(o.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi, rc.LEX), golden[p + "dc_lex3"])
     np.testing.assert_array_equal(o.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi, rc.RB), golden[p + "dc_rb3"])
+    _, xf, yf = o.compute_grad(geo, bc, homog, phi)
+    np.testing.assert_array_equal(xf, golden[p + "grad_x"])
+    np.testing.assert_array_equal(yf, golden[p + "grad_y"])
+    np.testing.assert_array_equal(o.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, xf, yf), golden[p + "div"])
+    for nodedir in (1, 2):
+        for d in (1, 2):
+            np.testing.assert_array_equal(o.compute_pd(geo.nx, geo.ny, geo.dx, geo.dy, phi, d, nodedir),
+                                          golden[p + f"pd{nodedir}{d}"])
 
 
 @pytest.mark.parametrize("k", range(len(CYCLE_CASES)))
@@ -112,6 +120,24 @@ def test_oracle_vs_reference_transfers_and_bottom(case, seed):
     dinv = o.dc_invdiags(geo)
     np.testing.assert_array_equal(o.bicgstab2(1, geo, lah, 1, dinv, 1e-6, 80, 5, 1, rhs, phi),
                                   r.bicgstab2(1, geo, lah, 1, dinv, 1e-6, 80, 5, 1, rhs, phi))
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [41, 42])
+@pytest.mark.parametrize("homog", [1, 0])
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+def test_oracle_vs_reference_flux_tools(case, homog, seed):
+    o, r = rc.oracle(), rc.reference()
+    geo, bc, phi, _ = setup(*case, homog=homog, seed=seed)
+    for nodedir in (1, 2):
+        for d in (1, 2):
+            np.testing.assert_array_equal(o.compute_pd(geo.nx, geo.ny, geo.dx, geo.dy, phi, d, nodedir),
+                                          r.compute_pd(geo.nx, geo.ny, geo.dx, geo.dy, phi, d, nodedir))
+    a, b = o.compute_grad(geo, bc, homog, phi), r.compute_grad(geo, bc, homog, phi)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(o.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, a[1], a[2]),
+                                  r.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, a[1], a[2]))
 
 
 @needs_ref
@@ -220,6 +246,68 @@ def test_quadratic_exactness_with_constant_cross_metric(types):
     lap, _ = o.laplacian(geo, bc, 0, phi.copy())
     rows = slice(0, ny - 1) if types[3] == 0 else slice(0, ny)
     np.testing.assert_allclose(lap[rows], exact, rtol=1e-9, atol=1e-8)
+
+
+@pytest.mark.parametrize("homog", [1, 0])
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+def test_div_of_grad_is_the_laplacian(case, homog):
+    """compute_grad uses the Laplacian's own face stencils (poisson2d.f90:804-805),
+    so J div(J grad phi) equals the golden-pinned Laplacian to rounding on every
+    boundary combination: a wrong face formula, a swapped metric or a Neumann
+    face left unset shows up as an O(1) difference."""
+    o = rc.oracle()
+    geo, bc, phi, _ = setup(*case, homog=homog, seed=7)
+    lap, _ = o.laplacian(geo, bc, homog, phi)
+    _, xf, yf = o.compute_grad(geo, bc, homog, phi)
+    div = o.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, xf, yf)
+    np.testing.assert_allclose(div, lap, rtol=0, atol=1e-13 * np.max(np.abs(lap)))
+
+
+def test_div_of_linear_fluxes_and_grad_of_quadratics():
+    """compute_div of fluxes linear along their normal is the constant slope sum;
+    compute_grad of a quadratic with constant metrics (cross term included) is
+    exact on every face -- interior centred differences, the one-sided boundary
+    stencils and the order-3 Dirichlet ghost are all quadratic-exact -- and
+    Neumann faces carry the bc data verbatim."""
+    o = rc.oracle()
+    nx, ny = 12, 10
+    dx, dy = 1.0 / nx, 1.0 / ny
+    xn, yn = np.arange(nx + 1) * dx, np.arange(ny + 1) * dy
+    xc, yc = (np.arange(nx) + 0.5) * dx, (np.arange(ny) + 0.5) * dy
+    rng = np.random.default_rng(3)
+    xf = 1.7 * xn[None, :] + rng.standard_normal((ny, 1))
+    yf = -0.6 * yn[:, None] + rng.standard_normal((1, nx))
+    np.testing.assert_allclose(o.compute_div(nx, ny, dx, dy, xf, yf), np.full((ny, nx), 1.1), atol=1e-12)
+
+    gxx, gxy, gyy = 1.3, 0.4, 0.7
+    geo = G.Geometry(nx, ny, dx, dy, np.ones((ny, nx)), np.full((ny, nx + 1), gxx), np.full((ny, nx + 1), gxy),
+                     np.full((ny + 1, nx), gxy), np.full((ny + 1, nx), gyy))
+    a, b, c, d, e, f0 = 0.7, -1.1, 0.9, 0.3, -0.5, 0.2
+    phi = quadratic(geo, a, b, c, d, e, f0)
+
+    def u(x, y):
+        return a * x * x + b * y * y + c * x * y + d * x + e * y + f0
+
+    def fx(x, y):  # J g^xj d_j u at an x-face
+        return gxx * (2 * a * x + c * y + d) + gxy * (2 * b * y + c * x + e)
+
+    def fy(x, y):
+        return gyy * (2 * b * y + c * x + e) + gxy * (2 * a * x + c * y + d)
+
+    types = (1, 0, 1, 0)
+    neum_xhi, neum_yhi = rng.standard_normal(ny), rng.standard_normal(nx)
+    data = (u(0.0, yc), neum_xhi, u(xc, 0.0), neum_yhi)
+    bc = G.BoundaryConditions(types, (G.BCMODE_NONUNIFORM,) * 4, tuple(np.asarray(v, float) for v in data))
+    _, gx, gy = o.compute_grad(geo, bc, 0, phi.copy())
+    want_x = fx(xn[None, :], yc[:, None])
+    want_y = fy(xc[None, :], yn[:, None])
+    np.testing.assert_allclose(gx[:, :-1], want_x[:, :-1], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(gy[:-1, :], want_y[:-1, :], rtol=0, atol=1e-10)
+    np.testing.assert_array_equal(gx[:, -1], neum_xhi)
+    np.testing.assert_array_equal(gy[-1, :], neum_yhi)
+    _, hx, hy = o.compute_grad(geo, bc, 1, phi.copy())
+    np.testing.assert_array_equal(hx[:, -1], 0.0)
+    np.testing.assert_array_equal(hy[-1, :], 0.0)
 
 
 def test_dirichlet_order3_ghost_exact_for_quadratics():
