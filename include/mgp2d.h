@@ -108,6 +108,22 @@ int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in,
                           int *refx, int *refy);
 /* arrayutils.f90:753 pnorm over the valid cells of a ghost-free field */
 int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, double *out);
+/* poisson2d.f90:404 compute_pd(pd, phi, dir): d phi / d(dir) on the faces normal to nodedir
+ * (1: x-faces, pd is (nx+1) x ny; 2: y-faces, pd is nx x (ny+1)); phi (nx+2) x (ny+2) is read
+ * with its ghosts as given.  Across the face (dir == nodedir) a two-point difference; along it
+ * compute_laplacian's cross-term stencil with its boundary closures.  nx, ny >= 3. */
+int mgp_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd);
+/* poisson2d.f90:809 compute_grad(xflux, yflux, phi, geo, bc, homog, xwk, ywk): fills phi's
+ * ghosts (do_neum false), then J grad phi on the faces -- xflux (nx+1) x ny, yflux nx x (ny+1)
+ * -- then fill_boundary_fluxes.  The workspaces xwk/ywk are internal. */
+int mgp_compute_grad(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *xflux,
+                     double *yflux);
+/* poisson2d.f90:860 compute_div(div, xflux, yflux): J div F on the nx x ny cells */
+int mgp_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
+                    double *div);
+/* arrayutils.f90:1571 fill_boundary_fluxes(xflux, yflux, bc, homog): Neumann faces take the BC
+ * flux (zero when homogeneous or BCMODE_LAH); other faces are left as they are. */
+int mgp_fill_boundary_fluxes(int nx, int ny, const mgp_bc *bc, int homog, double *xflux, double *yflux);
 /* poisson2d.f90:2253 (kind 0) / MGPoisson2D_DD.f90:339 (kind 1) solve_bicgstab2 */
 int mgp_solve_bicgstab2(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd,
                         double tol, int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,

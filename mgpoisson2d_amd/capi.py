@@ -65,7 +65,8 @@ EXPORTS = (
     "mgp_solver_default_opts", "mgp_solver_create", "mgp_solver_destroy", "mgp_solver_set_rhs",
     "mgp_solver_set_phi", "mgp_solver_get_phi", "mgp_solver_copy_rhs_from", "mgp_solver_solve",
     "mgp_solver_field", "mgp_solver_stream", "mgp_solver_last_cycles", "mgp_solver_time_kernel",
-    "mgp_device_synchronize", "mgp_xline_relax",
+    "mgp_device_synchronize", "mgp_xline_relax", "mgp_compute_pd", "mgp_compute_grad", "mgp_compute_div",
+    "mgp_fill_boundary_fluxes",
 )
 
 
@@ -181,6 +182,38 @@ def dc_relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_
     _check(lib.mgp_dc_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
                                _ptr(rhs), _ptr(phi), _i(ordering)))
     return phi
+
+
+def compute_pd(nx, ny, dx, dy, phi, direction, nodedir):
+    """Poisson2D::compute_pd: pd is (ny, nx+1) for nodedir 1, (ny+1, nx) for nodedir 2."""
+    phi = _f64(phi)
+    pd = np.zeros((ny, nx + 1) if nodedir == 1 else (ny + 1, nx))
+    _check(lib.mgp_compute_pd(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(phi), _i(direction), _i(nodedir), _ptr(pd)))
+    return pd
+
+
+def compute_grad(geo, bc, homog, phi):
+    """Poisson2D::compute_grad: returns (phi with ghosts filled, xflux (ny, nx+1), yflux (ny+1, nx))."""
+    a = _Args()
+    phi = _f64(phi).copy()
+    xf = np.zeros((geo.ny, geo.nx + 1))
+    yf = np.zeros((geo.ny + 1, geo.nx))
+    _check(lib.mgp_compute_grad(a.geo(geo), a.bc(bc), _i(homog), _ptr(phi), _ptr(xf), _ptr(yf)))
+    return phi, xf, yf
+
+
+def compute_div(nx, ny, dx, dy, xflux, yflux):
+    xflux, yflux = _f64(xflux), _f64(yflux)
+    div = np.zeros((ny, nx))
+    _check(lib.mgp_compute_div(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(xflux), _ptr(yflux), _ptr(div)))
+    return div
+
+
+def fill_boundary_fluxes(nx, ny, bc, homog, xflux, yflux):
+    a = _Args()
+    xf, yf = _f64(xflux).copy(), _f64(yflux).copy()
+    _check(lib.mgp_fill_boundary_fluxes(_i(nx), _i(ny), a.bc(bc), _i(homog), _ptr(xf), _ptr(yf)))
+    return xf, yf
 
 
 def restrict(kind, fnx, fny, fine, cnx, cny):

@@ -367,6 +367,87 @@ int mgp_prolong(const mgp_geo *fine_geo, const mgp_geo *crse_geo, const int *bc_
     });
 }
 
+int mgp_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd) {
+    return guarded([&] {
+        if (nx < 3 || ny < 3 || !(dx > 0) || !(dy > 0) || (dir != 1 && dir != 2) || (nodedir != 1 && nodedir != 2) ||
+            !phi || !pd) {
+            g_last_error = "compute_pd: need nx, ny >= 3, positive spacing, dir and nodedir in {1, 2}";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf f, out;
+        f.alloc(level_elems(nx, ny));
+        out.alloc(level_elems(nx, ny));
+        up_phi(s, f, g, phi);
+        launch_pd(s, out.p, f.p, g, dir, nodedir);
+        download_block(s, pd, out.p, g, 0, 0, nx + (nodedir == 1), ny + (nodedir == 2));
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int mgp_compute_grad(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *xflux, double *yflux) {
+    return guarded([&] {
+        if (!phi || !xflux || !yflux) {
+            g_last_error = "compute_grad: null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        Call c(geo, bc);
+        const Grid &g = c.geo.g;
+        DevBuf f = c.field(), xf = c.field(), yf = c.field();
+        up_phi(c.s, f, g, phi);
+        launch_fill_ghosts(c.s, f.p, g, c.geo.view(), c.bc.bc, homog, 0);
+        launch_grad(c.s, xf.p, yf.p, f.p, g, c.geo.view(), c.bc.bc, homog);
+        down_phi(c.s, phi, f, g);
+        download_block(c.s, xflux, xf.p, g, 0, 0, g.nx + 1, g.ny);
+        download_block(c.s, yflux, yf.p, g, 0, 0, g.nx, g.ny + 1);
+        c.finish();
+    });
+}
+
+int mgp_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux, double *div) {
+    return guarded([&] {
+        if (nx <= 0 || ny <= 0 || !(dx > 0) || !(dy > 0) || !xflux || !yflux || !div) {
+            g_last_error = "compute_div: bad shape, spacing or null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf xf, yf, out;
+        xf.alloc(level_elems(nx, ny));
+        yf.alloc(level_elems(nx, ny));
+        out.alloc(level_elems(nx, ny));
+        upload_block(s, xf.p, g, xflux, 0, 0, nx + 1, ny);
+        upload_block(s, yf.p, g, yflux, 0, 0, nx, ny + 1);
+        launch_div(s, out.p, xf.p, yf.p, g);
+        download_block(s, div, out.p, g, 0, 0, nx, ny);
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int mgp_fill_boundary_fluxes(int nx, int ny, const mgp_bc *bc, int homog, double *xflux, double *yflux) {
+    return guarded([&] {
+        if (nx <= 0 || ny <= 0 || !xflux || !yflux) {
+            g_last_error = "fill_boundary_fluxes: bad shape or null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        check_bc(bc);
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, 1.0, 1.0);
+        DevBc b;
+        b.upload(s, *bc, nx, ny);
+        DevBuf xf, yf;
+        xf.alloc(level_elems(nx, ny));
+        yf.alloc(level_elems(nx, ny));
+        upload_block(s, xf.p, g, xflux, 0, 0, nx + 1, ny);
+        upload_block(s, yf.p, g, yflux, 0, 0, nx, ny + 1);
+        launch_boundary_fluxes(s, xf.p, yf.p, g, b.bc, homog);
+        download_block(s, xflux, xf.p, g, 0, 0, nx + 1, ny);
+        download_block(s, yflux, yf.p, g, 0, 0, nx, ny + 1);
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
 int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out, int *refx,
                           int *refy) {
     return guarded([&] {

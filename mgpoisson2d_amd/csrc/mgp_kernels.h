@@ -98,4 +98,14 @@ void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, doub
                      const double *z, const Grid &g);
 void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sweeps, int homog);
 
+// Flux tools (mgp_flux.hip).  Face fields use the level layout: x-face (i, j)
+// and y-face (i, j) at ix(pitch, i, j).
+// xflux, yflux = J grad phi with Neumann faces from the BCs; phi's ghosts filled first.
+void launch_grad(hipStream_t s, double *xflux, double *yflux, const double *phi, const Grid &g, const Geo &geo,
+                 const Bc &bc, int homog);
+// One compute_pd field (nodedir 1: x-faces, 2: y-faces; dir: derivative direction).
+void launch_pd(hipStream_t s, double *pd, const double *phi, const Grid &g, int dir, int nodedir);
+void launch_div(hipStream_t s, double *div, const double *xflux, const double *yflux, const Grid &g);
+void launch_boundary_fluxes(hipStream_t s, double *xflux, double *yflux, const Grid &g, const Bc &bc, int homog);
+
 }  // namespace mgp

@@ -146,6 +146,13 @@ module ArrayUtils
             real(c_double), intent(in) :: a(*)
             real(c_double)             :: out
         end function
+        integer(c_int) function mgp_fill_boundary_fluxes_c(nx, ny, bc, homog, xflux, yflux) &
+                                                           bind(C, name="mgp_fill_boundary_fluxes")
+            import :: mgp_bc_c, c_int, c_double
+            integer(c_int), value      :: nx, ny, homog
+            type(mgp_bc_c), intent(in) :: bc
+            real(c_double)             :: xflux(*), yflux(*)
+        end function
         type(c_ptr) function mgp_last_error_c() bind(C, name="mgp_last_error")
             import :: c_ptr
         end function
@@ -577,6 +584,24 @@ contains
         phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
     end subroutine fill_ghosts
 
+    ! arrayutils.f90:1571 fill_boundary_fluxes (runs on the GPU).  Neumann
+    ! faces of the valid region take the BC flux; other faces are untouched.
+    subroutine fill_boundary_fluxes (xflux, yflux, bc, homog)
+        type(box_data), intent(inout) :: xflux, yflux
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        type(bc_pack) :: bp
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: xf(:,:), yf(:,:)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (xf, xflux)
+        call pack_cc (yf, yflux)
+        call mgp_check (mgp_fill_boundary_fluxes_c(yflux%valid%nx, xflux%valid%ny, bcc, merge(1, 0, homog), &
+                                                   xf, yf), 'fill_boundary_fluxes')
+        xflux%data(xflux%valid%ilo:xflux%valid%ihi, xflux%valid%jlo:xflux%valid%jhi) = xf
+        yflux%data(yflux%valid%ilo:yflux%valid%ihi, yflux%valid%jlo:yflux%valid%jhi) = yf
+    end subroutine fill_boundary_fluxes
+
 end module ArrayUtils
 
 
@@ -618,6 +643,27 @@ module Poisson2D
             real(c_double), intent(in)  :: invd(*), rhs(*)
             real(c_double)              :: phi(*)
         end function
+        integer(c_int) function c_pd(nx, ny, dx, dy, phi, dir, nodedir, pd) bind(C, name="mgp_compute_pd")
+            import :: c_int, c_double
+            integer(c_int), value       :: nx, ny, dir, nodedir
+            real(c_double), value       :: dx, dy
+            real(c_double), intent(in)  :: phi(*)
+            real(c_double)              :: pd(*)
+        end function
+        integer(c_int) function c_grad(geo, bc, homog, phi, xflux, yflux) bind(C, name="mgp_compute_grad")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog
+            real(c_double)              :: phi(*), xflux(*), yflux(*)
+        end function
+        integer(c_int) function c_div(nx, ny, dx, dy, xflux, yflux, div) bind(C, name="mgp_compute_div")
+            import :: c_int, c_double
+            integer(c_int), value       :: nx, ny
+            real(c_double), value       :: dx, dy
+            real(c_double), intent(in)  :: xflux(*), yflux(*)
+            real(c_double)              :: div(*)
+        end function
         integer(c_int) function c_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, &
                                             rhs, phi, ordering) bind(C, name="mgp_solve_bicgstab2")
             import :: mgp_geo_c, mgp_bc_c, c_int, c_double
@@ -649,6 +695,81 @@ contains
         call mgp_check (c_invdiags(gc, bcc, merge(1, 0, homog), a), 'Poisson2d::compute_invdiags')
         invdiags%data(invdiags%valid%ilo:invdiags%valid%ihi, invdiags%valid%jlo:invdiags%valid%jhi) = a
     end subroutine compute_invdiags
+
+    ! poisson2d.f90:404 -- pd must be nodal in exactly one direction; phi's
+    ! ghosts are read as they are.
+    subroutine compute_pd (pd, phi, dir)
+        type(box_data), intent(inout) :: pd
+        type(box_data), intent(in)    :: phi
+        integer, intent(in)           :: dir
+        real(dp), allocatable :: a(:,:), d(:,:)
+        integer :: nodedir, ilo, ihi, jlo, jhi
+        if ((pd%offi .ne. 0) .and. (pd%offj .ne. 0)) then
+            print*, 'compute_pd cannot handle totally cell-centered data.'
+            stop
+        endif
+        if ((pd%offi .eq. 0) .and. (pd%offj .eq. 0)) then
+            print*, 'compute_pd cannot handle totally nodal-centered data.'
+            stop
+        endif
+        nodedir = merge(1, 2, pd%offi .eq. 0)
+        ilo = phi%valid%ilo
+        ihi = phi%valid%ihi + merge(1, 0, nodedir .eq. 1)
+        jlo = phi%valid%jlo
+        jhi = phi%valid%jhi + merge(1, 0, nodedir .eq. 2)
+        call pack_phi (a, phi)
+        allocate (d(ilo:ihi, jlo:jhi))
+        call mgp_check (c_pd(phi%valid%nx, phi%valid%ny, phi%valid%dx, phi%valid%dy, a, dir, nodedir, d), &
+                        'Poisson2d::compute_pd')
+        pd%data(ilo:ihi, jlo:jhi) = d
+    end subroutine compute_pd
+
+    ! poisson2d.f90:809 -- xwk and ywk are accepted for the reference's
+    ! argument list; the GPU path needs no workspace from the caller.
+    subroutine compute_grad (xflux, yflux, phi, geo, bc, homog, xwk, ywk)
+        type(box_data), intent(inout) :: xflux, yflux
+        type(box_data), intent(inout) :: phi
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        type(box_data), intent(inout) :: xwk, ywk
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), xf(:,:), yf(:,:)
+        integer :: ilo, ihi, jlo, jhi
+        ilo = phi%valid%ilo
+        ihi = phi%valid%ihi
+        jlo = phi%valid%jlo
+        jhi = phi%valid%jhi
+        call pack_geo (gp, gc, geo, phi%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_phi (a, phi)
+        allocate (xf(ilo:ihi+1, jlo:jhi), yf(ilo:ihi, jlo:jhi+1))
+        call mgp_check (c_grad(gc, bcc, merge(1, 0, homog), a, xf, yf), 'Poisson2d::compute_grad')
+        phi%data(ilo-1:ihi+1, jlo-1:jhi+1) = a
+        xflux%data(ilo:ihi+1, jlo:jhi) = xf
+        yflux%data(ilo:ihi, jlo:jhi+1) = yf
+    end subroutine compute_grad
+
+    ! poisson2d.f90:860
+    subroutine compute_div (div, xflux, yflux)
+        type(box_data), intent(inout) :: div
+        type(box_data), intent(in)    :: xflux, yflux
+        real(dp), allocatable :: xf(:,:), yf(:,:), d(:,:)
+        integer :: ilo, ihi, jlo, jhi
+        ilo = div%valid%ilo
+        ihi = div%valid%ihi
+        jlo = div%valid%jlo
+        jhi = div%valid%jhi
+        allocate (xf(ilo:ihi+1, jlo:jhi), yf(ilo:ihi, jlo:jhi+1), d(ilo:ihi, jlo:jhi))
+        xf = xflux%data(ilo:ihi+1, jlo:jhi)
+        yf = yflux%data(ilo:ihi, jlo:jhi+1)
+        call mgp_check (c_div(div%valid%nx, div%valid%ny, div%valid%dx, div%valid%dy, xf, yf, d), &
+                        'Poisson2d::compute_div')
+        div%data(ilo:ihi, jlo:jhi) = d
+    end subroutine compute_div
 
     ! poisson2d.f90:886
     subroutine compute_laplacian (lap, phi, geo, bc, homog, opt_jscale)

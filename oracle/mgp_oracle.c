@@ -1497,6 +1497,13 @@ void mgo_compute_grad(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double 
     compute_grad(&G, &B, homog, phi, xflux, yflux);
 }
 
+void mgo_fill_boundary_fluxes(int nx, int ny, MGO_BC_ARGS, int homog, double *xflux, double *yflux)
+{
+    Geo G = mkgeo(nx, ny, 1.0, 1.0, 0, 0, 0, 0, 0);
+    MKB;
+    boundary_fluxes(&G, &B, homog, xflux, yflux);
+}
+
 void mgo_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
                      double *div)
 {

@@ -84,6 +84,8 @@ int mgo_vcycle(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double tol, int m
  * boundary fluxes.  compute_div: J div F into an nx x ny field. */
 void mgo_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd);
 void mgo_compute_grad(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double *xflux, double *yflux);
+/* fill_boundary_fluxes (arrayutils.f90:1571-1622): Neumann faces take the BC flux. */
+void mgo_fill_boundary_fluxes(int nx, int ny, MGO_BC_ARGS, int homog, double *xflux, double *yflux);
 void mgo_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
                      double *div);
 

@@ -253,6 +253,34 @@ contains
     end subroutine ref_compute_div
 
     ! --------------------------------------------------------------------------
+    ! ArrayUtils::fill_boundary_fluxes (arrayutils.f90:1571)
+    ! --------------------------------------------------------------------------
+    subroutine ref_fill_boundary_fluxes (nx, ny, bct, bcm, bd_xlo, bd_xhi, bd_ylo, bd_yhi, homog, xflux, yflux) &
+                                         bind(C, name="ref_fill_boundary_fluxes")
+        integer(c_int), value       :: nx, ny, homog
+        integer(c_int), intent(in)  :: bct(4), bcm(4)
+        type(c_ptr), value          :: bd_xlo, bd_xhi, bd_ylo, bd_yhi
+        real(c_double), intent(inout) :: xflux(0:nx,0:ny-1), yflux(0:nx-1,0:ny)
+
+        type(box) :: valid
+        type(bdry_data) :: bc
+        type(box_data) :: fx, fy
+
+        call mk_valid (valid, nx, ny, one, one)
+        call mk_bc (bc, valid, bct, bcm, bd_xlo, bd_xhi, bd_ylo, bd_yhi)
+        call define_box_data (fx, valid, 0, 0, BD_NODE, BD_CELL)
+        call define_box_data (fy, valid, 0, 0, BD_CELL, BD_NODE)
+        fx%data = xflux
+        fy%data = yflux
+        call fill_boundary_fluxes (fx, fy, bc, homog .ne. 0)
+        xflux = fx%data
+        yflux = fy%data
+        call undefine_box_data (fx)
+        call undefine_box_data (fy)
+        call undefine_bdry_data (bc)
+    end subroutine ref_fill_boundary_fluxes
+
+    ! --------------------------------------------------------------------------
     ! Poisson2D::compute_residual (poisson2d.f90:1327)
     ! --------------------------------------------------------------------------
     subroutine ref_residual (nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy, bct, bcm, &

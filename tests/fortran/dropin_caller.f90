@@ -17,6 +17,7 @@ program dropin_caller
     type(geo_data)     :: geo
     type(bdry_data)    :: bc
     type(box_data)     :: phi, rhs, lap, invd, cj, phi0
+    type(box_data)     :: xflux, yflux, xwk, ywk
     integer            :: i, j, u
     character(len=256) :: fname
     real(dp)           :: x, y
@@ -76,6 +77,26 @@ program dropin_caller
     phi%data = phi0%data
     call fmg (phi, rhs, geo, bc, .true., 0, 0, 1.0d30, 1, -1, 1, 2, 2, 2, .false., 0)
     write (u) phi%data(valid%ilo:valid%ihi, valid%jlo:valid%jhi)
+
+    ! projection tools with inhomogeneous data: J grad phi, J div of it, one
+    ! transverse derivative, then the homogeneous boundary fluxes
+    bc%data_xlo = 0.3_dp
+    bc%data_ylo = 0.25_dp
+    bc%data_yhi = -0.5_dp
+    call define_box_data (xflux, valid, 0, 0, BD_NODE, BD_CELL)
+    call define_box_data (yflux, valid, 0, 0, BD_CELL, BD_NODE)
+    call define_box_data (xwk, xflux)
+    call define_box_data (ywk, yflux)
+    phi%data = phi0%data
+    call compute_grad (xflux, yflux, phi, geo, bc, .false., xwk, ywk)
+    write (u) xflux%data
+    write (u) yflux%data
+    call compute_div (lap, xflux, yflux)
+    write (u) lap%data
+    call compute_pd (ywk, phi, 1)
+    write (u) ywk%data
+    call fill_boundary_fluxes (xflux, yflux, bc, .true.)
+    write (u) yflux%data
     close (u)
 
 contains

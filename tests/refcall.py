@@ -108,6 +108,11 @@ class Checker:
                                  _ptr(yf))
         return phi, xf, yf
 
+    def fill_boundary_fluxes(self, nx, ny, bc, homog, xflux, yflux):
+        xf, yf = np.ascontiguousarray(xflux).copy(), np.ascontiguousarray(yflux).copy()
+        self._fn("fill_boundary_fluxes")(_i(nx), _i(ny), *self._bc_args(bc), _i(homog), _ptr(xf), _ptr(yf))
+        return xf, yf
+
     def compute_div(self, nx, ny, dx, dy, xflux, yflux):
         div = np.zeros((ny, nx))
         xflux, yflux = np.ascontiguousarray(xflux), np.ascontiguousarray(yflux)

@@ -40,3 +40,9 @@ def test_bad_arguments_rejected_without_device_work():
         capi.compute_invdiags(geo, G.BoundaryConditions.uniform((1, 1, 1, 1)), 1)
     with pytest.raises(capi.MgpError):
         capi.restrict(0, 10, 10, np.zeros((10, 10)), 3, 5)
+    with pytest.raises(capi.MgpError):
+        capi.compute_pd(2, 8, 0.5, 0.125, np.zeros((10, 4)), 1, 1)
+    with pytest.raises(capi.MgpError):
+        capi.compute_pd(8, 8, 0.125, 0.125, np.zeros((10, 10)), 3, 1)
+    with pytest.raises(capi.MgpError):
+        capi.compute_div(8, 8, 0.0, 0.125, np.zeros((8, 9)), np.zeros((9, 8)))

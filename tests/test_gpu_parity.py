@@ -46,6 +46,74 @@ def test_operator_fill_invdiags(case, homog):
 
 
 @pytest.mark.parametrize("case", OPERATOR_CASES)
+@pytest.mark.parametrize("homog", [1, 0])
+def test_flux_tools(case, homog):
+    """compute_pd (all four face/derivative pairs), compute_grad, compute_div."""
+    geo, bc, phi, _ = setup(*case, homog=homog, seed=11)
+    nx, ny, dx, dy = geo.nx, geo.ny, geo.dx, geo.dy
+    pds = {(nd, d): mg.compute_pd(nx, ny, dx, dy, phi, d, nd) for nd in (1, 2) for d in (1, 2)}
+    phig, xf, yf = mg.compute_grad(geo, bc, homog, phi)
+    div = mg.compute_div(nx, ny, dx, dy, xf, yf)
+    for c in checkers():
+        for (nd, d), pd in pds.items():
+            same(pd, c.compute_pd(nx, ny, dx, dy, phi, d, nd))
+        p2, x2, y2 = c.compute_grad(geo, bc, homog, phi)
+        same(phig, p2)
+        same(xf, x2)
+        same(yf, y2)
+        same(div, c.compute_div(nx, ny, dx, dy, xf, yf))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("homog", [1, 0])
+def test_fill_boundary_fluxes(mode, homog):
+    from mgpoisson2d_amd import geometry as G
+    rng = np.random.default_rng(mode * 2 + homog)
+    for nx, ny, types in ((16, 12, (0, 0, 0, 0)), (33, 17, (1, 0, 0, 1)), (5, 300, (0, 1, 1, 0))):
+        bc = G.BoundaryConditions(types, (mode,) * 4, tuple(rng.standard_normal(ny if k < 2 else nx) for k in range(4)))
+        xf, yf = rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx))
+        a = mg.fill_boundary_fluxes(nx, ny, bc, homog, xf, yf)
+        for c in checkers():
+            b = c.fill_boundary_fluxes(nx, ny, bc, homog, xf, yf)
+            same(a[0], b[0])
+            same(a[1], b[1])
+
+
+@pytest.mark.parametrize("n", [(3, 3), (3, 9), (300, 4), (1030, 700)])
+def test_flux_tools_sizes(n):
+    """Smallest legal grids (every face is a boundary-closure face) and grids
+    wider than one workgroup tile, against the oracle."""
+    geo, bc, phi, _ = setup(n[0], n[1], 0.07, 2.5, (1, 0, 0, 1), homog=False, seed=12)
+    o = rc.oracle()
+    for a, b in zip(mg.compute_grad(geo, bc, 0, phi), o.compute_grad(geo, bc, 0, phi)):
+        same(a, b)
+    for nd in (1, 2):
+        for d in (1, 2):
+            same(mg.compute_pd(geo.nx, geo.ny, geo.dx, geo.dy, phi, d, nd),
+                 o.compute_pd(geo.nx, geo.ny, geo.dx, geo.dy, phi, d, nd))
+
+
+def test_div_of_grad_is_laplacian_2048():
+    """Full size (2048^2 stretched mapped grid, Dirichlet xi / Neumann eta):
+    J div(J grad phi) from the flux kernels equals the Laplacian kernel's
+    result to rounding (both use the same face stencils; only the operation
+    order differs).  Bound: 1e-14 of the largest flux difference quotient."""
+    import os, sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    geo, bc, _ = bench.config2_problem(2048, 2048)
+    rng = np.random.default_rng(0)
+    phi = np.zeros((2050, 2050))
+    phi[1:-1, 1:-1] = rng.standard_normal((2048, 2048))
+    for homog in (1, 0):
+        lap, _ = mg.compute_laplacian(geo, bc, homog, phi)
+        _, xf, yf = mg.compute_grad(geo, bc, homog, phi)
+        div = mg.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, xf, yf)
+        scale = np.max(np.abs(xf)) / geo.dx + np.max(np.abs(yf)) / geo.dy
+        assert np.max(np.abs(div - lap)) <= 1e-14 * scale
+
+
+@pytest.mark.parametrize("case", OPERATOR_CASES)
 def test_relax_gs_lex_and_mc(case):
     geo, bc, phi, rhs = setup(*case, seed=2)
     inv = mg.compute_invdiags(geo, bc, 1)

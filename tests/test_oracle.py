@@ -141,6 +141,25 @@ def test_oracle_vs_reference_flux_tools(case, homog, seed):
 
 
 @needs_ref
+@pytest.mark.parametrize("mode", [G.BCMODE_UNIFORM, G.BCMODE_NONUNIFORM, G.BCMODE_LAH])
+@pytest.mark.parametrize("homog", [1, 0])
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+def test_oracle_vs_reference_fill_boundary_fluxes(case, homog, mode):
+    """Neumann faces take the BC flux (zero when homogeneous or LAH); every
+    other face keeps its value."""
+    o, r = rc.oracle(), rc.reference()
+    nx, ny, types = case[0], case[1], case[4]
+    rng = np.random.default_rng(nx * 7 + homog)
+    bc = G.BoundaryConditions(types, (mode,) * 4, tuple(rng.standard_normal(ny if k < 2 else nx) for k in range(4)))
+    xf, yf = rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx))
+    a, b = o.fill_boundary_fluxes(nx, ny, bc, homog, xf, yf), r.fill_boundary_fluxes(nx, ny, bc, homog, xf, yf)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[0][:, 1:-1], xf[:, 1:-1])
+    np.testing.assert_array_equal(a[1][1:-1, :], yf[1:-1, :])
+
+
+@needs_ref
 @pytest.mark.parametrize("case", CYCLE_CASES)
 @pytest.mark.parametrize("kind", [0, 1, 2])
 @pytest.mark.parametrize("cycles", [1, 5])
