@@ -174,8 +174,20 @@ int mgp_solver_copy_rhs_from(mgp_solver *s, const mgp_solver *src);
 /* Solve in place on the resident phi/rhs; relres may be NULL (else maxiters+1 doubles). */
 int mgp_solver_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters,
                      int *outcome);
+/* Resident face fluxes for the projection step, in the host layouts of compute_grad:
+ * xflux (nx+1) x ny, yflux nx x (ny+1).  The first set_flux allocates them. */
+int mgp_solver_set_flux(mgp_solver *s, const double *xflux, const double *yflux);
+int mgp_solver_get_flux(mgp_solver *s, double *xflux, double *yflux);
+/* The divergence-cleaning step of the reference's projection workflow (test.f90:2817-3024)
+ * on the resident fields: rhs = J div F (compute_div); solve L phi = rhs from phi = 0 with the
+ * solver's method and BCs (homog from the options); F -= J grad phi (compute_grad, same BCs);
+ * divnorm (2 doubles, may be NULL) receives max|J div F| before and after.  The reference
+ * workflow's solver is fmg(tol, maxiters 10, numcycles 1, smoothing 8/8/8) with all-Neumann
+ * BCs: an MGP_METHOD_FMG solver created with those options reproduces it. */
+int mgp_solver_project(mgp_solver *s, double tol, int maxiters, double *divnorm, int *iters, int *outcome);
 /* Device pointer + pitch (doubles) of the resident fields, padded layout:
- * element (i, j) at ptr[(j + 1) * pitch + 32 + i], i, j = -1 .. n. which: 0 phi, 1 rhs. */
+ * element (i, j) at ptr[(j + 1) * pitch + 32 + i], i, j = -1 .. n.  which: 0 phi, 1 rhs,
+ * 2 xflux (x-face (i, j) at the same offset), 3 yflux (after the first set_flux). */
 int mgp_solver_field(mgp_solver *s, int which, double **ptr, int *pitch);
 /* HIP stream the solver enqueues on (a hipStream_t). */
 void *mgp_solver_stream(mgp_solver *s);

@@ -66,7 +66,7 @@ EXPORTS = (
     "mgp_solver_set_phi", "mgp_solver_get_phi", "mgp_solver_copy_rhs_from", "mgp_solver_solve",
     "mgp_solver_field", "mgp_solver_stream", "mgp_solver_last_cycles", "mgp_solver_time_kernel",
     "mgp_device_synchronize", "mgp_xline_relax", "mgp_compute_pd", "mgp_compute_grad", "mgp_compute_div",
-    "mgp_fill_boundary_fluxes",
+    "mgp_fill_boundary_fluxes", "mgp_solver_set_flux", "mgp_solver_get_flux", "mgp_solver_project",
 )
 
 
@@ -326,6 +326,26 @@ class Solver:
         _check(lib.mgp_solver_solve(self._h, _d(tol), _i(maxiters), _i(int(zerophi)), None,
                                     ctypes.byref(iters), ctypes.byref(outcome)))
         return iters.value, outcome.value
+
+    def set_flux(self, xflux, yflux):
+        """Resident face fluxes: xflux (ny, nx+1), yflux (ny+1, nx)."""
+        xflux, yflux = _f64(xflux), _f64(yflux)
+        _check(lib.mgp_solver_set_flux(self._h, _ptr(xflux), _ptr(yflux)))
+
+    def get_flux(self):
+        xf = np.zeros((self.geo.ny, self.geo.nx + 1))
+        yf = np.zeros((self.geo.ny + 1, self.geo.nx))
+        _check(lib.mgp_solver_get_flux(self._h, _ptr(xf), _ptr(yf)))
+        return xf, yf
+
+    def project(self, tol=1e-10, maxiters=100):
+        """Divergence cleaning of the resident fluxes: returns (max|J div F|
+        before, after, solver iterations, outcome)."""
+        dn = np.zeros(2)
+        iters, outcome = _i(0), _i(0)
+        _check(lib.mgp_solver_project(self._h, _d(tol), _i(maxiters), _ptr(dn), ctypes.byref(iters),
+                                      ctypes.byref(outcome)))
+        return dn[0], dn[1], iters.value, outcome.value
 
     def last_cycles(self):
         return lib.mgp_solver_last_cycles(self._h)

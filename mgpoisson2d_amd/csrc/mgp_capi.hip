@@ -100,6 +100,7 @@ struct mgp_solver {
     DevGeo geo;
     DevBc ubc;
     DevBuf phi, rhs;
+    DevBuf xflux, yflux, div;  // projection fields, allocated by the first set_flux
     Workspace W;
     RefHierarchy ref;
     KrylovMG kmg;
@@ -621,28 +622,89 @@ int mgp_solver_copy_rhs_from(mgp_solver *s, const mgp_solver *src) {
     });
 }
 
+namespace {
+
+int run_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters) {
+    const mgp_solver_opts &o = s->opts;
+    int it = 0, oc = MGP_MAXITERS;
+    if (o.method == MGP_METHOD_KRYLOV_MG) {
+        oc = s->kmg.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters, zerophi,
+                          relres, &it);
+        s->last_cycles = s->kmg.last_cycles;
+    } else {
+        oc = s->ref.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters,
+                          o.numcycles, o.smooth_down, o.smooth_up, o.smooth_bottom, zerophi, relres, &it);
+        s->last_cycles = it;
+    }
+    if (iters) *iters = it;
+    return oc;
+}
+
+void need_flux(const mgp_solver *s) {
+    if (!s->xflux.p) {
+        g_last_error = "no resident fluxes: call mgp_solver_set_flux first";
+        throw Failure(MGP_ERR_ARG);
+    }
+}
+
+}  // namespace
+
 int mgp_solver_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters,
                      int *outcome) {
     return guarded([&] {
-        const mgp_solver_opts &o = s->opts;
-        int it = 0, oc = MGP_MAXITERS;
-        if (o.method == MGP_METHOD_KRYLOV_MG) {
-            oc = s->kmg.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters,
-                              zerophi, relres, &it);
-            s->last_cycles = s->kmg.last_cycles;
-        } else {
-            oc = s->ref.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters,
-                              o.numcycles, o.smooth_down, o.smooth_up, o.smooth_bottom, zerophi, relres, &it);
-            s->last_cycles = it;
+        const int oc = run_solve(s, tol, maxiters, zerophi, relres, iters);
+        if (outcome) *outcome = oc;
+    });
+}
+
+int mgp_solver_set_flux(mgp_solver *s, const double *xflux, const double *yflux) {
+    return guarded([&] {
+        const Grid &g = s->geo.g;
+        if (!s->xflux.p) {
+            s->xflux.alloc(level_elems(g.nx, g.ny));
+            s->yflux.alloc(level_elems(g.nx, g.ny));
+            s->div.alloc(level_elems(g.nx, g.ny));
         }
-        if (iters) *iters = it;
+        upload_block(s->stream, s->xflux.p, g, xflux, 0, 0, g.nx + 1, g.ny);
+        upload_block(s->stream, s->yflux.p, g, yflux, 0, 0, g.nx, g.ny + 1);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_get_flux(mgp_solver *s, double *xflux, double *yflux) {
+    return guarded([&] {
+        need_flux(s);
+        const Grid &g = s->geo.g;
+        download_block(s->stream, xflux, s->xflux.p, g, 0, 0, g.nx + 1, g.ny);
+        download_block(s->stream, yflux, s->yflux.p, g, 0, 0, g.nx, g.ny + 1);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_project(mgp_solver *s, double tol, int maxiters, double *divnorm, int *iters, int *outcome) {
+    return guarded([&] {
+        need_flux(s);
+        const Grid &g = s->geo.g;
+        const int homog = s->opts.homog;
+        launch_div(s->stream, s->rhs.p, s->xflux.p, s->yflux.p, g);
+        const double d0 = s->W.reduce(s->stream, s->rhs.p, nullptr, nullptr, 1.0, g, 2);
+        const int oc = run_solve(s, tol, maxiters, 1, nullptr, iters);
+        launch_fill_ghosts(s->stream, s->phi.p, g, s->geo.view(), s->ubc.bc, homog, 0);
+        launch_grad_subtract(s->stream, s->xflux.p, s->yflux.p, s->phi.p, g, s->geo.view(), s->ubc.bc, homog);
+        launch_div(s->stream, s->div.p, s->xflux.p, s->yflux.p, g);
+        const double d1 = s->W.reduce(s->stream, s->div.p, nullptr, nullptr, 1.0, g, 2);
+        if (divnorm) {
+            divnorm[0] = d0;
+            divnorm[1] = d1;
+        }
         if (outcome) *outcome = oc;
     });
 }
 
 int mgp_solver_field(mgp_solver *s, int which, double **ptr, int *pitch) {
-    if (!s || (which != 0 && which != 1)) return MGP_ERR_ARG;
-    *ptr = which == 0 ? s->phi.p : s->rhs.p;
+    if (!s || which < 0 || which > 3 || (which >= 2 && !s->xflux.p)) return MGP_ERR_ARG;
+    const DevBuf *f[4] = {&s->phi, &s->rhs, &s->xflux, &s->yflux};
+    *ptr = f[which]->p;
     *pitch = s->geo.g.pitch;
     return MGP_OK;
 }

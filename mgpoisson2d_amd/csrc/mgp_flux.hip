@@ -194,7 +194,10 @@ constexpr int FLUX_TX = 64, FLUX_TY = 4;
 
 // xflux = Jxx dphi/dxi + Jxy dphi/deta on x-faces, yflux = Jyx dphi/dxi +
 // Jyy dphi/deta on y-faces, then Neumann faces take their BC flux.  phi's
-// ghosts must already hold fill_ghosts(do_neum = .false.).
+// ghosts must already hold fill_ghosts(do_neum = .false.).  SUB: the fluxes
+// are subtracted from xflux/yflux instead (F - J grad phi, the projection's
+// correction, test.f90:2990-2993).
+template <bool SUB>
 __global__ void __launch_bounds__(FLUX_TX * FLUX_TY)
 k_grad(double *__restrict__ xflux, double *__restrict__ yflux, const double *__restrict__ phi, Grid g, Geo geo,
        Bc bc, int homog)
@@ -214,7 +217,7 @@ k_grad(double *__restrict__ xflux, double *__restrict__ yflux, const double *__r
             const double along = pd_eta_at_xface(P, g.nx, g.ny, 0.25 / g.dy, i, j);
             f = geo.Jxx[k] * across + geo.Jxy[k] * along;
         }
-        xflux[k] = f;
+        xflux[k] = SUB ? xflux[k] - f : f;
     }
     if (i < g.nx) {
         double f;
@@ -226,7 +229,7 @@ k_grad(double *__restrict__ xflux, double *__restrict__ yflux, const double *__r
             const double across = (1.0 / g.dy) * (P(i, j) - P(i, j - 1));
             f = geo.Jyx[k] * along + geo.Jyy[k] * across;
         }
-        yflux[k] = f;
+        yflux[k] = SUB ? yflux[k] - f : f;
     }
 }
 
@@ -280,8 +283,15 @@ dim3 face_grid(int w, int h) { return dim3((w + FLUX_TX - 1) / FLUX_TX, (h + FLU
 void launch_grad(hipStream_t s, double *xflux, double *yflux, const double *phi, const Grid &g, const Geo &geo,
                  const Bc &bc, int homog)
 {
-    hipLaunchKernelGGL(k_grad, face_grid(g.nx + 1, g.ny + 1), dim3(FLUX_TX, FLUX_TY), 0, s, xflux, yflux, phi, g,
-                       geo, bc, homog);
+    hipLaunchKernelGGL(k_grad<false>, face_grid(g.nx + 1, g.ny + 1), dim3(FLUX_TX, FLUX_TY), 0, s, xflux, yflux,
+                       phi, g, geo, bc, homog);
+}
+
+void launch_grad_subtract(hipStream_t s, double *xflux, double *yflux, const double *phi, const Grid &g,
+                          const Geo &geo, const Bc &bc, int homog)
+{
+    hipLaunchKernelGGL(k_grad<true>, face_grid(g.nx + 1, g.ny + 1), dim3(FLUX_TX, FLUX_TY), 0, s, xflux, yflux,
+                       phi, g, geo, bc, homog);
 }
 
 void launch_pd(hipStream_t s, double *pd, const double *phi, const Grid &g, int dir, int nodedir)

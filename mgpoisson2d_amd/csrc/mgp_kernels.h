@@ -103,6 +103,9 @@ void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sw
 // xflux, yflux = J grad phi with Neumann faces from the BCs; phi's ghosts filled first.
 void launch_grad(hipStream_t s, double *xflux, double *yflux, const double *phi, const Grid &g, const Geo &geo,
                  const Bc &bc, int homog);
+// xflux -= J grad phi, yflux -= J grad phi (same face values as launch_grad).
+void launch_grad_subtract(hipStream_t s, double *xflux, double *yflux, const double *phi, const Grid &g,
+                          const Geo &geo, const Bc &bc, int homog);
 // One compute_pd field (nodedir 1: x-faces, 2: y-faces; dir: derivative direction).
 void launch_pd(hipStream_t s, double *pd, const double *phi, const Grid &g, int dir, int nodedir);
 void launch_div(hipStream_t s, double *div, const double *xflux, const double *yflux, const Grid &g);

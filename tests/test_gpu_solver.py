@@ -170,3 +170,75 @@ def test_xline_relax_matches_dense(case):
     np.testing.assert_allclose(got[1:-1, 1:-1], want, rtol=0, atol=1e-10 * scale)
     # ghost cells are left as they were
     np.testing.assert_array_equal(got[0, 1:-1], phi[0, 1:-1])
+
+
+def projection_fluxes(geo):
+    """Smooth face fluxes with zero net boundary flux (the compatibility the
+    all-Neumann solve needs), corrected as the reference's projection test
+    does: xflux -= (net flux) x (test.f90:2905-2913)."""
+    nx, ny, dx, dy = geo.nx, geo.ny, geo.dx, geo.dy
+    xn, yc = np.arange(nx + 1) * dx, (np.arange(ny) + 0.5) * dy
+    xc, yn = (np.arange(nx) + 0.5) * dx, np.arange(ny + 1) * dy
+    xf = np.sin(3 * xn)[None, :] * np.cos(2 * yc)[:, None] + 0.3 * np.cos(5 * yc)[:, None] * xn[None, :] ** 2
+    yf = np.cos(4 * xc)[None, :] * np.sin(yn)[:, None] - 0.2 * yn[:, None] * np.sin(2 * xc)[None, :]
+    net = np.sum(xf[:, -1] - xf[:, 0]) * dy + np.sum(yf[-1, :] - yf[0, :]) * dx
+    xf = xf - net * xn[None, :]
+    from mgpoisson2d_amd import geometry as G
+    bc = G.BoundaryConditions((0, 0, 0, 0), (1, 1, 1, 1),
+                              (xf[:, 0].copy(), xf[:, -1].copy(), yf[0, :].copy(), yf[-1, :].copy()))
+    return xf, yf, bc
+
+
+@pytest.mark.parametrize("amp,beta", [(0.0, 0.0), (0.1, 3.0)])
+def test_projection_reference_workflow(amp, beta):
+    """The reference's divergence cleaning (test.f90:2817-3024): all-Neumann
+    BCs from the flux itself, fmg(tol 1e-30, 10 cycles, smoothing 8/8/8),
+    F -= J grad phi.  Against the oracle composing compute_div, fmg,
+    compute_grad: max|div| before is exact, the cleaned fluxes agree to 1e-12
+    of their scale (the cycles' norms are summed in parallel)."""
+    from mgpoisson2d_amd import geometry as G
+    geo = G.mapped_geometry(64, 64, amp, beta)
+    xf, yf, bc = projection_fluxes(geo)
+    o = rc.oracle()
+    div0 = o.compute_div(64, 64, geo.dx, geo.dy, xf, yf)
+    phi, _, _, it, _ = o.vcycle(2, geo, bc, 1, 1e-30, 10, np.zeros((66, 66)), div0, numcycles=1, sdown=8, sup=8,
+                                sbottom=8, zerophi=True)
+    _, gx, gy = o.compute_grad(geo, bc, 1, phi)
+    want_x, want_y = xf - gx, yf - gy
+    div1 = o.compute_div(64, 64, geo.dx, geo.dy, want_x, want_y)
+
+    s = mg.Solver(geo, bc, method=mg.METHOD_FMG, ordering=mg.ORDER_LEX, smooth_down=8, smooth_up=8,
+                  smooth_bottom=8, homog=True)
+    s.set_flux(xf, yf)
+    d0, d1, iters, _ = s.project(1e-30, 10)
+    assert iters == it
+    assert d0 == np.max(np.abs(div0))
+    gxf, gyf = s.get_flux()
+    scale = max(np.max(np.abs(xf)), np.max(np.abs(yf)))
+    np.testing.assert_allclose(gxf, want_x, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(gyf, want_y, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(d1, np.max(np.abs(div1)), rtol=1e-6, atol=1e-12 * d0)
+    assert d1 < 1e-4 * d0
+
+
+def test_projection_fast_solver():
+    """KRYLOV_MG projection on a stretched grid: J div of the cleaned flux is
+    the solve's residual, so its 2-norm relative to J div F is within the
+    tolerance; recomputed on the host with the oracle's compute_div."""
+    from mgpoisson2d_amd import geometry as G
+    geo = G.mapped_geometry(256, 192, 0.1, 4.0)
+    xf, yf, bc = projection_fluxes(geo)
+    s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
+    s.set_flux(xf, yf)
+    d0, d1, iters, outcome = s.project(1e-10, 200)
+    assert outcome == mg.CONVERGED, iters
+    o = rc.oracle()
+    div0 = o.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, xf, yf)
+    gxf, gyf = s.get_flux()
+    div1 = o.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, gxf, gyf)
+    assert d0 == np.max(np.abs(div0))
+    assert d1 == np.max(np.abs(div1))
+    assert np.linalg.norm(div1) <= 2e-10 * np.linalg.norm(div0)
+    # Neumann faces keep the flux (the correction's boundary fluxes are zero)
+    np.testing.assert_array_equal(gxf[:, [0, -1]], xf[:, [0, -1]])
+    np.testing.assert_array_equal(gyf[[0, -1], :], yf[[0, -1], :])
