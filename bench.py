@@ -192,7 +192,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "avg_launch_ms": k_ms,
-            "timing": "HIP events around each launch after a 512 MiB cache flush",
+            "timing": "HIP events around each launch after a 512 MiB read-only cache flush",
             "warm_avg_launch_ms": kw_ms,
             "alg_bytes_per_launch": k_bytes,
         },

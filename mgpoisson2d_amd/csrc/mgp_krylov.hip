@@ -263,6 +263,27 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     return outcome;
 }
 
+namespace {
+
+// Cache flush for cold timing: streams `n` doubles through every L2 and the
+// MALL with loads only, so the timed launch starts with cold caches and no
+// dirty lines left to write back.  One double per workgroup is stored.
+__global__ void __launch_bounds__(256) k_flush_read(const double *__restrict__ src, size_t n,
+                                                     double *__restrict__ sink) {
+    double acc = 0.0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) acc += src[k];
+    __shared__ double part[256];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < 256; ++q) t += part[q];
+        sink[blockIdx.x] = t;
+    }
+}
+
+}  // namespace
+
 double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, double *bytes) {
     if (!built) {
         g_last_error = "solver not built";
@@ -309,12 +330,17 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, doub
         hip_check(hipEventElapsedTime(&ms, a, b), "event");
         total = ms;
     } else {
-        // each launch timed alone after 512 MiB of writes have evicted the
-        // operands from the L2s and the MALL, as inside a solve
-        DevBuf flush;
+        // each launch timed alone after 512 MiB of loads have evicted the
+        // operands from the L2s and the 256 MB MALL (clean lines only, so the
+        // timed launch pays no write-back of the flush)
+        constexpr int flush_blocks = 2048;
+        DevBuf flush, sink;
         flush.alloc((size_t)64 << 20);
+        sink.alloc(flush_blocks);
+        hip_check(hipMemsetAsync(flush.p, 0, flush.n * sizeof(double), s), "flush");
+        k_flush_read<<<flush_blocks, 256, 0, s>>>(flush.p, flush.n, sink.p);
         for (int k = 0; k < reps; ++k) {
-            hip_check(hipMemsetAsync(flush.p, k & 0xff, flush.n * sizeof(double), s), "flush");
+            k_flush_read<<<flush_blocks, 256, 0, s>>>(flush.p, flush.n, sink.p);
             hip_check(hipEventRecord(a, s), "event");
             launch();
             hip_check(hipEventRecord(b, s), "event");

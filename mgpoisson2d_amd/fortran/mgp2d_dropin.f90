@@ -664,6 +664,28 @@ module Poisson2D
             real(c_double), intent(in)  :: xflux(*), yflux(*)
             real(c_double)              :: div(*)
         end function
+        integer(c_int) function c_jacobi(geo, bc, homog, invd, omega, tol, maxiters, zerophi, rhs, phi, ordering, &
+                                         relres, iters) bind(C, name="mgp_relax_jacobi")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog, maxiters, zerophi, ordering
+            real(c_double), value       :: omega, tol
+            real(c_double), intent(in)  :: invd(*), rhs(*)
+            real(c_double)              :: phi(*), relres(*)
+            integer(c_int)              :: iters
+        end function
+        integer(c_int) function c_bicgstab1(geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi, &
+                                            ordering, relres, last) bind(C, name="mgp_solve_bicgstab")
+            import :: mgp_geo_c, mgp_bc_c, c_int, c_double
+            type(mgp_geo_c), intent(in) :: geo
+            type(mgp_bc_c), intent(in)  :: bc
+            integer(c_int), value       :: homog, maxiters, maxrestarts, zerophi, ordering
+            real(c_double), value       :: tol
+            real(c_double), intent(in)  :: invd(*), rhs(*)
+            real(c_double)              :: phi(*), relres(*)
+            integer(c_int)              :: last
+        end function
         integer(c_int) function c_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, &
                                             rhs, phi, ordering) bind(C, name="mgp_solve_bicgstab2")
             import :: mgp_geo_c, mgp_bc_c, c_int, c_double
@@ -884,6 +906,94 @@ contains
         enddo
         call undefine_box_data (r)
     end subroutine relax_gs
+
+    ! poisson2d.f90:1358 -- verbosity >= 3 prints the relres history as the
+    ! reference does; the optional ordering selects the reduction order.
+    subroutine relax_jacobi (phi, rhs, geo, bc, homog, invdiags, omega, tol, maxiters, zerophi, verbosity, &
+                             ordering)
+        type(box_data), intent(inout)   :: phi
+        type(box_data), intent(in)      :: rhs
+        type(geo_data), intent(in)      :: geo
+        type(bdry_data), intent(in)     :: bc
+        logical, intent(in)             :: homog
+        type(box_data), intent(in)      :: invdiags
+        real(dp), intent(in)            :: omega, tol
+        integer, intent(in)             :: maxiters
+        logical, intent(in)             :: zerophi
+        integer, intent(in)             :: verbosity
+        integer, intent(in), optional   :: ordering
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), f(:,:), iv(:,:)
+        real(dp) :: relres(0:max(maxiters, 0))
+        integer :: ord, n, k
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (maxiters .eq. 0) return
+        call pack_geo (gp, gc, geo, rhs%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (f, rhs)
+        call pack_cc (iv, invdiags)
+        call pack_phi (a, phi)
+        relres = zero
+        n = 0
+        call mgp_check (c_jacobi(gc, bcc, merge(1, 0, homog), iv, omega, tol, maxiters, merge(1, 0, zerophi), f, a, &
+                                 ord, relres, n), 'Poisson2d::relax_jacobi')
+        if (zerophi) phi%data = zero
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+        if (verbosity .ge. 3 .and. tol .gt. zero) then
+            do k = 0, n
+                print*, 'iter ', k, ': rel |res| = ', relres(k)
+            enddo
+            if (relres(n) .le. tol) print*, "Converged."
+        endif
+    end subroutine relax_jacobi
+
+    ! poisson2d.f90:1993 -- the first BiCGStab version; verbosity >= 3 prints
+    ! its relres history.  The optional ordering: ORDER_LEX (the reference's
+    ! order, bit-identical) or ORDER_MC (grid-wide, for large grids).
+    subroutine solve_bicgstab (phi, rhs, geo, bc, homog, invdiags, tol, max_iters, max_restarts, zerophi, &
+                               verbosity, ordering)
+        type(box_data), intent(inout) :: phi
+        type(box_data), intent(in)    :: rhs
+        type(geo_data), intent(in)    :: geo
+        type(bdry_data), intent(in)   :: bc
+        logical, intent(in)           :: homog
+        type(box_data), intent(in)    :: invdiags
+        real(dp), intent(in)          :: tol
+        integer, intent(in)           :: max_iters, max_restarts
+        logical, intent(in)           :: zerophi
+        integer, intent(in)           :: verbosity
+        integer, intent(in), optional :: ordering
+        type(geo_pack) :: gp
+        type(bc_pack) :: bp
+        type(mgp_geo_c) :: gc
+        type(mgp_bc_c) :: bcc
+        real(dp), allocatable :: a(:,:), f(:,:), iv(:,:)
+        real(dp) :: relres(0:max(max_iters, 0) + max(max_restarts, 0))
+        integer :: ord, last, k
+        ord = ORDER_LEX
+        if (present(ordering)) ord = ordering
+        if (max_iters .eq. 0) return
+        call pack_geo (gp, gc, geo, rhs%valid)
+        call pack_bc (bp, bcc, bc)
+        call pack_cc (f, rhs)
+        call pack_cc (iv, invdiags)
+        call pack_phi (a, phi)
+        relres = zero
+        last = 0
+        call mgp_check (c_bicgstab1(gc, bcc, merge(1, 0, homog), iv, tol, max_iters, max_restarts, &
+                                    merge(1, 0, zerophi), f, a, ord, relres, last), 'solve_bicgstab')
+        if (zerophi) phi%data = zero
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
+        if (verbosity .ge. 3) then
+            do k = 0, last
+                print*, 'iter ', k, ': rel |res| = ', relres(k)
+            enddo
+        endif
+    end subroutine solve_bicgstab
 
     ! poisson2d.f90:2253
     subroutine solve_bicgstab2 (phi, rhs, geo, bc, homog, invdiags, tol, max_iters, max_restarts, &

@@ -722,6 +722,77 @@ contains
     end subroutine ref_bicgstab2
 
     ! --------------------------------------------------------------------------
+    ! Poisson2D::relax_jacobi (poisson2d.f90:1358) and Poisson2D::solve_bicgstab
+    ! (poisson2d.f90:1993), verbosity 0: phi in, phi out.
+    ! --------------------------------------------------------------------------
+    subroutine ref_relax_jacobi (nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy, bct, bcm, &
+                                 bd_xlo, bd_xhi, bd_ylo, bd_yhi, homog, invd, omega, tol, maxiters, &
+                                 zerophi, rhs, phi) bind(C, name="ref_relax_jacobi")
+        use Poisson2D, only: relax_jacobi
+        integer(c_int), value       :: nx, ny, homog, maxiters, zerophi
+        real(c_double), value       :: dx, dy, omega, tol
+        real(c_double), intent(in)  :: J(0:nx-1,0:ny-1), Jxx(0:nx,0:ny-1), Jxy(0:nx,0:ny-1)
+        real(c_double), intent(in)  :: Jyx(0:nx-1,0:ny), Jyy(0:nx-1,0:ny)
+        integer(c_int), intent(in)  :: bct(4), bcm(4)
+        type(c_ptr), value          :: bd_xlo, bd_xhi, bd_ylo, bd_yhi
+        real(c_double), intent(in)  :: invd(0:nx-1,0:ny-1), rhs(0:nx-1,0:ny-1)
+        real(c_double), intent(inout) :: phi(-1:nx,-1:ny)
+
+        type(box) :: valid
+        type(geo_data) :: geo
+        type(bdry_data) :: bc
+        type(box_data) :: fphi, frhs, finv
+
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_geo (geo, valid, J, Jxx, Jxy, Jyx, Jyy)
+        call mk_bc (bc, valid, bct, bcm, bd_xlo, bd_xhi, bd_ylo, bd_yhi)
+        call mk_cc1 (fphi, valid, phi)
+        call mk_cc0 (frhs, valid, rhs)
+        call mk_cc0 (finv, valid, invd)
+        call relax_jacobi (fphi, frhs, geo, bc, homog .ne. 0, finv, omega, tol, maxiters, zerophi .ne. 0, 0)
+        phi = fphi%data
+        call undefine_box_data (fphi)
+        call undefine_box_data (frhs)
+        call undefine_box_data (finv)
+        call undefine_bdry_data (bc)
+        call free_geo (geo)
+    end subroutine ref_relax_jacobi
+
+    subroutine ref_bicgstab1 (nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy, bct, bcm, &
+                              bd_xlo, bd_xhi, bd_ylo, bd_yhi, homog, invd, tol, maxiters, &
+                              maxrestarts, zerophi, rhs, phi) bind(C, name="ref_bicgstab1")
+        use Poisson2D, only: solve_bicgstab
+        integer(c_int), value       :: nx, ny, homog, maxiters, maxrestarts, zerophi
+        real(c_double), value       :: dx, dy, tol
+        real(c_double), intent(in)  :: J(0:nx-1,0:ny-1), Jxx(0:nx,0:ny-1), Jxy(0:nx,0:ny-1)
+        real(c_double), intent(in)  :: Jyx(0:nx-1,0:ny), Jyy(0:nx-1,0:ny)
+        integer(c_int), intent(in)  :: bct(4), bcm(4)
+        type(c_ptr), value          :: bd_xlo, bd_xhi, bd_ylo, bd_yhi
+        real(c_double), intent(in)  :: invd(0:nx-1,0:ny-1), rhs(0:nx-1,0:ny-1)
+        real(c_double), intent(inout) :: phi(-1:nx,-1:ny)
+
+        type(box) :: valid
+        type(geo_data) :: geo
+        type(bdry_data) :: bc
+        type(box_data) :: fphi, frhs, finv
+
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_geo (geo, valid, J, Jxx, Jxy, Jyx, Jyy)
+        call mk_bc (bc, valid, bct, bcm, bd_xlo, bd_xhi, bd_ylo, bd_yhi)
+        call mk_cc1 (fphi, valid, phi)
+        call mk_cc0 (frhs, valid, rhs)
+        call mk_cc0 (finv, valid, invd)
+        call solve_bicgstab (fphi, frhs, geo, bc, homog .ne. 0, finv, tol, maxiters, maxrestarts, &
+                             zerophi .ne. 0, 0)
+        phi = fphi%data
+        call undefine_box_data (fphi)
+        call undefine_box_data (frhs)
+        call undefine_box_data (finv)
+        call undefine_bdry_data (bc)
+        call free_geo (geo)
+    end subroutine ref_bicgstab1
+
+    ! --------------------------------------------------------------------------
     ! ArrayUtils::pnorm (arrayutils.f90:753) over the valid box of a ghost-free
     ! cell-centred field.
     ! --------------------------------------------------------------------------

@@ -40,3 +40,29 @@ def coarse_geo(checker_restrict, geo):
     cnx, cny = geo.nx // 2, geo.ny // 2
     arrs = [checker_restrict(k, geo.nx, geo.ny, a, cnx, cny) for k, a in zip((0, 1, 1, 2, 2), geo.arrays())]
     return G.Geometry(cnx, cny, geo.dx * 2, geo.dy * 2, *arrs)
+
+
+def probed_matrix(geo, bc):
+    """Sparse matrix of the reference operator (homogeneous BCs) by probing the
+    oracle with 25 interleaved impulse patterns (the stencil reaches 2 cells)."""
+    import scipy.sparse as sp
+    import refcall as rc
+    o = rc.oracle()
+    nx, ny = geo.nx, geo.ny
+    rows, cols, vals = [], [], []
+    for ci in range(5):
+        for cj in range(5):
+            phi = np.zeros((ny + 2, nx + 2))
+            phi[1 + cj:ny + 1:5, 1 + ci:nx + 1:5] = 1.0
+            lap, _ = o.laplacian(geo, bc, 1, phi)
+            J, I = np.nonzero(lap)
+            d = (I - ci) % 5
+            d = np.where(d > 2, d - 5, d)
+            e = (J - cj) % 5
+            e = np.where(e > 2, e - 5, e)
+            si, sj = I - d, J - e
+            ok = (si >= 0) & (si < nx) & (sj >= 0) & (sj < ny)
+            rows.append(J[ok] * nx + I[ok])
+            cols.append(sj[ok] * nx + si[ok])
+            vals.append(lap[J[ok], I[ok]])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nx * ny,) * 2)

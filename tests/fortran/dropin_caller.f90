@@ -78,6 +78,14 @@ program dropin_caller
     call fmg (phi, rhs, geo, bc, .true., 0, 0, 1.0d30, 1, -1, 1, 2, 2, 2, .false., 0)
     write (u) phi%data(valid%ilo:valid%ihi, valid%jlo:valid%jhi)
 
+    ! single-level solvers: 12 Jacobi sweeps, then solve_bicgstab with restarts
+    phi%data = phi0%data
+    call relax_jacobi (phi, rhs, geo, bc, .true., invd, 0.8_dp, -one, 12, .false., 0)
+    write (u) phi%data(valid%ilo:valid%ihi, valid%jlo:valid%jhi)
+    phi%data = phi0%data
+    call solve_bicgstab (phi, rhs, geo, bc, .true., invd, 1.0d-8, 20, 2, .true., 0)
+    write (u) phi%data(valid%ilo:valid%ihi, valid%jlo:valid%jhi)
+
     ! projection tools with inhomogeneous data: J grad phi, J div of it, one
     ! transverse derivative, then the homogeneous boundary fluxes
     bc%data_xlo = 0.3_dp

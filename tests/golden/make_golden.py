@@ -58,6 +58,9 @@ def main():
             put(p + "dc_invd", dinv)
             put(p + "dc_lex3", ref.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi))
             put(p + "dc_rb3", ref_rb.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi))
+            # single-level solvers (poisson2d.f90:1358-2248)
+            put(p + "jacobi", ref.relax_jacobi(geo, bc, homog, inv, 0.8, 1e-3, 12, 0, rhs, phi)[0])
+            put(p + "bicgstab1", ref.bicgstab1(geo, bc, homog, inv, 1e-8, 20, 2, 0, rhs, phi)[0])
             # flux tools (poisson2d.f90:404-879): J grad phi on the faces, J div of it
             _, xf, yf = ref.compute_grad(geo, bc, homog, phi)
             put(p + "grad_x", xf)

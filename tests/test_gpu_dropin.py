@@ -3,9 +3,10 @@ libMGPoisson interface (tests/fortran/dropin_caller.f90), compiled against the
 reference's own modules and against the drop-in layer over the HIP library;
 both executables run and their outputs agree.
 
-Operators, inverse diagonals, Gauss-Seidel sweeps, restriction and the flux
-tools (compute_grad, compute_div, compute_pd, fill_boundary_fluxes) must match
-bit for bit; the V-cycle and FMG results to 1e-12 (their relres norms are
+Operators, inverse diagonals, Gauss-Seidel sweeps, restriction, the
+single-level solvers (relax_jacobi, solve_bicgstab) and the flux tools
+(compute_grad, compute_div, compute_pd, fill_boundary_fluxes) must match bit
+for bit; the V-cycle and FMG results to 1e-12 (their relres norms are
 summed in parallel on the GPU)."""
 import os
 import subprocess
@@ -28,7 +29,7 @@ def run(exe, out):
 def split(a):
     n2, c2 = N * N, (N // 2) ** 2
     nf = (N + 1) * N
-    sizes = [n2, n2, n2, c2, n2, n2, n2, n2, nf, nf, n2, nf, nf]
+    sizes = [n2, n2, n2, c2, n2, n2, n2, n2, n2, n2, nf, nf, n2, nf, nf]
     parts, k = [], 0
     for s in sizes:
         parts.append(a[k:k + s])
@@ -47,5 +48,5 @@ def test_fortran_caller_dropin_matches_reference(tmp_path):
     for k in (4, 6, 7):  # DC vcycle x4, plain vcycle x1, fmg x1
         np.testing.assert_allclose(g[k], r[k], rtol=0, atol=1e-12 * np.max(np.abs(r[k])))
     np.testing.assert_array_equal(g[5], r[5])  # rhs restored by the DC driver
-    for k in range(8, 13):  # compute_grad x/y, compute_div, compute_pd, fill_boundary_fluxes
+    for k in range(8, 15):  # jacobi, bicgstab v1, compute_grad x/y, compute_div, compute_pd, fill_boundary_fluxes
         np.testing.assert_array_equal(g[k], r[k])

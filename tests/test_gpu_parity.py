@@ -190,6 +190,66 @@ def test_bicgstab2_bottom(case, kind):
     same(p1, p2)
 
 
+@pytest.mark.parametrize("homog", [1, 0])
+@pytest.mark.parametrize("case", OPERATOR_CASES + CYCLE_CASES[:2])
+def test_relax_jacobi(case, homog):
+    """Reference reduction order (LEX): phi and the relres history bit for bit;
+    parallel reductions (MC): phi bit for bit, history to 1e-13."""
+    geo, bc, phi, rhs = setup(*case, homog=homog, seed=22)
+    o = rc.oracle()
+    inv = o.invdiags(geo, bc, homog)
+    for omega, tol, its, zero in ((0.8, -1.0, 5, 0), (0.6, 1e-3, 40, 1), (1.0, 0.5, 10, 0)):
+        want, hist = o.relax_jacobi(geo, bc, homog, inv, omega, tol, its, zero, rhs, phi)
+        got, ghist, n = mg.relax_jacobi(geo, bc, homog, inv, omega, tol, its, zero, rhs, phi, mg.ORDER_LEX)
+        same(got, want)
+        if tol > 0:
+            same(ghist, hist)
+        got, ghist, n = mg.relax_jacobi(geo, bc, homog, inv, omega, tol, its, zero, rhs, phi, mg.ORDER_MC)
+        same(got, want)
+        if tol > 0:
+            np.testing.assert_allclose(ghist, hist, rtol=1e-13, atol=0)
+        if rc.reference_available():
+            same(want, rc.reference().relax_jacobi(geo, bc, homog, inv, omega, tol, its, zero, rhs, phi)[0])
+
+
+@pytest.mark.parametrize("homog", [1, 0])
+@pytest.mark.parametrize("case", OPERATOR_CASES + CYCLE_CASES[:2])
+def test_solve_bicgstab_v1(case, homog):
+    """LEX: the reference's preconditioner and reduction order, bit for bit
+    (restarts and the divergence undo included).  MC: multi-colour sweeps and
+    parallel reductions against the oracle's MC run, 1e-9 over 6 iterations."""
+    geo, bc, phi, rhs = setup(*case, homog=homog, seed=23)
+    o = rc.oracle()
+    inv = o.invdiags(geo, bc, homog)
+    for tol, its, restarts, zero in ((1e-8, 30, 3, 1), (1e-3, 10, 0, 0), (1e-14, 60, 5, 1)):
+        want, hist = o.bicgstab1(geo, bc, homog, inv, tol, its, restarts, zero, rhs, phi)
+        got, ghist = mg.solve_bicgstab(geo, bc, homog, inv, tol, its, restarts, zero, rhs, phi, mg.ORDER_LEX)
+        same(got, want)
+        same(ghist, hist)
+        for c in checkers()[1:]:
+            same(got, c.bicgstab1(geo, bc, homog, inv, tol, its, restarts, zero, rhs, phi)[0])
+    want, hist = o.bicgstab1(geo, bc, homog, inv, 1e-30, 6, 0, 1, rhs, phi, order=rc.MC)
+    got, ghist = mg.solve_bicgstab(geo, bc, homog, inv, 1e-30, 6, 0, 1, rhs, phi, mg.ORDER_MC)
+    assert len(ghist) == len(hist)
+    np.testing.assert_allclose(ghist, hist, rtol=1e-9, atol=0)
+    scale = np.max(np.abs(want))
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * scale)
+
+
+@pytest.mark.parametrize("n", [(128, 96), (1024, 768)])
+def test_bicgstab2_grid_wide(n):
+    """solve_bicgstab2 with the multi-colour preconditioner on grid-wide
+    kernels (the full-size path) against the oracle's MC run, 4 iterations,
+    1e-10 of the solution's scale."""
+    geo, bc, phi, rhs = setup(n[0], n[1], 0.1, 3.0, (0, 0, 0, 0), seed=24)
+    o = rc.oracle()
+    inv = o.invdiags(geo, bc, 1)
+    want = o.bicgstab2(0, geo, bc, 1, inv, 1e-30, 4, 0, 1, rhs, phi, order=rc.MC)
+    got = mg.solve_bicgstab2(0, geo, bc, 1, inv, 1e-30, 4, 0, 1, rhs, phi, ordering=mg.ORDER_MC)
+    scale = np.max(np.abs(want))
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * scale)
+
+
 @pytest.mark.parametrize("case", CYCLE_CASES)
 @pytest.mark.parametrize("kind,ordering", [(0, 0), (0, 2), (1, 0), (1, 1), (2, 0), (2, 2)])
 def test_vcycle_history(case, kind, ordering):

@@ -66,6 +66,10 @@ def test_oracle_matches_golden_operators(golden, k, homog):
     np.testing.assert_array_equal(dinv, golden[p + "dc_invd"])
     np.testing.assert_array_equal(o.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi, rc.LEX), golden[p + "dc_lex3"])
     np.testing.assert_array_equal(o.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi, rc.RB), golden[p + "dc_rb3"])
+    np.testing.assert_array_equal(o.relax_jacobi(geo, bc, homog, inv, 0.8, 1e-3, 12, 0, rhs, phi)[0],
+                                  golden[p + "jacobi"])
+    np.testing.assert_array_equal(o.bicgstab1(geo, bc, homog, inv, 1e-8, 20, 2, 0, rhs, phi)[0],
+                                  golden[p + "bicgstab1"])
     _, xf, yf = o.compute_grad(geo, bc, homog, phi)
     np.testing.assert_array_equal(xf, golden[p + "grad_x"])
     np.testing.assert_array_equal(yf, golden[p + "grad_y"])
@@ -138,6 +142,24 @@ def test_oracle_vs_reference_flux_tools(case, homog, seed):
         np.testing.assert_array_equal(x, y)
     np.testing.assert_array_equal(o.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, a[1], a[2]),
                                   r.compute_div(geo.nx, geo.ny, geo.dx, geo.dy, a[1], a[2]))
+
+
+@needs_ref
+@pytest.mark.parametrize("homog", [1, 0])
+@pytest.mark.parametrize("case", OPERATOR_CASES + CYCLE_CASES[:2])
+def test_oracle_vs_reference_single_level_solvers(case, homog):
+    """relax_jacobi with and without residual monitoring, solve_bicgstab with
+    restarts, early convergence and the divergence undo, from a given phi and
+    from zero."""
+    o, r = rc.oracle(), rc.reference()
+    geo, bc, phi, rhs = setup(*case, homog=homog, seed=21)
+    inv = o.invdiags(geo, bc, homog)
+    for omega, tol, its, zero in ((0.8, -1.0, 5, 0), (0.6, 1e-3, 40, 1), (1.0, 0.5, 10, 0)):
+        np.testing.assert_array_equal(o.relax_jacobi(geo, bc, homog, inv, omega, tol, its, zero, rhs, phi)[0],
+                                      r.relax_jacobi(geo, bc, homog, inv, omega, tol, its, zero, rhs, phi)[0])
+    for tol, its, restarts, zero in ((1e-8, 30, 3, 1), (1e-3, 10, 0, 0), (1e-14, 60, 5, 1)):
+        np.testing.assert_array_equal(o.bicgstab1(geo, bc, homog, inv, tol, its, restarts, zero, rhs, phi)[0],
+                                      r.bicgstab1(geo, bc, homog, inv, tol, its, restarts, zero, rhs, phi)[0])
 
 
 @needs_ref
@@ -327,6 +349,36 @@ def test_div_of_linear_fluxes_and_grad_of_quadratics():
     _, hx, hy = o.compute_grad(geo, bc, 1, phi.copy())
     np.testing.assert_array_equal(hx[:, -1], 0.0)
     np.testing.assert_array_equal(hy[-1, :], 0.0)
+
+
+def test_jacobi_first_sweep_and_convergence():
+    """From phi = 0 one sweep with omega = 1 gives phi = rhs * invdiags exactly
+    (the residual of zero is rhs); on an all-Neumann problem (where the
+    diagonal carries the whole ghost coupling) the sweeps converge to a
+    solution of L phi = rhs, checked with the golden-pinned operator."""
+    o = rc.oracle()
+    geo, bc, phi, rhs = setup(16, 12, 0.1, 2.0, (0, 0, 0, 0), seed=3)
+    inv = o.invdiags(geo, bc, 1)
+    one, hist = o.relax_jacobi(geo, bc, 1, inv, 1.0, 1e-30, 1, 1, rhs, phi)
+    np.testing.assert_array_equal(one[1:-1, 1:-1], rhs * inv)
+    sol, hist = o.relax_jacobi(geo, bc, 1, inv, 0.8, 1e-10, 20000, 1, rhs, phi)
+    assert hist[-1] <= 1e-10 and len(hist) < 20000
+    res, _ = o.residual(geo, bc, 1, rhs, sol)
+    assert np.linalg.norm(res) <= 2e-10 * np.linalg.norm(rhs)
+
+
+def test_bicgstab1_reaches_the_direct_solution():
+    """solve_bicgstab on the identity map with Dirichlet sides converges to
+    the sparse direct solve of the probed operator."""
+    import scipy.sparse.linalg as spl
+    from cases import probed_matrix
+    o = rc.oracle()
+    geo, bc, phi, rhs = setup(16, 12, 0.0, 0.0, (1, 1, 1, 1), seed=3)
+    inv = o.invdiags(geo, bc, 1)
+    sol, hist = o.bicgstab1(geo, bc, 1, inv, 1e-13, 200, 10, 1, rhs, phi)
+    assert hist[-1] <= 1e-13
+    exact = spl.spsolve(probed_matrix(geo, bc).tocsc(), rhs.ravel()).reshape(geo.ny, geo.nx)
+    np.testing.assert_allclose(sol[1:-1, 1:-1], exact, rtol=0, atol=1e-10 * np.max(np.abs(exact)))
 
 
 def test_dirichlet_order3_ghost_exact_for_quadratics():
