@@ -35,7 +35,8 @@ typedef enum {
     MGP_ERR_ARG = 1,          /* bad shape / argument (reference: print* + stop) */
     MGP_ERR_UNSUPPORTED = 2,  /* e.g. BCTYPE_CF, non-2x2 prolongation order 2 */
     MGP_ERR_HIP = 3,          /* HIP runtime failure */
-    MGP_ERR_NOMEM = 4         /* device allocation failed (reference: "Out of memory") */
+    MGP_ERR_NOMEM = 4,        /* device allocation failed (reference: "Out of memory") */
+    MGP_ERR_BREAKDOWN = 5     /* Krylov breakdown where the reference stops (print* + stop) */
 } mgp_status;
 
 /* Outcome of an iterative solve. */
@@ -61,6 +62,19 @@ typedef struct {
     int mode[4];          /* BCMODE_UNIFORM / NONUNIFORM / LAH */
     const double *data[4];/* 1 value (UNIFORM, LAH) or ny (x sides) / nx (y sides) */
 } mgp_bc;
+
+/* A mapping whose metrics the library generates on the GPU (no counterpart in the
+ * reference's library; MGP_MAP_TERRAIN is its test geometry).  L, H: domain extents,
+ * <= 0 for nx*dx and ny*dy. */
+enum { MGP_MAP_TERRAIN = 0, MGP_MAP_WARPED = 1 };
+typedef struct {
+    int kind;     /* MGP_MAP_TERRAIN: test.f90:320-716, x = xi, y = el + (H - el) eta/H over the bottom
+                     el = amp sin^2(pi x / L) (amp 1/4 there), metrics by differencing point positions;
+                     MGP_MAP_WARPED: x = s + amp sin(2 pi s) sin(2 pi t), y = (e^(beta t) - 1)/(e^beta - 1),
+                     s = xi/L, t = eta/H, exact derivatives */
+    double amp, beta;
+    double L, H;
+} mgp_mapping;
 
 /* ---- 2. Drop-in entry points (host arrays) ------------------------------ */
 
@@ -124,7 +138,32 @@ int mgp_compute_div(int nx, int ny, double dx, double dy, const double *xflux, c
 /* arrayutils.f90:1571 fill_boundary_fluxes(xflux, yflux, bc, homog): Neumann faces take the BC
  * flux (zero when homogeneous or BCMODE_LAH); other faces are left as they are. */
 int mgp_fill_boundary_fluxes(int nx, int ny, const mgp_bc *bc, int homog, double *xflux, double *yflux);
-/* poisson2d.f90:2253 (kind 0) / MGPoisson2D_DD.f90:339 (kind 1) solve_bicgstab2 */
+/* poisson2d.f90:1358 relax_jacobi(phi, rhs, geo, bc, homog, invdiags, omega, tol, maxiters, zerophi,
+ * verbosity).  relres (maxiters+1 doubles, may be NULL) receives the history the reference prints
+ * at verbosity >= 3 (written when tol > 0); *iters the sweeps made.  ordering MGP_ORDER_LEX sums
+ * the norms in the reference's order (bit-identical stopping decisions); MGP_ORDER_MC uses the
+ * parallel reductions (large grids). */
+int mgp_relax_jacobi(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega,
+                     double tol, int maxiters, int zerophi, const double *rhs, double *phi, int ordering,
+                     double *relres, int *iters);
+/* poisson2d.f90:1993 solve_bicgstab(phi, rhs, geo, bc, homog, invdiags, tol, max_iters, max_restarts,
+ * zerophi, verbosity), the first BiCGStab version (relax_gs-preconditioned, max-norm residuals,
+ * restarts on hanging or divergence).  ordering MGP_ORDER_LEX: the reference's preconditioner order
+ * and reduction order, bit-identical iterates; MGP_ORDER_MC: multi-colour preconditioner and
+ * parallel reductions over the whole grid.  relres (maxiters+maxrestarts+1 doubles, may be NULL)
+ * receives the history the reference prints at verbosity >= 3, entries 0..*last (restarts take an
+ * entry).  Returns MGP_ERR_BREAKDOWN where the reference stops on beta == 0. */
+int mgp_solve_bicgstab(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double tol,
+                       int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi, int ordering,
+                       double *relres, int *last);
+/* test.f90:320 define_geometry (MGP_MAP_TERRAIN) or the analytic MGP_MAP_WARPED metrics, computed
+ * on the GPU: J (nx x ny), Jxx, Jxy ((nx+1) x ny), Jyx, Jyy (nx x (ny+1)) in the mgp_geo layouts. */
+int mgp_metric_fill(const mgp_mapping *m, int nx, int ny, double dx, double dy, double *J, double *Jxx,
+                    double *Jxy, double *Jyx, double *Jyy);
+/* poisson2d.f90:2253 (kind 0) / MGPoisson2D_DD.f90:339 (kind 1) solve_bicgstab2.  Kind 0 with
+ * ordering MGP_ORDER_MC runs grid-wide kernels with parallel reductions (full-size grids); every
+ * other combination runs the bottom-solver kernel (one workgroup, the reference's reduction order,
+ * bit-identical). */
 int mgp_solve_bicgstab2(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd,
                         double tol, int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
                         int ordering);
@@ -165,6 +204,10 @@ typedef struct {
 void mgp_solver_default_opts(mgp_solver_opts *opts);
 int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts);
 int mgp_solver_destroy(mgp_solver *s);
+/* As mgp_solver_create, with the metrics generated on the GPU from a mapping (nothing of the
+ * geometry crosses PCIe; removes host metric assembly from set-up). */
+int mgp_solver_create_mapped(mgp_solver **s, const mgp_mapping *m, int nx, int ny, double dx, double dy,
+                             const mgp_bc *bc, const mgp_solver_opts *opts);
 /* Upload / download in the host layouts above (phi: one ghost layer). */
 int mgp_solver_set_rhs(mgp_solver *s, const double *rhs);
 int mgp_solver_set_phi(mgp_solver *s, const double *phi);
@@ -196,11 +239,12 @@ int mgp_solver_last_cycles(const mgp_solver *s);
 /* Measurement hook (KRYLOV_MG): times `reps` launches of one finest-level
  * kernel with HIP events on the solver's stream.  kernel 0: x-line
  * relaxation of the even interior rows; 1: residual; 2: plain copy of one
- * finest-level field (bandwidth reference).  cold != 0: each launch is timed
- * alone after 512 MiB of writes have flushed the L2s and the MALL (as the
- * kernels run inside a solve); cold == 0: back-to-back launches on
- * cache-resident operands.  *avg_ms is the mean duration per launch, *bytes
- * the algorithmic bytes one launch moves. */
+ * finest-level field (bandwidth reference).  cold != 0: back-to-back launches
+ * that take four copies of the operands in turn, so that each launch finds its
+ * operands evicted from the L2s and the MALL (as the kernels run inside a
+ * solve); cold == 0: back-to-back launches on cache-resident operands.
+ * *avg_ms is the mean duration per launch, *bytes the algorithmic bytes one
+ * launch moves. */
 int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double *avg_ms, double *bytes);
 /* hipDeviceSynchronize on the current device. */
 int mgp_device_synchronize(void);

@@ -19,6 +19,7 @@ MGP_OK = 0
 CONVERGED, DIVERGED, MAXITERS = 0, 1, 2
 ORDER_LEX, ORDER_RB, ORDER_MC = 0, 1, 2
 METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG, METHOD_FMG = 0, 1, 2, 3
+MAP_TERRAIN, MAP_WARPED = 0, 1
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -37,6 +38,12 @@ class _Geo(ctypes.Structure):
 
 class _Bc(ctypes.Structure):
     _fields_ = [("type", _i * 4), ("mode", _i * 4), ("data", _dp * 4)]
+
+
+class Mapping(ctypes.Structure):
+    """mgp_mapping: MAP_TERRAIN (the reference's test geometry) or MAP_WARPED;
+    L, H <= 0 mean nx*dx and ny*dy."""
+    _fields_ = [("kind", _i), ("amp", _d), ("beta", _d), ("L", _d), ("H", _d)]
 
 
 class SolverOpts(ctypes.Structure):
@@ -67,6 +74,7 @@ EXPORTS = (
     "mgp_solver_field", "mgp_solver_stream", "mgp_solver_last_cycles", "mgp_solver_time_kernel",
     "mgp_device_synchronize", "mgp_xline_relax", "mgp_compute_pd", "mgp_compute_grad", "mgp_compute_div",
     "mgp_fill_boundary_fluxes", "mgp_solver_set_flux", "mgp_solver_get_flux", "mgp_solver_project",
+    "mgp_relax_jacobi", "mgp_solve_bicgstab", "mgp_metric_fill", "mgp_solver_create_mapped",
 )
 
 
@@ -216,6 +224,14 @@ def fill_boundary_fluxes(nx, ny, bc, homog, xflux, yflux):
     return xf, yf
 
 
+def metric_fill(mapping, nx, ny, dx, dy):
+    """Metrics of a mapping computed on the GPU: returns a Geometry."""
+    arrs = [np.zeros((ny, nx)), np.zeros((ny, nx + 1)), np.zeros((ny, nx + 1)), np.zeros((ny + 1, nx)),
+            np.zeros((ny + 1, nx))]
+    _check(lib.mgp_metric_fill(ctypes.byref(mapping), _i(nx), _i(ny), _d(dx), _d(dy), *[_ptr(a) for a in arrs]))
+    return Geometry(nx, ny, dx, dy, *arrs)
+
+
 def restrict(kind, fnx, fny, fine, cnx, cny):
     shape = {0: (cny, cnx), 1: (cny, cnx + 1), 2: (cny + 1, cnx)}[kind]
     crse = np.zeros(shape)
@@ -247,6 +263,31 @@ def pnorm(nx, ny, dx, dy, arr, p=2):
     arr = _f64(arr)
     _check(lib.mgp_pnorm(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(arr), _i(p), ctypes.byref(out)))
     return out.value
+
+
+def relax_jacobi(geo, bc, homog, invd, omega, tol, maxiters, zerophi, rhs, phi, ordering=ORDER_LEX):
+    """Poisson2D::relax_jacobi: returns (phi, relres history (when tol > 0), sweeps)."""
+    a = _Args()
+    phi = _f64(phi).copy()
+    invd, rhs = _f64(invd), _f64(rhs)
+    relres = np.zeros(max(maxiters, 0) + 1)
+    n = _i(0)
+    _check(lib.mgp_relax_jacobi(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _d(tol), _i(maxiters),
+                                _i(zerophi), _ptr(rhs), _ptr(phi), _i(ordering), _ptr(relres), ctypes.byref(n)))
+    return phi, relres[: n.value + 1], n.value
+
+
+def solve_bicgstab(geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi, ordering=ORDER_LEX):
+    """Poisson2D::solve_bicgstab (version 1): returns (phi, relres history)."""
+    a = _Args()
+    phi = _f64(phi).copy()
+    invd, rhs = _f64(invd), _f64(rhs)
+    relres = np.zeros(maxiters + maxrestarts + 1)
+    last = _i(0)
+    _check(lib.mgp_solve_bicgstab(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(tol), _i(maxiters),
+                                  _i(maxrestarts), _i(zerophi), _ptr(rhs), _ptr(phi), _i(ordering), _ptr(relres),
+                                  ctypes.byref(last)))
+    return phi, relres[: last.value + 1]
 
 
 def solve_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi,
@@ -281,13 +322,19 @@ class Solver:
     the device between solves."""
 
     def __init__(self, geo, bc, method=METHOD_KRYLOV_MG, ordering=ORDER_MC, numcycles=1, smooth_down=2,
-                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True, use_graph=True):
+                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True, use_graph=True, mapping=None):
+        """geo: a Geometry (metrics uploaded), or with mapping given, any object
+        with nx, ny, dx, dy (metrics generated on the GPU)."""
         self.geo = geo
         opts = SolverOpts(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth,
                           int(homog), int(use_graph))
         a = _Args()
         self._h = ctypes.c_void_p()
-        _check(lib.mgp_solver_create(ctypes.byref(self._h), a.geo(geo), a.bc(bc), ctypes.byref(opts)))
+        if mapping is None:
+            _check(lib.mgp_solver_create(ctypes.byref(self._h), a.geo(geo), a.bc(bc), ctypes.byref(opts)))
+        else:
+            _check(lib.mgp_solver_create_mapped(ctypes.byref(self._h), ctypes.byref(mapping), _i(geo.nx),
+                                                _i(geo.ny), _d(geo.dx), _d(geo.dy), a.bc(bc), ctypes.byref(opts)))
 
     def close(self):
         if self._h:

@@ -11,9 +11,11 @@
 #include <vector>
 
 #include "mgp2d.h"
+#include "mgp_geometry.h"
 #include "mgp_host.h"
 #include "mgp_kernels.h"
 #include "mgp_krylov.h"
+#include "mgp_single.h"
 
 using namespace mgp;
 
@@ -116,6 +118,7 @@ const char *mgp_status_string(int status) {
     case MGP_ERR_UNSUPPORTED: return "unsupported";
     case MGP_ERR_HIP: return "HIP runtime error";
     case MGP_ERR_NOMEM: return "out of device memory";
+    case MGP_ERR_BREAKDOWN: return "solver breakdown";
     default: return "unknown status";
     }
 }
@@ -482,12 +485,96 @@ int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, doub
     });
 }
 
+namespace {
+
+void single_ordering(const mgp_bc *bc, int ordering) {
+    if (ordering != MGP_ORDER_LEX && ordering != MGP_ORDER_MC) {
+        g_last_error = "ordering must be LEX or MC";
+        throw Failure(MGP_ERR_ARG);
+    }
+    if (ordering == MGP_ORDER_MC)
+        for (int k = 0; k < 4; ++k)
+            if (bc->type[k] == BC_PERIODIC) {
+                g_last_error = "MC ordering needs non-periodic sides";
+                throw Failure(MGP_ERR_UNSUPPORTED);
+            }
+}
+
+}  // namespace
+
+int mgp_relax_jacobi(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega, double tol,
+                     int maxiters, int zerophi, const double *rhs, double *phi, int ordering, double *relres,
+                     int *iters) {
+    return guarded([&] {
+        no_lah(bc);
+        if (maxiters < 0) {
+            g_last_error = "relax_jacobi: maxiters < 0";
+            throw Failure(MGP_ERR_ARG);
+        }
+        Call c(geo, bc);
+        single_ordering(bc, ordering);
+        const Grid &g = c.geo.g;
+        DevBuf f = c.field(), r = c.field(), iv = c.field();
+        up_phi(c.s, f, g, phi);
+        up_cc(c.s, r, g, rhs);
+        up_cc(c.s, iv, g, invd);
+        SingleLevel S(c.s, g, c.geo.view(), c.bc.bc, ordering);
+        std::vector<double> hist;
+        const int n = S.jacobi(homog, iv.p, omega, tol, maxiters, zerophi, r.p, f.p, hist);
+        down_phi(c.s, phi, f, g);
+        c.finish();
+        if (relres && tol > 0.0)
+            for (int k = 0; k <= n; ++k) relres[k] = hist[k];
+        if (iters) *iters = n;
+    });
+}
+
+int mgp_solve_bicgstab(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double tol,
+                       int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi, int ordering,
+                       double *relres, int *last) {
+    return guarded([&] {
+        no_lah(bc);
+        if (maxiters < 0 || maxrestarts < 0) {
+            g_last_error = "solve_bicgstab: negative iteration limits";
+            throw Failure(MGP_ERR_ARG);
+        }
+        Call c(geo, bc);
+        single_ordering(bc, ordering);
+        const Grid &g = c.geo.g;
+        DevBuf f = c.field(), r = c.field(), iv = c.field();
+        up_phi(c.s, f, g, phi);
+        up_cc(c.s, r, g, rhs);
+        up_cc(c.s, iv, g, invd);
+        SingleLevel S(c.s, g, c.geo.view(), c.bc.bc, ordering);
+        std::vector<double> hist;
+        const int n = S.bicgstab1(homog, iv.p, tol, maxiters, maxrestarts, zerophi, r.p, f.p, hist);
+        down_phi(c.s, phi, f, g);
+        c.finish();
+        if (relres)
+            for (int k = 0; k <= n; ++k) relres[k] = hist[k];
+        if (last) *last = n;
+    });
+}
+
 int mgp_solve_bicgstab2(int kind, const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double tol,
                         int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi, int ordering) {
     return guarded([&] {
         if (kind == 0) no_lah(bc);
         Call c(geo, bc);
         const Grid &g = c.geo.g;
+        if (kind == 0 && ordering == MGP_ORDER_MC) {
+            single_ordering(bc, ordering);
+            DevBuf f = c.field(), r = c.field(), iv = c.field();
+            up_phi(c.s, f, g, phi);
+            up_cc(c.s, r, g, rhs);
+            up_cc(c.s, iv, g, invd);
+            SingleLevel S(c.s, g, c.geo.view(), c.bc.bc, ordering);
+            std::vector<double> norms;
+            S.bicgstab2(homog, iv.p, tol, maxiters, maxrestarts, zerophi, r.p, f.p, norms);
+            down_phi(c.s, phi, f, g);
+            c.finish();
+            return;
+        }
         std::vector<DevBuf> w(13);
         for (auto &b : w) b.alloc(level_elems(g.nx, g.ny));
         up_phi(c.s, w[0], g, phi);
@@ -535,11 +622,15 @@ void mgp_solver_default_opts(mgp_solver_opts *o) {
     o->use_graph = 1;
 }
 
-int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts) {
+extern "C++" {
+namespace {
+
+// Shared body of the two create calls: make_geo(stream) fills s->geo.
+template <class MakeGeo>
+int create_solver(mgp_solver **out, int nx, int ny, const mgp_bc *bc, const mgp_solver_opts *opts, MakeGeo make_geo) {
     *out = nullptr;
     mgp_solver *s = new mgp_solver;
     const int st = guarded([&] {
-        check_geo(geo);
         check_bc(bc);
         if (opts) s->opts = *opts;
         else mgp_solver_default_opts(&s->opts);
@@ -555,9 +646,9 @@ int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, co
             throw Failure(MGP_ERR_ARG);
         }
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
-        s->geo.upload(s->stream, *geo);
-        s->ubc.upload(s->stream, *bc, geo->nx, geo->ny);
-        const size_t n = level_elems(geo->nx, geo->ny);
+        make_geo(s->stream, s->geo);
+        s->ubc.upload(s->stream, *bc, nx, ny);
+        const size_t n = level_elems(nx, ny);
         s->phi.alloc(n);
         s->rhs.alloc(n);
         if (plain || o.method == MGP_METHOD_DC_VCYCLE) {
@@ -577,6 +668,59 @@ int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, co
     }
     *out = s;
     return MGP_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts) {
+    if (!out) return MGP_ERR_ARG;
+    const int st = guarded([&] { check_geo(geo); });
+    if (st != MGP_OK) {
+        *out = nullptr;
+        return st;
+    }
+    return create_solver(out, geo->nx, geo->ny, bc, opts,
+                         [&](hipStream_t stream, DevGeo &dg) { dg.upload(stream, *geo); });
+}
+
+int mgp_solver_create_mapped(mgp_solver **out, const mgp_mapping *m, int nx, int ny, double dx, double dy,
+                             const mgp_bc *bc, const mgp_solver_opts *opts) {
+    if (!out) return MGP_ERR_ARG;
+    const int st = guarded([&] {
+        check_mapping(m);
+        if (nx < 3 || ny < 3 || !(dx > 0) || !(dy > 0)) {
+            g_last_error = "bad grid (need nx, ny >= 3, positive spacing)";
+            throw Failure(MGP_ERR_ARG);
+        }
+    });
+    if (st != MGP_OK) {
+        *out = nullptr;
+        return st;
+    }
+    return create_solver(out, nx, ny, bc, opts,
+                         [&](hipStream_t stream, DevGeo &dg) { dg.generate(stream, *m, nx, ny, dx, dy); });
+}
+
+int mgp_metric_fill(const mgp_mapping *m, int nx, int ny, double dx, double dy, double *J, double *Jxx, double *Jxy,
+                    double *Jyx, double *Jyy) {
+    return guarded([&] {
+        check_mapping(m);
+        if (nx < 1 || ny < 1 || !(dx > 0) || !(dy > 0) || !J || !Jxx || !Jxy || !Jyx || !Jyy) {
+            g_last_error = "metric_fill: bad grid or null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        DevGeo dg;
+        dg.generate(s, *m, nx, ny, dx, dy);
+        const Grid &g = dg.g;
+        download_block(s, J, dg.J.p, g, 0, 0, nx, ny);
+        download_block(s, Jxx, dg.Jxx.p, g, 0, 0, nx + 1, ny);
+        download_block(s, Jxy, dg.Jxy.p, g, 0, 0, nx + 1, ny);
+        download_block(s, Jyx, dg.Jyx.p, g, 0, 0, nx, ny + 1);
+        download_block(s, Jyy, dg.Jyy.p, g, 0, 0, nx, ny + 1);
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
 }
 
 int mgp_solver_destroy(mgp_solver *s) {

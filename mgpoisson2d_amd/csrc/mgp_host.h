@@ -46,6 +46,8 @@ struct DevGeo {
     Grid g;
     DevBuf J, Jxx, Jxy, Jyx, Jyy;
     void upload(hipStream_t s, const mgp_geo &h);
+    // metrics of a mapping generated on the device (mgp_geometry.hip)
+    void generate(hipStream_t s, const mgp_mapping &m, int nx, int ny, double dx, double dy);
     Geo view() const;
 };
 

@@ -13,6 +13,7 @@
 
 #include "mgp2d.h"
 #include "mgp_host.h"
+#include "mgp_geometry.h"
 #include "mgp_kernels.h"
 
 namespace mgp {
@@ -90,6 +91,16 @@ void DevGeo::upload(hipStream_t s, const mgp_geo &h) {
     upload_block(s, Jxy.p, g, h.Jxy, 0, 0, h.nx + 1, h.ny);
     upload_block(s, Jyx.p, g, h.Jyx, 0, 0, h.nx, h.ny + 1);
     upload_block(s, Jyy.p, g, h.Jyy, 0, 0, h.nx, h.ny + 1);
+}
+void DevGeo::generate(hipStream_t s, const mgp_mapping &m, int nx, int ny, double dx, double dy) {
+    g = make_grid(nx, ny, dx, dy);
+    const size_t n = level_elems(nx, ny);
+    J.alloc(n);
+    Jxx.alloc(n);
+    Jxy.alloc(n);
+    Jyx.alloc(n);
+    Jyy.alloc(n);
+    launch_metric(s, m, g, J.p, Jxx.p, Jxy.p, Jyx.p, Jyy.p);
 }
 Geo DevGeo::view() const { return Geo{J.p, Jxx.p, Jxy.p, Jyx.p, Jyy.p}; }
 

@@ -263,27 +263,6 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     return outcome;
 }
 
-namespace {
-
-// Cache flush for cold timing: streams `n` doubles through every L2 and the
-// MALL with loads only, so the timed launch starts with cold caches and no
-// dirty lines left to write back.  One double per workgroup is stored.
-__global__ void __launch_bounds__(256) k_flush_read(const double *__restrict__ src, size_t n,
-                                                     double *__restrict__ sink) {
-    double acc = 0.0;
-    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) acc += src[k];
-    __shared__ double part[256];
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int q = 0; q < 256; ++q) t += part[q];
-        sink[blockIdx.x] = t;
-    }
-}
-
-}  // namespace
-
 double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, double *bytes) {
     if (!built) {
         g_last_error = "solver not built";
@@ -330,25 +309,44 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, doub
         hip_check(hipEventElapsedTime(&ms, a, b), "event");
         total = ms;
     } else {
-        // each launch timed alone after 512 MiB of loads have evicted the
-        // operands from the L2s and the 256 MB MALL (clean lines only, so the
-        // timed launch pays no write-back of the flush)
-        constexpr int flush_blocks = 2048;
-        DevBuf flush, sink;
-        flush.alloc((size_t)64 << 20);
-        sink.alloc(flush_blocks);
-        hip_check(hipMemsetAsync(flush.p, 0, flush.n * sizeof(double), s), "flush");
-        k_flush_read<<<flush_blocks, 256, 0, s>>>(flush.p, flush.n, sink.p);
-        for (int k = 0; k < reps; ++k) {
-            k_flush_read<<<flush_blocks, 256, 0, s>>>(flush.p, flush.n, sink.p);
-            hip_check(hipEventRecord(a, s), "event");
-            launch();
-            hip_check(hipEventRecord(b, s), "event");
-            hip_check(hipEventSynchronize(b), "event");
-            float ms = 0.0f;
-            hip_check(hipEventElapsedTime(&ms, a, b), "event");
-            total += ms;
+        // back to back over four copies of the operands, used in turn: between
+        // two uses of a copy the other three launches move more than 500 MB,
+        // more than the L2s and the 256 MB MALL hold, so every launch starts
+        // on cold caches, and the events around the batch see no per-launch
+        // dispatch gap
+        constexpr int copies = 4;
+        const size_t n = level_elems(g.nx, g.ny), nb = n * sizeof(double);
+        struct Copy {
+            DevBuf e, r, w, J, Jxx, Jxy, Jyx, Jyy;
+        };
+        std::vector<Copy> extra(copies - 1);
+        for (Copy &c : extra) {
+            DevBuf *dst[8] = {&c.e, &c.r, &c.w, &c.J, &c.Jxx, &c.Jxy, &c.Jyx, &c.Jyy};
+            const DevBuf *src[8] = {&l.e, &l.r, &l.w, &l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy};
+            for (int q = 0; q < 8; ++q) {
+                dst[q]->alloc(n);
+                hip_check(hipMemcpyAsync(dst[q]->p, src[q]->p, nb, hipMemcpyDeviceToDevice, s), "copy");
+            }
         }
+        auto launch_copy = [&](int k) {
+            if (k == 0) {
+                launch();
+                return;
+            }
+            Copy &c = extra[k - 1];
+            const Geo geo{c.J.p, c.Jxx.p, c.Jxy.p, c.Jyx.p, c.Jyy.p};
+            if (kernel == 0) launch_xline(s, c.e.p, c.r.p, l.K.p, g, geo, 2, 2, n_even, -1, -1);
+            else if (kernel == 1) launch_laplacian(s, c.w.p, c.e.p, c.r.p, g, geo, l.bc, 1, 0);
+            else launch_lincomb(s, c.w.p, 1.0, c.e.p, 0.0, nullptr, g, 0);
+        };
+        for (int k = 0; k < copies; ++k) launch_copy(k);  // first touch of every copy
+        hip_check(hipEventRecord(a, s), "event");
+        for (int k = 0; k < reps; ++k) launch_copy(k % copies);
+        hip_check(hipEventRecord(b, s), "event");
+        hip_check(hipEventSynchronize(b), "event");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, a, b), "event");
+        total = ms;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -946,6 +946,182 @@ static void bicgstab2(int dc, const Geo *g, const Bc *bc, int homog, const doubl
 }
 
 /* ---------------------------------------------------------------------------
+ * Single-level solvers: Poisson2D::relax_jacobi (poisson2d.f90:1358-1451) and
+ * Poisson2D::solve_bicgstab (:1993-2248).  Work fields the reference defines
+ * like phi carry a ghost ring; the ones it never writes there stay zero (the
+ * reference build allocates zeroed memory).
+ * ------------------------------------------------------------------------- */
+
+/* inner_prod (arrayutils.f90:637-661) over the valid cells of ghosted fields */
+static double ip_ph(const Geo *g, const double *a, const double *b)
+{
+    const double vol = g->dx * g->dy;
+    double ip = 0.0;
+    for (int j = 0; j < g->ny; ++j) {
+        double dot = 0.0;
+        for (int i = 0; i < g->nx; ++i) dot = dot + PH(a, i, j) * PH(b, i, j);
+        ip = ip + vol * dot;
+    }
+    return ip;
+}
+
+/* pnorm(.., 0) (arrayutils.f90:763-768) over the valid cells of a ghosted field */
+static double maxnorm_ph(const Geo *g, const double *a)
+{
+    double pn = 0.0;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const double v = fabs(PH(a, i, j));
+            pn = pn > v ? pn : v;
+        }
+    return pn;
+}
+
+static void valid_to_c0(const Geo *g, const double *ph, double *c0)
+{
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) C0(c0, i, j) = PH(ph, i, j);
+}
+
+static void c0_to_valid(const Geo *g, const double *c0, double *ph)
+{
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) PH(ph, i, j) = C0(c0, i, j);
+}
+
+/* relax_jacobi: returns the sweeps made; relres[0..sweeps] is written when
+ * tol > 0 (2-norm of the residual relative to the initial one). */
+static int relax_jacobi(const Geo *g, const Bc *bc, int homog, const double *invd, double omega, double tol,
+                        int maxiters, int zerophi, const double *rhs, double *phi, double *relres)
+{
+    const int nx = g->nx, ny = g->ny;
+    const size_t n1 = (size_t)(nx + 2) * (size_t)(ny + 2), n0 = (size_t)nx * (size_t)ny;
+    if (maxiters == 0) return 0;
+    double *r = calloc(n0, sizeof(double));
+    if (!r) die("out of memory");
+    if (zerophi) memset(phi, 0, sizeof(double) * n1);
+    residual(g, bc, homog, rhs, phi, r);
+    double rscale = 1.0;
+    if (tol > 0.0) {
+        rscale = pnorm(nx, ny, g->dx, g->dy, r, 2);
+        if (rscale == 0.0) {
+            relres[0] = rscale;
+            rscale = 1.0;
+        } else {
+            relres[0] = 1.0;
+        }
+    }
+    int it;
+    for (it = 1; it <= maxiters; ++it) {
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double newphi = PH(phi, i, j) + C0(r, i, j) * C0(invd, i, j);
+                PH(phi, i, j) = omega * newphi + (1.0 - omega) * PH(phi, i, j);
+            }
+        residual(g, bc, homog, rhs, phi, r);
+        if (tol > 0.0) {
+            relres[it] = pnorm(nx, ny, g->dx, g->dy, r, 2) / rscale;
+            if (relres[it] <= tol) break;
+        }
+    }
+    free(r);
+    return it > maxiters ? maxiters : it;
+}
+
+/* solve_bicgstab (version 1): returns the last bookkeeping index i (relres
+ * holds entries 0..i; restarts take an index of their own), or -1 where the
+ * reference stops the program (beta == 0). */
+static int bicgstab1(const Geo *g, const Bc *bc, int homog, const double *invd, double tol, int max_iters,
+                     int max_restarts, int zerophi, const double *rhs, double *phi, int order, double *relres)
+{
+    const double hang = 1.0e-8;
+    const double relax_omega = 1.0 - 1.0 / 3.0;
+    const int nx = g->nx, ny = g->ny;
+    const size_t n1 = (size_t)(nx + 2) * (size_t)(ny + 2), n0 = (size_t)nx * (size_t)ny;
+    if (max_iters == 0) return 0;
+    const int nb = max_iters + max_restarts + 1;
+    double *r = calloc(n1, sizeof(double)), *r0 = calloc(n1, sizeof(double)), *nu = calloc(n1, sizeof(double));
+    double *p = calloc(n1, sizeof(double)), *t = calloc(n1, sizeof(double)), *y = calloc(n1, sizeof(double));
+    double *z = calloc(n1, sizeof(double)), *c0a = calloc(n0, sizeof(double)), *c0b = calloc(n0, sizeof(double));
+    double *rho = calloc((size_t)nb, sizeof(double)), *om = calloc((size_t)nb, sizeof(double));
+    if (!r || !r0 || !nu || !p || !t || !y || !z || !c0a || !c0b || !rho || !om) die("out of memory");
+    if (zerophi) memset(phi, 0, sizeof(double) * n1);
+
+    int i = 0, is_restart = 0, num_restarts = 0;
+    residual(g, bc, homog, rhs, phi, c0a);
+    c0_to_valid(g, c0a, r);
+    memcpy(r0, r, sizeof(double) * n1);
+    const double rscale = maxnorm_ph(g, r0);
+    relres[0] = 1.0;
+    if (rscale != 0.0) {
+        double alpha = 1.0, lastres;
+        rho[0] = 1.0;
+        om[0] = 1.0;
+        for (int iter = 1; iter <= max_iters; ++iter) {
+            ++i;
+            rho[i] = ip_ph(g, r0, r);
+            const double beta = (rho[i] / rho[i - 1]) * (alpha / om[i - 1]);
+            if (beta == 0.0) {
+                i = -1;
+                break;
+            }
+            for (size_t k = 0; k < n1; ++k) p[k] = beta * p[k];
+            for (size_t k = 0; k < n1; ++k) p[k] = p[k] - beta * om[i - 1] * nu[k] + r[k];
+            for (int jj = 0; jj < ny; ++jj)
+                for (int ii = 0; ii < nx; ++ii) PH(y, ii, jj) = PH(p, ii, jj) * C0(invd, ii, jj);
+            valid_to_c0(g, p, c0a);
+            relax_gs(g, bc, homog, invd, relax_omega, 2, c0a, y, order);
+            laplacian(g, bc, homog, 0, y, c0b);
+            c0_to_valid(g, c0b, nu);
+            alpha = ip_ph(g, r0, nu);
+            alpha = rho[i] / alpha;
+            for (size_t k = 0; k < n1; ++k) r[k] = r[k] - alpha * nu[k];
+            for (int jj = 0; jj < ny; ++jj)
+                for (int ii = 0; ii < nx; ++ii) PH(z, ii, jj) = PH(r, ii, jj) * C0(invd, ii, jj);
+            valid_to_c0(g, r, c0a);
+            relax_gs(g, bc, homog, invd, relax_omega, 2, c0a, z, order);
+            laplacian(g, bc, homog, 0, z, c0b);
+            c0_to_valid(g, c0b, t);
+            om[i] = ip_ph(g, t, r) / ip_ph(g, t, t);
+            for (size_t k = 0; k < n1; ++k) phi[k] = phi[k] + alpha * y[k] + om[i] * z[k];
+            for (size_t k = 0; k < n1; ++k) r[k] = r[k] - om[i] * t[k];
+            lastres = is_restart ? 1.0e200 : relres[i - 1];
+            relres[i] = maxnorm_ph(g, r) / rscale;
+            if (relres[i] <= tol) break;
+            const int hanging = fabs(relres[i] - lastres) < tol * hang;
+            const int diverging = !hanging && relres[i] > lastres;
+            if (hanging || diverging) {
+                if (num_restarts < max_restarts) {
+                    /* restart: a fresh residual takes the next bookkeeping slot */
+                    ++num_restarts;
+                    ++i;
+                    residual(g, bc, homog, rhs, phi, c0a);
+                    c0_to_valid(g, c0a, r);
+                    memcpy(r0, r, sizeof(double) * n1);
+                    relres[i] = maxnorm_ph(g, r) / rscale;
+                    alpha = 1.0;
+                    rho[i] = 1.0;
+                    om[i] = 1.0;
+                    memset(nu, 0, sizeof(double) * n1);
+                    memset(p, 0, sizeof(double) * n1);
+                    is_restart = 1;
+                    continue;
+                }
+                if (diverging) {
+                    /* undo the last correction */
+                    for (size_t k = 0; k < n1; ++k) r[k] = r[k] + om[i] * t[k];
+                    for (size_t k = 0; k < n1; ++k) phi[k] = phi[k] - alpha * y[k] - om[i] * z[k];
+                }
+                break;
+            }
+            is_restart = 0;
+        }
+    }
+    free(r); free(r0); free(nu); free(p); free(t); free(y); free(z); free(c0a); free(c0b); free(rho); free(om);
+    return i;
+}
+
+/* ---------------------------------------------------------------------------
  * Cycle drivers: MGPoisson2D::vcycle / vcycle_noinit (MGPoisson2D.f90:400-854)
  * and DeferredMGPoisson2D::vcycle / vcycle_noinit (MGPoisson2D_DD.f90:589-1045).
  * Level workspace is zero-initialised, so ghosts that the reference leaves
@@ -1368,6 +1544,54 @@ static void compute_div(const Geo *g, const double *xflux, const double *yflux, 
 }
 
 /* ---------------------------------------------------------------------------
+ * Metric assembly of the reference's test geometry (test.f90:320-716): x = xi,
+ * y = el(x) + (H - el(x)) eta/H over the bottom el(x) = amp sin^2(pi x / L)
+ * (amp = 1/4 there), J and J g^ij by differencing positions of the staggered
+ * points (fill_dxdXi, :566-646):
+ *   J        = y_eta at cells,  from y at (cell, node) points
+ *   J g^11   = y_eta at x-faces, from y at (node, node)
+ *   J g^12   = -y_xi at x-faces, from y at (cell, cell)
+ *   J g^21   = -y_xi at y-faces, from y at (node, node)
+ *   J g^22   = (1 + y_xi^2) / y_eta at y-faces, y_eta from y at (cell, cell)
+ * L and H are nx dx and ny dy, as the reference's box_data L and H are.
+ * ------------------------------------------------------------------------- */
+
+/* fill_x / fill_y / fill_elevation (test.f90:370-469, sinusoidal bottom at :530) */
+static double terrain_y(double dx, double dy, double Lx, double Hy, double amp, int i, double offx, int j,
+                        double offy)
+{
+    const double pi = 4.0 * atan(1.0);
+    const double xx = (i + offx) * dx;
+    const double sn = sin(pi * xx / Lx);
+    const double el = amp * (sn * sn);
+    const double yfrac = (j + offy) * dy / Hy;
+    return el + (Hy - el) * yfrac;
+}
+
+static void metric_terrain(const Geo *g, double amp, double *J, double *Jxx, double *Jxy, double *Jyx, double *Jyy)
+{
+    const int nx = g->nx, ny = g->ny;
+    const double dx = g->dx, dy = g->dy, Lx = nx * dx, Hy = ny * dy;
+    const double sx = 1.0 / dx, sy = 1.0 / dy;
+#define TY(i, ox, j, oy) terrain_y(dx, dy, Lx, Hy, amp, (i), (ox), (j), (oy))
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) C0(J, i, j) = (TY(i, 0.5, j + 1, 0.0) - TY(i, 0.5, j, 0.0)) * sy;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+            XF(Jxx, i, j) = (TY(i, 0.0, j + 1, 0.0) - TY(i, 0.0, j, 0.0)) * sy;
+            XF(Jxy, i, j) = -((TY(i, 0.5, j, 0.5) - TY(i - 1, 0.5, j, 0.5)) * sx);
+        }
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const double yxi = (TY(i + 1, 0.0, j, 0.0) - TY(i, 0.0, j, 0.0)) * sx;
+            const double yeta = (TY(i, 0.5, j, 0.5) - TY(i, 0.5, j - 1, 0.5)) * sy;
+            YF(Jyx, i, j) = -yxi;
+            YF(Jyy, i, j) = (1.0 + yxi * yxi) / yeta;
+        }
+#undef TY
+}
+
+/* ---------------------------------------------------------------------------
  * Exported entry points (argument order of oracle/ref_bridge.f90).
  * ------------------------------------------------------------------------- */
 #define MKG Geo G = mkgeo(nx, ny, dx, dy, J, Jxx, Jxy, Jyx, Jyy)
@@ -1482,6 +1706,29 @@ int mgo_vcycle(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double tol, int m
     MKG; MKB;
     return vcycle(kind, &G, &B, homog, tol, maxiters, maxdepth, numcycles, sdown, sup, sbottom,
                   zerophi, phi, rhs, order, relres, iters);
+}
+
+int mgo_relax_jacobi(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double omega, double tol,
+                     int maxiters, int zerophi, const double *rhs, double *phi, double *relres)
+{
+    MKG; MKB;
+    check_lah(&B);
+    return relax_jacobi(&G, &B, homog, invd, omega, tol, maxiters, zerophi, rhs, phi, relres);
+}
+
+int mgo_bicgstab1(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol, int max_iters,
+                  int max_restarts, int zerophi, const double *rhs, double *phi, int order, double *relres)
+{
+    MKG; MKB;
+    check_lah(&B);
+    return bicgstab1(&G, &B, homog, invd, tol, max_iters, max_restarts, zerophi, rhs, phi, order, relres);
+}
+
+void mgo_metric_terrain(int nx, int ny, double dx, double dy, double amp, double *J, double *Jxx, double *Jxy,
+                        double *Jyx, double *Jyy)
+{
+    Geo G = mkgeo(nx, ny, dx, dy, 0, 0, 0, 0, 0);
+    metric_terrain(&G, amp, J, Jxx, Jxy, Jyx, Jyy);
 }
 
 void mgo_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd)

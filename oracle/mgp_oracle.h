@@ -84,6 +84,19 @@ int mgo_vcycle(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double tol, int m
  * boundary fluxes.  compute_div: J div F into an nx x ny field. */
 void mgo_compute_pd(int nx, int ny, double dx, double dy, const double *phi, int dir, int nodedir, double *pd);
 void mgo_compute_grad(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double *xflux, double *yflux);
+/* relax_jacobi (poisson2d.f90:1358-1451): returns the sweeps made; relres (maxiters+1)
+ * gets the relative 2-norm history when tol > 0. */
+int mgo_relax_jacobi(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double omega, double tol,
+                     int maxiters, int zerophi, const double *rhs, double *phi, double *relres);
+/* solve_bicgstab, version 1 (poisson2d.f90:1993-2248): returns the last bookkeeping
+ * index i (relres, max_iters+max_restarts+1 doubles, holds 0..i), or -1 where the
+ * reference stops on beta == 0.  order: relax_gs ordering of the preconditioner. */
+int mgo_bicgstab1(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol, int max_iters,
+                  int max_restarts, int zerophi, const double *rhs, double *phi, int order, double *relres);
+/* Metric assembly of the reference's test geometry (test.f90:320-716): terrain-following
+ * y over el(x) = amp sin^2(pi x / L), J and J g^ij from differences of point positions. */
+void mgo_metric_terrain(int nx, int ny, double dx, double dy, double amp, double *J, double *Jxx, double *Jxy,
+                        double *Jyx, double *Jyy);
 /* fill_boundary_fluxes (arrayutils.f90:1571-1622): Neumann faces take the BC flux. */
 void mgo_fill_boundary_fluxes(int nx, int ny, MGO_BC_ARGS, int homog, double *xflux, double *yflux);
 void mgo_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,

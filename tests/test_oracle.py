@@ -381,6 +381,48 @@ def test_bicgstab1_reaches_the_direct_solution():
     np.testing.assert_allclose(sol[1:-1, 1:-1], exact, rtol=0, atol=1e-10 * np.max(np.abs(exact)))
 
 
+def terrain_analytic(n, amp=0.25):
+    """Exact J and J g^ij of y = el(x) + (1 - el(x)) eta, el = amp sin^2(pi x)
+    on the unit square, at cells and faces."""
+    dx = 1.0 / n
+    xc, xn = (np.arange(n) + 0.5) * dx, np.arange(n + 1) * dx
+    yc, yn = xc, xn
+    el = lambda x: amp * np.sin(math.pi * x) ** 2
+    elp = lambda x: amp * math.pi * np.sin(2 * math.pi * x)
+    J = np.broadcast_to((1 - el(xc))[None, :], (n, n))
+    Jxx = np.broadcast_to((1 - el(xn))[None, :], (n, n + 1))
+    Jxy = -(elp(xn)[None, :] * (1 - yc)[:, None])
+    Jyx = -(elp(xc)[None, :] * (1 - yn)[:, None])
+    Jyy = (1 + (elp(xc)[None, :] * (1 - yn)[:, None]) ** 2) / (1 - el(xc))[None, :]
+    return J, Jxx, Jxy, Jyx, Jyy
+
+
+def test_terrain_metrics_closed_forms():
+    """The reference test geometry's metric assembly (test.f90:320-716):
+    (1) a flat bottom is the identity map; (2) the discrete metrics satisfy the
+    geometric conservation law d_xi(J g^11) + d_eta(J g^21) = 0 to rounding
+    (the operator annihilates phi = x), which any wrong stagger, sign or index
+    breaks; (3) y is linear in eta, so J and J g^11 are exact, and J g^12,
+    J g^21, J g^22 converge to the exact derivatives at second order."""
+    o = rc.oracle()
+    flat = o.metric_terrain(16, 12, 1 / 16, 1 / 12, 0.0)
+    for a, want in zip(flat.arrays(), (1, 1, 0, 0, 1)):
+        np.testing.assert_allclose(a, want, rtol=0, atol=4e-15)
+    for n in (16, 128):
+        g = o.metric_terrain(n, n, 1 / n, 1 / n, 0.25)
+        gcl = (g.Jxx[:, 1:] - g.Jxx[:, :-1]) / g.dx + (g.Jyx[1:, :] - g.Jyx[:-1, :]) / g.dy
+        assert np.max(np.abs(gcl)) <= 1e-16 * n * n
+    errs = []
+    for n in (32, 64, 128):
+        g = o.metric_terrain(n, n, 1 / n, 1 / n, 0.25)
+        errs.append([np.max(np.abs(a - b)) for a, b in zip(g.arrays(), terrain_analytic(n))])
+    for k in (0, 1):
+        assert max(e[k] for e in errs) <= 1e-13
+    for k in (2, 3, 4):
+        rates = [math.log2(errs[m][k] / errs[m + 1][k]) for m in range(2)]
+        assert min(rates) > 1.95, (k, rates)
+
+
 def test_dirichlet_order3_ghost_exact_for_quadratics():
     o = rc.oracle()
     geo = G.mapped_geometry(10, 8)
