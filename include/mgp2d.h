@@ -120,8 +120,18 @@ int mgp_prolong(const mgp_geo *fine_geo, const mgp_geo *crse_geo, const int *bc_
 /* MGPoisson2D.f90:38,67 estimate_maxdepth + create_mgschedule; refx/refy hold >= 64 entries */
 int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out,
                           int *refx, int *refy);
-/* arrayutils.f90:753 pnorm over the valid cells of a ghost-free field */
+/* arrayutils.f90:753 pnorm over the valid cells of a ghost-free field: p = 0 max |a|, p >= 1
+ * (sum dx dy |a|^p)^(1/p).  Sums are parallel and compensated (within an ulp or two of the
+ * reference's sequential sums); the max norm is exact. */
 int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, double *out);
+/* arrayutils.f90:637 inner_prod(bd1, bd2): sum of dx dy a b over the nx x ny cells. */
+int mgp_inner_prod(int nx, int ny, double dx, double dy, const double *a, const double *b, double *out);
+/* arrayutils.f90:670 integrate2d(bd, bx, geo, opt_jscale): sum of dx dy J a (J given, jscale true)
+ * or dx dy a (J == NULL). */
+int mgp_integrate2d(int nx, int ny, double dx, double dy, const double *a, const double *J, double *out);
+/* arrayutils.f90:1497 extrapolate_ghosts(phi, order), order 0..3, on a one-ghost field
+ * (nx+2) x (ny+2): x-side ghost columns first (over all rows), then y-side rows. */
+int mgp_extrapolate_ghosts(int nx, int ny, int order, double *phi);
 /* poisson2d.f90:404 compute_pd(pd, phi, dir): d phi / d(dir) on the faces normal to nodedir
  * (1: x-faces, pd is (nx+1) x ny; 2: y-faces, pd is nx x (ny+1)); phi (nx+2) x (ny+2) is read
  * with its ghosts as given.  Across the face (dir == nodedir) a two-point difference; along it

@@ -75,6 +75,7 @@ EXPORTS = (
     "mgp_device_synchronize", "mgp_xline_relax", "mgp_compute_pd", "mgp_compute_grad", "mgp_compute_div",
     "mgp_fill_boundary_fluxes", "mgp_solver_set_flux", "mgp_solver_get_flux", "mgp_solver_project",
     "mgp_relax_jacobi", "mgp_solve_bicgstab", "mgp_metric_fill", "mgp_solver_create_mapped",
+    "mgp_inner_prod", "mgp_integrate2d", "mgp_extrapolate_ghosts",
 )
 
 
@@ -256,6 +257,29 @@ def create_mgschedule(nx, ny, dx, dy, maxdepth=-1):
     md = _i(0)
     _check(lib.mgp_create_mgschedule(_i(nx), _i(ny), _d(dx), _d(dy), _i(maxdepth), ctypes.byref(md), rx, ry))
     return md.value, list(rx[: md.value]), list(ry[: md.value])
+
+
+def inner_prod(nx, ny, dx, dy, a, b):
+    out = _d(0.0)
+    a, b = _f64(a), _f64(b)
+    _check(lib.mgp_inner_prod(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), _ptr(b), ctypes.byref(out)))
+    return out.value
+
+
+def integrate2d(nx, ny, dx, dy, a, J=None):
+    """arrayutils integrate2d: sum dx dy J a (J given) or dx dy a."""
+    out = _d(0.0)
+    a = _f64(a)
+    J = None if J is None else _f64(J)
+    _check(lib.mgp_integrate2d(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), None if J is None else _ptr(J),
+                               ctypes.byref(out)))
+    return out.value
+
+
+def extrapolate_ghosts(nx, ny, order, phi):
+    phi = _f64(phi).copy()
+    _check(lib.mgp_extrapolate_ghosts(_i(nx), _i(ny), _i(order), _ptr(phi)))
+    return phi
 
 
 def pnorm(nx, ny, dx, dy, arr, p=2):

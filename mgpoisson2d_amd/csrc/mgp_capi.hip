@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "mgp2d.h"
@@ -469,11 +470,23 @@ int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in,
     });
 }
 
+namespace {
+
+void check_field(int nx, int ny, double dx, double dy, const void *a, const void *out, const char *what) {
+    if (nx <= 0 || ny <= 0 || !(dx > 0) || !(dy > 0) || !a || !out) {
+        g_last_error = std::string(what) + ": bad shape, spacing or null array";
+        throw Failure(MGP_ERR_ARG);
+    }
+}
+
+}  // namespace
+
 int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, double *out) {
     return guarded([&] {
-        if (p != 0 && p != 2) {
-            g_last_error = "pnorm: p must be 0 or 2";
-            throw Failure(MGP_ERR_UNSUPPORTED);
+        check_field(nx, ny, dx, dy, a, out, "pnorm");
+        if (p < 0) {
+            g_last_error = "pnorm: p must be >= 0";
+            throw Failure(MGP_ERR_ARG);
         }
         hipStream_t s = thread_stream();
         const Grid g = make_grid(nx, ny, dx, dy);
@@ -481,7 +494,64 @@ int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, doub
         d.alloc(level_elems(nx, ny));
         upload_block(s, d.p, g, a, 0, 0, nx, ny);
         Workspace W;
-        *out = p == 2 ? pnorm2(W, s, d.p, g) : W.reduce(s, d.p, nullptr, nullptr, 1.0, g, 2);
+        const double vol = dx * dy;
+        if (p == 0) *out = W.reduce(s, d.p, nullptr, nullptr, 1.0, g, 2);
+        else if (p == 2) *out = pnorm2(W, s, d.p, g);
+        else if (p == 1) *out = W.reduce(s, d.p, nullptr, nullptr, vol, g, 4, 1);
+        else *out = std::pow(W.reduce(s, d.p, nullptr, nullptr, vol, g, 4, p), 1.0 / p);
+    });
+}
+
+int mgp_inner_prod(int nx, int ny, double dx, double dy, const double *a, const double *b, double *out) {
+    return guarded([&] {
+        check_field(nx, ny, dx, dy, a, out, "inner_prod");
+        if (!b) {
+            g_last_error = "inner_prod: null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf da, db;
+        da.alloc(level_elems(nx, ny));
+        db.alloc(level_elems(nx, ny));
+        upload_block(s, da.p, g, a, 0, 0, nx, ny);
+        upload_block(s, db.p, g, b, 0, 0, nx, ny);
+        Workspace W;
+        *out = W.reduce(s, da.p, db.p, nullptr, dx * dy, g, 0);
+    });
+}
+
+int mgp_integrate2d(int nx, int ny, double dx, double dy, const double *a, const double *J, double *out) {
+    return guarded([&] {
+        check_field(nx, ny, dx, dy, a, out, "integrate2d");
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf da, dj;
+        da.alloc(level_elems(nx, ny));
+        upload_block(s, da.p, g, a, 0, 0, nx, ny);
+        if (J) {
+            dj.alloc(level_elems(nx, ny));
+            upload_block(s, dj.p, g, J, 0, 0, nx, ny);
+        }
+        Workspace W;
+        *out = W.reduce(s, da.p, nullptr, J ? dj.p : nullptr, dx * dy, g, 3);
+    });
+}
+
+int mgp_extrapolate_ghosts(int nx, int ny, int order, double *phi) {
+    return guarded([&] {
+        if (order < 0 || order > 3 || !phi || nx < order + 1 || ny < order + 1) {
+            g_last_error = "extrapolate_ghosts: order must be 0..3 and nx, ny > order";
+            throw Failure(order > 3 ? MGP_ERR_UNSUPPORTED : MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, 1.0, 1.0);
+        DevBuf d;
+        d.alloc(level_elems(nx, ny));
+        up_phi(s, d, g, phi);
+        launch_extrapolate(s, d.p, g, order);
+        down_phi(s, phi, d, g);
+        hip_check(hipStreamSynchronize(s), "sync");
     });
 }
 

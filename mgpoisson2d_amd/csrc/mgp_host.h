@@ -72,7 +72,7 @@ struct Workspace {
     Workspace(const Workspace &) = delete;
     ~Workspace();
     double reduce(hipStream_t s, const double *a, const double *b, const double *J, double vol, const Grid &g,
-                  int kind);
+                  int kind, int p = 1);
     const double *ones(hipStream_t s, const Grid &g);
 };
 

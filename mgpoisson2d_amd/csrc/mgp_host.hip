@@ -183,12 +183,12 @@ int reference_schedule(int nx, int ny, double dx, double dy, int maxdepth, std::
 // Scalars
 // --------------------------------------------------------------------------
 double Workspace::reduce(hipStream_t s, const double *a, const double *b, const double *J, double vol,
-                         const Grid &g, int kind) {
+                         const Grid &g, int kind, int p) {
     const size_t need = (size_t)reduce_blocks(g) + 8;
     if (partial.n < need) partial.alloc(need);
     if (!host_scalar) hip_check(hipHostMalloc((void **)&host_scalar, 8 * sizeof(double)), "hipHostMalloc");
     // result in partial.p[0], per-block partial sums from partial.p[8] on
-    launch_reduce(s, partial.p, partial.p + 8, a, b, J, vol, g, kind);
+    launch_reduce(s, partial.p, partial.p + 8, a, b, J, vol, g, kind, p);
     hip_check(hipMemcpyAsync(host_scalar, partial.p, sizeof(double), hipMemcpyDeviceToHost, s), "scalar");
     hip_check(hipStreamSynchronize(s), "sync");
     return host_scalar[0];

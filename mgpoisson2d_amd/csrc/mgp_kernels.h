@@ -49,11 +49,14 @@ void launch_setval(hipStream_t s, double *y, double v, const Grid &g, int ghoste
 // out = a * b / out = a / b over valid cells
 void launch_lincomb_mul(hipStream_t s, double *out, const double *a, const double *b, const Grid &g, int unused);
 void launch_lincomb_div(hipStream_t s, double *out, const double *a, const double *b, const Grid &g, int unused);
-// kind 0: sum w a b; 1: sum w a^2; 2: max |a|; 3: sum w a.  w = vol (or vol*J).
-// Writes the scalar to out (device); partial needs reduce_blocks(g) doubles.
+// kind 0: sum w a b; 1: sum w a^2; 2: max |a|; 3: sum w a; 4: sum w |a|^p.
+// w = vol (or vol*J).  Writes the scalar to out (device); partial needs
+// reduce_blocks(g) doubles.
 int reduce_blocks(const Grid &g);
 void launch_reduce(hipStream_t s, double *out, double *partial, const double *a, const double *b,
-                   const double *J, double vol, const Grid &g, int kind);
+                   const double *J, double vol, const Grid &g, int kind, int p = 1);
+// extrapolate_ghosts of order 0..3, in place (one workgroup).
+void launch_extrapolate(hipStream_t s, double *phi, const Grid &g, int order);
 
 void launch_seq_mean(hipStream_t s, double *out, const double *fine, const double *J, const Grid &g, double dxprod);
 void launch_sub_mean(hipStream_t s, double *fine, const double *sums, const Grid &g);

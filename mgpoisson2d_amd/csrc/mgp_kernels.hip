@@ -489,6 +489,32 @@ __global__ void __launch_bounds__(256) k_extrap3(Fld dst, CFld src, Grid g) {
     }
 }
 
+// extrapolate_ghosts (arrayutils.f90:1497-1566), in place, orders 0..3: the
+// x-side ghost columns over every row of the array (ghost rows included),
+// then the y-side ghost rows over every column, so the corners come from the
+// y rule applied to the new x ghosts, as in the reference's array syntax.
+__device__ __forceinline__ double extrap(int order, double p0, double p1, double p2, double p3) {
+    switch (order) {
+    case 0: return p0;
+    case 1: return 2.0 * p0 - p1;
+    case 2: return 3.0 * (p0 - p1) + p2;
+    default: return 4.0 * (p0 + p2) - 6.0 * p1 - p3;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_extrapolate(Fld f, Grid g, int order) {
+    const int nx = g.nx, ny = g.ny;
+    for (int j = (int)threadIdx.x - 1; j <= ny; j += blockDim.x) {
+        f(-1, j) = extrap(order, f(0, j), f(1, j), f(2, j), order == 3 ? f(3, j) : 0.0);
+        f(nx, j) = extrap(order, f(nx - 1, j), f(nx - 2, j), f(nx - 3, j), order == 3 ? f(nx - 4, j) : 0.0);
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x - 1; i <= nx; i += blockDim.x) {
+        f(i, -1) = extrap(order, f(i, 0), f(i, 1), f(i, 2), order == 3 ? f(i, 3) : 0.0);
+        f(i, ny) = extrap(order, f(i, ny - 1), f(i, ny - 2), f(i, ny - 3), order == 3 ? f(i, ny - 4) : 0.0);
+    }
+}
+
 // ===========================================================================
 // Pointwise helpers
 // ===========================================================================
@@ -578,10 +604,24 @@ __device__ __forceinline__ double block_combine(double *ss, double *sc, Acc acc,
     return kind == 2 ? ss[0] : ss[0] + sc[0];
 }
 
-// kind 0: sum w*a*b, kind 1: sum w*a*a, kind 2: max |a|, kind 3: sum w*a  (w = vol or vol*J)
-// Block b takes rows b, b + gridDim.x, ...; threads stride along the row.
+// |x|^p for an integer p >= 1 as Fortran's real**integer evaluates it:
+// binary powering from the lowest bit.
+__device__ __forceinline__ double abs_pow(double x, int p) {
+    double b = fabs(x), r = 1.0;
+    while (true) {
+        if (p & 1) r *= b;
+        p /= 2;
+        if (p == 0) break;
+        b *= b;
+    }
+    return r;
+}
+
+// kind 0: sum w*a*b, 1: sum w*a*a, 2: max |a|, 3: sum w*a, 4: sum w*|a|^p
+// (w = vol or vol*J).  Block b takes rows b, b + gridDim.x, ...; threads
+// stride along the row.
 __global__ void __launch_bounds__(256) k_reduce_partial(double *partial, const double *a, const double *b,
-                                                        const double *J, double vol, Grid g, int kind) {
+                                                        const double *J, double vol, Grid g, int kind, int p) {
     __shared__ double ss[256], sc[256];
     Acc acc{0.0, 0.0};
     double mx = 0.0;
@@ -595,7 +635,8 @@ __global__ void __launch_bounds__(256) k_reduce_partial(double *partial, const d
             if (kind == 0) acc.add(w * (x * b[c]));
             else if (kind == 1) acc.add(w * (x * x));
             else if (kind == 2) mx = fmax(mx, fabs(x));
-            else acc.add(w * x);
+            else if (kind == 3) acc.add(w * x);
+            else acc.add(w * abs_pow(x, p));
         }
     }
     const double v = block_combine(ss, sc, acc, mx, kind);
@@ -1004,6 +1045,9 @@ void launch_prolong2x2(hipStream_t s, double *fine, int fpitch, const double *cr
 void launch_extrap3(hipStream_t s, double *dst, const double *src, const Grid &g) {
     k_extrap3<<<1, 256, 0, s>>>(Fld{dst, g.pitch}, CFld{src, g.pitch}, g);
 }
+void launch_extrapolate(hipStream_t s, double *phi, const Grid &g, int order) {
+    k_extrapolate<<<1, 256, 0, s>>>(Fld{phi, g.pitch}, g, order);
+}
 void launch_einit(hipStream_t s, double *e, const double *r, const double *invd, const Grid &g) {
     k_einit<<<tiles(g.nx, g.ny), 256, 0, s>>>(e, r, invd, g);
 }
@@ -1033,9 +1077,9 @@ int reduce_blocks(const Grid &g) {
     return (int)b;
 }
 void launch_reduce(hipStream_t s, double *out, double *partial, const double *a, const double *b,
-                   const double *J, double vol, const Grid &g, int kind) {
+                   const double *J, double vol, const Grid &g, int kind, int p) {
     const int nb = reduce_blocks(g) < g.ny ? reduce_blocks(g) : g.ny;
-    k_reduce_partial<<<nb, 256, 0, s>>>(partial, a, b, J, vol, g, kind);
+    k_reduce_partial<<<nb, 256, 0, s>>>(partial, a, b, J, vol, g, kind, p);
     k_reduce_final<<<1, 256, 0, s>>>(out, partial, nb, kind);
 }
 void launch_bicg_scalars(hipStream_t s, double *sc, int op) { k_bicg_scalars<<<1, 64, 0, s>>>(sc, op); }

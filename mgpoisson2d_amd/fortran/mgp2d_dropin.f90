@@ -146,6 +146,26 @@ module ArrayUtils
             real(c_double), intent(in) :: a(*)
             real(c_double)             :: out
         end function
+        integer(c_int) function mgp_inner_prod_c(nx, ny, dx, dy, a, b, out) bind(C, name="mgp_inner_prod")
+            import :: c_int, c_double
+            integer(c_int), value      :: nx, ny
+            real(c_double), value      :: dx, dy
+            real(c_double), intent(in) :: a(*), b(*)
+            real(c_double)             :: out
+        end function
+        integer(c_int) function mgp_integrate2d_c(nx, ny, dx, dy, a, J, out) bind(C, name="mgp_integrate2d")
+            import :: c_int, c_double, c_ptr
+            integer(c_int), value      :: nx, ny
+            real(c_double), value      :: dx, dy
+            real(c_double), intent(in) :: a(*)
+            type(c_ptr), value         :: J
+            real(c_double)             :: out
+        end function
+        integer(c_int) function mgp_extrapolate_ghosts_c(nx, ny, order, phi) bind(C, name="mgp_extrapolate_ghosts")
+            import :: c_int, c_double
+            integer(c_int), value      :: nx, ny, order
+            real(c_double)             :: phi(*)
+        end function
         integer(c_int) function mgp_fill_boundary_fluxes_c(nx, ny, bc, homog, xflux, yflux) &
                                                            bind(C, name="mgp_fill_boundary_fluxes")
             import :: mgp_bc_c, c_int, c_double
@@ -380,10 +400,13 @@ contains
         bd%data(bd%valid%ilo:bd%valid%ihi, bd%valid%jlo:bd%valid%jhi) = val
     end subroutine setval_valid
 
+    ! arrayutils.f90:637 (runs on the GPU)
     function inner_prod (bd1, bd2) result (ip)
         type(box_data), intent(in) :: bd1, bd2
-        real(dp) :: ip, vol
-        integer  :: ilo, ihi, jlo, jhi, j
+        real(dp) :: ip
+        real(dp), allocatable :: a(:,:), b(:,:)
+        real(c_double) :: out
+        integer  :: ilo, ihi, jlo, jhi
         ilo = max(bd1%valid%ilo, bd2%valid%ilo)
         ihi = min(bd1%valid%ihi, bd2%valid%ihi)
         jlo = max(bd1%valid%jlo, bd2%valid%jlo)
@@ -392,14 +415,15 @@ contains
             print*, 'ERROR: inner_prod_box_data: given box_data do not overlap.'
             stop
         endif
-        vol = bd1%valid%dx * bd1%valid%dy
-        ip = zero
-        do j = jlo, jhi
-            ip = ip + vol * dot_product (bd1%data(ilo:ihi,j), bd2%data(ilo:ihi,j))
-        enddo
+        allocate (a(ilo:ihi, jlo:jhi), b(ilo:ihi, jlo:jhi))
+        a = bd1%data(ilo:ihi, jlo:jhi)
+        b = bd2%data(ilo:ihi, jlo:jhi)
+        call mgp_check (mgp_inner_prod_c(ihi - ilo + 1, jhi - jlo + 1, bd1%valid%dx, bd1%valid%dy, a, b, out), &
+                        'inner_prod')
+        ip = out
     end function inner_prod
 
-    ! 2-norm and max norm run on the GPU (mgp_pnorm); other p on the host.
+    ! arrayutils.f90:753 (runs on the GPU)
     function pnorm (bd, bx, p) result (pn)
         type(box_data), intent(in) :: bd
         type(box), intent(in)      :: bx
@@ -407,83 +431,52 @@ contains
         real(dp)                   :: pn
         real(dp), allocatable      :: a(:,:)
         real(c_double)             :: out
-        integer :: i, j
-        if (p .eq. 0 .or. p .eq. 2) then
-            allocate (a(bx%ilo:bx%ihi, bx%jlo:bx%jhi))
-            a = bd%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi)
-            call mgp_check (mgp_pnorm_c(bx%nx, bx%ny, bd%bx%dx, bd%bx%dy, a, p, out), 'pnorm')
-            pn = out
-            return
-        endif
-        pn = zero
-        do j = bx%jlo, bx%jhi
-            do i = bx%ilo, bx%ihi
-                pn = pn + bd%bx%dx * bd%bx%dy * abs(bd%data(i,j))**p
-            enddo
-        enddo
-        pn = pn**(one/p)
+        allocate (a(bx%ilo:bx%ihi, bx%jlo:bx%jhi))
+        a = bd%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi)
+        call mgp_check (mgp_pnorm_c(bx%nx, bx%ny, bd%bx%dx, bd%bx%dy, a, p, out), 'pnorm')
+        pn = out
     end function pnorm
 
+    ! arrayutils.f90:670 (runs on the GPU)
     function integrate2d (bd, bx, geo, opt_jscale) result (sum)
         type(box_data), intent(in) :: bd
         type(box), intent(in)      :: bx
         type(geo_data), intent(in) :: geo
         logical, optional          :: opt_jscale
         real(dp) :: sum
+        real(dp), allocatable, target :: a(:,:), jac(:,:)
+        real(c_double) :: out
         logical  :: jscale
-        integer  :: i, j
+        if ((bd%offi .ne. BD_CELL) .or. (bd%offj .ne. BD_CELL)) then
+            print*, 'pnorm: Only works with cell-centered data for now.'
+            stop
+        endif
         jscale = .true.
         if (present(opt_jscale)) jscale = opt_jscale
-        sum = zero
-        do j = bx%jlo, bx%jhi
-            do i = bx%ilo, bx%ihi
-                if (jscale) then
-                    sum = sum + geo%dx * geo%dy * geo%J%data(i,j) * bd%data(i,j)
-                else
-                    sum = sum + geo%dx * geo%dy * bd%data(i,j)
-                endif
-            enddo
-        enddo
+        allocate (a(bx%ilo:bx%ihi, bx%jlo:bx%jhi))
+        a = bd%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi)
+        if (jscale) then
+            allocate (jac(bx%ilo:bx%ihi, bx%jlo:bx%jhi))
+            jac = geo%J%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi)
+            call mgp_check (mgp_integrate2d_c(bx%nx, bx%ny, geo%dx, geo%dy, a, c_loc(jac), out), 'integrate2d')
+        else
+            call mgp_check (mgp_integrate2d_c(bx%nx, bx%ny, geo%dx, geo%dy, a, c_null_ptr, out), 'integrate2d')
+        endif
+        sum = out
     end function integrate2d
 
-    ! arrayutils.f90:1497 (order 0..3), host side
+    ! arrayutils.f90:1497 (orders 0..3, runs on the GPU)
     subroutine extrapolate_ghosts (phi, order)
         type(box_data), intent(inout) :: phi
         integer, intent(in)           :: order
-        integer :: ilo, ihi, jlo, jhi
-        ilo = phi%valid%ilo
-        ihi = phi%valid%ihi
-        jlo = phi%valid%jlo
-        jhi = phi%valid%jhi
-        select case (order)
-            case (0)
-                phi%data(ilo-1,:) = phi%data(ilo,:)
-                phi%data(ihi+1,:) = phi%data(ihi,:)
-                phi%data(:,jlo-1) = phi%data(:,jlo)
-                phi%data(:,jhi+1) = phi%data(:,jhi)
-            case (1)
-                phi%data(ilo-1,:) = two*phi%data(ilo,:) - phi%data(ilo+1,:)
-                phi%data(ihi+1,:) = two*phi%data(ihi,:) - phi%data(ihi-1,:)
-                phi%data(:,jlo-1) = two*phi%data(:,jlo) - phi%data(:,jlo+1)
-                phi%data(:,jhi+1) = two*phi%data(:,jhi) - phi%data(:,jhi-1)
-            case (2)
-                phi%data(ilo-1,:) = three*(phi%data(ilo,:) - phi%data(ilo+1,:)) + phi%data(ilo+2,:)
-                phi%data(ihi+1,:) = three*(phi%data(ihi,:) - phi%data(ihi-1,:)) + phi%data(ihi-2,:)
-                phi%data(:,jlo-1) = three*(phi%data(:,jlo) - phi%data(:,jlo+1)) + phi%data(:,jlo+2)
-                phi%data(:,jhi+1) = three*(phi%data(:,jhi) - phi%data(:,jhi-1)) + phi%data(:,jhi-2)
-            case (3)
-                phi%data(ilo-1,:) = four*(phi%data(ilo,:) + phi%data(ilo+2,:)) - six*phi%data(ilo+1,:) &
-                                  - phi%data(ilo+3,:)
-                phi%data(ihi+1,:) = four*(phi%data(ihi,:) + phi%data(ihi-2,:)) - six*phi%data(ihi-1,:) &
-                                  - phi%data(ihi-3,:)
-                phi%data(:,jlo-1) = four*(phi%data(:,jlo) + phi%data(:,jlo+2)) - six*phi%data(:,jlo+1) &
-                                  - phi%data(:,jlo+3)
-                phi%data(:,jhi+1) = four*(phi%data(:,jhi) + phi%data(:,jhi-2)) - six*phi%data(:,jhi-1) &
-                                  - phi%data(:,jhi-3)
-            case default
-                print*, 'extrapolate_ghosts: order = ', order, ' is not supported.'
-                stop
-        end select
+        real(dp), allocatable :: a(:,:)
+        if (order .lt. 0 .or. order .gt. 3) then
+            print*, 'extrapolate_ghosts: order = ', order, ' is not supported.'
+            stop
+        endif
+        call pack_phi (a, phi)
+        call mgp_check (mgp_extrapolate_ghosts_c(phi%valid%nx, phi%valid%ny, order, a), 'extrapolate_ghosts')
+        phi%data(phi%valid%ilo-1:phi%valid%ihi+1, phi%valid%jlo-1:phi%valid%jhi+1) = a
     end subroutine extrapolate_ghosts
 
     ! ---- marshalling helpers -------------------------------------------------

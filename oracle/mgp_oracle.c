@@ -780,6 +780,20 @@ static int schedule(int nx, int ny, double dx, double dy, int maxdepth, int *ref
     return d;
 }
 
+/* Fortran's real**integer (abs(x)**p in pnorm): binary powering, squaring the
+ * base and multiplying the set bits into the result from the lowest up. */
+static double powi(double x, int n)
+{
+    double r = 1.0;
+    while (1) {
+        if (n & 1) r *= x;
+        n /= 2;
+        if (n == 0) break;
+        x *= x;
+    }
+    return r;
+}
+
 /* arrayutils.f90 pnorm (:753-796) over a ghost-free field */
 static double pnorm(int nx, int ny, double dx, double dy, const double *a, int p)
 {
@@ -803,7 +817,7 @@ static double pnorm(int nx, int ny, double dx, double dy, const double *a, int p
         pn = sqrt(pn);
     } else {
         for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) pn = pn + vs * pow(fabs(a[i + (size_t)j * nx]), p);
+            for (int i = 0; i < nx; ++i) pn = pn + vs * powi(fabs(a[i + (size_t)j * nx]), p);
         pn = pow(pn, 1.0 / p);
     }
     return pn;
@@ -1689,6 +1703,24 @@ void mgo_schedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *ma
 double mgo_pnorm(int nx, int ny, double dx, double dy, const double *a, int p)
 {
     return pnorm(nx, ny, dx, dy, a, p);
+}
+
+double mgo_inner_prod(int nx, int ny, double dx, double dy, const double *a, const double *b)
+{
+    return inner_prod(nx, ny, dx, dy, a, b);
+}
+
+/* arrayutils.f90 integrate2d (:670-710): sum volScale*J*a (J given) or volScale*a */
+double mgo_integrate2d(int nx, int ny, double dx, double dy, const double *a, const double *J)
+{
+    const double vs = dx * dy;
+    double sum = 0.0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const size_t c = (size_t)i + (size_t)j * (size_t)nx;
+            sum = J ? sum + vs * J[c] * a[c] : sum + vs * a[c];
+        }
+    return sum;
 }
 
 void mgo_bicgstab2(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol,

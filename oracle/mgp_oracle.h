@@ -65,6 +65,10 @@ void mgo_prolong(int fnx, int fny, double fdx, double fdy, const double *fJ, con
 void mgo_schedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out,
                   int *refx, int *refy);
 double mgo_pnorm(int nx, int ny, double dx, double dy, const double *a, int p);
+/* inner_prod (arrayutils.f90:637-661) and integrate2d (:670-710, J == NULL: no J scaling)
+ * over ghost-free nx x ny fields, in the reference's summation order. */
+double mgo_inner_prod(int nx, int ny, double dx, double dy, const double *a, const double *b);
+double mgo_integrate2d(int nx, int ny, double dx, double dy, const double *a, const double *J);
 void mgo_bicgstab2(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol,
                    int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
                    int order);

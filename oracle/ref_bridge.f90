@@ -809,4 +809,41 @@ contains
         call undefine_box_data (f)
     end function ref_pnorm
 
+    ! --------------------------------------------------------------------------
+    ! ArrayUtils::inner_prod (arrayutils.f90:637) and integrate2d (:670) over
+    ! ghost-free cell-centred fields; jscale 0 passes opt_jscale = .false.
+    ! --------------------------------------------------------------------------
+    function ref_inner_prod (nx, ny, dx, dy, a, b) result (ip) bind(C, name="ref_inner_prod")
+        integer(c_int), value      :: nx, ny
+        real(c_double), value      :: dx, dy
+        real(c_double), intent(in) :: a(0:nx-1,0:ny-1), b(0:nx-1,0:ny-1)
+        real(c_double)             :: ip
+        type(box) :: valid
+        type(box_data) :: fa, fb
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_cc0 (fa, valid, a)
+        call mk_cc0 (fb, valid, b)
+        ip = inner_prod (fa, fb)
+        call undefine_box_data (fa)
+        call undefine_box_data (fb)
+    end function ref_inner_prod
+
+    function ref_integrate2d (nx, ny, dx, dy, a, J, jscale) result (sm) bind(C, name="ref_integrate2d")
+        integer(c_int), value      :: nx, ny, jscale
+        real(c_double), value      :: dx, dy
+        real(c_double), intent(in) :: a(0:nx-1,0:ny-1), J(0:nx-1,0:ny-1)
+        real(c_double)             :: sm
+        type(box) :: valid
+        type(box_data) :: fa
+        type(geo_data) :: geo
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_cc0 (fa, valid, a)
+        call mk_cc0 (geo%J, valid, J)
+        geo%dx = dx
+        geo%dy = dy
+        sm = integrate2d (fa, valid, geo, jscale .ne. 0)
+        call undefine_box_data (fa)
+        call undefine_box_data (geo%J)
+    end function ref_integrate2d
+
 end module ref_bridge

@@ -105,6 +105,13 @@ program dropin_caller
     write (u) ywk%data
     call fill_boundary_fluxes (xflux, yflux, bc, .true.)
     write (u) yflux%data
+
+    ! reductions and ghost extrapolation
+    write (u) pnorm (rhs, valid, 0), pnorm (rhs, valid, 1), pnorm (rhs, valid, 2), pnorm (rhs, valid, 3)
+    write (u) inner_prod (rhs, lap), integrate2d (rhs, valid, geo), integrate2d (rhs, valid, geo, .false.)
+    phi%data = phi0%data
+    call extrapolate_ghosts (phi, 2)
+    write (u) phi%data
     close (u)
 
 contains

@@ -58,6 +58,14 @@ def main():
             put(p + "dc_invd", dinv)
             put(p + "dc_lex3", ref.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi))
             put(p + "dc_rb3", ref_rb.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi))
+            # reductions and ghost extrapolation (arrayutils.f90:637-796, 1497-1566)
+            v = np.ascontiguousarray(phi[1:-1, 1:-1])
+            nx_, ny_, dx_, dy_ = geo.nx, geo.ny, geo.dx, geo.dy
+            put(p + "reductions", [ref.pnorm(nx_, ny_, dx_, dy_, v, q) for q in range(5)]
+                + [ref.inner_prod(nx_, ny_, dx_, dy_, v, rhs), ref.integrate2d(nx_, ny_, dx_, dy_, v, geo.J),
+                   ref.integrate2d(nx_, ny_, dx_, dy_, v)])
+            for order in range(4):
+                put(p + f"extrap{order}", ref.extrapolate_ghosts(nx_, ny_, order, phi))
             # single-level solvers (poisson2d.f90:1358-2248)
             put(p + "jacobi", ref.relax_jacobi(geo, bc, homog, inv, 0.8, 1e-3, 12, 0, rhs, phi)[0])
             put(p + "bicgstab1", ref.bicgstab1(geo, bc, homog, inv, 1e-8, 20, 2, 0, rhs, phi)[0])

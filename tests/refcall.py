@@ -190,6 +190,19 @@ class Checker:
     def pnorm(self, nx, ny, dx, dy, a, p):
         return self._fn("pnorm", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(np.ascontiguousarray(a)), _i(p))
 
+    def inner_prod(self, nx, ny, dx, dy, a, b):
+        return self._fn("inner_prod", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(np.ascontiguousarray(a)),
+                                          _ptr(np.ascontiguousarray(b)))
+
+    def integrate2d(self, nx, ny, dx, dy, a, J=None):
+        a = np.ascontiguousarray(a)
+        if self.is_oracle:
+            return self._fn("integrate2d", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a),
+                                               None if J is None else _ptr(np.ascontiguousarray(J)))
+        Jarr = np.ones_like(a) if J is None else np.ascontiguousarray(J)
+        return self._fn("integrate2d", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), _ptr(Jarr),
+                                           _i(0 if J is None else 1))
+
     def bicgstab2(self, kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi, order=LEX):
         phi = phi.copy()
         args = [_i(kind), *self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(np.ascontiguousarray(invd)),

@@ -46,3 +46,9 @@ def test_bad_arguments_rejected_without_device_work():
         capi.compute_pd(8, 8, 0.125, 0.125, np.zeros((10, 10)), 3, 1)
     with pytest.raises(capi.MgpError):
         capi.compute_div(8, 8, 0.0, 0.125, np.zeros((8, 9)), np.zeros((9, 8)))
+    with pytest.raises(capi.MgpError):
+        capi.extrapolate_ghosts(8, 8, 4, np.zeros((10, 10)))
+    with pytest.raises(capi.MgpError):
+        capi.pnorm(8, 8, 0.125, 0.125, np.zeros((8, 8)), -1)
+    with pytest.raises(capi.MgpError):
+        capi.metric_fill(capi.Mapping(7, 0.0, 0.0, 0.0, 0.0), 8, 8, 0.125, 0.125)

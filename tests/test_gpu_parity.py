@@ -113,6 +113,30 @@ def test_div_of_grad_is_laplacian_2048():
         assert np.max(np.abs(div - lap)) <= 1e-14 * scale
 
 
+@pytest.mark.parametrize("n", [(16, 12), (33, 17), (1000, 700)])
+def test_reductions_and_extrapolation(n):
+    """pnorm p = 0..4, inner_prod, integrate2d against the oracle (sequential
+    sums; the GPU's are parallel and compensated: 1e-13 relative, the max norm
+    exact); extrapolate_ghosts orders 0..3 bit for bit."""
+    nx, ny = n
+    dx, dy = 1.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(nx)
+    a, b, J = rng.standard_normal((ny, nx)), rng.standard_normal((ny, nx)), 1 + rng.random((ny, nx))
+    o = rc.oracle()
+    assert mg.pnorm(nx, ny, dx, dy, a, 0) == o.pnorm(nx, ny, dx, dy, a, 0)
+    for p in (1, 2, 3, 4):
+        np.testing.assert_allclose(mg.pnorm(nx, ny, dx, dy, a, p), o.pnorm(nx, ny, dx, dy, a, p), rtol=1e-13)
+    np.testing.assert_allclose(mg.inner_prod(nx, ny, dx, dy, a, b), o.inner_prod(nx, ny, dx, dy, a, b),
+                               rtol=1e-12, atol=1e-14 * np.sqrt(nx * ny) * dx * dy)
+    np.testing.assert_allclose(mg.integrate2d(nx, ny, dx, dy, a, J), o.integrate2d(nx, ny, dx, dy, a, J),
+                               rtol=1e-12, atol=1e-14 * np.sqrt(nx * ny) * dx * dy)
+    np.testing.assert_allclose(mg.integrate2d(nx, ny, dx, dy, a), o.integrate2d(nx, ny, dx, dy, a),
+                               rtol=1e-12, atol=1e-14 * np.sqrt(nx * ny) * dx * dy)
+    phi = rng.standard_normal((ny + 2, nx + 2))
+    for order in range(4):
+        same(mg.extrapolate_ghosts(nx, ny, order, phi), o.extrapolate_ghosts(nx, ny, order, phi))
+
+
 @pytest.mark.parametrize("case", OPERATOR_CASES)
 def test_relax_gs_lex_and_mc(case):
     geo, bc, phi, rhs = setup(*case, seed=2)

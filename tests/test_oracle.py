@@ -66,6 +66,13 @@ def test_oracle_matches_golden_operators(golden, k, homog):
     np.testing.assert_array_equal(dinv, golden[p + "dc_invd"])
     np.testing.assert_array_equal(o.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi, rc.LEX), golden[p + "dc_lex3"])
     np.testing.assert_array_equal(o.dc_relax_gs(geo, lah, 1, dinv, 0.8, 3, rhs, phi, rc.RB), golden[p + "dc_rb3"])
+    v = np.ascontiguousarray(phi[1:-1, 1:-1])
+    red = [o.pnorm(geo.nx, geo.ny, geo.dx, geo.dy, v, q) for q in range(5)] + [
+        o.inner_prod(geo.nx, geo.ny, geo.dx, geo.dy, v, rhs), o.integrate2d(geo.nx, geo.ny, geo.dx, geo.dy, v, geo.J),
+        o.integrate2d(geo.nx, geo.ny, geo.dx, geo.dy, v)]
+    np.testing.assert_array_equal(red, golden[p + "reductions"])
+    for order in range(4):
+        np.testing.assert_array_equal(o.extrapolate_ghosts(geo.nx, geo.ny, order, phi), golden[p + f"extrap{order}"])
     np.testing.assert_array_equal(o.relax_jacobi(geo, bc, homog, inv, 0.8, 1e-3, 12, 0, rhs, phi)[0],
                                   golden[p + "jacobi"])
     np.testing.assert_array_equal(o.bicgstab1(geo, bc, homog, inv, 1e-8, 20, 2, 0, rhs, phi)[0],
@@ -215,6 +222,50 @@ def test_oracle_vs_reference_schedule_and_norms():
     a = np.random.default_rng(35).standard_normal((12, 20))
     for p in (0, 1, 2):
         assert o.pnorm(20, 12, 0.05, 0.1, a, p) == r.pnorm(20, 12, 0.05, 0.1, a, p)
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [(16, 12), (33, 17), (200, 150)])
+def test_oracle_vs_reference_reductions_and_extrapolation(n):
+    """pnorm for p = 0..4, inner_prod, integrate2d with and without J, and
+    extrapolate_ghosts of orders 0..3, bit for bit."""
+    o, r = rc.oracle(), rc.reference()
+    nx, ny = n
+    dx, dy = 1.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(nx)
+    a, b, J = rng.standard_normal((ny, nx)), rng.standard_normal((ny, nx)), 1 + rng.random((ny, nx))
+    for p in range(5):
+        assert o.pnorm(nx, ny, dx, dy, a, p) == r.pnorm(nx, ny, dx, dy, a, p)
+    assert o.inner_prod(nx, ny, dx, dy, a, b) == r.inner_prod(nx, ny, dx, dy, a, b)
+    assert o.integrate2d(nx, ny, dx, dy, a, J) == r.integrate2d(nx, ny, dx, dy, a, J)
+    assert o.integrate2d(nx, ny, dx, dy, a) == r.integrate2d(nx, ny, dx, dy, a)
+    phi = rng.standard_normal((ny + 2, nx + 2))
+    for order in range(4):
+        np.testing.assert_array_equal(o.extrapolate_ghosts(nx, ny, order, phi), r.extrapolate_ghosts(nx, ny, order, phi))
+
+
+def test_reductions_and_extrapolation_closed_forms():
+    """Extrapolation of order k reproduces polynomials of degree k along each
+    line (ghost cells, corners included); integrate2d of 1 with J is the sum
+    of J dx dy; inner_prod(a, a) = pnorm(a, 2)^2; the max norm is the largest
+    magnitude; pnorm(|a| const, p) = const (area 1)."""
+    o = rc.oracle()
+    nx, ny = 10, 8
+    dx, dy = 1.0 / nx, 1.0 / ny
+    ii, jj = np.meshgrid(np.arange(-1, nx + 1), np.arange(-1, ny + 1))
+    for order in range(4):
+        poly = (0.3 + 0.2 * ii - 0.1 * jj) ** order + 0.7
+        phi = poly.copy()
+        phi[0, :] = phi[-1, :] = phi[:, 0] = phi[:, -1] = 99.0
+        out = o.extrapolate_ghosts(nx, ny, order, phi)
+        np.testing.assert_allclose(out, poly, rtol=0, atol=1e-9 * np.max(np.abs(poly)))
+    rng = np.random.default_rng(5)
+    a, J = rng.standard_normal((ny, nx)), 1 + rng.random((ny, nx))
+    np.testing.assert_allclose(o.integrate2d(nx, ny, dx, dy, np.ones((ny, nx)), J), np.sum(J) * dx * dy, rtol=1e-14)
+    np.testing.assert_allclose(o.inner_prod(nx, ny, dx, dy, a, a), o.pnorm(nx, ny, dx, dy, a, 2) ** 2, rtol=1e-14)
+    assert o.pnorm(nx, ny, dx, dy, a, 0) == np.max(np.abs(a))
+    for p in (1, 2, 3, 4):
+        np.testing.assert_allclose(o.pnorm(nx, ny, dx, dy, np.full((ny, nx), -0.75), p), 0.75, rtol=1e-14)
 
 
 # --- 3. closed forms ----------------------------------------------------------
