@@ -264,6 +264,134 @@ __device__ __forceinline__ void row_update_add(const RowCtx &X, double *e, int j
 }
 
 // ---------------------------------------------------------------------------
+// Interface solve of the row system, one wave.  The 512 interface rows (2
+// per thread of the workgroup) sit in LDS lane-major: row q at
+// iface_pos(q) = (q % 8) * 64 + q / 8, so lane l reads its rows 8l .. 8l+7
+// without bank conflicts.  Lane l
+//  1. eliminates its 8 rows to 2 (the partition step of the chunks): its
+//     first row in terms of x(8l-1), x(8l), x(8l+7), its last in terms of
+//     x(8l), x(8l+7), x(8l+8);
+//  2. takes part in parallel cyclic reduction of those 128 rows (7 steps,
+//     neighbours exchanged with cross-lane shuffles, no barrier);
+//  3. back-substitutes its 6 inner unknowns and writes x(8l .. 8l+7) to X
+//     in natural order.
+// ---------------------------------------------------------------------------
+constexpr int IFACE = 2 * XL_THREADS;  // interface rows, padded
+
+// Quotients of the line solves: v_rcp_f64 refined by one Newton step (error
+// of order eps^2 relative, three dependent instructions instead of the
+// correctly rounded division sequence).  The line solves are part of the
+// preconditioner, so they need accuracy, not bit-exact rounding.
+__device__ __forceinline__ double rcp(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double quot(double a, double b) { return a * rcp(b); }
+
+__device__ __forceinline__ int iface_pos(int q) { return (q & 7) * 64 + (q >> 3); }
+
+struct Row4 {
+    double a, b, c, d;
+};
+
+__device__ __forceinline__ Row4 shfl_up4(const Row4 &r, int s) {
+    return Row4{__shfl_up(r.a, s), __shfl_up(r.b, s), __shfl_up(r.c, s), __shfl_up(r.d, s)};
+}
+__device__ __forceinline__ Row4 shfl_down4(const Row4 &r, int s) {
+    return Row4{__shfl_down(r.a, s), __shfl_down(r.b, s), __shfl_down(r.c, s), __shfl_down(r.d, s)};
+}
+
+// One PCR update of row r from its neighbours m (q - st) and p (q + st);
+// a missing neighbour contributes nothing.
+__device__ __forceinline__ Row4 pcr_update(const Row4 &r, const Row4 &m, bool has_m, const Row4 &p, bool has_p) {
+    const double k1 = has_m ? quot(r.a, m.b) : 0.0, k2 = has_p ? quot(r.c, p.b) : 0.0;
+    Row4 o;
+    o.a = has_m ? -m.a * k1 : 0.0;
+    o.c = has_p ? -p.c * k2 : 0.0;
+    o.b = r.b - (has_m ? m.c * k1 : 0.0) - (has_p ? p.a * k2 : 0.0);
+    o.d = r.d - (has_m ? m.d * k1 : 0.0) - (has_p ? p.d * k2 : 0.0);
+    return o;
+}
+
+__device__ __forceinline__ void interface_solve(double *IA, double *IB, const double *IC, double *ID, double *IX,
+                                               int l) {
+    constexpr int L = 8;
+    double a[L], b[L], c[L], d[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const int q = k * 64 + l;
+        a[k] = IA[q];
+        b[k] = IB[q];
+        c[k] = IC[q];
+        d[k] = ID[q];
+    }
+    // 1. partition: first row (backward sweep), last row (forward sweep)
+    double bu = b[L - 2], cu = c[L - 2], du = d[L - 2];
+#pragma unroll
+    for (int k = L - 3; k >= 0; --k) {
+        const double r = quot(c[k], bu);
+        bu = b[k] - r * a[k + 1];
+        cu = -r * cu;
+        du = d[k] - r * du;
+    }
+    Row4 f{a[0], bu, cu, du};
+#pragma unroll
+    for (int k = 2; k < L; ++k) {
+        const double r = quot(a[k], b[k - 1]);
+        a[k] = -r * a[k - 1];
+        b[k] = b[k] - r * c[k - 1];
+        d[k] = d[k] - r * d[k - 1];
+    }
+    Row4 g{a[L - 1], b[L - 1], c[L - 1], d[L - 1]};
+    // the inner rows go back to this lane's own LDS slots for step 3, so the
+    // PCR runs with only the two lane rows in registers (the clobber keeps
+    // the compiler from carrying them in registers across it)
+#pragma unroll
+    for (int k = 1; k < L - 1; ++k) {
+        const int q = k * 64 + l;
+        IA[q] = a[k];
+        IB[q] = b[k];
+        ID[q] = d[k];
+    }
+    asm volatile("" ::: "memory");
+    // 2. PCR over the 128 lane rows, Q = 2l (first) and 2l+1 (last)
+#pragma unroll
+    for (int st = 1; st < 128; st *= 2) {
+        Row4 fm, fp, gm, gp;
+        if (st == 1) {
+            fm = shfl_up4(g, 1);   // Q-1 of the first row: the previous lane's last
+            fp = g;                // Q+1: this lane's last
+            gm = f;                // Q-1 of the last row: this lane's first
+            gp = shfl_down4(f, 1); // Q+1: the next lane's first
+        } else {
+            const int sl = st / 2;
+            fm = shfl_up4(f, sl);
+            fp = shfl_down4(f, sl);
+            gm = shfl_up4(g, sl);
+            gp = shfl_down4(g, sl);
+        }
+        const int Qf = 2 * l, Qg = 2 * l + 1;
+        const Row4 nf = pcr_update(f, fm, Qf - st >= 0, fp, Qf + st < 128);
+        const Row4 ng = pcr_update(g, gm, Qg - st >= 0, gp, Qg + st < 128);
+        f = nf;
+        g = ng;
+    }
+    const double xs = quot(f.d, f.b), xe = quot(g.d, g.b);
+    // 3. back substitution of the inner unknowns
+    asm volatile("" ::: "memory");
+    IX[L * l] = xs;
+    IX[L * l + L - 1] = xe;
+    double xnext = xe;
+#pragma unroll
+    for (int k = L - 2; k >= 1; --k) {
+        const int q = k * 64 + l;
+        const double xk = quot(ID[q] - IA[q] * xs - IC[q] * xnext, IB[q]);
+        IX[L * l + k] = xk;
+        xnext = xk;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // On-the-fly solve of one row (fine levels, bandwidth-bound): coefficients
 // from the metrics, elimination, PCR and back substitution per call.
 // LDS: 4 nt (C+1) doubles.
@@ -303,7 +431,7 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
 #pragma unroll
         for (int k = C - 2; k >= 0; --k)
             if (k <= len - 3) {
-                const double r = cc[k] / bu;
+                const double r = quot(cc[k], bu);
                 bu = cb[k] - r * ca[k + 1];
                 cu = -r * cu;
                 du = cd[k] - r * du;
@@ -315,7 +443,7 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
 #pragma unroll
         for (int k = 2; k <= C; ++k)
             if (k <= len - 1) {
-                const double r = ca[k] / cb[k - 1];
+                const double r = quot(ca[k], cb[k - 1]);
                 ca[k] = -r * ca[k - 1];
                 cb[k] = cb[k] - r * cc[k - 1];
                 cd[k] = cd[k] - r * cd[k - 1];
@@ -326,70 +454,36 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
         od = longc ? cd[C] : cd[C - 1];
     }
 
-    // PCR on the 2 nt reduced unknowns (even: chunk starts).  LDS:
-    // [Ea Eb Ec Ed Oa Ob Oc Od], nt doubles each; entry q lives in half
-    // (q & 1) at index q >> 1.
-    const int n = 2 * nt;
-    for (int st = 1; st < n; st *= 2) {
-        if (tid < nt) {
-            lds[tid] = ea;
-            lds[nt + tid] = eb;
-            lds[2 * nt + tid] = ec;
-            lds[3 * nt + tid] = ed;
-            lds[4 * nt + tid] = oa;
-            lds[5 * nt + tid] = ob;
-            lds[6 * nt + tid] = oc;
-            lds[7 * nt + tid] = od;
-        }
-        __syncthreads();
-        if (tid < nt) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int qq = 2 * tid + h;
-                const double a = h ? oa : ea, bb = h ? ob : eb, c = h ? oc : ec, d = h ? od : ed;
-                double k1 = 0.0, k2 = 0.0, am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
-                const int qm = qq - st, qp = qq + st;
-                if (qm >= 0) {
-                    const int o = (qm & 1) * 4 * nt + (qm >> 1);
-                    k1 = a / lds[o + nt];
-                    am = lds[o];
-                    cm = lds[o + 2 * nt];
-                    dm = lds[o + 3 * nt];
-                }
-                if (qp < n) {
-                    const int o = (qp & 1) * 4 * nt + (qp >> 1);
-                    k2 = c / lds[o + nt];
-                    ap = lds[o];
-                    cp = lds[o + 2 * nt];
-                    dp = lds[o + 3 * nt];
-                }
-                const double na = -am * k1, nc = -cp * k2;
-                const double nbb = bb - cm * k1 - ap * k2, nd = d - dm * k1 - dp * k2;
-                if (h) {
-                    oa = na;
-                    ob = nbb;
-                    oc = nc;
-                    od = nd;
-                } else {
-                    ea = na;
-                    eb = nbb;
-                    ec = nc;
-                    ed = nd;
-                }
-            }
-        }
-        __syncthreads();
+    // Interface system: the 2 nt chunk-end unknowns (start of chunk t at
+    // q = 2t, end at q = 2t+1; tridiagonal in that order), padded to 512 rows
+    // with identity rows by the threads past nt.  Staged once through LDS,
+    // solved by wave 0 (interface_solve), read back once: two barriers.
+    double *IA = lds + NT1, *IB = IA + IFACE, *IC = IB + IFACE, *ID = IC + IFACE, *IX = ID + IFACE;
+    {
+        const int q0 = iface_pos(2 * tid), q1 = iface_pos(2 * tid + 1);
+        IA[q0] = ea;
+        IB[q0] = eb;
+        IC[q0] = ec;
+        ID[q0] = ed;
+        IA[q1] = oa;
+        IB[q1] = ob;
+        IC[q1] = oc;
+        ID[q1] = od;
     }
+    __syncthreads();
+    if (tid < 64) interface_solve(IA, IB, IC, ID, IX, tid);
+    __syncthreads();
+    const double x_start = IX[2 * tid], x_end = IX[2 * tid + 1];
 
     // back substitution, then the coalesced update
     if (tid < nt) {
-        const double xs = ed / eb, xe = od / ob;
+        const double xs = x_start, xe = x_end;
         const bool longc = len == CP;
         double xnext = xe;
 #pragma unroll
         for (int k = C - 1; k >= 1; --k)
             if (k <= len - 2) {
-                const double xk = (cd[k] - ca[k] * xs - cc[k] * xnext) / cb[k];
+                const double xk = quot(cd[k] - ca[k] * xs - cc[k] * xnext, cb[k]);
                 lds[tid * CP + k] = xk;
                 xnext = xk;
             }
@@ -488,8 +582,9 @@ size_t xline_lds_bytes(int nx) {
     const int C = xline_chunk(nx);
     if (C == 0) return (size_t)-1;
     const size_t nt = (size_t)(nx / C);
-    const size_t tr = 4 * nt * (C + 1), pcr = 8 * nt;
-    return (tr > pcr ? tr : pcr) * sizeof(double);
+    // chunk staging (4 arrays), or one chunk array plus the interface rows
+    const size_t tr = 4 * nt * (C + 1), iface = nt * (C + 1) + 5 * IFACE;
+    return (tr > iface ? tr : iface) * sizeof(double);
 }
 
 bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
