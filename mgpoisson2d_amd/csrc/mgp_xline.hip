@@ -273,8 +273,9 @@ __device__ __forceinline__ void row_update_add(const RowCtx &X, double *e, int j
 //     x(8l), x(8l+7), x(8l+8);
 //  2. takes part in parallel cyclic reduction of those 128 rows (7 steps,
 //     neighbours exchanged with cross-lane shuffles, no barrier);
-//  3. back-substitutes its 6 inner unknowns and writes x(8l .. 8l+7) to X
-//     in natural order.
+//  3. evaluates its 6 inner unknowns from the first and last ones (their
+//     linear forms were set up in step 1) and writes x(8l .. 8l+7) to X in
+//     natural order.
 // ---------------------------------------------------------------------------
 constexpr int IFACE = 2 * XL_THREADS;  // interface rows, padded
 
@@ -343,15 +344,22 @@ __device__ __forceinline__ void interface_solve(double *IA, double *IB, const do
         d[k] = d[k] - r * d[k - 1];
     }
     Row4 g{a[L - 1], b[L - 1], c[L - 1], d[L - 1]};
-    // the inner rows go back to this lane's own LDS slots for step 3, so the
-    // PCR runs with only the two lane rows in registers (the clobber keeps
-    // the compiler from carrying them in registers across it)
+    // inner unknowns as x_k = G_k + H_k x_first + M_k x_last, kept in this
+    // lane's own LDS slots (IA, IB, ID) for step 3, so that the PCR runs with
+    // only the two lane rows in registers and step 3 has no dependent chain
+    {
+        double G = 0.0, H = 0.0, M = 1.0;
 #pragma unroll
-    for (int k = 1; k < L - 1; ++k) {
-        const int q = k * 64 + l;
-        IA[q] = a[k];
-        IB[q] = b[k];
-        ID[q] = d[k];
+        for (int k = L - 2; k >= 1; --k) {
+            const double ib = rcp(b[k]);
+            G = (d[k] - c[k] * G) * ib;
+            H = (-a[k] - c[k] * H) * ib;
+            M = (-c[k] * M) * ib;
+            const int q = k * 64 + l;
+            IA[q] = G;
+            IB[q] = H;
+            ID[q] = M;
+        }
     }
     asm volatile("" ::: "memory");
     // 2. PCR over the 128 lane rows, Q = 2l (first) and 2l+1 (last)
@@ -381,13 +389,10 @@ __device__ __forceinline__ void interface_solve(double *IA, double *IB, const do
     asm volatile("" ::: "memory");
     IX[L * l] = xs;
     IX[L * l + L - 1] = xe;
-    double xnext = xe;
 #pragma unroll
-    for (int k = L - 2; k >= 1; --k) {
+    for (int k = 1; k <= L - 2; ++k) {
         const int q = k * 64 + l;
-        const double xk = quot(ID[q] - IA[q] * xs - IC[q] * xnext, IB[q]);
-        IX[L * l + k] = xk;
-        xnext = xk;
+        IX[L * l + k] = IA[q] + IB[q] * xs + ID[q] * xe;
     }
 }
 
@@ -423,6 +428,7 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
     // start row:  a_0 x_{prev end} + bu x_s + cu x_e = du
     // end row:    a'_L x_s + b'_L x_e + c_L x_{next start} = d'_L
     double ea = 0.0, eb = 1.0, ec = 0.0, ed = 0.0, oa = 0.0, ob = 1.0, oc = 0.0, od = 0.0;
+    double sg[CP], sh[CP], sm[CP];
     if (tid < nt) {
         const bool longc = len == CP;
         double bu = longc ? cb[C - 1] : cb[C - 2];
@@ -452,6 +458,21 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
         ob = longc ? cb[C] : cb[C - 1];
         oc = longc ? cc[C] : cc[C - 1];
         od = longc ? cd[C] : cd[C - 1];
+        // inner unknowns as x_k = sg_k + sh_k x_s + sm_k x_e (the back
+        // substitution unrolled once here, off the critical path of the
+        // interface solve): ca/cb/cc/cd are dead after this
+        double G = 0.0, H = 0.0, M = 1.0;
+#pragma unroll
+        for (int k = C - 1; k >= 1; --k)
+            if (k <= len - 2) {
+                const double ib = rcp(cb[k]);
+                G = (cd[k] - cc[k] * G) * ib;
+                H = (-ca[k] - cc[k] * H) * ib;
+                M = (-cc[k] * M) * ib;
+                sg[k] = G;
+                sh[k] = H;
+                sm[k] = M;
+            }
     }
 
     // Interface system: the 2 nt chunk-end unknowns (start of chunk t at
@@ -479,14 +500,9 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
     if (tid < nt) {
         const double xs = x_start, xe = x_end;
         const bool longc = len == CP;
-        double xnext = xe;
 #pragma unroll
-        for (int k = C - 1; k >= 1; --k)
-            if (k <= len - 2) {
-                const double xk = quot(cd[k] - ca[k] * xs - cc[k] * xnext, cb[k]);
-                lds[tid * CP + k] = xk;
-                xnext = xk;
-            }
+        for (int k = 1; k <= C - 1; ++k)
+            if (k <= len - 2) lds[tid * CP + k] = sg[k] + sh[k] * xs + sm[k] * xe;
         lds[tid * CP] = xs;
         lds[tid * CP + (longc ? C : C - 1)] = xe;
     }
