@@ -522,7 +522,7 @@ __device__ __forceinline__ void xline_rows(double *__restrict__ e, const double 
                                            const double *__restrict__ K, const Grid &g, const Geo &geo, int row0,
                                            int rowstep, int nt, int extra0, int extra1) {
     extern __shared__ double lds[];
-    const RowCtx X = row_ctx(g, nt, C);
+    const RowCtx X0 = row_ctx(g, nt, C);
     const int slot = launch_slot();
     const int ex0 = slot == 0 ? extra0 : -1, ex1 = slot == (int)gridDim.x - 1 ? extra1 : -1;
 #pragma unroll 1
@@ -530,6 +530,11 @@ __device__ __forceinline__ void xline_rows(double *__restrict__ e, const double 
         const int jr = r == 0 ? row0 + slot * rowstep : (r == 1 ? ex0 : ex1);
         if (jr < 0) continue;  // uniform over the workgroup
         if (r) __syncthreads();
+        // the chunk remainder is made opaque per row, so the per-thread LDS
+        // positions derived from it are recomputed in the row (a few integer
+        // ops) instead of being hoisted out of the loop and spilled
+        RowCtx X = X0;
+        asm volatile("" : "+s"(X.rem));
         xline_row<C>(X, e, f, K, g, geo, jr, lds);
     }
 }
