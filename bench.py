@@ -15,6 +15,10 @@ stretched wall are ~1e7 times the J-scaled rhs), so no FP64 solution vector
 reaches 1e-10 there; 1e-10 is reachable up to 1024^2 (--n 1024 --tol 1e-10).
 
 A step is one complete solve from phi = 0 with rhs already resident in HBM.
+The same solves with the preconditioner V-cycle in FP32 (KRYLOV_MG32; the
+Krylov vectors, operator, residual and solution stay FP64) are timed beside
+them and reported under "mixed", with their own relative residual and their
+distance from the FP64 solution.
 Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
 """
 from __future__ import annotations
@@ -92,6 +96,9 @@ def main():
     ap.add_argument("--sd", type=int, default=2, help="pre-smoothing sweeps per level")
     ap.add_argument("--su", type=int, default=2, help="post-smoothing sweeps per level")
     ap.add_argument("--sb", type=int, default=2, help="sweeps on the coarsest level")
+    ap.add_argument("--method", choices=("fp64", "mixed"), default="fp64",
+                    help="solver of the headline value: FP64 cycles (default) or FP32 cycles in FP64 BiCGStab")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--kernels-only", action="store_true",
@@ -106,51 +113,89 @@ def main():
 
     from mgpoisson2d_amd import capi
     geo, bc, rhs = config2_problem(args.n, args.n)
-    solver = capi.Solver(geo, bc, method=capi.METHOD_KRYLOV_MG, smooth_down=args.sd, smooth_up=args.su,
-                         smooth_bottom=args.sb, use_graph=not args.no_graph)
-    solver.set_rhs(rhs)
+    main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
+    other_method = capi.METHOD_KRYLOV_MG if args.method == "mixed" else capi.METHOD_KRYLOV_MG32
+
+    def make(method):
+        s = capi.Solver(geo, bc, method=method, smooth_down=args.sd, smooth_up=args.su, smooth_bottom=args.sb,
+                        use_graph=not args.no_graph)
+        s.set_rhs(rhs)
+        return s
+
+    solver = make(main_method)
     print(f"# setup done: {args.n}x{args.n}", file=sys.stderr, flush=True)
     if args.kernels_only:
+        mixed = make(capi.METHOD_KRYLOV_MG32) if not args.no_mixed else None
         for cold in (True, False):
-            for k, name in ((0, "x-line even rows"), (1, "residual"), (2, "copy")):
-                ms, nbytes = solver.time_kernel(k, reps=5, cold=cold)
+            for s, k, name in ((solver, 0, "x-line even rows"), (solver, 1, "residual"), (solver, 2, "copy"),
+                               (mixed, 3, "x-line even rows FP32")):
+                if s is None:
+                    continue
+                ms, nbytes = s.time_kernel(k, reps=5, cold=cold)
                 print(f"# {'cold' if cold else 'warm'} {name}: {ms:.4f} ms/launch, {nbytes / 1e6:.1f} MB "
                       f"algorithmic, {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+        if mixed is not None:
+            mixed.close()
         solver.close()
         return
 
-    for _ in range(args.warmup):
-        iters, outcome = solver.solve_fast(args.tol, 400, zerophi=True)
-        if outcome != capi.CONVERGED:
-            raise SystemExit(f"warmup solve did not converge (outcome {outcome}, {iters} iterations)")
-    capi.device_synchronize()
-    t0 = time.perf_counter()
-    cycles = 0
-    for _ in range(args.steps):
-        iters, outcome = solver.solve_fast(args.tol, 400, zerophi=True)
-        cycles += solver.last_cycles()
-        if outcome != capi.CONVERGED:
-            raise SystemExit(f"solve did not converge (outcome {outcome})")
-    capi.device_synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = 1e3 * dt / args.steps
-    cycles_per_solve = cycles / args.steps
-    print(f"# {args.steps} solves in {dt:.3f} s, {cycles_per_solve:.1f} V-cycles per solve",
-          file=sys.stderr, flush=True)
+    def timed(s, label):
+        for _ in range(args.warmup):
+            iters, outcome = s.solve_fast(args.tol, 400, zerophi=True)
+            if outcome != capi.CONVERGED:
+                raise SystemExit(f"{label} warmup solve did not converge (outcome {outcome}, {iters} iterations)")
+        capi.device_synchronize()
+        t0 = time.perf_counter()
+        cycles = 0
+        for _ in range(args.steps):
+            iters, outcome = s.solve_fast(args.tol, 400, zerophi=True)
+            cycles += s.last_cycles()
+            if outcome != capi.CONVERGED:
+                raise SystemExit(f"{label} solve did not converge (outcome {outcome})")
+        capi.device_synchronize()
+        dt = time.perf_counter() - t0
+        print(f"# {label}: {args.steps} solves in {dt:.3f} s, {cycles / args.steps:.1f} V-cycles per solve",
+              file=sys.stderr, flush=True)
+        # verify the last solution against the operator itself
+        relres, _, _ = s.solve(args.tol, 400, zerophi=True)
+        return dt, cycles / args.steps, float(relres[-1])
 
-    # verify the last solution against the operator itself (1e-10 relative)
-    relres, _, _ = solver.solve(args.tol, 400, zerophi=True)
+    dt, cycles_per_solve, final_relres = timed(solver, args.method)
+    ms_per_step = 1e3 * dt / args.steps
+
+    compare = None
+    if not args.no_mixed:
+        other = make(other_method)
+        odt, ocycles, orelres = timed(other, "fp64" if args.method == "mixed" else "mixed")
+        a, b = solver.get_phi()[1:-1, 1:-1], other.get_phi()[1:-1, 1:-1]
+        diff = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        mixed_s, fp64_s = (solver, other) if args.method == "mixed" else (other, solver)
+        mixed_dt, fp64_dt = (dt, odt) if args.method == "mixed" else (odt, dt)
+        mk_ms, mk_bytes = mixed_s.time_kernel(3, reps=20, cold=True)
+        compare = {
+            "mixed_solves_per_s": args.steps / mixed_dt,
+            "fp64_solves_per_s": args.steps / fp64_dt,
+            "mixed_over_fp64": fp64_dt / mixed_dt,
+            "mixed_v_cycles_per_solve": ocycles if args.method == "fp64" else cycles_per_solve,
+            "mixed_final_relres": orelres if args.method == "fp64" else final_relres,
+            "max_rel_diff_mixed_vs_fp64_phi": diff,
+            "xline_fp32_cold_ms": mk_ms,
+            "xline_fp32_GBps": mk_bytes / (mk_ms * 1e-3) / 1e9,
+        }
+        other.close()
 
     # dominant kernel: finest-level x-line relaxation (even rows), HIP events
     # around back-to-back launches whose operands are cache-cold (as inside a solve)
-    k_ms, k_bytes = solver.time_kernel(0, reps=20, cold=True)
+    mixed_main = args.method == "mixed"
+    rk = 3 if mixed_main else 0
+    k_ms, k_bytes = solver.time_kernel(rk, reps=20, cold=True)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
-    kw_ms, _ = solver.time_kernel(0, reps=20, cold=False)
+    kw_ms, _ = solver.time_kernel(rk, reps=20, cold=False)
     r_ms, r_bytes = solver.time_kernel(1, reps=20, cold=True)
     c_ms, c_bytes = solver.time_kernel(2, reps=20, cold=True)
 
     traffic = None
-    if args.n == NX and os.path.exists(args.traffic):
+    if args.n == NX and not mixed_main and os.path.exists(args.traffic):
         with open(args.traffic) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
@@ -169,23 +214,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32" if mixed_main else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"FP64 single solve, {args.n}x{args.n} warped/stretched mapped grid "
+            "workload": f"{'mixed-precision' if mixed_main else 'FP64'} single solve, {args.n}x{args.n} "
+                        f"warped/stretched mapped grid "
                         f"(A=0.1, beta=5), Dirichlet xi / Neumann eta, to {args.tol:g} relative residual",
             "tol": args.tol,
-            "method": f"FP64 BiCGStab + V({args.sd},{args.su}) y-semicoarsening x-line multigrid preconditioner "
-                      f"({args.sb} sweeps on the coarsest level)",
+            "method": f"FP64 BiCGStab + {'FP32 ' if mixed_main else ''}V({args.sd},{args.su}) y-semicoarsening "
+                      f"x-line multigrid preconditioner ({args.sb} sweeps on the coarsest level)",
             "v_cycles_per_solve": cycles_per_solve,
-            "final_relres": float(relres[-1]),
+            "final_relres": final_relres,
             "residual_kernel_ms": r_ms,
             "residual_kernel_GBps": r_bytes / (r_ms * 1e-3) / 1e9,
             "copy_kernel_GBps": c_bytes / (c_ms * 1e-3) / 1e9,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_xline (finest level, even rows)",
+            "kernel": f"k_xline{' FP32' if mixed_main else ''} (finest level, even rows)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -197,6 +243,7 @@ def main():
             "alg_bytes_per_launch": k_bytes,
         },
         "cpu_baseline": cpu,
+        "mixed": compare,
     }
     print(json.dumps(line), flush=True)
     solver.close()

@@ -198,7 +198,10 @@ enum {
     MGP_METHOD_DC_VCYCLE = 1,  /* DeferredMGPoisson2D::vcycle on device data */
     MGP_METHOD_KRYLOV_MG = 2,  /* FP64 BiCGStab on the reference operator, preconditioned by one
                                   semicoarsening (y) x-line-relaxation V-cycle per application */
-    MGP_METHOD_FMG = 3         /* MGPoisson2D::fmg on device data */
+    MGP_METHOD_FMG = 3,        /* MGPoisson2D::fmg on device data */
+    MGP_METHOD_KRYLOV_MG32 = 4 /* KRYLOV_MG with the preconditioner V-cycle in FP32 (metrics,
+                                  stencil tables, corrections, residuals); the Krylov vectors, the
+                                  operator, the true residual and the solution stay FP64 */
 };
 
 typedef struct {
@@ -246,10 +249,11 @@ int mgp_solver_field(mgp_solver *s, int which, double **ptr, int *pitch);
 void *mgp_solver_stream(mgp_solver *s);
 /* Number of V-cycle applications made by the last solve. */
 int mgp_solver_last_cycles(const mgp_solver *s);
-/* Measurement hook (KRYLOV_MG): times `reps` launches of one finest-level
+/* Measurement hook (KRYLOV_MG, KRYLOV_MG32): times `reps` launches of one finest-level
  * kernel with HIP events on the solver's stream.  kernel 0: x-line
  * relaxation of the even interior rows; 1: residual; 2: plain copy of one
- * finest-level field (bandwidth reference).  cold != 0: back-to-back launches
+ * finest-level field (bandwidth reference); 3 (KRYLOV_MG32 only): the FP32
+ * x-line relaxation of the even interior rows.  cold != 0: back-to-back launches
  * that take four copies of the operands in turn, so that each launch finds its
  * operands evicted from the L2s and the MALL (as the kernels run inside a
  * solve); cold == 0: back-to-back launches on cache-resident operands.

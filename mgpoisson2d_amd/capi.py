@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmgp2d.so
 MGP_OK = 0
 CONVERGED, DIVERGED, MAXITERS = 0, 1, 2
 ORDER_LEX, ORDER_RB, ORDER_MC = 0, 1, 2
-METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG, METHOD_FMG = 0, 1, 2, 3
+METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG, METHOD_FMG, METHOD_KRYLOV_MG32 = 0, 1, 2, 3, 4
 MAP_TERRAIN, MAP_WARPED = 0, 1
 
 _d = ctypes.c_double

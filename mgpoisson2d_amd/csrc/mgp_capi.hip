@@ -724,7 +724,7 @@ int create_solver(mgp_solver **out, int nx, int ny, const mgp_bc *bc, const mgp_
         if (plain || o.method == MGP_METHOD_DC_VCYCLE) {
             s->ref.build(s->stream, s->geo, s->ubc.bc, o.method == MGP_METHOD_DC_VCYCLE, o.ordering, o.maxdepth);
             s->ref.fmg = o.method == MGP_METHOD_FMG;
-        } else if (o.method == MGP_METHOD_KRYLOV_MG)
+        } else if (o.method == MGP_METHOD_KRYLOV_MG || o.method == MGP_METHOD_KRYLOV_MG32)
             s->kmg.build(s->stream, s->geo, s->ubc.bc, o);
         else {
             g_last_error = "unknown method";
@@ -841,7 +841,7 @@ namespace {
 int run_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters) {
     const mgp_solver_opts &o = s->opts;
     int it = 0, oc = MGP_MAXITERS;
-    if (o.method == MGP_METHOD_KRYLOV_MG) {
+    if (o.method == MGP_METHOD_KRYLOV_MG || o.method == MGP_METHOD_KRYLOV_MG32) {
         oc = s->kmg.solve(s->W, s->stream, s->geo, s->ubc.bc, o.homog, s->phi.p, s->rhs.p, tol, maxiters, zerophi,
                           relres, &it);
         s->last_cycles = s->kmg.last_cycles;
@@ -929,8 +929,9 @@ int mgp_solver_last_cycles(const mgp_solver *s) { return s ? s->last_cycles : 0;
 
 int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double *avg_ms, double *bytes) {
     return guarded([&] {
-        if (s->opts.method != MGP_METHOD_KRYLOV_MG || reps <= 0 || kernel < 0 || kernel > 2) {
-            g_last_error = "time_kernel: KRYLOV_MG solvers, kernel 0, 1 or 2, reps > 0";
+        const bool kmg = s->opts.method == MGP_METHOD_KRYLOV_MG || s->opts.method == MGP_METHOD_KRYLOV_MG32;
+        if (!kmg || reps <= 0 || kernel < 0 || kernel > 3) {
+            g_last_error = "time_kernel: KRYLOV_MG / KRYLOV_MG32 solvers, kernel 0 to 3, reps > 0";
             throw Failure(MGP_ERR_ARG);
         }
         *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, cold, bytes);

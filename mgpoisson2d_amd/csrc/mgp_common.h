@@ -42,6 +42,12 @@ struct Geo {
     const double *J, *Jxx, *Jxy, *Jyx, *Jyy;
 };
 
+// FP32 copy of a level's face metrics, same layout (pitch and offsets in
+// elements): the mixed-precision preconditioner reads these.
+struct GeoF {
+    const float *J, *Jxx, *Jxy, *Jyx, *Jyy;
+};
+
 // Boundary description as the kernels see it.  val[] is the UNIFORM value;
 // data[] points at device NONUNIFORM face data (ny values for x sides, nx for
 // y sides) or is null.

@@ -21,22 +21,39 @@ struct Failure {
 
 void hip_check(hipError_t e, const char *what);
 
-// Owning device allocation of doubles, zero-filled on allocation.
-struct DevBuf {
-    double *p = nullptr;
+// Zero-filled device allocation of `bytes` (throws MGP_ERR_NOMEM); dev_free
+// releases it.
+void *dev_alloc_zero(size_t bytes);
+void dev_free(void *p);
+
+// Owning device allocation of `count` elements of T, zero-filled on allocation.
+template <typename T>
+struct DevArr {
+    T *p = nullptr;
     size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    DevBuf &operator=(DevBuf &&o) noexcept {
+    DevArr() = default;
+    DevArr(const DevArr &) = delete;
+    DevArr &operator=(const DevArr &) = delete;
+    DevArr(DevArr &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevArr &operator=(DevArr &&o) noexcept {
         if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
         return *this;
     }
-    ~DevBuf() { release(); }
-    void alloc(size_t count);
-    void release();
+    ~DevArr() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        p = static_cast<T *>(dev_alloc_zero(count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) dev_free(p);
+        p = nullptr;
+        n = 0;
+    }
 };
+using DevBuf = DevArr<double>;
+using DevBufF = DevArr<float>;
 
 Grid make_grid(int nx, int ny, double dx, double dy);
 void upload_block(hipStream_t s, double *dev, const Grid &g, const double *host, int i0, int j0, int w, int h);

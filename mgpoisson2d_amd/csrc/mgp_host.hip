@@ -30,25 +30,22 @@ void hip_check(hipError_t e, const char *what) {
 // --------------------------------------------------------------------------
 // Device buffers
 // --------------------------------------------------------------------------
-void DevBuf::alloc(size_t count) {
-    release();
-    if (count == 0) return;
-    if (hipMalloc(&p, count * sizeof(double)) != hipSuccess) {
-        p = nullptr;
+void *dev_alloc_zero(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
         g_last_error = "hipMalloc failed";
         throw Failure(MGP_ERR_NOMEM);
     }
-    n = count;
     // zero-fill, then wait: the library's streams are non-blocking, so work
     // they enqueue next is not ordered after the null stream otherwise
-    hip_check(hipMemset(p, 0, count * sizeof(double)), "hipMemset");
-    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (hipMemset(p, 0, bytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(p);
+        g_last_error = "hipMemset of a new allocation failed";
+        throw Failure(MGP_ERR_HIP);
+    }
+    return p;
 }
-void DevBuf::release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-}
+void dev_free(void *p) { (void)hipFree(p); }
 
 Grid make_grid(int nx, int ny, double dx, double dy) {
     Grid g;

@@ -87,16 +87,33 @@ void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, c
 // rc = 1x2 (y) average of the fine residual f - L e (table at ring cells); ec = 0.
 void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
                                 const double *K, const Grid &gf, const Grid &gc, const Geo &geo);
+// FP32 form (mixed-precision preconditioner): same layout and formulas.
+void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
+                                const Grid &gf, const Grid &gc, const GeoF &geo);
 bool xline_supported(int nx);
-size_t xline_lds_bytes(int nx);
+// dynamic LDS of one x-line workgroup for elements of `elem` bytes
+size_t xline_lds_bytes(int nx, size_t elem);
 // Relax rows j = row0 + k*rowstep, k < nrows (one workgroup each; no row of a
 // launch may read another: interior rows at least two apart, boundary rows
 // three).  The first workgroup then also relaxes row extra0, the last one row
 // extra1 (< 0: none), each after its own row.
 void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
                   int row0, int rowstep, int nrows, int extra0, int extra1);
+void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
+                  int rowstep, int nrows, int extra0, int extra1);
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
                       double shi);
+void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
+                      double shi);
+// Precision conversions over a level array.  ghosted: also the ghost ring.
+void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted);
+void launch_convert(hipStream_t s, float *out, const double *in, const Grid &g, int ghosted);
+void launch_convert(hipStream_t s, double *out, const float *in, const Grid &g, int ghosted);
+// out[k] = (float)in[k], k < n (flat arrays: stencil tables, whole padded levels)
+void launch_convert_flat(hipStream_t s, float *out, const double *in, size_t n);
+// Start of an FP32 V-cycle: f32 = (float)f over valid cells, e32 = 0 over
+// valid and ghost cells.
+void launch_begin32(hipStream_t s, float *f32, float *e32, const double *f, const Grid &g);
 void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, double b, const double *y, double c,
                      const double *z, const Grid &g);
 void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sweeps, int homog);

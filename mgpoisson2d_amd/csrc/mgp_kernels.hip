@@ -899,17 +899,49 @@ namespace mgp {
 
 // y-direction prolongation for 1x2 semicoarsening: linear in y, coarse
 // ghosts reflected (Neumann, sign +1) or negated (Dirichlet, sign -1).
-__global__ void __launch_bounds__(256) k_prolong_y(double *ef, const double *ec, Grid gf, Grid gc, double slo,
-                                                   double shi) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong_y(T *ef, const T *ec, Grid gf, Grid gc, T slo, T shi) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= gf.nx || j >= gf.ny) return;
     const int J = j >> 1;
-    const double c = ec[ix(gc.pitch, i, J)];
-    double nb;
+    const T c = ec[ix(gc.pitch, i, J)];
+    T nb;
     if (j & 1) nb = J + 1 < gc.ny ? ec[ix(gc.pitch, i, J + 1)] : shi * c;
     else nb = J - 1 >= 0 ? ec[ix(gc.pitch, i, J - 1)] : slo * c;
-    ef[ix(gf.pitch, i, j)] += 0.75 * c + 0.25 * nb;
+    ef[ix(gf.pitch, i, j)] += T(0.75) * c + T(0.25) * nb;
+}
+
+template <typename TO, typename TI>
+__global__ void __launch_bounds__(256) k_convert(TO *out, const TI *in, Grid g, int ghosted) {
+    const int o = ghosted ? 1 : 0;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63) - o;
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6) - o;
+    if (i >= g.nx + o || j >= g.ny + o) return;
+    const size_t c = ix(g.pitch, i, j);
+    out[c] = (TO)in[c];
+}
+
+__global__ void __launch_bounds__(256) k_convert_flat(float *out, const double *in, size_t n) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) out[k] = (float)in[k];
+}
+
+__global__ void __launch_bounds__(256) k_setval_f(float *y, float v, Grid g, int ghosted) {
+    const int o = ghosted ? 1 : 0;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63) - o;
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6) - o;
+    if (i >= g.nx + o || j >= g.ny + o) return;
+    y[ix(g.pitch, i, j)] = v;
+}
+
+__global__ void __launch_bounds__(256) k_begin32(float *f32, float *e32, const double *f, Grid g) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63) - 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6) - 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t c = ix(g.pitch, i, j);
+    e32[c] = 0.0f;
+    if (i >= 0 && i < g.nx && j >= 0 && j < g.ny) f32[c] = (float)f[c];
 }
 
 // out = a*x + b*y + c*z over valid cells (null operands skipped).
@@ -1104,6 +1136,35 @@ namespace mgp {
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
                       double shi) {
     k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, slo, shi);
+}
+
+void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
+                      double shi) {
+    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, (float)slo, (float)shi);
+}
+
+void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted) {
+    const int o = ghosted ? 2 : 0;
+    k_setval_f<<<tiles(g.nx + o, g.ny + o), 256, 0, s>>>(y, v, g, ghosted);
+}
+
+void launch_convert(hipStream_t s, float *out, const double *in, const Grid &g, int ghosted) {
+    const int o = ghosted ? 2 : 0;
+    k_convert<<<tiles(g.nx + o, g.ny + o), 256, 0, s>>>(out, in, g, ghosted);
+}
+
+void launch_convert(hipStream_t s, double *out, const float *in, const Grid &g, int ghosted) {
+    const int o = ghosted ? 2 : 0;
+    k_convert<<<tiles(g.nx + o, g.ny + o), 256, 0, s>>>(out, in, g, ghosted);
+}
+
+void launch_convert_flat(hipStream_t s, float *out, const double *in, size_t n) {
+    if (n == 0) return;
+    k_convert_flat<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(out, in, n);
+}
+
+void launch_begin32(hipStream_t s, float *f32, float *e32, const double *f, const Grid &g) {
+    k_begin32<<<tiles(g.nx + 2, g.ny + 2), 256, 0, s>>>(f32, e32, f, g);
 }
 void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, double b, const double *y, double c,
                      const double *z, const Grid &g) {

@@ -13,6 +13,15 @@
 // whose extrapolated cross terms reach two rows in), 1x2 averaging
 // restriction and linear-in-y prolongation (new: the reference has no
 // semicoarsened prolongation, MGPoisson2D.f90:385-388).
+//
+// Mixed precision (MGP_METHOD_KRYLOV_MG32): the same V-cycle on FP32 copies
+// of the level metrics and stencil tables, with FP32 corrections and
+// residuals.  The Krylov vectors, the operator A, the true residual and the
+// solution update stay FP64; the preconditioner's input is rounded to FP32 on
+// entry and its output widened on exit.  A preconditioner only has to
+// approximate A^-1, so its rounding changes the iteration count, not the
+// accuracy of the converged solution, which is set by the FP64 residual
+// (with restarts from the FP64 true residual: iterative refinement).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,12 +42,15 @@ constexpr int MIN_COARSE_NY = 4;
 struct KLevel {
     Grid g;
     Geo geo;
+    GeoF geo32;
     Bc bc;
     DevBuf J, Jxx, Jxy, Jyx, Jyy, e, r, w, K;
+    DevBufF Jxx32, Jxy32, Jyx32, Jyy32, e32, r32, K32;  // mixed precision only
 };
 
 struct KrylovMG::Impl {
     std::vector<KLevel> L;
+    bool mixed = false;
     DevBuf r, rhat, p, v, s, t, phat, shat, scal, partial;
     double *host = nullptr;  // pinned copy of the device scalars
     int sd = 2, su = 2, sb = 2;
@@ -69,8 +81,10 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         g_last_error = "KRYLOV_MG: row too long for one workgroup's line solve (nx <= 4111 in FP64)";
         throw Failure(MGP_ERR_UNSUPPORTED);
     }
+    const bool mixed = o.method == MGP_METHOD_KRYLOV_MG32;
     delete impl;
     impl = new Impl;
+    impl->mixed = mixed;
     impl->sd = o.smooth_down > 0 ? o.smooth_down : 1;
     impl->su = o.smooth_up > 0 ? o.smooth_up : 1;
     impl->sb = o.smooth_bottom > 0 ? o.smooth_bottom : 1;
@@ -83,7 +97,11 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         KLevel &l = L.back();
         l.g = make_grid(top.g.nx, ny, top.g.dx, dy);
         const size_t n = level_elems(l.g.nx, l.g.ny);
-        for (DevBuf *b : {&l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy, &l.e, &l.r, &l.w}) b->alloc(n);
+        for (DevBuf *b : {&l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy}) b->alloc(n);
+        // FP64 level fields: the FP64 cycle, and on the finest level the
+        // kernel timings (time_kernel)
+        if (!mixed || L.size() == 1)
+            for (DevBuf *b : {&l.e, &l.r, &l.w}) b->alloc(n);
         l.K.alloc((size_t)ring_table_elems(l.g));
         l.bc = mg_bc(user_bc, MODE_UNIFORM);
         const size_t bytes = n * sizeof(double);
@@ -103,6 +121,21 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         }
         l.geo = Geo{l.J.p, l.Jxx.p, l.Jxy.p, l.Jyx.p, l.Jyy.p};
         launch_ring_coef(s, l.K.p, l.g, l.geo, l.bc);
+        if (mixed) {
+            // FP32 copies of the whole padded arrays (the metrics and the
+            // stencil table are formed in FP64 and rounded once)
+            DevBuf *src[4] = {&l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy};
+            DevBufF *dst[4] = {&l.Jxx32, &l.Jxy32, &l.Jyx32, &l.Jyy32};
+            for (int q = 0; q < 4; ++q) {
+                dst[q]->alloc(n);
+                launch_convert_flat(s, dst[q]->p, src[q]->p, n);
+            }
+            l.K32.alloc(l.K.n);
+            launch_convert_flat(s, l.K32.p, l.K.p, l.K.n);
+            l.e32.alloc(n);
+            l.r32.alloc(n);
+            l.geo32 = GeoF{nullptr, l.Jxx32.p, l.Jxy32.p, l.Jyx32.p, l.Jyy32.p};
+        }
         if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || (int)L.size() >= maxlev) break;
         ny /= 2;
         dy *= 2.0;
@@ -117,7 +150,8 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
     built = true;
 }
 
-void xline_zebra(hipStream_t s, const Grid &g, const Geo &geo, const double *K, double *e, const double *f) {
+template <typename T, typename G>
+static void zebra(hipStream_t s, const Grid &g, const G &geo, const T *K, T *e, const T *f) {
     const int ny = g.ny;
     // Even interior rows 2, 4, .. <= ny-2, then odd interior rows 1, 3, ..
     // <= ny-2, then rows 0 and ny-1 (whose extrapolated stencils reach two
@@ -137,23 +171,54 @@ void xline_zebra(hipStream_t s, const Grid &g, const Geo &geo, const double *K, 
     }
 }
 
-static void zebra_sweep(hipStream_t s, KLevel &l, double *e, const double *f) {
-    xline_zebra(s, l.g, l.geo, l.K.p, e, f);
+void xline_zebra(hipStream_t s, const Grid &g, const Geo &geo, const double *K, double *e, const double *f) {
+    zebra(s, g, geo, K, e, f);
 }
 
-static void vcycle(hipStream_t s, KrylovMG::Impl &K, size_t d, double *e, const double *f) {
+static void zebra_sweep(hipStream_t s, KLevel &l, double *e, const double *f) { zebra(s, l.g, l.geo, l.K.p, e, f); }
+static void zebra_sweep(hipStream_t s, KLevel &l, float *e, const float *f) {
+    zebra(s, l.g, l.geo32, l.K32.p, e, f);
+}
+
+// The coarse level's correction and residual fields of the cycle's type.
+static float *coarse_e(KLevel &c, float *) { return c.e32.p; }
+static float *coarse_r(KLevel &c, float *) { return c.r32.p; }
+static double *coarse_e(KLevel &c, double *) { return c.e.p; }
+static double *coarse_r(KLevel &c, double *) { return c.r.p; }
+static const float *table(KLevel &l, float *) { return l.K32.p; }
+static const double *table(KLevel &l, double *) { return l.K.p; }
+static const GeoF &metrics(KLevel &l, float *) { return l.geo32; }
+static const Geo &metrics(KLevel &l, double *) { return l.geo; }
+
+// V-cycle from level d down, e = 0 on entry (set by the caller).
+template <typename T>
+static void vcycle_t(hipStream_t s, KrylovMG::Impl &K, size_t d, T *e, const T *f) {
     KLevel &l = K.L[d];
-    if (d == 0) launch_setval(s, e, 0.0, l.g, 1);
     if (d + 1 == K.L.size()) {
         for (int k = 0; k < K.sb; ++k) zebra_sweep(s, l, e, f);
         return;
     }
     KLevel &c = K.L[d + 1];
+    T *ce = coarse_e(c, e), *cr = coarse_r(c, e);
     for (int k = 0; k < K.sd; ++k) zebra_sweep(s, l, e, f);
-    launch_residual_restrict_y(s, c.r.p, c.e.p, e, f, l.K.p, l.g, c.g, l.geo);
-    vcycle(s, K, d + 1, c.e.p, c.r.p);
-    launch_prolong_y(s, e, c.e.p, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
+    launch_residual_restrict_y(s, cr, ce, e, f, table(l, e), l.g, c.g, metrics(l, e));
+    vcycle_t(s, K, d + 1, ce, cr);
+    launch_prolong_y(s, e, ce, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
     for (int k = 0; k < K.su; ++k) zebra_sweep(s, l, e, f);
+}
+
+// Preconditioner application e = M f (one V-cycle from e = 0), FP64 in and
+// out; in mixed precision the cycle runs on FP32 copies.
+static void precondition(hipStream_t s, KrylovMG::Impl &K, double *e, const double *f) {
+    KLevel &l = K.L[0];
+    if (!K.mixed) {
+        launch_setval(s, e, 0.0, l.g, 1);
+        vcycle_t(s, K, 0, e, f);
+        return;
+    }
+    launch_begin32(s, l.r32.p, l.e32.p, f, l.g);
+    vcycle_t(s, K, 0, l.e32.p, (const float *)l.r32.p);
+    launch_convert(s, e, l.e32.p, l.g, 0);
 }
 
 // out = A_h v (homogeneous user BCs); v's ghosts are refilled.
@@ -184,13 +249,13 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
         launch_reduce(s, sc + SC_RHO_NEW, partial, K.rhat.p, K.r.p, nullptr, vol, g, 0);
         launch_bicg_scalars(s, sc, 0);
         launch_bicg_update(s, K.p.p, K.r.p, K.v.p, sc, g, 0);
-        vcycle(s, K, 0, K.phat.p, K.p.p);
+        precondition(s, K, K.phat.p, K.p.p);
         apply_h(s, top, user_bc, K.phat.p, K.v.p);
         launch_reduce(s, sc + SC_RV, partial, K.rhat.p, K.v.p, nullptr, vol, g, 0);
         launch_bicg_scalars(s, sc, 1);
         launch_bicg_update(s, K.s.p, K.r.p, K.v.p, sc, g, 1);
         launch_bicg_update(s, phi, K.phat.p, nullptr, sc, g, 2);
-        vcycle(s, K, 0, K.shat.p, K.s.p);
+        precondition(s, K, K.shat.p, K.s.p);
         apply_h(s, top, user_bc, K.shat.p, K.t.p);
         launch_reduce(s, sc + SC_TS, partial, K.t.p, K.s.p, nullptr, vol, g, 0);
         launch_reduce(s, sc + SC_TT, partial, K.t.p, K.t.p, nullptr, vol, g, 0);
@@ -223,6 +288,17 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     if (r0 == 0.0) {
         outcome = MGP_CONVERGED;
     } else {
+        // Iterative refinement: each BiCGStab run stops once its recursive
+        // residual has fallen by `refine` below the true residual it started
+        // from, and the next starts from the FP64 true residual.  The FP32
+        // preconditioner is consistent to about 1e-7 of its input, so the
+        // recurrences of one run lose their accuracy a few decades down (at
+        // 2048^2 a run stalls near 1e-7); 1e-3 per run measured fastest on
+        // the 2048^2 stretched-grid problem (74 V-cycles to 1e-9, against 168
+        // without restarts).  The FP64 cycle restarts after 1e-6 (62 V-cycles
+        // against 64).
+        const double refine = K.mixed ? 1e-3 : 1e-6;
+        double base = 1.0;  // relres at the last restart
         bool restart = true;
         for (int it = 1; it <= maxiters; ++it) {
             if (restart) {
@@ -239,7 +315,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
             hip_check(hipStreamSynchronize(s), "sync");
             const double rn = std::sqrt(K.host[SC_RN]) / r0;
             const double omega = K.host[SC_OMEGA];
-            if (rn <= tol || omega == 0.0 || !std::isfinite(rn)) {
+            if (rn <= tol || omega == 0.0 || !std::isfinite(rn) || rn <= refine * base) {
                 const double tr = true_residual() / r0;
                 hist.push_back(tr);
                 if (tr <= tol) {
@@ -251,6 +327,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
                     break;
                 }
                 restart = true;  // recursive residual drifted: restart from the true one
+                base = tr;
                 continue;
             }
             hist.push_back(rn);
@@ -288,14 +365,21 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, doub
     } else if (kernel == 2) {
         // plain copy w = e over the valid cells: the bandwidth reference for the two above
         *bytes = nx * (double)g.ny * 16.0;
+    } else if (kernel == 3 && impl->mixed) {
+        // the FP32 x-line kernel of the mixed-precision cycle: kernel 0's
+        // array elements at 4 bytes each
+        const double R = n_even;
+        *bytes = nx * 4.0 * ((2.0 * R + 1.0) + R + R + R + 2.0 * R + 2.0 * R + R) + R * 2.0 * 9.0 * 4.0;
     } else {
-        g_last_error = "time_kernel: kernel must be 0 (x-line, even rows), 1 (residual) or 2 (copy)";
+        g_last_error = "time_kernel: kernel must be 0 (x-line, even rows), 1 (residual), 2 (copy) or, for a "
+                       "mixed-precision solver, 3 (FP32 x-line, even rows)";
         throw Failure(MGP_ERR_ARG);
     }
     auto launch = [&]() {
         if (kernel == 0) launch_xline(s, l.e.p, l.r.p, l.K.p, g, l.geo, 2, 2, n_even, -1, -1);
         else if (kernel == 1) launch_laplacian(s, l.w.p, l.e.p, l.r.p, g, l.geo, l.bc, 1, 0);
-        else launch_lincomb(s, l.w.p, 1.0, l.e.p, 0.0, nullptr, g, 0);
+        else if (kernel == 2) launch_lincomb(s, l.w.p, 1.0, l.e.p, 0.0, nullptr, g, 0);
+        else launch_xline(s, l.e32.p, l.r32.p, l.K32.p, g, l.geo32, 2, 2, n_even, -1, -1);
     };
     launch();  // warm
     double total = 0.0;
@@ -318,9 +402,20 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, doub
         const size_t n = level_elems(g.nx, g.ny), nb = n * sizeof(double);
         struct Copy {
             DevBuf e, r, w, J, Jxx, Jxy, Jyx, Jyy;
+            DevBufF e32, r32, Jxx32, Jxy32, Jyx32, Jyy32;
         };
         std::vector<Copy> extra(copies - 1);
         for (Copy &c : extra) {
+            if (kernel == 3) {
+                DevBufF *dst[6] = {&c.e32, &c.r32, &c.Jxx32, &c.Jxy32, &c.Jyx32, &c.Jyy32};
+                const DevBufF *src[6] = {&l.e32, &l.r32, &l.Jxx32, &l.Jxy32, &l.Jyx32, &l.Jyy32};
+                for (int q = 0; q < 6; ++q) {
+                    dst[q]->alloc(n);
+                    hip_check(hipMemcpyAsync(dst[q]->p, src[q]->p, n * sizeof(float), hipMemcpyDeviceToDevice, s),
+                              "copy");
+                }
+                continue;
+            }
             DevBuf *dst[8] = {&c.e, &c.r, &c.w, &c.J, &c.Jxx, &c.Jxy, &c.Jyx, &c.Jyy};
             const DevBuf *src[8] = {&l.e, &l.r, &l.w, &l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy};
             for (int q = 0; q < 8; ++q) {
@@ -335,7 +430,10 @@ double KrylovMG::time_kernel(hipStream_t s, int kernel, int reps, int cold, doub
             }
             Copy &c = extra[k - 1];
             const Geo geo{c.J.p, c.Jxx.p, c.Jxy.p, c.Jyx.p, c.Jyy.p};
-            if (kernel == 0) launch_xline(s, c.e.p, c.r.p, l.K.p, g, geo, 2, 2, n_even, -1, -1);
+            if (kernel == 3) {
+                const GeoF geo32{nullptr, c.Jxx32.p, c.Jxy32.p, c.Jyx32.p, c.Jyy32.p};
+                launch_xline(s, c.e32.p, c.r32.p, l.K32.p, g, geo32, 2, 2, n_even, -1, -1);
+            } else if (kernel == 0) launch_xline(s, c.e.p, c.r.p, l.K.p, g, geo, 2, 2, n_even, -1, -1);
             else if (kernel == 1) launch_laplacian(s, c.w.p, c.e.p, c.r.p, g, geo, l.bc, 1, 0);
             else launch_lincomb(s, c.w.p, 1.0, c.e.p, 0.0, nullptr, g, 0);
         };

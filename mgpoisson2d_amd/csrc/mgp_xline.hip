@@ -66,8 +66,12 @@ struct ImpulseFilled {
 // Device-memory pointers typed as global: the compiler then knows they
 // cannot alias LDS and may issue loads ahead of LDS stores (with generic
 // pointers every LDS store would wait for the loads before it).
-typedef const __attribute__((address_space(1))) double gdouble;
-__device__ __forceinline__ gdouble *gptr(const double *p) { return (gdouble *)p; }
+template <typename T>
+using gconst = const __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ gconst<T> *gptr(const T *p) {
+    return (gconst<T> *)p;
+}
 
 __host__ __device__ inline int ring_count(int nx, int ny) { return 2 * nx + 2 * (ny - 2); }
 
@@ -101,16 +105,17 @@ __device__ __forceinline__ int ring_index(int nx, int ny, int i, int j) {
 
 // (L e)(i, j) at a ring cell from the table.  All nine table entries and
 // field values are loaded before any is used (one memory round trip).
-__device__ __forceinline__ double ring_lap(gdouble *e, gdouble *K, int nring, const Grid &g, int i, int j) {
+template <typename T>
+__device__ __forceinline__ T ring_lap(gconst<T> *e, gconst<T> *K, int nring, const Grid &g, int i, int j) {
     const int w0 = window_origin(i, g.nx), v0 = window_origin(j, g.ny);
     const int r = ring_index(g.nx, g.ny, i, j);
-    double kv[9], ev[9];
+    T kv[9], ev[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         kv[q] = K[(size_t)q * nring + r];
         ev[q] = e[ix(g.pitch, i + w0 + q % 3, j + v0 + q / 3)];
     }
-    double lap = 0.0;
+    T lap = 0;
 #pragma unroll
     for (int q = 0; q < 9; ++q) lap += kv[q] * ev[q];
     return lap;
@@ -118,14 +123,15 @@ __device__ __forceinline__ double ring_lap(gdouble *e, gdouble *K, int nring, co
 
 // Coefficients of e(i-1, j), e(i, j), e(i+1, j) at a ring cell (0 where the
 // neighbour is outside the window, i.e. not a valid cell).
-__device__ __forceinline__ void ring_row_coef(gdouble *K, int nring, const Grid &g, int i, int j, double &a,
-                                              double &b, double &c) {
+template <typename T>
+__device__ __forceinline__ void ring_row_coef(gconst<T> *K, int nring, const Grid &g, int i, int j, T &a, T &b,
+                                              T &c) {
     const int w0 = window_origin(i, g.nx), v0 = window_origin(j, g.ny);
     const int r = ring_index(g.nx, g.ny, i, j);
     const int base = 3 * (-v0);  // window row of j itself
-    a = w0 <= -1 ? K[(size_t)(base + (-1 - w0)) * nring + r] : 0.0;
+    a = w0 <= -1 ? K[(size_t)(base + (-1 - w0)) * nring + r] : T(0);
     b = K[(size_t)(base + (0 - w0)) * nring + r];
-    c = w0 + 2 >= 1 ? K[(size_t)(base + (1 - w0)) * nring + r] : 0.0;
+    c = w0 + 2 >= 1 ? K[(size_t)(base + (1 - w0)) * nring + r] : T(0);
 }
 
 // K[q * nring + r]: coefficient of e(i + w0 + q%3, j + v0 + q/3) in the
@@ -166,24 +172,28 @@ __device__ __forceinline__ int launch_slot() {
     return x * q8 + (x < r8 ? x : r8) + (b >> 3);
 }
 
+template <typename T>
 struct RowCtx {
     int nx, ny, P, nt, rem, nring;
-    double idx, idy, idx2, idy2, ixy4;
+    T idx, idy, idx2, idy2, ixy4;
 };
 
-__device__ __forceinline__ RowCtx row_ctx(const Grid &g, int nt, int C) {
-    RowCtx r;
+// The inverse spacings are formed in FP64 and rounded once to T.
+template <typename T>
+__device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
+    RowCtx<T> r;
     r.nx = g.nx;
     r.ny = g.ny;
     r.P = g.pitch;
     r.nt = nt;
     r.rem = g.nx - nt * C;
     r.nring = ring_count(g.nx, g.ny);
-    r.idx = 1.0 / g.dx;
-    r.idy = 1.0 / g.dy;
-    r.idx2 = r.idx * r.idx;
-    r.idy2 = r.idy * r.idy;
-    r.ixy4 = 0.25 * r.idx * r.idy;
+    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy;
+    r.idx = (T)idx;
+    r.idy = (T)idy;
+    r.idx2 = (T)(idx * idx);
+    r.idy2 = (T)(idy * idy);
+    r.ixy4 = (T)(0.25 * idx * idy);
     return r;
 }
 
@@ -193,15 +203,14 @@ __device__ __forceinline__ RowCtx row_ctx(const Grid &g, int nt, int C) {
 // table.  Lanes past the row end evaluate the last cell and store into the
 // chunk layout's last slot, which no chunk uses (the last chunk has C cells):
 // the loop stays branch-free.
-template <int C>
-__device__ __forceinline__ void row_pass(const RowCtx &X, const double *e_, const double *f_, const double *K_,
-                                         const Grid &g, const Geo &geo_, int j, double *Ld, double *La, double *Lb,
-                                         double *Lc) {
+template <int C, typename T, typename GM>
+__device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const T *f_, const T *K_, const Grid &g,
+                                         const GM &geo_, int j, T *Ld, T *La, T *Lb, T *Lc) {
     constexpr int U = C + 1;
-    gdouble *e = gptr(e_), *f = gptr(f_), *K = gptr(K_);
-    gdouble *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
+    gconst<T> *e = gptr(e_), *f = gptr(f_), *K = gptr(K_);
+    gconst<T> *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
     const int tid = threadIdx.x, nx = X.nx, ny = X.ny, P = X.P;
-    const double fourth = 0.25;
+    const T fourth = T(0.25);
     const int dump = X.nt * (C + 1) - 1;
     if (j != 0 && j != ny - 1) {
 #pragma unroll
@@ -211,15 +220,15 @@ __device__ __forceinline__ void row_pass(const RowCtx &X, const double *e_, cons
             const size_t c = ix(P, i, j);
             // interior stencil; for i = 0, nx-1 the reads stay inside the
             // padded level and the values are replaced below
-            gdouble *q = e + c;
-            const double p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
-            const double ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
-            const double Jxx_e = Jxx[c + 1], Jxx_w = Jxx[c], Jxy_e = Jxy[c + 1], Jxy_w = Jxy[c];
-            const double Jyy_n = Jyy[c + P], Jyy_s = Jyy[c], Jyx_n = Jyx[c + P], Jyx_s = Jyx[c];
-            const double xfe = Jxx_e * (pe - p) * X.idx + Jxy_e * fourth * (ene - ese + pn - ps) * X.idy;
-            const double xfw = Jxx_w * (p - pw) * X.idx + Jxy_w * fourth * (pn - ps + enw - esw) * X.idy;
-            const double yfn = Jyy_n * (pn - p) * X.idy + Jyx_n * fourth * (ene - enw + pe - pw) * X.idx;
-            const double yfs = Jyy_s * (p - ps) * X.idy + Jyx_s * fourth * (pe - pw + ese - esw) * X.idx;
+            gconst<T> *q = e + c;
+            const T p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
+            const T ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
+            const T Jxx_e = Jxx[c + 1], Jxx_w = Jxx[c], Jxy_e = Jxy[c + 1], Jxy_w = Jxy[c];
+            const T Jyy_n = Jyy[c + P], Jyy_s = Jyy[c], Jyx_n = Jyx[c + P], Jyx_s = Jyx[c];
+            const T xfe = Jxx_e * (pe - p) * X.idx + Jxy_e * fourth * (ene - ese + pn - ps) * X.idy;
+            const T xfw = Jxx_w * (p - pw) * X.idx + Jxy_w * fourth * (pn - ps + enw - esw) * X.idy;
+            const T yfn = Jyy_n * (pn - p) * X.idy + Jyx_n * fourth * (ene - enw + pe - pw) * X.idx;
+            const T yfs = Jyy_s * (p - ps) * X.idy + Jyx_s * fourth * (pe - pw + ese - esw) * X.idx;
             const int o = iraw < nx ? chunk_pos<C>(i, X.rem) : dump;
             Ld[o] = f[c] - ((xfe - xfw) * X.idx + (yfn - yfs) * X.idy);
             La[o] = Jxx_w * X.idx2 - (Jyx_n - Jyx_s) * X.ixy4;
@@ -240,9 +249,9 @@ __device__ __forceinline__ void row_pass(const RowCtx &X, const double *e_, cons
         for (int u = 0; u < U; ++u) {
             const int iraw = tid + XL_THREADS * u;
             const int i = iraw < nx ? iraw : nx - 1;
-            const double fv = f[ix(P, i, j)];
-            const double lap = ring_lap(e, K, X.nring, g, i, j);
-            double a, b, c;
+            const T fv = f[ix(P, i, j)];
+            const T lap = ring_lap(e, K, X.nring, g, i, j);
+            T a, b, c;
             ring_row_coef(K, X.nring, g, i, j, a, b, c);
             const int o = iraw < nx ? chunk_pos<C>(i, X.rem) : dump;
             Ld[o] = fv - lap;
@@ -254,8 +263,8 @@ __device__ __forceinline__ void row_pass(const RowCtx &X, const double *e_, cons
 }
 
 // Coalesced update e(:, j) += x, x in the chunk layout (e re-read).
-template <int C>
-__device__ __forceinline__ void row_update_add(const RowCtx &X, double *e, int j, const double *Lx) {
+template <int C, typename T>
+__device__ __forceinline__ void row_update_add(const RowCtx<T> &X, T *e, int j, const T *Lx) {
 #pragma unroll
     for (int u = 0; u < C + 1; ++u) {
         const int i = threadIdx.x + XL_THREADS * u;
@@ -287,37 +296,47 @@ __device__ __forceinline__ double rcp(double x) {
     const double r = __builtin_amdgcn_rcp(x);
     return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
 }
-__device__ __forceinline__ double quot(double a, double b) { return a * rcp(b); }
+// FP32: v_rcp_f32 alone (about 1 ulp).
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename T>
+__device__ __forceinline__ T quot(T a, T b) {
+    return a * rcp(b);
+}
 
 __device__ __forceinline__ int iface_pos(int q) { return (q & 7) * 64 + (q >> 3); }
 
+template <typename T>
 struct Row4 {
-    double a, b, c, d;
+    T a, b, c, d;
 };
 
-__device__ __forceinline__ Row4 shfl_up4(const Row4 &r, int s) {
-    return Row4{__shfl_up(r.a, s), __shfl_up(r.b, s), __shfl_up(r.c, s), __shfl_up(r.d, s)};
+template <typename T>
+__device__ __forceinline__ Row4<T> shfl_up4(const Row4<T> &r, int s) {
+    return Row4<T>{__shfl_up(r.a, s), __shfl_up(r.b, s), __shfl_up(r.c, s), __shfl_up(r.d, s)};
 }
-__device__ __forceinline__ Row4 shfl_down4(const Row4 &r, int s) {
-    return Row4{__shfl_down(r.a, s), __shfl_down(r.b, s), __shfl_down(r.c, s), __shfl_down(r.d, s)};
+template <typename T>
+__device__ __forceinline__ Row4<T> shfl_down4(const Row4<T> &r, int s) {
+    return Row4<T>{__shfl_down(r.a, s), __shfl_down(r.b, s), __shfl_down(r.c, s), __shfl_down(r.d, s)};
 }
 
 // One PCR update of row r from its neighbours m (q - st) and p (q + st);
 // a missing neighbour contributes nothing.
-__device__ __forceinline__ Row4 pcr_update(const Row4 &r, const Row4 &m, bool has_m, const Row4 &p, bool has_p) {
-    const double k1 = has_m ? quot(r.a, m.b) : 0.0, k2 = has_p ? quot(r.c, p.b) : 0.0;
-    Row4 o;
-    o.a = has_m ? -m.a * k1 : 0.0;
-    o.c = has_p ? -p.c * k2 : 0.0;
-    o.b = r.b - (has_m ? m.c * k1 : 0.0) - (has_p ? p.a * k2 : 0.0);
-    o.d = r.d - (has_m ? m.d * k1 : 0.0) - (has_p ? p.d * k2 : 0.0);
+template <typename T>
+__device__ __forceinline__ Row4<T> pcr_update(const Row4<T> &r, const Row4<T> &m, bool has_m, const Row4<T> &p,
+                                              bool has_p) {
+    const T k1 = has_m ? quot(r.a, m.b) : T(0), k2 = has_p ? quot(r.c, p.b) : T(0);
+    Row4<T> o;
+    o.a = has_m ? -m.a * k1 : T(0);
+    o.c = has_p ? -p.c * k2 : T(0);
+    o.b = r.b - (has_m ? m.c * k1 : T(0)) - (has_p ? p.a * k2 : T(0));
+    o.d = r.d - (has_m ? m.d * k1 : T(0)) - (has_p ? p.d * k2 : T(0));
     return o;
 }
 
-__device__ __forceinline__ void interface_solve(double *IA, double *IB, const double *IC, double *ID, double *IX,
-                                               int l) {
+template <typename T>
+__device__ __forceinline__ void interface_solve(T *IA, T *IB, const T *IC, T *ID, T *IX, int l) {
     constexpr int L = 8;
-    double a[L], b[L], c[L], d[L];
+    T a[L], b[L], c[L], d[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) {
         const int q = k * 64 + l;
@@ -327,31 +346,31 @@ __device__ __forceinline__ void interface_solve(double *IA, double *IB, const do
         d[k] = ID[q];
     }
     // 1. partition: first row (backward sweep), last row (forward sweep)
-    double bu = b[L - 2], cu = c[L - 2], du = d[L - 2];
+    T bu = b[L - 2], cu = c[L - 2], du = d[L - 2];
 #pragma unroll
     for (int k = L - 3; k >= 0; --k) {
-        const double r = quot(c[k], bu);
+        const T r = quot(c[k], bu);
         bu = b[k] - r * a[k + 1];
         cu = -r * cu;
         du = d[k] - r * du;
     }
-    Row4 f{a[0], bu, cu, du};
+    Row4<T> f{a[0], bu, cu, du};
 #pragma unroll
     for (int k = 2; k < L; ++k) {
-        const double r = quot(a[k], b[k - 1]);
+        const T r = quot(a[k], b[k - 1]);
         a[k] = -r * a[k - 1];
         b[k] = b[k] - r * c[k - 1];
         d[k] = d[k] - r * d[k - 1];
     }
-    Row4 g{a[L - 1], b[L - 1], c[L - 1], d[L - 1]};
+    Row4<T> g{a[L - 1], b[L - 1], c[L - 1], d[L - 1]};
     // inner unknowns as x_k = G_k + H_k x_first + M_k x_last, kept in this
     // lane's own LDS slots (IA, IB, ID) for step 3, so that the PCR runs with
     // only the two lane rows in registers and step 3 has no dependent chain
     {
-        double G = 0.0, H = 0.0, M = 1.0;
+        T G = 0, H = 0, M = 1;
 #pragma unroll
         for (int k = L - 2; k >= 1; --k) {
-            const double ib = rcp(b[k]);
+            const T ib = rcp(b[k]);
             G = (d[k] - c[k] * G) * ib;
             H = (-a[k] - c[k] * H) * ib;
             M = (-c[k] * M) * ib;
@@ -365,7 +384,7 @@ __device__ __forceinline__ void interface_solve(double *IA, double *IB, const do
     // 2. PCR over the 128 lane rows, Q = 2l (first) and 2l+1 (last)
 #pragma unroll
     for (int st = 1; st < 128; st *= 2) {
-        Row4 fm, fp, gm, gp;
+        Row4<T> fm, fp, gm, gp;
         if (st == 1) {
             fm = shfl_up4(g, 1);   // Q-1 of the first row: the previous lane's last
             fp = g;                // Q+1: this lane's last
@@ -379,12 +398,12 @@ __device__ __forceinline__ void interface_solve(double *IA, double *IB, const do
             gp = shfl_down4(g, sl);
         }
         const int Qf = 2 * l, Qg = 2 * l + 1;
-        const Row4 nf = pcr_update(f, fm, Qf - st >= 0, fp, Qf + st < 128);
-        const Row4 ng = pcr_update(g, gm, Qg - st >= 0, gp, Qg + st < 128);
+        const Row4<T> nf = pcr_update(f, fm, Qf - st >= 0, fp, Qf + st < 128);
+        const Row4<T> ng = pcr_update(g, gm, Qg - st >= 0, gp, Qg + st < 128);
         f = nf;
         g = ng;
     }
-    const double xs = quot(f.d, f.b), xe = quot(g.d, g.b);
+    const T xs = quot(f.d, f.b), xe = quot(g.d, g.b);
     // 3. back substitution of the inner unknowns
     asm volatile("" ::: "memory");
     IX[L * l] = xs;
@@ -401,18 +420,18 @@ __device__ __forceinline__ void interface_solve(double *IA, double *IB, const do
 // from the metrics, elimination, PCR and back substitution per call.
 // LDS: 4 nt (C+1) doubles.
 // ---------------------------------------------------------------------------
-template <int C>
-__device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double *__restrict__ f,
-                          const double *__restrict__ K, const Grid &g, const Geo &geo, int j, double *lds) {
+template <int C, typename T, typename GM>
+__device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
+                          const Grid &g, const GM &geo, int j, T *lds) {
     constexpr int CP = C + 1;
     const int tid = threadIdx.x, nt = X.nt;
     const int NT1 = nt * CP;
-    double *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
+    T *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
     row_pass<C>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
     __syncthreads();
 
     // each thread takes its chunk into registers
-    double ca[CP], cb[CP], cc[CP], cd[CP];
+    T ca[CP], cb[CP], cc[CP], cd[CP];
     const int len = tid < nt ? C + (tid < X.rem ? 1 : 0) : 0;
 #pragma unroll
     for (int k = 0; k < CP; ++k)
@@ -427,17 +446,17 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
     // partition elimination in registers
     // start row:  a_0 x_{prev end} + bu x_s + cu x_e = du
     // end row:    a'_L x_s + b'_L x_e + c_L x_{next start} = d'_L
-    double ea = 0.0, eb = 1.0, ec = 0.0, ed = 0.0, oa = 0.0, ob = 1.0, oc = 0.0, od = 0.0;
-    double sg[CP], sh[CP], sm[CP];
+    T ea = 0, eb = 1, ec = 0, ed = 0, oa = 0, ob = 1, oc = 0, od = 0;
+    T sg[CP], sh[CP], sm[CP];
     if (tid < nt) {
         const bool longc = len == CP;
-        double bu = longc ? cb[C - 1] : cb[C - 2];
-        double cu = longc ? cc[C - 1] : cc[C - 2];
-        double du = longc ? cd[C - 1] : cd[C - 2];
+        T bu = longc ? cb[C - 1] : cb[C - 2];
+        T cu = longc ? cc[C - 1] : cc[C - 2];
+        T du = longc ? cd[C - 1] : cd[C - 2];
 #pragma unroll
         for (int k = C - 2; k >= 0; --k)
             if (k <= len - 3) {
-                const double r = quot(cc[k], bu);
+                const T r = quot(cc[k], bu);
                 bu = cb[k] - r * ca[k + 1];
                 cu = -r * cu;
                 du = cd[k] - r * du;
@@ -449,7 +468,7 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
 #pragma unroll
         for (int k = 2; k <= C; ++k)
             if (k <= len - 1) {
-                const double r = quot(ca[k], cb[k - 1]);
+                const T r = quot(ca[k], cb[k - 1]);
                 ca[k] = -r * ca[k - 1];
                 cb[k] = cb[k] - r * cc[k - 1];
                 cd[k] = cd[k] - r * cd[k - 1];
@@ -461,11 +480,11 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
         // inner unknowns as x_k = sg_k + sh_k x_s + sm_k x_e (the back
         // substitution unrolled once here, off the critical path of the
         // interface solve): ca/cb/cc/cd are dead after this
-        double G = 0.0, H = 0.0, M = 1.0;
+        T G = 0, H = 0, M = 1;
 #pragma unroll
         for (int k = C - 1; k >= 1; --k)
             if (k <= len - 2) {
-                const double ib = rcp(cb[k]);
+                const T ib = rcp(cb[k]);
                 G = (cd[k] - cc[k] * G) * ib;
                 H = (-ca[k] - cc[k] * H) * ib;
                 M = (-cc[k] * M) * ib;
@@ -479,7 +498,7 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
     // q = 2t, end at q = 2t+1; tridiagonal in that order), padded to 512 rows
     // with identity rows by the threads past nt.  Staged once through LDS,
     // solved by wave 0 (interface_solve), read back once: two barriers.
-    double *IA = lds + NT1, *IB = IA + IFACE, *IC = IB + IFACE, *ID = IC + IFACE, *IX = ID + IFACE;
+    T *IA = lds + NT1, *IB = IA + IFACE, *IC = IB + IFACE, *ID = IC + IFACE, *IX = ID + IFACE;
     {
         const int q0 = iface_pos(2 * tid), q1 = iface_pos(2 * tid + 1);
         IA[q0] = ea;
@@ -494,11 +513,11 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
     __syncthreads();
     if (tid < 64) interface_solve(IA, IB, IC, ID, IX, tid);
     __syncthreads();
-    const double x_start = IX[2 * tid], x_end = IX[2 * tid + 1];
+    const T x_start = IX[2 * tid], x_end = IX[2 * tid + 1];
 
     // back substitution, then the coalesced update
     if (tid < nt) {
-        const double xs = x_start, xe = x_end;
+        const T xs = x_start, xe = x_end;
         const bool longc = len == CP;
 #pragma unroll
         for (int k = 1; k <= C - 1; ++k)
@@ -515,14 +534,13 @@ __device__ void xline_row(const RowCtx &X, double *__restrict__ e, const double 
 // row.  The boundary rows ride on the odd-row launch this way: row 0 after
 // row 1 and row ny-1 after the last odd row, the order of a separate third
 // phase, without its launch.
-// Rows j = row0 + slot*rowstep; the workgroup of slot 0 then also relaxes row
-// extra0 and the last slot row extra1 (< 0: none), each after its own row.
-template <int C>
-__device__ __forceinline__ void xline_rows(double *__restrict__ e, const double *__restrict__ f,
-                                           const double *__restrict__ K, const Grid &g, const Geo &geo, int row0,
-                                           int rowstep, int nt, int extra0, int extra1) {
-    extern __shared__ double lds[];
-    const RowCtx X0 = row_ctx(g, nt, C);
+template <int C, typename T, typename GM>
+__device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
+                                           const Grid &g, const GM &geo, int row0, int rowstep, int nt, int extra0,
+                                           int extra1) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_bytes[];
+    T *lds = reinterpret_cast<T *>(lds_bytes);
+    const RowCtx<T> X0 = row_ctx<T>(g, nt, C);
     const int slot = launch_slot();
     const int ex0 = slot == 0 ? extra0 : -1, ex1 = slot == (int)gridDim.x - 1 ? extra1 : -1;
 #pragma unroll 1
@@ -533,52 +551,77 @@ __device__ __forceinline__ void xline_rows(double *__restrict__ e, const double 
         // the chunk remainder is made opaque per row, so the per-thread LDS
         // positions derived from it are recomputed in the row (a few integer
         // ops) instead of being hoisted out of the loop and spilled
-        RowCtx X = X0;
+        RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
         xline_row<C>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
-// Rows up to 2055 cells: two waves per SIMD (two workgroups per CU, as the
-// 73.7 KB of LDS allow): the fine levels are bandwidth-bound and need the
-// second workgroup to cover the elimination and PCR phases of the first.
-template <int C>
-__global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_xline(double *__restrict__ e, const double *__restrict__ f, const double *__restrict__ K, Grid g, Geo geo,
-        int row0, int rowstep, int nt, int extra0, int extra1) {
+// Waves per SIMD of the fine-level kernel: the LDS of one workgroup at
+// C = 8 (73.7 KB in FP64, 36.9 KB in FP32) allows two workgroups per CU in
+// FP64 and four in FP32, and the register budget is set to match.
+template <typename T>
+constexpr int xl_waves() {
+    return sizeof(T) == 8 ? 2 : 3;
+}
+
+// Rows up to 2055 cells: the fine levels are bandwidth-bound and need the
+// other workgroups of the CU to cover the elimination and interface phases
+// of each one.
+template <int C, typename T, typename GM>
+__global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(xl_waves<T>(), xl_waves<T>())))
+k_xline(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K, Grid g, GM geo, int row0, int rowstep,
+        int nt, int extra0, int extra1) {
     xline_rows<C>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
 }
 
-// Longer rows (C = 16): one workgroup per CU (139 KB of LDS), all registers.
-template <int C>
-__global__ void __launch_bounds__(XL_THREADS) k_xline_long(double *__restrict__ e, const double *__restrict__ f,
-                                                           const double *__restrict__ K, Grid g, Geo geo, int row0,
+// Longer rows (C = 16): one workgroup per CU in FP64 (139 KB of LDS), all registers.
+template <int C, typename T, typename GM>
+__global__ void __launch_bounds__(XL_THREADS) k_xline_long(T *__restrict__ e, const T *__restrict__ f,
+                                                           const T *__restrict__ K, Grid g, GM geo, int row0,
                                                            int rowstep, int nt, int extra0, int extra1) {
     xline_rows<C>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+}
+
+// (L e)(i, j) at an interior cell from the face metrics: the stencil of
+// cell_lap_interior in the element type T (inverse spacings idx, idy).
+template <typename T, typename G>
+__device__ __forceinline__ T interior_lap(gconst<T> *e, const G &geo_, int P, size_t c, T idx, T idy) {
+    gconst<T> *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
+    const T fourth = T(0.25);
+    gconst<T> *q = e + c;
+    const T p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
+    const T ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
+    const T xfe = Jxx[c + 1] * (pe - p) * idx + Jxy[c + 1] * fourth * (ene - ese + pn - ps) * idy;
+    const T xfw = Jxx[c] * (p - pw) * idx + Jxy[c] * fourth * (pn - ps + enw - esw) * idy;
+    const T yfn = Jyy[c + P] * (pn - p) * idy + Jyx[c + P] * fourth * (ene - enw + pe - pw) * idx;
+    const T yfs = Jyy[c] * (p - ps) * idy + Jyx[c] * fourth * (pe - pw + ese - esw) * idx;
+    return (xfe - xfw) * idx + (yfn - yfs) * idy;
 }
 
 // Preconditioner residual restricted 1x2 in y, with the coarse correction
 // cleared:  rc(i, J) = (r(i, 2J) + r(i, 2J+1)) / 2,  r = f - L e  (interior
 // stencil from the metrics, ring cells from the table),  ec(i, J) = 0.
-__global__ void __launch_bounds__(256) k_residual_restrict_y(double *rc, double *ec, const double *e,
-                                                             const double *f, const double *K, Grid gf, Grid gc,
-                                                             Geo geo) {
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const T *e, const T *f, const T *K,
+                                                             Grid gf, Grid gc, G geo) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= gc.nx || J >= gc.ny) return;
     const int nring = ring_count(gf.nx, gf.ny);
-    double s = 0.0;
+    const T idx = (T)(1.0 / gf.dx), idy = (T)(1.0 / gf.dy);
+    T s = 0;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const int j = 2 * J + b;
         const bool ring = i == 0 || i == gf.nx - 1 || j == 0 || j == gf.ny - 1;
-        const double lap =
-            ring ? ring_lap(gptr(e), gptr(K), nring, gf, i, j) : cell_lap_interior(CFld{e, gf.pitch}, gf, geo, i, j);
-        s = s + 0.5 * (f[ix(gf.pitch, i, j)] - lap);
+        const T lap = ring ? ring_lap(gptr(e), gptr(K), nring, gf, i, j)
+                           : interior_lap(gptr(e), geo, gf.pitch, ix(gf.pitch, i, j), idx, idy);
+        s = s + T(0.5) * (f[ix(gf.pitch, i, j)] - lap);
     }
     const size_t c = ix(gc.pitch, i, J);
     rc[c] = s;
-    ec[c] = 0.0;
+    ec[c] = 0;
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
@@ -599,33 +642,45 @@ void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, c
     k_ring_coef<<<(n + 255) / 256, 256, 0, s>>>(K, g, geo, bc);
 }
 
-size_t xline_lds_bytes(int nx) {
+size_t xline_lds_bytes(int nx, size_t elem) {
     const int C = xline_chunk(nx);
     if (C == 0) return (size_t)-1;
     const size_t nt = (size_t)(nx / C);
     // chunk staging (4 arrays), or one chunk array plus the interface rows
     const size_t tr = 4 * nt * (C + 1), iface = nt * (C + 1) + 5 * IFACE;
-    return (tr > iface ? tr : iface) * sizeof(double);
+    return (tr > iface ? tr : iface) * elem;
 }
 
 bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
 
-void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
-                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo) {
+template <typename T, typename G>
+static void residual_restrict_y(hipStream_t s, T *rc, T *ec, const T *e, const T *f, const T *K, const Grid &gf,
+                                const Grid &gc, const G &geo) {
     const dim3 grid((unsigned)((gc.nx + 63) / 64), (unsigned)((gc.ny + 3) / 4));
     k_residual_restrict_y<<<grid, 256, 0, s>>>(rc, ec, e, f, K, gf, gc, geo);
 }
 
-void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
-                  int row0, int rowstep, int nrows, int extra0, int extra1) {
+void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
+                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo) {
+    residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo);
+}
+
+void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
+                                const Grid &gf, const Grid &gc, const GeoF &geo) {
+    residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo);
+}
+
+template <typename T, typename G>
+static void xline(hipStream_t s, T *e, const T *f, const T *K, const Grid &g, const G &geo, int row0, int rowstep,
+                  int nrows, int extra0, int extra1) {
     if (nrows <= 0) return;
     const int C = xline_chunk(g.nx);
-    if (C == 0) {
+    const size_t lds = xline_lds_bytes(g.nx, sizeof(T));
+    if (C == 0 || lds > 160 * 1024) {
         g_last_error = "k_xline: row length not supported";
         throw Failure(MGP_ERR_UNSUPPORTED);
     }
     const int nt = g.nx / C;
-    const size_t lds = xline_lds_bytes(g.nx);
     switch (C) {
         case 2: k_xline<2><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
         case 4: k_xline<4><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
@@ -637,6 +692,16 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
         g_last_error = std::string("k_xline launch: ") + hipGetErrorString(err);
         throw Failure(MGP_ERR_HIP);
     }
+}
+
+void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
+                  int row0, int rowstep, int nrows, int extra0, int extra1) {
+    xline(s, e, f, K, g, geo, row0, rowstep, nrows, extra0, extra1);
+}
+
+void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
+                  int rowstep, int nrows, int extra0, int extra1) {
+    xline(s, e, f, K, g, geo, row0, rowstep, nrows, extra0, extra1);
 }
 
 }  // namespace mgp
