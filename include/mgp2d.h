@@ -263,6 +263,32 @@ int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double
 /* hipDeviceSynchronize on the current device. */
 int mgp_device_synchronize(void);
 
+/* ---- 4. Batched solver (no counterpart in the reference, whose callers solve
+ *         one problem per call: many independent problems of uneven sizes,
+ *         each solved as MGP_METHOD_KRYLOV_MG solves it, with one GPU launch
+ *         per level operation for all of them) ---------------------------- */
+
+typedef struct mgp_batch mgp_batch;
+
+/* n problems with host metrics (mgp_geo layouts) and BCs (Dirichlet or Neumann sides,
+ * homogeneous).  opts: method MGP_METHOD_KRYLOV_MG, homog 1; smoothing counts, maxdepth and
+ * use_graph as for mgp_solver_create.  Each problem keeps its own hierarchy in HBM. */
+int mgp_batch_create(mgp_batch **b, int n, const mgp_geo *geos, const mgp_bc *bcs, const mgp_solver_opts *opts);
+/* As mgp_batch_create, with problem k's metrics generated on the GPU from maps[k] on an
+ * nx[k] x ny[k] grid of spacing dx[k], dy[k]. */
+int mgp_batch_create_mapped(mgp_batch **b, int n, const mgp_mapping *maps, const int *nx, const int *ny,
+                            const double *dx, const double *dy, const mgp_bc *bcs, const mgp_solver_opts *opts);
+int mgp_batch_destroy(mgp_batch *b);
+int mgp_batch_count(const mgp_batch *b);
+/* Problem k's rhs (nx x ny, the mgp_solver_set_rhs layout) / phi (valid cells, nx x ny). */
+int mgp_batch_set_rhs(mgp_batch *b, int k, const double *rhs);
+int mgp_batch_get_phi(mgp_batch *b, int k, double *phi);
+/* Solves every problem to ||rhs - L phi|| <= tol ||rhs - L phi0||; iters[k], outcomes[k]
+ * (may be NULL) per problem.  A problem's iterates do not depend on the others in the batch. */
+int mgp_batch_solve(mgp_batch *b, double tol, int maxiters, int zerophi, int *iters, int *outcomes);
+/* V-cycle applications of the longest-running problem in the last solve. */
+int mgp_batch_last_cycles(const mgp_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

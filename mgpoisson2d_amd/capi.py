@@ -75,7 +75,9 @@ EXPORTS = (
     "mgp_device_synchronize", "mgp_xline_relax", "mgp_compute_pd", "mgp_compute_grad", "mgp_compute_div",
     "mgp_fill_boundary_fluxes", "mgp_solver_set_flux", "mgp_solver_get_flux", "mgp_solver_project",
     "mgp_relax_jacobi", "mgp_solve_bicgstab", "mgp_metric_fill", "mgp_solver_create_mapped",
-    "mgp_inner_prod", "mgp_integrate2d", "mgp_extrapolate_ghosts",
+    "mgp_inner_prod", "mgp_integrate2d", "mgp_extrapolate_ghosts", "mgp_batch_create",
+    "mgp_batch_create_mapped", "mgp_batch_destroy", "mgp_batch_count", "mgp_batch_set_rhs", "mgp_batch_get_phi",
+    "mgp_batch_solve", "mgp_batch_last_cycles",
 )
 
 
@@ -434,3 +436,75 @@ class Solver:
 
 def device_synchronize():
     _check(lib.mgp_device_synchronize())
+
+
+class Batch:
+    """Many independent problems solved together (mgp_batch): each is solved as
+    KRYLOV_MG solves it, with one GPU launch per level operation for all.
+
+    geos: a list of Geometry (metrics uploaded), or with mappings given, a list
+    of objects with nx, ny, dx, dy (metrics generated on the GPU from
+    mappings[k]).  bcs: one BoundaryConditions per problem (Dirichlet or
+    Neumann sides; homogeneous)."""
+
+    def __init__(self, geos, bcs, mappings=None, smooth_down=2, smooth_up=2, smooth_bottom=2, maxdepth=-1,
+                 use_graph=True):
+        n = len(geos)
+        self.shapes = [(g.ny, g.nx) for g in geos]
+        opts = SolverOpts(METHOD_KRYLOV_MG, ORDER_MC, 1, smooth_down, smooth_up, smooth_bottom, maxdepth, 1,
+                          int(use_graph))
+        a = _Args()
+        bc_arr = (_Bc * n)()
+        for k, b in enumerate(bcs):
+            bc_arr[k] = a.bc(b)._obj
+        self._h = ctypes.c_void_p()
+        if mappings is None:
+            geo_arr = (_Geo * n)()
+            for k, g in enumerate(geos):
+                geo_arr[k] = a.geo(g)._obj
+            _check(lib.mgp_batch_create(ctypes.byref(self._h), _i(n), geo_arr, bc_arr, ctypes.byref(opts)))
+        else:
+            maps = (Mapping * n)(*mappings)
+            nx = (_i * n)(*[g.nx for g in geos])
+            ny = (_i * n)(*[g.ny for g in geos])
+            dx = (_d * n)(*[g.dx for g in geos])
+            dy = (_d * n)(*[g.dy for g in geos])
+            _check(lib.mgp_batch_create_mapped(ctypes.byref(self._h), _i(n), maps, nx, ny, dx, dy, bc_arr,
+                                               ctypes.byref(opts)))
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def close(self):
+        if self._h:
+            lib.mgp_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rhs(self, k, rhs):
+        rhs = _f64(rhs)
+        if rhs.shape != self.shapes[k]:
+            raise ValueError(f"rhs of problem {k} must have shape {self.shapes[k]}")
+        _check(lib.mgp_batch_set_rhs(self._h, _i(k), _ptr(rhs)))
+
+    def get_phi(self, k):
+        phi = np.zeros(self.shapes[k])
+        _check(lib.mgp_batch_get_phi(self._h, _i(k), _ptr(phi)))
+        return phi
+
+    def solve(self, tol=1e-10, maxiters=200, zerophi=True):
+        """Returns (iterations, outcomes) per problem."""
+        n = len(self)
+        iters = np.zeros(n, dtype=np.int32)
+        outc = np.zeros(n, dtype=np.int32)
+        _check(lib.mgp_batch_solve(self._h, _d(tol), _i(maxiters), _i(int(zerophi)),
+                                   iters.ctypes.data_as(ctypes.POINTER(_i)), outc.ctypes.data_as(ctypes.POINTER(_i))))
+        return iters, outc
+
+    def last_cycles(self):
+        return lib.mgp_batch_last_cycles(self._h)

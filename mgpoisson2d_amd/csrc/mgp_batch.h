@@ -1,0 +1,137 @@
+// mgp_batch.h -- the batched fast solver: many independent problems of
+// uneven sizes, each solved as KRYLOV_MG solves it, with one launch per level
+// operation for all of them.  Launches are driven by tables built once per
+// batch (row tasks for the line relaxation, cell tiles for everything else);
+// every workgroup reads its problem's entry and works on that problem alone,
+// so a problem's arithmetic does not depend on the others in the batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mgp_common.h"
+#include "mgp_host.h"
+
+namespace mgp {
+
+// One level of one problem as the batched level kernels see it.  e/f are the
+// correction and right-hand side of the level's cycle.  On the finest level
+// the two entries are the two preconditioner applications of an iteration
+// (phat from p, shat from s); on the others both are the level's own fields.
+struct BLev {
+    Grid g;
+    Geo geo;
+    double *e[2];
+    double *f[2];
+    const double *K;    // ring stencil table
+    double slo, shi;    // ghost signs of the y sides in the prolongation
+    int prob, nt;       // problem index; x-line chunks (nx / C)
+};
+
+// Krylov vectors of one problem (finest level, level layout).
+enum BVec : int { BV_PHI = 0, BV_RHS, BV_R, BV_RHAT, BV_P, BV_V, BV_S, BV_T, BV_PHAT, BV_SHAT, BV_COUNT };
+
+struct BProb {
+    Grid g;
+    Geo geo;
+    Bc bc;             // the caller's BCs (homogeneous use)
+    double *v[BV_COUNT];
+    double *sc;        // SC_COUNT BiCGStab scalars
+    double *partial;   // nb partial sums of the reductions
+    int nb;            // reduction row blocks: block b sums rows b, b + nb, ...
+    int lev0;          // index of the finest level in the level table
+};
+
+// Row task: row `row` of level `lev`, then row `extra` (< 0: none) by the
+// same workgroup (the boundary-row fold of the single-problem sweep).
+struct XlTask {
+    int lev, row, extra, pad;
+};
+
+// Cell tile: cells (64 tx .. 64 tx + 63, 4 ty .. 4 ty + 3) of item `item`
+// (a problem or a level, as the kernel says); for reductions, tx is the row
+// block.
+struct BTile {
+    int item, tx, ty, pad;
+};
+
+// Level kernels (mgp_xline.hip).  `active` is indexed by problem; sel picks
+// the e/f pair of the finest level.
+void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
+                        int C, size_t lds);
+// coarse (lev+1) r = 1x2 restriction of f - L e on lev, coarse e = 0; tiles over the coarse cells
+void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
+                           int sel);
+// e(lev) += linear-in-y prolongation of e(lev+1); tiles over the fine cells
+void launch_prolong_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
+                          int sel);
+
+// The batched solver.
+class BatchKMG {
+  public:
+    struct Problem {
+        int nx, ny;
+        double dx, dy;
+        const mgp_geo *geo;         // host metrics, or
+        const mgp_mapping *map;     // metrics generated on the GPU
+        Bc bc;                      // types only (homogeneous)
+    };
+    BatchKMG(hipStream_t s, const std::vector<Problem> &probs, const mgp_solver_opts &o);
+    ~BatchKMG();
+    BatchKMG(const BatchKMG &) = delete;
+    BatchKMG &operator=(const BatchKMG &) = delete;
+
+    int count() const { return (int)P.size(); }
+    const Grid &grid(int k) const { return P[k].g; }
+    double *vec(int k, int which) { return P[k].v[which]; }
+    // Solves every problem to tol (relative to its own initial residual).
+    // iters[k] / outcome[k] per problem; returns the V-cycle applications of
+    // the busiest problem.
+    int solve(double tol, int maxiters, int zerophi, int *iters, int *outcome);
+    int last_cycles = 0;
+
+  private:
+    void body();
+    void precondition(int sel);
+    void zebra(int d, int set, int sel);
+    void reduce(int a, int b, int slot, const int *m);
+    void restart(const int *m);
+    void true_residual(const int *m);
+    void upload_flags(int *dst, const std::vector<int> &h);
+    void fetch_scalars();
+
+    hipStream_t s;
+    int sd, su, sb;
+    std::vector<BProb> P;
+    std::vector<BLev> L;
+    std::vector<int> nlev;      // levels per problem
+    int depth = 0;              // max levels
+    void *arena = nullptr;
+    double *scal = nullptr;     // all problems' BiCGStab scalars, SC_COUNT each
+    BProb *dP = nullptr;
+    BLev *dL = nullptr;
+    int *active = nullptr, *mask = nullptr;
+    double *host_sc = nullptr;  // pinned: all problems' scalars
+    // launch tables (device) with their lengths
+    struct Table {
+        void *p = nullptr;
+        int n = 0;
+        int C = 0;        // x-line tables: chunk length
+        size_t lds = 0;   // x-line tables: dynamic LDS
+    };
+    enum { SET_MID = 0, SET_BOTTOM = 1 };
+    struct XlTable {
+        int d, set, colour;
+        Table t;
+    };
+    Table upload(const void *host, size_t bytes, int n);
+    Table t0;                    // finest-level tiles, item = problem
+    Table tred;                  // reduction row blocks, item = problem (tx = block)
+    std::vector<XlTable> xl;     // x-line task tables per (level, set, colour, chunk class)
+    std::vector<Table> tr, tp;   // per level: restriction (coarse tiles) / prolongation (fine tiles), item = level
+    std::vector<void *> owned;   // device tables to free
+    hipGraphExec_t graph = nullptr;
+};
+
+}  // namespace mgp

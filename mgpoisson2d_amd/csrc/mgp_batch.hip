@@ -1,0 +1,559 @@
+// mgp_batch.hip -- the batched fast solver (mgp_batch.h).
+//
+// Every problem runs the KRYLOV_MG iteration of mgp_krylov.hip: FP64
+// right-preconditioned BiCGStab on the reference operator with homogeneous
+// BCs, one y-semicoarsening x-line V-cycle per preconditioner application,
+// restarts from the true residual.  What the batch changes is the launch
+// structure: one launch per operation covers every problem that takes part
+// in it, the workgroups finding their problem (and level) through tables
+// built once.  Per-problem state (scalars, convergence, restarts) lives in
+// device arrays and a per-problem active flag; problems that have converged
+// keep their results while the others go on.
+//
+// Reductions are per problem and independent of the batch: problem p's
+// partial sums come from nb(p) row blocks fixed by its size and are summed
+// in block order, so a problem's iterates are the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <string>
+#include <tuple>
+
+#include "mgp_batch.h"
+#include "mgp_geometry.h"
+#include "mgp_kernels.h"
+#include "mgp_reduce.h"
+#include "mgp_stencil.h"
+
+namespace mgp {
+
+namespace {
+
+constexpr int MIN_COARSE_NY = 4;
+
+__device__ __forceinline__ bool tile_cell(const BTile &t, const Grid &g, int &i, int &j) {
+    i = t.tx * 64 + (threadIdx.x & 63);
+    j = t.ty * 4 + (threadIdx.x >> 6);
+    return i < g.nx && j < g.ny;
+}
+
+// v[out] = (v[rhs] -) A v[in], A the reference operator with the problem's
+// homogeneous BCs (ghosts synthesised from the valid cells).
+__global__ void __launch_bounds__(256) k_b_apply(const BTile *tiles, const BProb *P, const int *mask, int out, int in,
+                                                 int rhs) {
+    const BTile t = tiles[blockIdx.x];
+    if (!mask[t.item]) return;
+    const BProb *p = P + t.item;
+    const Grid g = p->g;
+    int i, j;
+    if (!tile_cell(t, g, i, j)) return;
+    const Geo geo = p->geo;
+    const Bc bc = p->bc;
+    const double *phi = p->v[in];
+    const size_t c = ix(g.pitch, i, j);
+    const bool interior = i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+    const double v = interior ? cell_lap_interior(CFld{phi, g.pitch}, g, geo, i, j)
+                              : cell_lap(synth(phi, g, bc), g, geo, bc, 1, i, j);
+    p->v[out][c] = rhs >= 0 ? p->v[rhs][c] - v : v;
+}
+
+// Partial sums of vol * v[a] * v[b] over the rows b, b + nb, ... of a problem.
+__global__ void __launch_bounds__(256) k_b_reduce_partial(const BTile *tiles, const BProb *P, const int *mask, int a,
+                                                          int b) {
+    __shared__ double ss[256], sc[256];
+    const BTile t = tiles[blockIdx.x];
+    if (!mask[t.item]) return;  // uniform over the workgroup
+    const BProb *p = P + t.item;
+    const Grid g = p->g;
+    const double vol = g.dx * g.dy;
+    const double *A = p->v[a], *B = p->v[b];
+    const int nb = p->nb;
+    Acc acc{0.0, 0.0};
+    for (int j = t.tx; j < g.ny; j += nb) {
+        const size_t row = ix(g.pitch, 0, j);
+        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) acc.add(vol * (A[row + i] * B[row + i]));
+    }
+    const double v = block_combine(ss, sc, acc, 0.0, 0);
+    if (threadIdx.x == 0) p->partial[t.tx] = v;
+}
+
+// One workgroup per problem: sc[slot] = sum of the nb partials in block order.
+__global__ void __launch_bounds__(256) k_b_reduce_final(const BProb *P, const int *mask, int slot) {
+    __shared__ double ss[256], sc[256];
+    if (!mask[blockIdx.x]) return;
+    const BProb *p = P + blockIdx.x;
+    Acc acc{0.0, 0.0};
+    for (int k = threadIdx.x; k < p->nb; k += blockDim.x) acc.add(p->partial[k]);
+    const double v = block_combine(ss, sc, acc, 0.0, 0);
+    if (threadIdx.x == 0) p->sc[slot] = v;
+}
+
+// BiCGStab scalars of every masked problem (the ops of k_bicg_scalars).
+__global__ void k_b_scalars(const BProb *P, const int *mask, int n, int op) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || !mask[k]) return;
+    double *sc = P[k].sc;
+    if (op == 0) {
+        sc[SC_BETA] = (sc[SC_RHO_NEW] / sc[SC_RHO]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
+        sc[SC_RHO] = sc[SC_RHO_NEW];
+    } else if (op == 1) {
+        sc[SC_ALPHA] = sc[SC_RHO] / sc[SC_RV];
+    } else if (op == 2) {
+        sc[SC_OMEGA] = sc[SC_TT] > 0.0 ? sc[SC_TS] / sc[SC_TT] : 0.0;
+    } else {
+        sc[SC_RHO] = 1.0;
+        sc[SC_ALPHA] = 1.0;
+        sc[SC_OMEGA] = 1.0;
+        sc[SC_BETA] = 0.0;
+    }
+}
+
+// Vector updates (the ops of k_bicg_update):
+//  0: p = r + beta (p - omega v)   1: s = r - alpha v   2: phi += alpha phat
+//  3: phi += omega shat            4: r = s - omega t
+__global__ void __launch_bounds__(256) k_b_update(const BTile *tiles, const BProb *P, const int *mask, int op) {
+    const BTile t = tiles[blockIdx.x];
+    if (!mask[t.item]) return;
+    const BProb *p = P + t.item;
+    const Grid g = p->g;
+    int i, j;
+    if (!tile_cell(t, g, i, j)) return;
+    const size_t c = ix(g.pitch, i, j);
+    const double *sc = p->sc;
+    double *const *v = p->v;
+    switch (op) {
+    case 0: v[BV_P][c] = v[BV_R][c] + sc[SC_BETA] * (v[BV_P][c] - sc[SC_OMEGA] * v[BV_V][c]); break;
+    case 1: v[BV_S][c] = v[BV_R][c] - sc[SC_ALPHA] * v[BV_V][c]; break;
+    case 2: v[BV_PHI][c] = v[BV_PHI][c] + sc[SC_ALPHA] * v[BV_PHAT][c]; break;
+    case 3: v[BV_PHI][c] = v[BV_PHI][c] + sc[SC_OMEGA] * v[BV_SHAT][c]; break;
+    default: v[BV_R][c] = v[BV_S][c] - sc[SC_OMEGA] * v[BV_T][c]; break;
+    }
+}
+
+// v[dst] = v[src] (src >= 0) or 0 over the valid cells.
+__global__ void __launch_bounds__(256) k_b_copy(const BTile *tiles, const BProb *P, const int *mask, int dst, int src) {
+    const BTile t = tiles[blockIdx.x];
+    if (!mask[t.item]) return;
+    const BProb *p = P + t.item;
+    const Grid g = p->g;
+    int i, j;
+    if (!tile_cell(t, g, i, j)) return;
+    const size_t c = ix(g.pitch, i, j);
+    p->v[dst][c] = src >= 0 ? p->v[src][c] : 0.0;
+}
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+void tiles_of(std::vector<BTile> &out, int item, int nx, int ny) {
+    for (int ty = 0; ty < (ny + 3) / 4; ++ty)
+        for (int tx = 0; tx < (nx + 63) / 64; ++tx) out.push_back(BTile{item, tx, ty, 0});
+}
+
+}  // namespace
+
+BatchKMG::Table BatchKMG::upload(const void *host, size_t bytes, int n) {
+    Table t;
+    t.n = n;
+    if (n == 0) return t;
+    t.p = dev_alloc_zero(bytes);
+    owned.push_back(t.p);
+    hip_check(hipMemcpyAsync(t.p, host, bytes, hipMemcpyHostToDevice, s), "table upload");
+    return t;
+}
+
+BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_solver_opts &o) : s(s_) {
+    const int n = (int)probs.size();
+    if (n <= 0) {
+        g_last_error = "batch: at least one problem";
+        throw Failure(MGP_ERR_ARG);
+    }
+    sd = o.smooth_down > 0 ? o.smooth_down : 1;
+    su = o.smooth_up > 0 ? o.smooth_up : 1;
+    sb = o.smooth_bottom > 0 ? o.smooth_bottom : 1;
+    const int maxlev = o.maxdepth < 0 ? 64 : o.maxdepth + 1;
+    for (const Problem &q : probs) {
+        if (q.nx < 4 || q.ny < 4 || !(q.dx > 0.0) || !(q.dy > 0.0)) {
+            g_last_error = "batch: every problem needs nx, ny >= 4 and dx, dy > 0";
+            throw Failure(MGP_ERR_ARG);
+        }
+        if (!xline_supported(q.nx)) {
+            g_last_error = "batch: row too long for one workgroup's line solve (nx <= 4111)";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        for (int k = 0; k < 4; ++k)
+            if (q.bc.type[k] != BC_DIRI && q.bc.type[k] != BC_NEUM) {
+                g_last_error = "batch: Dirichlet and Neumann sides only";
+                throw Failure(MGP_ERR_UNSUPPORTED);
+            }
+    }
+
+    // level counts and the arena
+    nlev.resize(n);
+    size_t bytes = 0;
+    auto take = [&](size_t elems) {
+        const size_t off = bytes;
+        bytes += round256(elems * sizeof(double));
+        return off;
+    };
+    struct Off {
+        size_t v[BV_COUNT], partial;
+        std::vector<size_t> J, Jxx, Jxy, Jyx, Jyy, e, r, K;
+    };
+    std::vector<Off> off(n);
+    for (int p = 0; p < n; ++p) {
+        const Problem &q = probs[p];
+        int ny = q.ny, lev = 0;
+        Off &O = off[p];
+        const size_t n0 = level_elems(q.nx, q.ny);
+        for (int k = 0; k < BV_COUNT; ++k) O.v[k] = take(n0);
+        const int nb = std::max(1, std::min(q.ny, (q.nx * q.ny + 8191) / 8192));
+        O.partial = take((size_t)nb);
+        while (true) {
+            const size_t nl = level_elems(q.nx, ny);
+            O.J.push_back(take(nl));
+            O.Jxx.push_back(take(nl));
+            O.Jxy.push_back(take(nl));
+            O.Jyx.push_back(take(nl));
+            O.Jyy.push_back(take(nl));
+            O.e.push_back(lev ? take(nl) : 0);
+            O.r.push_back(lev ? take(nl) : 0);
+            const Grid gl = make_grid(q.nx, ny, q.dx, q.dy);
+            O.K.push_back(take((size_t)ring_table_elems(gl)));
+            ++lev;
+            if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || lev >= maxlev) break;
+            ny /= 2;
+        }
+        nlev[p] = lev;
+        depth = std::max(depth, lev);
+    }
+    arena = dev_alloc_zero(bytes);
+    scal = static_cast<double *>(dev_alloc_zero((size_t)n * SC_COUNT * sizeof(double)));
+    char *base = static_cast<char *>(arena);
+    auto at = [&](size_t o) { return reinterpret_cast<double *>(base + o); };
+
+    // problems, levels, metrics and stencil tables
+    P.resize(n);
+    for (int p = 0; p < n; ++p) {
+        const Problem &q = probs[p];
+        const Off &O = off[p];
+        BProb &B = P[p];
+        B.g = make_grid(q.nx, q.ny, q.dx, q.dy);
+        for (int k = 0; k < 4; ++k) {
+            B.bc.type[k] = q.bc.type[k];
+            B.bc.mode[k] = MODE_UNIFORM;
+            B.bc.val[k] = 0.0;
+            B.bc.data[k] = nullptr;
+        }
+        for (int k = 0; k < BV_COUNT; ++k) B.v[k] = at(O.v[k]);
+        B.sc = scal + (size_t)p * SC_COUNT;
+        B.partial = at(O.partial);
+        B.nb = std::max(1, std::min(q.ny, (q.nx * q.ny + 8191) / 8192));
+        B.lev0 = (int)L.size();
+        int ny = q.ny;
+        double dy = q.dy;
+        const Bc lbc = mg_bc(B.bc, MODE_UNIFORM);
+        for (int d = 0; d < nlev[p]; ++d) {
+            BLev l;
+            l.g = make_grid(q.nx, ny, q.dx, dy);
+            l.geo = Geo{at(O.J[d]), at(O.Jxx[d]), at(O.Jxy[d]), at(O.Jyx[d]), at(O.Jyy[d])};
+            double *J = at(O.J[d]), *Jxx = at(O.Jxx[d]), *Jxy = at(O.Jxy[d]), *Jyx = at(O.Jyx[d]),
+                   *Jyy = at(O.Jyy[d]);
+            if (d == 0) {
+                if (q.map) {
+                    launch_metric(s, *q.map, l.g, J, Jxx, Jxy, Jyx, Jyy);
+                } else {
+                    const mgp_geo &h = *q.geo;
+                    upload_block(s, J, l.g, h.J, 0, 0, q.nx, q.ny);
+                    upload_block(s, Jxx, l.g, h.Jxx, 0, 0, q.nx + 1, q.ny);
+                    upload_block(s, Jxy, l.g, h.Jxy, 0, 0, q.nx + 1, q.ny);
+                    upload_block(s, Jyx, l.g, h.Jyx, 0, 0, q.nx, q.ny + 1);
+                    upload_block(s, Jyy, l.g, h.Jyy, 0, 0, q.nx, q.ny + 1);
+                }
+                l.e[0] = B.v[BV_PHAT];
+                l.e[1] = B.v[BV_SHAT];
+                l.f[0] = B.v[BV_P];
+                l.f[1] = B.v[BV_S];
+                B.geo = l.geo;
+            } else {
+                const BLev &f = L.back();
+                launch_restrict_cc(s, J, f.geo.J, l.g, f.g.pitch, 1, 2, nullptr, nullptr);
+                launch_restrict_xf(s, Jxx, f.geo.Jxx, l.g, f.g.pitch, 1, 2);
+                launch_restrict_xf(s, Jxy, f.geo.Jxy, l.g, f.g.pitch, 1, 2);
+                launch_restrict_yf(s, Jyx, f.geo.Jyx, l.g, f.g.pitch, 1, 2);
+                launch_restrict_yf(s, Jyy, f.geo.Jyy, l.g, f.g.pitch, 1, 2);
+                l.e[0] = l.e[1] = at(O.e[d]);
+                l.f[0] = l.f[1] = at(O.r[d]);
+            }
+            launch_ring_coef(s, at(O.K[d]), l.g, l.geo, lbc);
+            l.K = at(O.K[d]);
+            l.slo = B.bc.type[YLO] == BC_DIRI ? -1.0 : 1.0;
+            l.shi = B.bc.type[YHI] == BC_DIRI ? -1.0 : 1.0;
+            l.prob = p;
+            l.nt = 0;  // set with the chunk class below
+            L.push_back(l);
+            ny /= 2;
+            dy *= 2.0;
+        }
+    }
+
+    // chunk class of each problem: the largest chunk length any problem of
+    // the batch needs, where it also fits this problem's rows (remainder
+    // below the chunk count), else the problem's own
+    int Cb = 2;
+    for (const BProb &B : P) Cb = std::max(Cb, xline_chunk_length(B.g.nx));
+    std::vector<int> Cp(n);
+    for (int p = 0; p < n; ++p) {
+        const int nx = P[p].g.nx;
+        Cp[p] = (nx % Cb < nx / Cb) ? Cb : xline_chunk_length(nx);
+        for (int d = 0; d < nlev[p]; ++d) L[P[p].lev0 + d].nt = nx / Cp[p];
+    }
+
+    // launch tables
+    std::vector<BTile> t0h, tredh;
+    for (int p = 0; p < n; ++p) {
+        tiles_of(t0h, p, P[p].g.nx, P[p].g.ny);
+        for (int b = 0; b < P[p].nb; ++b) tredh.push_back(BTile{p, b, 0, 0});
+    }
+    t0 = upload(t0h.data(), t0h.size() * sizeof(BTile), (int)t0h.size());
+    tred = upload(tredh.data(), tredh.size() * sizeof(BTile), (int)tredh.size());
+    tr.resize(depth);
+    tp.resize(depth);
+    for (int d = 0; d < depth; ++d) {
+        std::map<std::tuple<int, int, int>, std::vector<XlTask>> tabs;  // (set, colour, C)
+        std::map<std::tuple<int, int, int>, size_t> lds;
+        std::vector<BTile> trh, tph;
+        for (int p = 0; p < n; ++p) {
+            if (d >= nlev[p]) continue;
+            const int lev = P[p].lev0 + d;
+            const int ny = L[lev].g.ny, nx = L[lev].g.nx;
+            const int set = d == nlev[p] - 1 ? SET_BOTTOM : SET_MID;
+            const int C = Cp[p];
+            auto add = [&](int colour, XlTask t) {
+                const auto key = std::make_tuple(set, colour, C);
+                tabs[key].push_back(t);
+                lds[key] = std::max(lds[key], xline_lds_bytes_for(nx, C, sizeof(double)));
+            };
+            // the order of xline_zebra: even interior rows, odd interior rows
+            // (the first and last relaxing rows 0 and ny-1 after their own),
+            // else rows 0 and ny-1 in a launch of their own
+            const int n_even = ny >= 4 ? (ny - 4) / 2 + 1 : 0;
+            const int n_odd = ny >= 3 ? (ny - 3) / 2 + 1 : 0;
+            for (int k = 0; k < n_even; ++k) add(0, XlTask{lev, 2 + 2 * k, -1, 0});
+            for (int k = 0; k < n_odd; ++k) {
+                int extra = -1;
+                if (n_odd >= 2 && k == 0) extra = 0;
+                if (n_odd >= 2 && k == n_odd - 1) extra = ny - 1;
+                add(1, XlTask{lev, 1 + 2 * k, extra, 0});
+            }
+            if (n_odd < 2) {
+                add(2, XlTask{lev, 0, -1, 0});
+                add(2, XlTask{lev, ny - 1, -1, 0});
+            }
+            if (set == SET_MID) {
+                tiles_of(trh, lev, nx, L[lev + 1].g.ny);
+                tiles_of(tph, lev, nx, ny);
+            }
+        }
+        for (auto &kv : tabs) {
+            XlTable x;
+            x.d = d;
+            x.set = std::get<0>(kv.first);
+            x.colour = std::get<1>(kv.first);
+            x.t = upload(kv.second.data(), kv.second.size() * sizeof(XlTask), (int)kv.second.size());
+            x.t.C = std::get<2>(kv.first);
+            x.t.lds = lds[kv.first];
+            xl.push_back(x);
+        }
+        tr[d] = upload(trh.data(), trh.size() * sizeof(BTile), (int)trh.size());
+        tp[d] = upload(tph.data(), tph.size() * sizeof(BTile), (int)tph.size());
+    }
+    std::sort(xl.begin(), xl.end(), [](const XlTable &a, const XlTable &b) {
+        return std::make_tuple(a.d, a.set, a.colour, a.t.C) < std::make_tuple(b.d, b.set, b.colour, b.t.C);
+    });
+
+    dP = static_cast<BProb *>(dev_alloc_zero(n * sizeof(BProb)));
+    dL = static_cast<BLev *>(dev_alloc_zero(L.size() * sizeof(BLev)));
+    hip_check(hipMemcpyAsync(dP, P.data(), n * sizeof(BProb), hipMemcpyHostToDevice, s), "problem table");
+    hip_check(hipMemcpyAsync(dL, L.data(), L.size() * sizeof(BLev), hipMemcpyHostToDevice, s), "level table");
+    active = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
+    mask = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
+    hip_check(hipHostMalloc((void **)&host_sc, (size_t)n * SC_COUNT * sizeof(double)), "hipHostMalloc");
+    hip_check(hipStreamSynchronize(s), "batch set-up");
+    if (o.use_graph) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(s, &g), "capture end");
+        hip_check(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+    }
+}
+
+BatchKMG::~BatchKMG() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    (void)hipStreamSynchronize(s);
+    for (void *p : owned) dev_free(p);
+    if (dP) dev_free(dP);
+    if (dL) dev_free(dL);
+    if (active) dev_free(active);
+    if (mask) dev_free(mask);
+    if (arena) dev_free(arena);
+    if (scal) dev_free(scal);
+    if (host_sc) (void)hipHostFree(host_sc);
+}
+
+void BatchKMG::zebra(int d, int set, int sel) {
+    for (int colour = 0; colour < 3; ++colour)
+        for (const XlTable &x : xl)
+            if (x.d == d && x.set == set && x.colour == colour)
+                launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.lds);
+}
+
+// e = M f for the pair `sel` of every active problem (phat from p: 0, shat
+// from s: 1): the V-cycle of mgp_krylov.hip, level by level over the batch.
+// Problems whose coarsest level is d make their bottom sweeps there while
+// the others go on down.
+void BatchKMG::precondition(int sel) {
+    k_b_copy<<<t0.n, 256, 0, s>>>((const BTile *)t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
+    for (int d = 0; d < depth; ++d) {
+        for (int k = 0; k < sd; ++k) zebra(d, SET_MID, sel);
+        for (int k = 0; k < sb; ++k) zebra(d, SET_BOTTOM, sel);
+        if (d + 1 < depth) launch_restrict_batch(s, (const BTile *)tr[d].p, tr[d].n, dL, active, sel);
+    }
+    for (int d = depth - 2; d >= 0; --d) {
+        launch_prolong_batch(s, (const BTile *)tp[d].p, tp[d].n, dL, active, sel);
+        for (int k = 0; k < su; ++k) zebra(d, SET_MID, sel);
+    }
+}
+
+void BatchKMG::reduce(int a, int b, int slot, const int *m) {
+    k_b_reduce_partial<<<tred.n, 256, 0, s>>>((const BTile *)tred.p, dP, m, a, b);
+    k_b_reduce_final<<<count(), 256, 0, s>>>(dP, m, slot);
+}
+
+// One BiCGStab iteration of every active problem (the body of
+// KrylovMG::solve); ends with ||r||^2 vol in each problem's SC_RN.
+void BatchKMG::body() {
+    const int n = count();
+    const BTile *T = (const BTile *)t0.p;
+    const int sb_ = (n + 63) / 64;
+    reduce(BV_RHAT, BV_R, SC_RHO_NEW, active);
+    k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 0);
+    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 0);
+    precondition(0);
+    k_b_apply<<<t0.n, 256, 0, s>>>(T, dP, active, BV_V, BV_PHAT, -1);
+    reduce(BV_RHAT, BV_V, SC_RV, active);
+    k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 1);
+    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 1);
+    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 2);
+    precondition(1);
+    k_b_apply<<<t0.n, 256, 0, s>>>(T, dP, active, BV_T, BV_SHAT, -1);
+    reduce(BV_T, BV_S, SC_TS, active);
+    reduce(BV_T, BV_T, SC_TT, active);
+    k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 2);
+    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 3);
+    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 4);
+    reduce(BV_R, BV_R, SC_RN, active);
+}
+
+// r = rhs - A phi and ||r||^2 vol into SC_RN, for the masked problems.
+void BatchKMG::true_residual(const int *m) {
+    k_b_apply<<<t0.n, 256, 0, s>>>((const BTile *)t0.p, dP, m, BV_R, BV_PHI, BV_RHS);
+    reduce(BV_R, BV_R, SC_RN, m);
+}
+
+// New BiCGStab run from the current r for the masked problems.
+void BatchKMG::restart(const int *m) {
+    const int n = count();
+    const BTile *T = (const BTile *)t0.p;
+    k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, m, BV_RHAT, BV_R);
+    k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, m, BV_P, -1);
+    k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, m, BV_V, -1);
+    k_b_scalars<<<(n + 63) / 64, 64, 0, s>>>(dP, m, n, 3);
+}
+
+void BatchKMG::upload_flags(int *dst, const std::vector<int> &h) {
+    hip_check(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, s), "flags");
+}
+
+void BatchKMG::fetch_scalars() {
+    hip_check(hipMemcpyAsync(host_sc, scal, (size_t)count() * SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s),
+              "scalars");
+    hip_check(hipStreamSynchronize(s), "sync");
+}
+
+int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outcome) {
+    const int n = count();
+    const BTile *T = (const BTile *)t0.p;
+    std::vector<int> act(n, 1), msk(n, 0), it(n, 0), oc(n, MGP_MAXITERS);
+    std::vector<double> r0(n, 0.0), base(n, 1.0);
+    upload_flags(active, act);
+    if (zerophi) k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, active, BV_PHI, -1);
+    true_residual(active);
+    fetch_scalars();
+    for (int p = 0; p < n; ++p) {
+        r0[p] = std::sqrt(host_sc[p * SC_COUNT + SC_RN]);
+        if (r0[p] == 0.0) {
+            oc[p] = MGP_CONVERGED;
+            act[p] = 0;
+        }
+    }
+    upload_flags(active, act);
+    restart(active);
+    // iterative refinement factor of the FP64 cycle (as KrylovMG::solve)
+    const double refine = 1e-6;
+    last_cycles = 0;
+    for (int k = 1; k <= maxiters; ++k) {
+        if (std::find(act.begin(), act.end(), 1) == act.end()) break;
+        if (graph) hip_check(hipGraphLaunch(graph, s), "graph launch");
+        else body();
+        last_cycles += 2;
+        fetch_scalars();
+        bool check = false;
+        for (int p = 0; p < n; ++p) {
+            msk[p] = 0;
+            if (!act[p]) continue;
+            it[p] = k;
+            const double rn = std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0[p];
+            const double omega = host_sc[p * SC_COUNT + SC_OMEGA];
+            if (rn <= tol || omega == 0.0 || !std::isfinite(rn) || rn <= refine * base[p]) {
+                msk[p] = 1;
+                check = true;
+            }
+        }
+        if (!check) continue;
+        upload_flags(mask, msk);
+        true_residual(mask);
+        fetch_scalars();
+        for (int p = 0; p < n; ++p) {
+            if (!msk[p]) continue;
+            const double tr = std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0[p];
+            if (tr <= tol || !std::isfinite(tr)) {
+                oc[p] = tr <= tol ? MGP_CONVERGED : MGP_DIVERGED;
+                act[p] = 0;
+                msk[p] = 0;
+            } else {
+                base[p] = tr;  // restart from the true residual
+            }
+        }
+        upload_flags(mask, msk);
+        restart(mask);
+        upload_flags(active, act);
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    for (int p = 0; p < n; ++p) {
+        if (iters) iters[p] = it[p];
+        if (outcome) outcome[p] = oc[p];
+    }
+    return last_cycles;
+}
+
+}  // namespace mgp

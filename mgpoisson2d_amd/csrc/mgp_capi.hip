@@ -8,10 +8,12 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "mgp2d.h"
+#include "mgp_batch.h"
 #include "mgp_geometry.h"
 #include "mgp_host.h"
 #include "mgp_kernels.h"
@@ -96,6 +98,15 @@ struct Call {
 };
 
 }  // namespace
+
+struct mgp_batch {
+    hipStream_t stream = nullptr;
+    mgp::BatchKMG *b = nullptr;
+    ~mgp_batch() {
+        delete b;
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
 
 struct mgp_solver {
     hipStream_t stream = nullptr;
@@ -937,6 +948,137 @@ int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double
         *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, cold, bytes);
     });
 }
+
+extern "C++" {
+namespace {
+
+int create_batch(mgp_batch **out, int n, const mgp_bc *bcs, const mgp_solver_opts *opts,
+                 const std::function<void(int, BatchKMG::Problem &)> &fill) {
+    if (!out) return MGP_ERR_ARG;
+    *out = nullptr;
+    mgp_batch *b = new mgp_batch;
+    const int st = guarded([&] {
+        if (n <= 0 || !bcs) {
+            g_last_error = "batch: n > 0 problems and their BCs";
+            throw Failure(MGP_ERR_ARG);
+        }
+        mgp_solver_opts o;
+        if (opts) o = *opts;
+        else mgp_solver_default_opts(&o);
+        if (o.method != MGP_METHOD_KRYLOV_MG || !o.homog) {
+            g_last_error = "batch: method KRYLOV_MG with homogeneous BCs (homog = 1)";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        std::vector<BatchKMG::Problem> probs(n);
+        for (int k = 0; k < n; ++k) {
+            check_bc(&bcs[k]);
+            BatchKMG::Problem &q = probs[k];
+            q.geo = nullptr;
+            q.map = nullptr;
+            for (int side = 0; side < 4; ++side) {
+                q.bc.type[side] = bcs[k].type[side];
+                q.bc.mode[side] = MODE_UNIFORM;
+                q.bc.val[side] = 0.0;
+                q.bc.data[side] = nullptr;
+            }
+            fill(k, q);
+        }
+        hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        b->b = new BatchKMG(b->stream, probs, o);
+    });
+    if (st != MGP_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return MGP_OK;
+}
+
+void batch_index(const mgp_batch *b, int k) {
+    if (!b || !b->b || k < 0 || k >= b->b->count()) {
+        g_last_error = "batch: problem index out of range";
+        throw Failure(MGP_ERR_ARG);
+    }
+}
+
+}  // namespace
+}  // extern "C++"
+
+int mgp_batch_create(mgp_batch **out, int n, const mgp_geo *geos, const mgp_bc *bcs, const mgp_solver_opts *opts) {
+    return create_batch(out, n, bcs, opts, [&](int k, BatchKMG::Problem &q) {
+        if (!geos) {
+            g_last_error = "batch: null geometry array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        check_geo(&geos[k]);
+        q.nx = geos[k].nx;
+        q.ny = geos[k].ny;
+        q.dx = geos[k].dx;
+        q.dy = geos[k].dy;
+        q.geo = &geos[k];
+    });
+}
+
+int mgp_batch_create_mapped(mgp_batch **out, int n, const mgp_mapping *maps, const int *nx, const int *ny,
+                            const double *dx, const double *dy, const mgp_bc *bcs, const mgp_solver_opts *opts) {
+    return create_batch(out, n, bcs, opts, [&](int k, BatchKMG::Problem &q) {
+        if (!maps || !nx || !ny || !dx || !dy) {
+            g_last_error = "batch: null mapping or grid arrays";
+            throw Failure(MGP_ERR_ARG);
+        }
+        check_mapping(&maps[k]);
+        q.nx = nx[k];
+        q.ny = ny[k];
+        q.dx = dx[k];
+        q.dy = dy[k];
+        q.map = &maps[k];
+    });
+}
+
+int mgp_batch_destroy(mgp_batch *b) {
+    delete b;
+    return MGP_OK;
+}
+
+int mgp_batch_count(const mgp_batch *b) { return b && b->b ? b->b->count() : 0; }
+
+int mgp_batch_set_rhs(mgp_batch *b, int k, const double *rhs) {
+    return guarded([&] {
+        batch_index(b, k);
+        if (!rhs) {
+            g_last_error = "batch: null rhs";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const Grid &g = b->b->grid(k);
+        upload_block(b->stream, b->b->vec(k, BV_RHS), g, rhs, 0, 0, g.nx, g.ny);
+        hip_check(hipStreamSynchronize(b->stream), "sync");
+    });
+}
+
+int mgp_batch_get_phi(mgp_batch *b, int k, double *phi) {
+    return guarded([&] {
+        batch_index(b, k);
+        if (!phi) {
+            g_last_error = "batch: null phi";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const Grid &g = b->b->grid(k);
+        download_block(b->stream, phi, b->b->vec(k, BV_PHI), g, 0, 0, g.nx, g.ny);
+        hip_check(hipStreamSynchronize(b->stream), "sync");
+    });
+}
+
+int mgp_batch_solve(mgp_batch *b, double tol, int maxiters, int zerophi, int *iters, int *outcomes) {
+    return guarded([&] {
+        if (!b || !b->b || maxiters < 0 || !(tol >= 0.0)) {
+            g_last_error = "batch solve: bad arguments";
+            throw Failure(MGP_ERR_ARG);
+        }
+        b->b->solve(tol, maxiters, zerophi, iters, outcomes);
+    });
+}
+
+int mgp_batch_last_cycles(const mgp_batch *b) { return b && b->b ? b->b->last_cycles : 0; }
 
 int mgp_device_synchronize(void) {
     return guarded([&] { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });

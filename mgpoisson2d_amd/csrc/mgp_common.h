@@ -10,6 +10,8 @@
 // data(i, j) (arrayutils.f90:107-116).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstddef>
 #include <cstdint>
 
@@ -60,6 +62,20 @@ struct Bc {
 
 __host__ __device__ inline size_t ix(int pitch, int i, int j) {
     return (size_t)(j + 1) * (size_t)pitch + (size_t)(OFFX + i);
+}
+
+// y-direction prolongation for 1x2 semicoarsening at fine cell (i, j): linear
+// in y, coarse ghosts reflected (Neumann, sign +1) or negated (Dirichlet,
+// sign -1); ef += the interpolated coarse correction.
+template <typename T>
+__device__ __forceinline__ void prolong_y_cell(T *ef, const T *ec, const Grid &gf, const Grid &gc, T slo, T shi,
+                                               int i, int j) {
+    const int J = j >> 1;
+    const T c = ec[ix(gc.pitch, i, J)];
+    T nb;
+    if (j & 1) nb = J + 1 < gc.ny ? ec[ix(gc.pitch, i, J + 1)] : shi * c;
+    else nb = J - 1 >= 0 ? ec[ix(gc.pitch, i, J - 1)] : slo * c;
+    ef[ix(gf.pitch, i, j)] += T(0.75) * c + T(0.25) * nb;
 }
 
 }  // namespace mgp

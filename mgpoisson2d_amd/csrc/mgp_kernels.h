@@ -91,8 +91,12 @@ void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const dou
 void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
                                 const Grid &gf, const Grid &gc, const GeoF &geo);
 bool xline_supported(int nx);
+// chunk length C of the line solve of an nx-cell row (2, 4, 8 or 16; 0: too long)
+int xline_chunk_length(int nx);
 // dynamic LDS of one x-line workgroup for elements of `elem` bytes
 size_t xline_lds_bytes(int nx, size_t elem);
+// the same for the chunk length C (a batch may use a longer one than the row's own)
+size_t xline_lds_bytes_for(int nx, int C, size_t elem);
 // Relax rows j = row0 + k*rowstep, k < nrows (one workgroup each; no row of a
 // launch may read another: interior rows at least two apart, boundary rows
 // three).  The first workgroup then also relaxes row extra0, the last one row

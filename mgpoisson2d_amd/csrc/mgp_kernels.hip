@@ -10,6 +10,7 @@
 
 #include "mgp_host.h"
 #include "mgp_kernels.h"
+#include "mgp_reduce.h"
 #include "mgp_stencil.h"
 
 namespace mgp {
@@ -570,40 +571,6 @@ __global__ void __launch_bounds__(256) k_setval(double *y, double v, Grid g, int
 // Reductions.  Two passes: per-block partial sums in a fixed order, then one
 // block sums the partials.  Compensated (Neumaier) FP64 accumulation.
 // ===========================================================================
-struct Acc {
-    double s, c;
-    __device__ void add(double x) {
-        const double t = s + x;
-        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
-        s = t;
-    }
-    __device__ double val() const { return s + c; }
-};
-
-// Fixed-order tree combine of per-thread (s, c) pairs in LDS; thread 0 ends
-// with the block's compensated sum (or max for kind 2).
-__device__ __forceinline__ double block_combine(double *ss, double *sc, Acc acc, double mx, int kind) {
-    const int t = threadIdx.x;
-    ss[t] = kind == 2 ? mx : acc.s;
-    sc[t] = kind == 2 ? 0.0 : acc.c;
-    __syncthreads();
-    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (t < off) {
-            if (kind == 2) {
-                ss[t] = fmax(ss[t], ss[t + off]);
-            } else {
-                Acc u{ss[t], sc[t]};
-                u.add(ss[t + off]);
-                u.c += sc[t + off];
-                ss[t] = u.s;
-                sc[t] = u.c;
-            }
-        }
-        __syncthreads();
-    }
-    return kind == 2 ? ss[0] : ss[0] + sc[0];
-}
-
 // |x|^p for an integer p >= 1 as Fortran's real**integer evaluates it:
 // binary powering from the lowest bit.
 __device__ __forceinline__ double abs_pow(double x, int p) {
@@ -897,19 +864,13 @@ __global__ void __launch_bounds__(256) k_relax_block(Bottom B, double omega, int
 
 namespace mgp {
 
-// y-direction prolongation for 1x2 semicoarsening: linear in y, coarse
-// ghosts reflected (Neumann, sign +1) or negated (Dirichlet, sign -1).
+// y-direction prolongation for 1x2 semicoarsening (prolong_y_cell).
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong_y(T *ef, const T *ec, Grid gf, Grid gc, T slo, T shi) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= gf.nx || j >= gf.ny) return;
-    const int J = j >> 1;
-    const T c = ec[ix(gc.pitch, i, J)];
-    T nb;
-    if (j & 1) nb = J + 1 < gc.ny ? ec[ix(gc.pitch, i, J + 1)] : shi * c;
-    else nb = J - 1 >= 0 ? ec[ix(gc.pitch, i, J - 1)] : slo * c;
-    ef[ix(gf.pitch, i, j)] += T(0.75) * c + T(0.25) * nb;
+    prolong_y_cell(ef, ec, gf, gc, slo, shi, i, j);
 }
 
 template <typename TO, typename TI>

@@ -28,6 +28,7 @@
 
 #include <string>
 
+#include "mgp_batch.h"
 #include "mgp_host.h"
 #include "mgp_kernels.h"
 #include "mgp_stencil.h"
@@ -603,11 +604,8 @@ __device__ __forceinline__ T interior_lap(gconst<T> *e, const G &geo_, int P, si
 // cleared:  rc(i, J) = (r(i, 2J) + r(i, 2J+1)) / 2,  r = f - L e  (interior
 // stencil from the metrics, ring cells from the table),  ec(i, J) = 0.
 template <typename T, typename G>
-__global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const T *e, const T *f, const T *K,
-                                                             Grid gf, Grid gc, G geo) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= gc.nx || J >= gc.ny) return;
+__device__ __forceinline__ void restrict_cell(T *rc, T *ec, const T *e, const T *f, const T *K, const Grid &gf,
+                                              const Grid &gc, const G &geo, int i, int J) {
     const int nring = ring_count(gf.nx, gf.ny);
     const T idx = (T)(1.0 / gf.dx), idy = (T)(1.0 / gf.dy);
     T s = 0;
@@ -622,6 +620,77 @@ __global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const
     const size_t c = ix(gc.pitch, i, J);
     rc[c] = s;
     ec[c] = 0;
+}
+
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const T *e, const T *f, const T *K,
+                                                             Grid gf, Grid gc, G geo) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= gc.nx || J >= gc.ny) return;
+    restrict_cell(rc, ec, e, f, K, gf, gc, geo, i, J);
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms: the workgroup's task or tile names the level (and through
+// it the problem); workgroups of inactive problems return at once.
+// ---------------------------------------------------------------------------
+template <int C>
+__device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs, const int *active, int sel) {
+    const XlTask t = tasks[launch_slot()];
+    const BLev *L = levs + t.lev;
+    if (!active[L->prob]) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_bytes[];
+    double *lds = reinterpret_cast<double *>(lds_bytes);
+    const Grid g = L->g;
+    const Geo geo = L->geo;
+    double *e = L->e[sel];
+    const double *f = L->f[sel], *K = L->K;
+    const RowCtx<double> X0 = row_ctx<double>(g, L->nt, C);
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {
+        const int jr = r == 0 ? t.row : t.extra;
+        if (jr < 0) continue;  // uniform over the workgroup
+        if (r) __syncthreads();
+        RowCtx<double> X = X0;
+        asm volatile("" : "+s"(X.rem));
+        xline_row<C>(X, e, f, K, g, geo, jr, lds);
+    }
+}
+
+template <int C>
+__global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_xline_batch(const XlTask *tasks, const BLev *levs, const int *active, int sel) {
+    xline_task<C>(tasks, levs, active, sel);
+}
+
+template <int C>
+__global__ void __launch_bounds__(XL_THREADS) k_xline_batch_long(const XlTask *tasks, const BLev *levs,
+                                                                 const int *active, int sel) {
+    xline_task<C>(tasks, levs, active, sel);
+}
+
+__global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, const BLev *levs, const int *active,
+                                                        int sel) {
+    const BTile t = tiles[blockIdx.x];
+    const BLev *F = levs + t.item, *Cl = levs + t.item + 1;
+    if (!active[F->prob]) return;
+    const int i = t.tx * 64 + (threadIdx.x & 63), J = t.ty * 4 + (threadIdx.x >> 6);
+    const Grid gc = Cl->g;
+    if (i >= gc.nx || J >= gc.ny) return;
+    restrict_cell(Cl->f[0], Cl->e[0], (const double *)F->e[sel], (const double *)F->f[sel], F->K, F->g, gc, F->geo,
+                  i, J);
+}
+
+__global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const BLev *levs, const int *active,
+                                                       int sel) {
+    const BTile t = tiles[blockIdx.x];
+    const BLev *F = levs + t.item, *Cl = levs + t.item + 1;
+    if (!active[F->prob]) return;
+    const int i = t.tx * 64 + (threadIdx.x & 63), j = t.ty * 4 + (threadIdx.x >> 6);
+    const Grid gf = F->g, gc = Cl->g;
+    if (i >= gf.nx || j >= gf.ny) return;
+    prolong_y_cell(F->e[sel], (const double *)Cl->e[0], gf, gc, F->slo, F->shi, i, j);
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
@@ -642,8 +711,9 @@ void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, c
     k_ring_coef<<<(n + 255) / 256, 256, 0, s>>>(K, g, geo, bc);
 }
 
-size_t xline_lds_bytes(int nx, size_t elem) {
-    const int C = xline_chunk(nx);
+size_t xline_lds_bytes(int nx, size_t elem) { return xline_lds_bytes_for(nx, xline_chunk(nx), elem); }
+
+size_t xline_lds_bytes_for(int nx, int C, size_t elem) {
     if (C == 0) return (size_t)-1;
     const size_t nt = (size_t)(nx / C);
     // chunk staging (4 arrays), or one chunk array plus the interface rows
@@ -652,6 +722,8 @@ size_t xline_lds_bytes(int nx, size_t elem) {
 }
 
 bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
+
+int xline_chunk_length(int nx) { return xline_chunk(nx); }
 
 template <typename T, typename G>
 static void residual_restrict_y(hipStream_t s, T *rc, T *ec, const T *e, const T *f, const T *K, const Grid &gf,
@@ -697,6 +769,35 @@ static void xline(hipStream_t s, T *e, const T *f, const T *K, const Grid &g, co
 void launch_xline(hipStream_t s, double *e, const double *f, const double *K, const Grid &g, const Geo &geo,
                   int row0, int rowstep, int nrows, int extra0, int extra1) {
     xline(s, e, f, K, g, geo, row0, rowstep, nrows, extra0, extra1);
+}
+
+void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
+                        int C, size_t lds) {
+    if (ntasks <= 0) return;
+    switch (C) {
+        case 2: k_xline_batch<2><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
+        case 4: k_xline_batch<4><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
+        case 8: k_xline_batch<8><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
+        case 16: k_xline_batch_long<16><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
+        default:
+            g_last_error = "k_xline_batch: chunk length must be 2, 4, 8 or 16";
+            throw Failure(MGP_ERR_ARG);
+    }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        g_last_error = std::string("k_xline_batch launch: ") + hipGetErrorString(err);
+        throw Failure(MGP_ERR_HIP);
+    }
+}
+
+void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
+                           int sel) {
+    if (ntiles > 0) k_restrict_batch<<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
+}
+
+void launch_prolong_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
+                          int sel) {
+    if (ntiles > 0) k_prolong_batch<<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
 }
 
 void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
