@@ -53,6 +53,97 @@ def config2_problem(nx=NX, ny=NY, seed=SEED):
     return geo, bc, np.ascontiguousarray(rhs)
 
 
+def config4_mix(n=1024, seed=SEED):
+    """The batched problem mix: n problems, nx, ny = 16 round(exp(U[ln 32, ln 1024]) / 16)
+    each, mapping x = xi + A sin(2 pi xi) sin(2 pi eta), y = (exp(beta eta) - 1)/(exp(beta) - 1)
+    on [0,1]^2 with beta ~ U[0, 6], A ~ U[0, 0.15]; BCs all-Dirichlet or Dirichlet xi /
+    Neumann eta with equal odds; rhs N(0,1) per cell."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(32.0), math.log(1024.0)
+    probs = []
+    for _ in range(n):
+        nx = int(16 * round(math.exp(rng.uniform(lo, hi)) / 16))
+        ny = int(16 * round(math.exp(rng.uniform(lo, hi)) / 16))
+        beta, amp = float(rng.uniform(0.0, 6.0)), float(rng.uniform(0.0, 0.15))
+        types = (1, 1, 1, 1) if rng.uniform() < 0.5 else (1, 1, 0, 0)
+        probs.append((nx, ny, amp, beta, types))
+    rhs = [rng.standard_normal((ny, nx)) for nx, ny, _, _, _ in probs]
+    return probs, rhs
+
+
+def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2):
+    """Solves/s of the batched solver on the problem mix, against the plain FP64
+    single-problem solver (KRYLOV_MG, one problem at a time) timed on a sample of
+    the same mix; per-solve latencies of the plain path give p50/p99."""
+    from types import SimpleNamespace
+    from mgpoisson2d_amd import geometry as G
+    probs, rhs = config4_mix(n)
+    geos = [SimpleNamespace(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny) for nx, ny, _, _, _ in probs]
+    maps = [capi.Mapping(capi.MAP_WARPED, amp, beta, 0.0, 0.0) for _, _, amp, beta, _ in probs]
+    bcs = [G.BoundaryConditions.uniform(t) for *_, t in probs]
+    t0 = time.perf_counter()
+    b = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb)
+    for k in range(n):
+        b.set_rhs(k, rhs[k])
+    capi.device_synchronize()
+    setup_s = time.perf_counter() - t0
+    iters, outc = b.solve(tol, 400)  # warm-up
+    if not np.all(outc == capi.CONVERGED):
+        raise SystemExit(f"batched warm-up: {int(np.sum(outc != capi.CONVERGED))} problems did not converge")
+    times = []
+    for _ in range(steps):
+        capi.device_synchronize()
+        t1 = time.perf_counter()
+        iters, outc = b.solve(tol, 400)
+        capi.device_synchronize()
+        times.append(time.perf_counter() - t1)
+    if not np.all(outc == capi.CONVERGED):
+        raise SystemExit("batched solve did not converge")
+    cycles = b.last_cycles()
+    print(f"# batched: {n} problems, setup {setup_s:.2f} s, solve {min(times):.3f} s, "
+          f"{2 * float(np.mean(iters)):.1f} V-cycles mean, {cycles} max", file=sys.stderr, flush=True)
+    # plain FP64 path on a sample, one problem at a time (hierarchy set-up not timed)
+    lat = []
+    diffs = []
+    for k in range(min(sample, n)):
+        s1 = capi.Solver(geos[k], bcs[k], method=capi.METHOD_KRYLOV_MG, mapping=maps[k], smooth_down=sd,
+                         smooth_up=su, smooth_bottom=sb)
+        s1.set_rhs(rhs[k])
+        s1.solve_fast(tol, 400, zerophi=True)
+        capi.device_synchronize()
+        t1 = time.perf_counter()
+        _, oc = s1.solve_fast(tol, 400, zerophi=True)
+        capi.device_synchronize()
+        lat.append(time.perf_counter() - t1)
+        if oc != capi.CONVERGED:
+            raise SystemExit(f"plain FP64 solve of problem {k} did not converge")
+        ref = s1.get_phi()[1:-1, 1:-1]
+        diffs.append(float(np.max(np.abs(b.get_phi(k) - ref)) / max(np.max(np.abs(ref)), 1e-300)))
+        s1.close()
+    b.close()
+    lat = np.array(lat)
+    batched = n / min(times)
+    plain = len(lat) / float(np.sum(lat))
+    cells = int(sum(nx * ny for nx, ny, *_ in probs))
+    return {
+        "workload": f"{n} mapped-grid problems, nx, ny log-uniform in [32, 1024] (multiples of 16), "
+                    f"beta U[0,6], A U[0,0.15], BCs all-Dirichlet or Dirichlet xi / Neumann eta, rhs N(0,1), "
+                    f"FP64 to {tol:g} relative residual each",
+        "cells": cells,
+        "batched_solves_per_s": batched,
+        "batched_batch_s": min(times),
+        "batched_setup_s": setup_s,
+        "plain_fp64_solves_per_s": plain,
+        "plain_sample": len(lat),
+        "batched_over_plain": batched / plain,
+        "plain_latency_p50_ms": 1e3 * float(np.percentile(lat, 50)),
+        "plain_latency_p99_ms": 1e3 * float(np.percentile(lat, 99)),
+        "v_cycles_mean": 2 * float(np.mean(iters)),
+        "v_cycles_max": cycles,
+        "max_rel_diff_batched_vs_plain": max(diffs),
+    }
+
+
 def cpu_baseline(geo, bc, rhs, gpu_cycles, budget_s=20.0):
     """Time the reference Fortran (oracle/_ref/fast, -O3 -march=native, one
     core) on the same 2048^2 problem: DeferredMGPoisson2D::vcycle, one V(2,2)
@@ -99,6 +190,9 @@ def main():
     ap.add_argument("--method", choices=("fp64", "mixed"), default="fp64",
                     help="solver of the headline value: FP64 cycles (default) or FP32 cycles in FP64 BiCGStab")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched problem-mix comparison")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched problem-mix comparison")
+    ap.add_argument("--batch-n", type=int, default=1024, help="problems in the batched mix")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--kernels-only", action="store_true",
@@ -112,6 +206,9 @@ def main():
         sys.exit(2)
 
     from mgpoisson2d_amd import capi
+    if args.batch_only:
+        print(json.dumps(batched_bench(capi, args.batch_n)), flush=True)
+        return
     geo, bc, rhs = config2_problem(args.n, args.n)
     main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
     other_method = capi.METHOD_KRYLOV_MG if args.method == "mixed" else capi.METHOD_KRYLOV_MG32
@@ -199,6 +296,10 @@ def main():
         with open(args.traffic) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(capi, args.batch_n)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(geo, bc, rhs, int(round(cycles_per_solve)))
@@ -244,6 +345,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "mixed": compare,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     solver.close()

@@ -27,6 +27,7 @@ struct BLev {
     const double *K;    // ring stencil table
     double slo, shi;    // ghost signs of the y sides in the prolongation
     int prob, nt;       // problem index; x-line chunks (nx / C)
+    int sweeps;         // coarsest level of its problem: zebra sweeps there (else 0)
 };
 
 // Krylov vectors of one problem (finest level, level layout).
@@ -58,8 +59,9 @@ struct BTile {
 
 // Level kernels (mgp_xline.hip).  `active` is indexed by problem; sel picks
 // the e/f pair of the finest level.
+// k >= 0: sweep k of a level's coarsest-level sweeps (levels with fewer return)
 void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
-                        int C, size_t lds);
+                        int C, size_t lds, int k);
 // coarse (lev+1) r = 1x2 restriction of f - L e on lev, coarse e = 0; tiles over the coarse cells
 void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
                            int sel);
@@ -94,7 +96,7 @@ class BatchKMG {
   private:
     void body();
     void precondition(int sel);
-    void zebra(int d, int set, int sel);
+    void zebra(int d, int set, int sel, int k);
     void reduce(int a, int b, int slot, const int *m);
     void restart(const int *m);
     void true_residual(const int *m);
@@ -125,13 +127,25 @@ class BatchKMG {
         int d, set, colour;
         Table t;
     };
-    Table upload(const void *host, size_t bytes, int n);
-    Table t0;                    // finest-level tiles, item = problem
-    Table tred;                  // reduction row blocks, item = problem (tx = block)
-    std::vector<XlTable> xl;     // x-line task tables per (level, set, colour, chunk class)
-    std::vector<Table> tr, tp;   // per level: restriction (coarse tiles) / prolongation (fine tiles), item = level
-    std::vector<void *> owned;   // device tables to free
-    hipGraphExec_t graph = nullptr;
+    // the launch tables of a set of problems, with the captured iteration
+    struct TableSet {
+        Table t0;                    // finest-level tiles, item = problem
+        Table tred;                  // reduction row blocks, item = problem (tx = block)
+        std::vector<XlTable> xl;     // x-line task tables per (level, set, colour, chunk class)
+        std::vector<Table> tr, tp;   // per level: restriction (coarse tiles) / prolongation (fine tiles)
+        std::vector<int> bottom;     // per level: most coarsest-level sweeps of a problem ending there
+        std::vector<void *> owned;   // device allocations of the tables
+        hipGraphExec_t graph = nullptr;
+        int nprob = 0, depth = 0;
+    };
+    Table upload(TableSet &ts, const void *host, size_t bytes, int n);
+    void build_tables(TableSet &ts, const std::vector<int> &which);
+    void release(TableSet &ts);
+    TableSet full;               // every problem (built once)
+    TableSet part;               // the problems still active, once they are at most half of `full`
+    TableSet *T = &full;         // the set the launches use
+    std::vector<int> Cp;         // chunk length of each problem's line solves
+    bool use_graph = true;
 };
 
 }  // namespace mgp

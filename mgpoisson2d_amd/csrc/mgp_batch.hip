@@ -153,12 +153,12 @@ void tiles_of(std::vector<BTile> &out, int item, int nx, int ny) {
 
 }  // namespace
 
-BatchKMG::Table BatchKMG::upload(const void *host, size_t bytes, int n) {
+BatchKMG::Table BatchKMG::upload(TableSet &ts, const void *host, size_t bytes, int n) {
     Table t;
     t.n = n;
     if (n == 0) return t;
     t.p = dev_alloc_zero(bytes);
-    owned.push_back(t.p);
+    ts.owned.push_back(t.p);
     hip_check(hipMemcpyAsync(t.p, host, bytes, hipMemcpyHostToDevice, s), "table upload");
     return t;
 }
@@ -292,44 +292,60 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             l.shi = B.bc.type[YHI] == BC_DIRI ? -1.0 : 1.0;
             l.prob = p;
             l.nt = 0;  // set with the chunk class below
+            l.sweeps = d == nlev[p] - 1 ? coarsest_sweeps(sb, ny) : 0;
             L.push_back(l);
             ny /= 2;
             dy *= 2.0;
         }
     }
 
-    // chunk class of each problem: the largest chunk length any problem of
-    // the batch needs, where it also fits this problem's rows (remainder
-    // below the chunk count), else the problem's own
-    int Cb = 2;
-    for (const BProb &B : P) Cb = std::max(Cb, xline_chunk_length(B.g.nx));
-    std::vector<int> Cp(n);
+    // each problem's line solves use its own chunk length, whatever the batch
+    // (the partition of a row into chunks decides the rounding of its solve)
+    Cp.resize(n);
     for (int p = 0; p < n; ++p) {
-        const int nx = P[p].g.nx;
-        Cp[p] = (nx % Cb < nx / Cb) ? Cb : xline_chunk_length(nx);
-        for (int d = 0; d < nlev[p]; ++d) L[P[p].lev0 + d].nt = nx / Cp[p];
+        Cp[p] = xline_chunk_length(P[p].g.nx);
+        for (int d = 0; d < nlev[p]; ++d) L[P[p].lev0 + d].nt = P[p].g.nx / Cp[p];
     }
 
-    // launch tables
+    dP = static_cast<BProb *>(dev_alloc_zero(n * sizeof(BProb)));
+    dL = static_cast<BLev *>(dev_alloc_zero(L.size() * sizeof(BLev)));
+    hip_check(hipMemcpyAsync(dP, P.data(), n * sizeof(BProb), hipMemcpyHostToDevice, s), "problem table");
+    hip_check(hipMemcpyAsync(dL, L.data(), L.size() * sizeof(BLev), hipMemcpyHostToDevice, s), "level table");
+    active = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
+    mask = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
+    hip_check(hipHostMalloc((void **)&host_sc, (size_t)n * SC_COUNT * sizeof(double)), "hipHostMalloc");
+    use_graph = o.use_graph != 0;
+    std::vector<int> all(n);
+    for (int p = 0; p < n; ++p) all[p] = p;
+    build_tables(full, all);
+}
+
+// Launch tables of the problems `which` (and the captured iteration over them).
+void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
+    ts.nprob = (int)which.size();
+    ts.depth = 0;
+    for (int p : which) ts.depth = std::max(ts.depth, nlev[p]);
     std::vector<BTile> t0h, tredh;
-    for (int p = 0; p < n; ++p) {
+    for (int p : which) {
         tiles_of(t0h, p, P[p].g.nx, P[p].g.ny);
         for (int b = 0; b < P[p].nb; ++b) tredh.push_back(BTile{p, b, 0, 0});
     }
-    t0 = upload(t0h.data(), t0h.size() * sizeof(BTile), (int)t0h.size());
-    tred = upload(tredh.data(), tredh.size() * sizeof(BTile), (int)tredh.size());
-    tr.resize(depth);
-    tp.resize(depth);
-    for (int d = 0; d < depth; ++d) {
+    ts.t0 = upload(ts, t0h.data(), t0h.size() * sizeof(BTile), (int)t0h.size());
+    ts.tred = upload(ts, tredh.data(), tredh.size() * sizeof(BTile), (int)tredh.size());
+    ts.tr.assign(ts.depth, Table());
+    ts.tp.assign(ts.depth, Table());
+    ts.bottom.assign(ts.depth, 0);
+    for (int d = 0; d < ts.depth; ++d) {
         std::map<std::tuple<int, int, int>, std::vector<XlTask>> tabs;  // (set, colour, C)
         std::map<std::tuple<int, int, int>, size_t> lds;
         std::vector<BTile> trh, tph;
-        for (int p = 0; p < n; ++p) {
+        for (int p : which) {
             if (d >= nlev[p]) continue;
             const int lev = P[p].lev0 + d;
             const int ny = L[lev].g.ny, nx = L[lev].g.nx;
             const int set = d == nlev[p] - 1 ? SET_BOTTOM : SET_MID;
             const int C = Cp[p];
+            if (set == SET_BOTTOM) ts.bottom[d] = std::max(ts.bottom[d], L[lev].sweeps);
             auto add = [&](int colour, XlTask t) {
                 const auto key = std::make_tuple(set, colour, C);
                 tabs[key].push_back(t);
@@ -361,27 +377,21 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             x.d = d;
             x.set = std::get<0>(kv.first);
             x.colour = std::get<1>(kv.first);
-            x.t = upload(kv.second.data(), kv.second.size() * sizeof(XlTask), (int)kv.second.size());
+            x.t = upload(ts, kv.second.data(), kv.second.size() * sizeof(XlTask), (int)kv.second.size());
             x.t.C = std::get<2>(kv.first);
             x.t.lds = lds[kv.first];
-            xl.push_back(x);
+            ts.xl.push_back(x);
         }
-        tr[d] = upload(trh.data(), trh.size() * sizeof(BTile), (int)trh.size());
-        tp[d] = upload(tph.data(), tph.size() * sizeof(BTile), (int)tph.size());
+        ts.tr[d] = upload(ts, trh.data(), trh.size() * sizeof(BTile), (int)trh.size());
+        ts.tp[d] = upload(ts, tph.data(), tph.size() * sizeof(BTile), (int)tph.size());
     }
-    std::sort(xl.begin(), xl.end(), [](const XlTable &a, const XlTable &b) {
+    std::sort(ts.xl.begin(), ts.xl.end(), [](const XlTable &a, const XlTable &b) {
         return std::make_tuple(a.d, a.set, a.colour, a.t.C) < std::make_tuple(b.d, b.set, b.colour, b.t.C);
     });
-
-    dP = static_cast<BProb *>(dev_alloc_zero(n * sizeof(BProb)));
-    dL = static_cast<BLev *>(dev_alloc_zero(L.size() * sizeof(BLev)));
-    hip_check(hipMemcpyAsync(dP, P.data(), n * sizeof(BProb), hipMemcpyHostToDevice, s), "problem table");
-    hip_check(hipMemcpyAsync(dL, L.data(), L.size() * sizeof(BLev), hipMemcpyHostToDevice, s), "level table");
-    active = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
-    mask = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
-    hip_check(hipHostMalloc((void **)&host_sc, (size_t)n * SC_COUNT * sizeof(double)), "hipHostMalloc");
-    hip_check(hipStreamSynchronize(s), "batch set-up");
-    if (o.use_graph) {
+    hip_check(hipStreamSynchronize(s), "batch tables");
+    if (use_graph) {
+        TableSet *const saved = T;
+        T = &ts;
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
         try {
@@ -389,18 +399,26 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
         } catch (...) {
             (void)hipStreamEndCapture(s, &g);
             if (g) (void)hipGraphDestroy(g);
+            T = saved;
             throw;
         }
+        T = saved;
         hip_check(hipStreamEndCapture(s, &g), "capture end");
-        hip_check(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0), "graph instantiate");
+        hip_check(hipGraphInstantiate(&ts.graph, g, nullptr, nullptr, 0), "graph instantiate");
         (void)hipGraphDestroy(g);
     }
 }
 
-BatchKMG::~BatchKMG() {
-    if (graph) (void)hipGraphExecDestroy(graph);
+void BatchKMG::release(TableSet &ts) {
     (void)hipStreamSynchronize(s);
-    for (void *p : owned) dev_free(p);
+    if (ts.graph) (void)hipGraphExecDestroy(ts.graph);
+    for (void *p : ts.owned) dev_free(p);
+    ts = TableSet();
+}
+
+BatchKMG::~BatchKMG() {
+    release(part);
+    release(full);
     if (dP) dev_free(dP);
     if (dL) dev_free(dL);
     if (active) dev_free(active);
@@ -410,11 +428,11 @@ BatchKMG::~BatchKMG() {
     if (host_sc) (void)hipHostFree(host_sc);
 }
 
-void BatchKMG::zebra(int d, int set, int sel) {
+void BatchKMG::zebra(int d, int set, int sel, int k) {
     for (int colour = 0; colour < 3; ++colour)
-        for (const XlTable &x : xl)
+        for (const XlTable &x : T->xl)
             if (x.d == d && x.set == set && x.colour == colour)
-                launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.lds);
+                launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.lds, k);
 }
 
 // e = M f for the pair `sel` of every active problem (phat from p: 0, shat
@@ -422,20 +440,22 @@ void BatchKMG::zebra(int d, int set, int sel) {
 // Problems whose coarsest level is d make their bottom sweeps there while
 // the others go on down.
 void BatchKMG::precondition(int sel) {
-    k_b_copy<<<t0.n, 256, 0, s>>>((const BTile *)t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
+    k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
+    const int depth = T->depth;
     for (int d = 0; d < depth; ++d) {
-        for (int k = 0; k < sd; ++k) zebra(d, SET_MID, sel);
-        for (int k = 0; k < sb; ++k) zebra(d, SET_BOTTOM, sel);
-        if (d + 1 < depth) launch_restrict_batch(s, (const BTile *)tr[d].p, tr[d].n, dL, active, sel);
+        for (int k = 0; k < sd; ++k) zebra(d, SET_MID, sel, -1);
+        // each problem ending at level d makes its own number of sweeps there
+        for (int k = 0; k < T->bottom[d]; ++k) zebra(d, SET_BOTTOM, sel, k);
+        if (d + 1 < depth) launch_restrict_batch(s, (const BTile *)T->tr[d].p, T->tr[d].n, dL, active, sel);
     }
     for (int d = depth - 2; d >= 0; --d) {
-        launch_prolong_batch(s, (const BTile *)tp[d].p, tp[d].n, dL, active, sel);
-        for (int k = 0; k < su; ++k) zebra(d, SET_MID, sel);
+        launch_prolong_batch(s, (const BTile *)T->tp[d].p, T->tp[d].n, dL, active, sel);
+        for (int k = 0; k < su; ++k) zebra(d, SET_MID, sel, -1);
     }
 }
 
 void BatchKMG::reduce(int a, int b, int slot, const int *m) {
-    k_b_reduce_partial<<<tred.n, 256, 0, s>>>((const BTile *)tred.p, dP, m, a, b);
+    k_b_reduce_partial<<<T->tred.n, 256, 0, s>>>((const BTile *)T->tred.p, dP, m, a, b);
     k_b_reduce_final<<<count(), 256, 0, s>>>(dP, m, slot);
 }
 
@@ -443,40 +463,42 @@ void BatchKMG::reduce(int a, int b, int slot, const int *m) {
 // KrylovMG::solve); ends with ||r||^2 vol in each problem's SC_RN.
 void BatchKMG::body() {
     const int n = count();
-    const BTile *T = (const BTile *)t0.p;
+    const BTile *tl = (const BTile *)T->t0.p;
+    const int nt = T->t0.n;
     const int sb_ = (n + 63) / 64;
     reduce(BV_RHAT, BV_R, SC_RHO_NEW, active);
     k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 0);
-    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 0);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 0);
     precondition(0);
-    k_b_apply<<<t0.n, 256, 0, s>>>(T, dP, active, BV_V, BV_PHAT, -1);
+    k_b_apply<<<nt, 256, 0, s>>>(tl, dP, active, BV_V, BV_PHAT, -1);
     reduce(BV_RHAT, BV_V, SC_RV, active);
     k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 1);
-    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 1);
-    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 2);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 1);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 2);
     precondition(1);
-    k_b_apply<<<t0.n, 256, 0, s>>>(T, dP, active, BV_T, BV_SHAT, -1);
+    k_b_apply<<<nt, 256, 0, s>>>(tl, dP, active, BV_T, BV_SHAT, -1);
     reduce(BV_T, BV_S, SC_TS, active);
     reduce(BV_T, BV_T, SC_TT, active);
     k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 2);
-    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 3);
-    k_b_update<<<t0.n, 256, 0, s>>>(T, dP, active, 4);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 3);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 4);
     reduce(BV_R, BV_R, SC_RN, active);
 }
 
 // r = rhs - A phi and ||r||^2 vol into SC_RN, for the masked problems.
 void BatchKMG::true_residual(const int *m) {
-    k_b_apply<<<t0.n, 256, 0, s>>>((const BTile *)t0.p, dP, m, BV_R, BV_PHI, BV_RHS);
+    k_b_apply<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, m, BV_R, BV_PHI, BV_RHS);
     reduce(BV_R, BV_R, SC_RN, m);
 }
 
 // New BiCGStab run from the current r for the masked problems.
 void BatchKMG::restart(const int *m) {
     const int n = count();
-    const BTile *T = (const BTile *)t0.p;
-    k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, m, BV_RHAT, BV_R);
-    k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, m, BV_P, -1);
-    k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, m, BV_V, -1);
+    const BTile *tl = (const BTile *)T->t0.p;
+    const int nt = T->t0.n;
+    k_b_copy<<<nt, 256, 0, s>>>(tl, dP, m, BV_RHAT, BV_R);
+    k_b_copy<<<nt, 256, 0, s>>>(tl, dP, m, BV_P, -1);
+    k_b_copy<<<nt, 256, 0, s>>>(tl, dP, m, BV_V, -1);
     k_b_scalars<<<(n + 63) / 64, 64, 0, s>>>(dP, m, n, 3);
 }
 
@@ -492,11 +514,14 @@ void BatchKMG::fetch_scalars() {
 
 int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outcome) {
     const int n = count();
-    const BTile *T = (const BTile *)t0.p;
+    if (T != &full) {
+        release(part);
+        T = &full;
+    }
     std::vector<int> act(n, 1), msk(n, 0), it(n, 0), oc(n, MGP_MAXITERS);
     std::vector<double> r0(n, 0.0), base(n, 1.0);
     upload_flags(active, act);
-    if (zerophi) k_b_copy<<<t0.n, 256, 0, s>>>(T, dP, active, BV_PHI, -1);
+    if (zerophi) k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, BV_PHI, -1);
     true_residual(active);
     fetch_scalars();
     for (int p = 0; p < n; ++p) {
@@ -513,7 +538,7 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
     last_cycles = 0;
     for (int k = 1; k <= maxiters; ++k) {
         if (std::find(act.begin(), act.end(), 1) == act.end()) break;
-        if (graph) hip_check(hipGraphLaunch(graph, s), "graph launch");
+        if (T->graph) hip_check(hipGraphLaunch(T->graph, s), "graph launch");
         else body();
         last_cycles += 2;
         fetch_scalars();
@@ -547,6 +572,17 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
         upload_flags(mask, msk);
         restart(mask);
         upload_flags(active, act);
+        // once at most half of the problems in the tables are still active,
+        // launch over those alone (the finished ones' workgroups would only
+        // return, but every level launch would still cost its latency)
+        std::vector<int> live;
+        for (int p = 0; p < n; ++p)
+            if (act[p]) live.push_back(p);
+        if (!live.empty() && 2 * (int)live.size() <= T->nprob) {
+            release(part);
+            build_tables(part, live);
+            T = &part;
+        }
     }
     hip_check(hipStreamSynchronize(s), "sync");
     for (int p = 0; p < n; ++p) {

@@ -91,6 +91,12 @@ void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const dou
 void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
                                 const Grid &gf, const Grid &gc, const GeoF &geo);
 bool xline_supported(int nx);
+// Sweeps on the coarsest level of the semicoarsening hierarchy: at least
+// `sb`, and half the level's rows.  The hierarchy stops at an odd row count,
+// and an x-line zebra sweep reduces errors that are smooth in y like one
+// Gauss-Seidel sweep in y, so a coarsest level of many rows needs more of
+// them (ny = 16 k with k odd leaves k rows).
+inline int coarsest_sweeps(int sb, int ny) { return sb > ny / 2 ? sb : ny / 2; }
 // chunk length C of the line solve of an nx-cell row (2, 4, 8 or 16; 0: too long)
 int xline_chunk_length(int nx);
 // dynamic LDS of one x-line workgroup for elements of `elem` bytes
@@ -105,10 +111,12 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
                   int row0, int rowstep, int nrows, int extra0, int extra1);
 void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
                   int rowstep, int nrows, int extra0, int extra1);
+// ef += 1x2 (y) prolongation of ec; Jyy (the fine level's, or null): the
+// weights of prolong_y_cell
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi);
+                      double shi, const double *Jyy);
 void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi);
+                      double shi, const float *Jyy);
 // Precision conversions over a level array.  ghosted: also the ghost ring.
 void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted);
 void launch_convert(hipStream_t s, float *out, const double *in, const Grid &g, int ghosted);

@@ -636,10 +636,10 @@ __global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const
 // it the problem); workgroups of inactive problems return at once.
 // ---------------------------------------------------------------------------
 template <int C>
-__device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs, const int *active, int sel) {
+__device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
     const XlTask t = tasks[launch_slot()];
     const BLev *L = levs + t.lev;
-    if (!active[L->prob]) return;
+    if (!active[L->prob] || (k >= 0 && k >= L->sweeps)) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_bytes[];
     double *lds = reinterpret_cast<double *>(lds_bytes);
     const Grid g = L->g;
@@ -660,14 +660,14 @@ __device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs
 
 template <int C>
 __global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_xline_batch(const XlTask *tasks, const BLev *levs, const int *active, int sel) {
-    xline_task<C>(tasks, levs, active, sel);
+k_xline_batch(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
+    xline_task<C>(tasks, levs, active, sel, k);
 }
 
 template <int C>
 __global__ void __launch_bounds__(XL_THREADS) k_xline_batch_long(const XlTask *tasks, const BLev *levs,
-                                                                 const int *active, int sel) {
-    xline_task<C>(tasks, levs, active, sel);
+                                                                 const int *active, int sel, int k) {
+    xline_task<C>(tasks, levs, active, sel, k);
 }
 
 __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, const BLev *levs, const int *active,
@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const
     const int i = t.tx * 64 + (threadIdx.x & 63), j = t.ty * 4 + (threadIdx.x >> 6);
     const Grid gf = F->g, gc = Cl->g;
     if (i >= gf.nx || j >= gf.ny) return;
-    prolong_y_cell(F->e[sel], (const double *)Cl->e[0], gf, gc, F->slo, F->shi, i, j);
+    prolong_y_cell(F->e[sel], (const double *)Cl->e[0], gf, gc, F->slo, F->shi, i, j, F->geo.Jyy);
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
@@ -772,13 +772,13 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
 }
 
 void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
-                        int C, size_t lds) {
+                        int C, size_t lds, int k) {
     if (ntasks <= 0) return;
     switch (C) {
-        case 2: k_xline_batch<2><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
-        case 4: k_xline_batch<4><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
-        case 8: k_xline_batch<8><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
-        case 16: k_xline_batch_long<16><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel); break;
+        case 2: k_xline_batch<2><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 4: k_xline_batch<4><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 8: k_xline_batch<8><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 16: k_xline_batch_long<16><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
         default:
             g_last_error = "k_xline_batch: chunk length must be 2, 4, 8 or 16";
             throw Failure(MGP_ERR_ARG);

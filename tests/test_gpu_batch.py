@@ -6,7 +6,8 @@
   a sparse direct solve of that operator to 1e-10 relative.
 * Each result is the same alone and in the mix: a problem solved in a batch
   of one gives the bits (and the iteration count) it gives inside the batch.
-* Graph replay and eager launches give the same bits.
+* Graph replay and eager launches give the same bits, and so do the launch
+  tables rebuilt over the problems still running once half have finished.
 * Metrics generated on the GPU (mgp_batch_create_mapped) give the solution
   of the single-problem solver built the same way, to the solve tolerance.
 """
@@ -32,6 +33,7 @@ CASES = [
     (32, 32, 0.00, 0.0, (1, 1, 0, 0)),
     (200, 16, 0.10, 3.0, (1, 1, 0, 0)),
     (40, 24, 0.15, 6.0, (1, 1, 1, 1)),
+    (576, 32, 0.05, 1.0, (1, 1, 0, 0)),  # a longer chunk class (C = 4) in the same batch
 ]
 
 
@@ -86,7 +88,7 @@ def test_batch_matches_direct_solve():
 def test_batch_alone_equals_in_mix():
     probs = problems(CASES)
     phis, iters, outc, _ = run_batch(probs)
-    for k in (0, 2, 5):
+    for k in (0, 2, 5, 7):
         alone, it1, oc1, _ = run_batch([probs[k]])
         assert (int(it1[0]), int(oc1[0])) == (int(iters[k]), int(outc[k]))
         np.testing.assert_array_equal(alone[0], phis[k])
