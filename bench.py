@@ -144,6 +144,107 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2)
     }
 
 
+def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_requests=200, tol=1e-10,
+                  budget=200e9, seed=SEED + 5):
+    """Continuous batching: n_small resident geometries from the batched mix plus one
+    big_n^2 (beta = 6) geometry that takes big_share of the requests; each request is the
+    slot's base rhs plus a 5 % perturbation, warm-started from the slot's last solution.
+    Capacity is measured closed-loop (one request in flight per slot); then requests
+    arrive as a Poisson process at `load` x capacity and their latency (arrival to result)
+    is recorded."""
+    from types import SimpleNamespace
+    from mgpoisson2d_amd import geometry as G
+    probs, base = config4_mix(n_small, seed)
+    probs.append((big_n, big_n, 0.1, 6.0, (1, 1, 0, 0)))
+    rng = np.random.default_rng(seed + 1)
+    base.append(rng.standard_normal((big_n, big_n)))
+    pert = [[rng.standard_normal(b.shape) for _ in range(3)] for b in base]
+    geos = [SimpleNamespace(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny) for nx, ny, *_ in probs]
+    maps = [capi.Mapping(capi.MAP_WARPED, amp, beta, 0.0, 0.0) for _, _, amp, beta, _ in probs]
+    bcs = [G.BoundaryConditions.uniform(t) for *_, t in probs]
+    v = capi.Server(geos, bcs, maps, hbm_budget=budget)
+    big = len(probs) - 1
+
+    def draw():
+        slot = big if rng.uniform() < big_share else int(rng.integers(n_small))
+        return slot, base[slot] + 0.05 * pert[slot][int(rng.integers(3))]
+
+    # cold solves of every slot (hierarchies and phi become resident)
+    pend = {v.submit(k, base[k], tol, 400, False): k for k in range(len(probs))}
+    while pend:
+        v.step()
+        for t in list(pend):
+            if v.result(t, pend[t]) is not None:
+                del pend[t]
+    # capacity, closed loop: as many requests in flight as slots
+    t0 = time.perf_counter()
+    pend = {}
+    for _ in range(len(probs)):
+        slot, r = draw()
+        pend[v.submit(slot, r, tol, 400, True)] = slot
+    done = 0
+    while done < n_requests // 2:
+        v.step()
+        for t in list(pend):
+            if v.result(t, pend[t]) is not None:
+                del pend[t]
+                done += 1
+                slot, r = draw()
+                pend[v.submit(slot, r, tol, 400, True)] = slot
+    while pend:
+        v.step()
+        for t in list(pend):
+            if v.result(t, pend[t]) is not None:
+                del pend[t]
+                done += 1
+    capacity = done / (time.perf_counter() - t0)
+    # open loop at load x capacity
+    rate = load * capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = [draw() for _ in range(n_requests)]
+    lat, big_lat, pend, nxt = [], [], {}, 0
+    t0 = time.perf_counter()
+    while nxt < n_requests or pend:
+        now = time.perf_counter() - t0
+        while nxt < n_requests and arrivals[nxt] <= now:
+            slot, r = reqs[nxt]
+            pend[v.submit(slot, r, tol, 400, True)] = (slot, arrivals[nxt])
+            nxt += 1
+        running, queued = v.step()
+        now = time.perf_counter() - t0
+        for t in list(pend):
+            slot, arr = pend[t]
+            res = v.result(t, slot)
+            if res is not None:
+                if res[2] != capi.CONVERGED:
+                    raise SystemExit(f"served solve on slot {slot} did not converge")
+                (big_lat if slot == big else lat).append(now - arr)
+                del pend[t]
+        if running == 0 and queued == 0 and not pend and nxt < n_requests:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+    span = time.perf_counter() - t0
+    footprint = v.footprint()
+    v.close()
+    all_lat = np.array(lat + big_lat)
+    return {
+        "workload": f"{n_small} resident geometries from the batched mix plus one {big_n}^2 beta=6 geometry "
+                    f"({100 * big_share:.0f} % of requests); rhs = base + 5 % perturbation, warm start from the "
+                    f"slot's last solution, FP64 to {tol:g} relative residual; Poisson arrivals at {load} x the "
+                    f"closed-loop capacity",
+        "capacity_solves_per_s": capacity,
+        "offered_per_s": rate,
+        "served_per_s": n_requests / span,
+        "requests": n_requests,
+        "latency_p50_ms": 1e3 * float(np.percentile(all_lat, 50)),
+        "latency_p99_ms": 1e3 * float(np.percentile(all_lat, 99)),
+        "small_latency_p50_ms": 1e3 * float(np.percentile(lat, 50)) if lat else None,
+        "small_latency_p99_ms": 1e3 * float(np.percentile(lat, 99)) if lat else None,
+        "big_latency_p50_ms": 1e3 * float(np.percentile(big_lat, 50)) if big_lat else None,
+        "hbm_footprint_GB": footprint / 1e9,
+        "hbm_budget_GB": budget / 1e9,
+    }
+
+
 def cpu_baseline(geo, bc, rhs, gpu_cycles, budget_s=20.0):
     """Time the reference Fortran (oracle/_ref/fast, -O3 -march=native, one
     core) on the same 2048^2 problem: DeferredMGPoisson2D::vcycle, one V(2,2)
@@ -193,6 +294,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched problem-mix comparison")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched problem-mix comparison")
     ap.add_argument("--batch-n", type=int, default=1024, help="problems in the batched mix")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--kernels-only", action="store_true",
@@ -208,6 +311,9 @@ def main():
     from mgpoisson2d_amd import capi
     if args.batch_only:
         print(json.dumps(batched_bench(capi, args.batch_n)), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps(serving_bench(capi)), flush=True)
         return
     geo, bc, rhs = config2_problem(args.n, args.n)
     main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
@@ -299,6 +405,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_bench(capi, args.batch_n)
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench(capi)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -346,6 +455,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": compare,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     solver.close()

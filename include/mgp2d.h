@@ -283,11 +283,39 @@ int mgp_batch_count(const mgp_batch *b);
 /* Problem k's rhs (nx x ny, the mgp_solver_set_rhs layout) / phi (valid cells, nx x ny). */
 int mgp_batch_set_rhs(mgp_batch *b, int k, const double *rhs);
 int mgp_batch_get_phi(mgp_batch *b, int k, double *phi);
-/* Solves every problem to ||rhs - L phi|| <= tol ||rhs - L phi0||; iters[k], outcomes[k]
- * (may be NULL) per problem.  A problem's iterates do not depend on the others in the batch. */
+/* Solves every problem to ||rhs - L phi|| <= tol ||rhs|| from phi = 0, or (zerophi == 0) from
+ * the resident phi; iters[k], outcomes[k] (may be NULL) per problem.  A problem's iterates do
+ * not depend on the others in the batch. */
 int mgp_batch_solve(mgp_batch *b, double tol, int maxiters, int zerophi, int *iters, int *outcomes);
 /* V-cycle applications of the longest-running problem in the last solve. */
 int mgp_batch_last_cycles(const mgp_batch *b);
+
+/* ---- 5. Serving loop (no counterpart in the reference): resident geometries
+ *         ("slots") with their hierarchies and last solutions in HBM, requests
+ *         queued per slot, continuous batching: every step admits queued
+ *         requests to idle slots and advances all running solves by one
+ *         iteration in the batched launches. ------------------------------ */
+
+typedef struct mgp_server mgp_server;
+
+/* n slots with metrics generated on the GPU (as mgp_batch_create_mapped).  hbm_budget > 0:
+ * MGP_ERR_NOMEM if the resident hierarchies would need more device bytes than that. */
+int mgp_server_create(mgp_server **v, int n, const mgp_mapping *maps, const int *nx, const int *ny,
+                      const double *dx, const double *dy, const mgp_bc *bcs, const mgp_solver_opts *opts,
+                      double hbm_budget);
+int mgp_server_destroy(mgp_server *v);
+/* Device bytes the resident slots hold. */
+double mgp_server_footprint(const mgp_server *v);
+/* Queues a solve of slot `slot` with this rhs (nx x ny, copied): ||rhs - L phi|| <= tol ||rhs||
+ * within maxiters iterations, from phi = 0 or (warm != 0) from the slot's last solution. */
+int mgp_server_submit(mgp_server *v, int slot, const double *rhs, double tol, int maxiters, int warm,
+                      long long *ticket);
+/* Admits the oldest queued request of every idle slot and advances every running solve by one
+ * iteration; *running / *queued (may be NULL) receive the solves on the device and waiting. */
+int mgp_server_step(mgp_server *v, int *running, int *queued);
+/* *done = 1 and phi (nx x ny), *iters, *outcome when the request has finished (its result is
+ * then released), else *done = 0.  A result does not depend on the other requests in flight. */
+int mgp_server_result(mgp_server *v, long long ticket, double *phi, int *iters, int *outcome, int *done);
 
 #ifdef __cplusplus
 }

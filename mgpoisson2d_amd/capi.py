@@ -58,6 +58,7 @@ def _load():
     lib.mgp_last_error.restype = ctypes.c_char_p
     lib.mgp_status_string.restype = ctypes.c_char_p
     lib.mgp_solver_stream.restype = ctypes.c_void_p
+    lib.mgp_server_footprint.restype = ctypes.c_double
     return lib
 
 
@@ -77,7 +78,8 @@ EXPORTS = (
     "mgp_relax_jacobi", "mgp_solve_bicgstab", "mgp_metric_fill", "mgp_solver_create_mapped",
     "mgp_inner_prod", "mgp_integrate2d", "mgp_extrapolate_ghosts", "mgp_batch_create",
     "mgp_batch_create_mapped", "mgp_batch_destroy", "mgp_batch_count", "mgp_batch_set_rhs", "mgp_batch_get_phi",
-    "mgp_batch_solve", "mgp_batch_last_cycles",
+    "mgp_batch_solve", "mgp_batch_last_cycles", "mgp_server_create", "mgp_server_destroy", "mgp_server_footprint",
+    "mgp_server_submit", "mgp_server_step", "mgp_server_result",
 )
 
 
@@ -508,3 +510,65 @@ class Batch:
 
     def last_cycles(self):
         return lib.mgp_batch_last_cycles(self._h)
+
+
+class Server:
+    """Serving loop over resident geometries (mgp_server): submit(slot, rhs)
+    queues a solve; step() admits queued requests to idle slots and advances
+    every running solve by one iteration; result(ticket) returns
+    (phi, iterations, outcome) once finished, else None."""
+
+    def __init__(self, geos, bcs, mappings, hbm_budget=0.0, smooth_down=2, smooth_up=2, smooth_bottom=2,
+                 use_graph=True):
+        n = len(geos)
+        self.shapes = [(g.ny, g.nx) for g in geos]
+        opts = SolverOpts(METHOD_KRYLOV_MG, ORDER_MC, 1, smooth_down, smooth_up, smooth_bottom, -1, 1,
+                          int(use_graph))
+        a = _Args()
+        bc_arr = (_Bc * n)()
+        for k, b in enumerate(bcs):
+            bc_arr[k] = a.bc(b)._obj
+        maps = (Mapping * n)(*mappings)
+        nx = (_i * n)(*[g.nx for g in geos])
+        ny = (_i * n)(*[g.ny for g in geos])
+        dx = (_d * n)(*[g.dx for g in geos])
+        dy = (_d * n)(*[g.dy for g in geos])
+        self._h = ctypes.c_void_p()
+        _check(lib.mgp_server_create(ctypes.byref(self._h), _i(n), maps, nx, ny, dx, dy, bc_arr, ctypes.byref(opts),
+                                     _d(hbm_budget)))
+
+    def close(self):
+        if self._h:
+            lib.mgp_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def footprint(self):
+        return lib.mgp_server_footprint(self._h)
+
+    def submit(self, slot, rhs, tol=1e-10, maxiters=400, warm=False):
+        rhs = _f64(rhs)
+        if rhs.shape != self.shapes[slot]:
+            raise ValueError(f"rhs of slot {slot} must have shape {self.shapes[slot]}")
+        t = ctypes.c_longlong(0)
+        _check(lib.mgp_server_submit(self._h, _i(slot), _ptr(rhs), _d(tol), _i(maxiters), _i(int(warm)),
+                                     ctypes.byref(t)))
+        return t.value
+
+    def step(self):
+        """Returns (solves running on the device, requests waiting)."""
+        r, q = _i(0), _i(0)
+        _check(lib.mgp_server_step(self._h, ctypes.byref(r), ctypes.byref(q)))
+        return r.value, q.value
+
+    def result(self, ticket, slot):
+        phi = np.zeros(self.shapes[slot])
+        it, oc, done = _i(0), _i(0), _i(0)
+        _check(lib.mgp_server_result(self._h, ctypes.c_longlong(ticket), _ptr(phi), ctypes.byref(it),
+                                     ctypes.byref(oc), ctypes.byref(done)))
+        return (phi, it.value, oc.value) if done.value else None

@@ -87,11 +87,23 @@ class BatchKMG {
     int count() const { return (int)P.size(); }
     const Grid &grid(int k) const { return P[k].g; }
     double *vec(int k, int which) { return P[k].v[which]; }
-    // Solves every problem to tol (relative to its own initial residual).
-    // iters[k] / outcome[k] per problem; returns the V-cycle applications of
-    // the busiest problem.
+    // Solves every problem to ||rhs - L phi|| <= tol ||rhs||, from phi = 0 or
+    // (zerophi == 0) from the resident phi.  iters[k] / outcome[k] per
+    // problem; returns the V-cycle applications of the busiest problem.
     int solve(double tol, int maxiters, int zerophi, int *iters, int *outcome);
     int last_cycles = 0;
+
+    // Step-wise use (the serving loop): start some problems while others run,
+    // advance all running ones by one iteration, collect the finished ones.
+    void start(const std::vector<int> &which, const std::vector<int> &warm, const std::vector<double> &tol,
+               const std::vector<int> &maxit);
+    std::vector<int> step();
+    int running() const;
+    bool is_running(int k) const { return act_[k] != 0; }
+    int iterations(int k) const { return it_[k]; }
+    int outcome(int k) const { return oc_[k]; }
+    // device bytes held by the batch (hierarchies, vectors, scalars, tables excluded)
+    size_t footprint() const { return footprint_; }
 
   private:
     void body();
@@ -105,6 +117,10 @@ class BatchKMG {
 
     hipStream_t s;
     int sd, su, sb;
+    // per-problem iteration state
+    std::vector<int> act_, it_, oc_, maxit_;
+    std::vector<double> tol_, r0_, base_;
+    size_t footprint_ = 0;
     std::vector<BProb> P;
     std::vector<BLev> L;
     std::vector<int> nlev;      // levels per problem
@@ -146,6 +162,45 @@ class BatchKMG {
     TableSet *T = &full;         // the set the launches use
     std::vector<int> Cp;         // chunk length of each problem's line solves
     bool use_graph = true;
+};
+
+// The serving loop over a batch of resident geometries ("slots"): requests
+// (a slot and a rhs) queue up; each step admits queued requests to idle
+// slots, advances every running solve by one iteration and retires the
+// finished ones, whose solutions stay resident as the slot's warm start.
+// Requests that arrive while others run join at the next step, so their
+// work overlaps on the device; a request's result does not depend on what
+// else is running (the batch keeps problems independent).
+class Server {
+  public:
+    Server(hipStream_t s, const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget);
+    int slots() const { return B.count(); }
+    size_t footprint() const { return B.footprint(); }
+    long long submit(int slot, const double *rhs, double tol, int maxiters, int warm);
+    // admits and advances; returns the solves on the device after the step
+    int step();
+    // done: 1 and the result (phi: nx x ny valid cells) if the request has
+    // finished, then forgotten; 0 if it is queued or running
+    int result(long long ticket, double *phi, int *iters, int *outcome);
+    int queued() const { return (int)queue.size(); }
+
+  private:
+    struct Req {
+        long long ticket;
+        int slot, maxiters, warm;
+        double tol;
+        std::vector<double> rhs;
+    };
+    struct Res {
+        std::vector<double> phi;
+        int iters, outcome;
+    };
+    hipStream_t s;
+    BatchKMG B;
+    std::vector<Req> queue;
+    std::vector<long long> owner;  // per slot: running ticket, or 0
+    std::vector<std::pair<long long, Res>> done;
+    long long next = 1;
 };
 
 }  // namespace mgp

@@ -315,6 +315,15 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
     mask = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
     hip_check(hipHostMalloc((void **)&host_sc, (size_t)n * SC_COUNT * sizeof(double)), "hipHostMalloc");
     use_graph = o.use_graph != 0;
+    act_.assign(n, 0);
+    it_.assign(n, 0);
+    oc_.assign(n, MGP_MAXITERS);
+    maxit_.assign(n, 0);
+    tol_.assign(n, 0.0);
+    r0_.assign(n, 0.0);
+    base_.assign(n, 1.0);
+    footprint_ = bytes + (size_t)n * (SC_COUNT * sizeof(double) + sizeof(BProb) + 2 * sizeof(int)) +
+                 L.size() * sizeof(BLev);
     std::vector<int> all(n);
     for (int p = 0; p < n; ++p) all[p] = p;
     build_tables(full, all);
@@ -512,72 +521,120 @@ void BatchKMG::fetch_scalars() {
     hip_check(hipStreamSynchronize(s), "sync");
 }
 
+// Starts the problems `which` (their rhs already set): phi from zero, or from
+// its current value where warm[k] != 0; tolerance tol[k] relative to ||rhs||
+// and maxit[k] iterations.  Problems whose initial residual already meets the
+// tolerance finish at once (converged, 0 iterations).
+void BatchKMG::start(const std::vector<int> &which, const std::vector<int> &warm, const std::vector<double> &tol,
+                     const std::vector<int> &maxit) {
+    const int n = count();
+    std::vector<int> msk(n, 0), cold(n, 0);
+    for (size_t q = 0; q < which.size(); ++q) {
+        const int p = which[q];
+        msk[p] = 1;
+        cold[p] = warm[q] ? 0 : 1;
+        it_[p] = 0;
+        oc_[p] = MGP_MAXITERS;
+        tol_[p] = tol[q];
+        maxit_[p] = maxit[q];
+        base_[p] = 1.0;
+    }
+    const BTile *tl = (const BTile *)T->t0.p;
+    upload_flags(mask, cold);
+    k_b_copy<<<T->t0.n, 256, 0, s>>>(tl, dP, mask, BV_PHI, -1);
+    upload_flags(mask, msk);
+    reduce(BV_RHS, BV_RHS, SC_RN, mask);
+    fetch_scalars();
+    for (int p : which) r0_[p] = std::sqrt(host_sc[p * SC_COUNT + SC_RN]);
+    true_residual(mask);
+    fetch_scalars();
+    for (int p : which) {
+        const double rel = r0_[p] > 0.0 ? std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0_[p] : 0.0;
+        if (rel <= tol_[p]) {
+            oc_[p] = MGP_CONVERGED;
+            msk[p] = 0;
+        } else {
+            act_[p] = 1;
+            base_[p] = rel;
+        }
+    }
+    upload_flags(mask, msk);
+    restart(mask);
+    upload_flags(active, act_);
+}
+
+// One BiCGStab iteration of every running problem, then their convergence
+// checks (true residual where the recursive one says done, or has fallen by
+// the refinement factor: restart from it).  Returns the problems that
+// finished in this step.
+std::vector<int> BatchKMG::step() {
+    const int n = count();
+    // iterative refinement factor of the FP64 cycle (as KrylovMG::solve)
+    const double refine = 1e-6;
+    std::vector<int> done, msk(n, 0);
+    if (T->graph) hip_check(hipGraphLaunch(T->graph, s), "graph launch");
+    else body();
+    fetch_scalars();
+    bool check = false;
+    for (int p = 0; p < n; ++p) {
+        if (!act_[p]) continue;
+        ++it_[p];
+        const double rn = std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0_[p];
+        const double omega = host_sc[p * SC_COUNT + SC_OMEGA];
+        if (rn <= tol_[p] || omega == 0.0 || !std::isfinite(rn) || rn <= refine * base_[p]) {
+            msk[p] = 1;
+            check = true;
+        } else if (it_[p] >= maxit_[p]) {
+            act_[p] = 0;
+            done.push_back(p);
+        }
+    }
+    if (check) {
+        upload_flags(mask, msk);
+        true_residual(mask);
+        fetch_scalars();
+        for (int p = 0; p < n; ++p) {
+            if (!msk[p]) continue;
+            const double tr = std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0_[p];
+            if (tr <= tol_[p] || !std::isfinite(tr) || it_[p] >= maxit_[p]) {
+                oc_[p] = tr <= tol_[p] ? MGP_CONVERGED : (std::isfinite(tr) ? MGP_MAXITERS : MGP_DIVERGED);
+                act_[p] = 0;
+                msk[p] = 0;
+                done.push_back(p);
+            } else {
+                base_[p] = tr;  // restart from the true residual
+            }
+        }
+        upload_flags(mask, msk);
+        restart(mask);
+    }
+    if (!done.empty()) upload_flags(active, act_);
+    return done;
+}
+
+int BatchKMG::running() const {
+    int r = 0;
+    for (int a : act_) r += a;
+    return r;
+}
+
 int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outcome) {
     const int n = count();
     if (T != &full) {
         release(part);
         T = &full;
     }
-    std::vector<int> act(n, 1), msk(n, 0), it(n, 0), oc(n, MGP_MAXITERS);
-    std::vector<double> r0(n, 0.0), base(n, 1.0);
-    upload_flags(active, act);
-    if (zerophi) k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, BV_PHI, -1);
-    true_residual(active);
-    fetch_scalars();
-    for (int p = 0; p < n; ++p) {
-        r0[p] = std::sqrt(host_sc[p * SC_COUNT + SC_RN]);
-        if (r0[p] == 0.0) {
-            oc[p] = MGP_CONVERGED;
-            act[p] = 0;
-        }
-    }
-    upload_flags(active, act);
-    restart(active);
-    // iterative refinement factor of the FP64 cycle (as KrylovMG::solve)
-    const double refine = 1e-6;
-    last_cycles = 0;
-    for (int k = 1; k <= maxiters; ++k) {
-        if (std::find(act.begin(), act.end(), 1) == act.end()) break;
-        if (T->graph) hip_check(hipGraphLaunch(T->graph, s), "graph launch");
-        else body();
-        last_cycles += 2;
-        fetch_scalars();
-        bool check = false;
-        for (int p = 0; p < n; ++p) {
-            msk[p] = 0;
-            if (!act[p]) continue;
-            it[p] = k;
-            const double rn = std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0[p];
-            const double omega = host_sc[p * SC_COUNT + SC_OMEGA];
-            if (rn <= tol || omega == 0.0 || !std::isfinite(rn) || rn <= refine * base[p]) {
-                msk[p] = 1;
-                check = true;
-            }
-        }
-        if (!check) continue;
-        upload_flags(mask, msk);
-        true_residual(mask);
-        fetch_scalars();
-        for (int p = 0; p < n; ++p) {
-            if (!msk[p]) continue;
-            const double tr = std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0[p];
-            if (tr <= tol || !std::isfinite(tr)) {
-                oc[p] = tr <= tol ? MGP_CONVERGED : MGP_DIVERGED;
-                act[p] = 0;
-                msk[p] = 0;
-            } else {
-                base[p] = tr;  // restart from the true residual
-            }
-        }
-        upload_flags(mask, msk);
-        restart(mask);
-        upload_flags(active, act);
-        // once at most half of the problems in the tables are still active,
+    std::vector<int> all(n);
+    for (int p = 0; p < n; ++p) all[p] = p;
+    start(all, std::vector<int>(n, zerophi ? 0 : 1), std::vector<double>(n, tol), std::vector<int>(n, maxiters));
+    while (running() > 0) {
+        step();
+        // once at most half of the problems in the tables are still running,
         // launch over those alone (the finished ones' workgroups would only
         // return, but every level launch would still cost its latency)
         std::vector<int> live;
         for (int p = 0; p < n; ++p)
-            if (act[p]) live.push_back(p);
+            if (act_[p]) live.push_back(p);
         if (!live.empty() && 2 * (int)live.size() <= T->nprob) {
             release(part);
             build_tables(part, live);
@@ -585,11 +642,100 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
         }
     }
     hip_check(hipStreamSynchronize(s), "sync");
+    int most = 0;
     for (int p = 0; p < n; ++p) {
-        if (iters) iters[p] = it[p];
-        if (outcome) outcome[p] = oc[p];
+        if (iters) iters[p] = it_[p];
+        if (outcome) outcome[p] = oc_[p];
+        most = std::max(most, it_[p]);
     }
+    last_cycles = 2 * most;
     return last_cycles;
+}
+
+// ---------------------------------------------------------------------------
+// Serving loop
+// ---------------------------------------------------------------------------
+Server::Server(hipStream_t s_, const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget)
+    : s(s_), B(s_, slots, o) {
+    if (budget > 0.0 && (double)B.footprint() > budget) {
+        g_last_error = "server: the resident hierarchies need " + std::to_string(B.footprint()) +
+                       " bytes of HBM, over the budget of " + std::to_string((long long)budget);
+        throw Failure(MGP_ERR_NOMEM);
+    }
+    owner.assign(slots.size(), 0);
+}
+
+long long Server::submit(int slot, const double *rhs, double tol, int maxiters, int warm) {
+    if (slot < 0 || slot >= B.count() || !rhs || !(tol >= 0.0) || maxiters < 0) {
+        g_last_error = "server submit: bad slot, rhs, tol or maxiters";
+        throw Failure(MGP_ERR_ARG);
+    }
+    const Grid &g = B.grid(slot);
+    Req r;
+    r.ticket = next++;
+    r.slot = slot;
+    r.tol = tol;
+    r.maxiters = maxiters;
+    r.warm = warm;
+    r.rhs.assign(rhs, rhs + (size_t)g.nx * g.ny);
+    queue.push_back(std::move(r));
+    return queue.back().ticket;
+}
+
+int Server::step() {
+    // admit: the oldest queued request of every idle slot
+    std::vector<int> which, warm, maxit;
+    std::vector<double> tol;
+    for (size_t q = 0; q < queue.size();) {
+        Req &r = queue[q];
+        if (owner[r.slot] != 0) {
+            ++q;
+            continue;
+        }
+        const Grid &g = B.grid(r.slot);
+        upload_block(s, B.vec(r.slot, BV_RHS), g, r.rhs.data(), 0, 0, g.nx, g.ny);
+        owner[r.slot] = r.ticket;
+        which.push_back(r.slot);
+        warm.push_back(r.warm);
+        tol.push_back(r.tol);
+        maxit.push_back(r.maxiters);
+        queue.erase(queue.begin() + (long)q);
+    }
+    std::vector<int> finished;
+    if (!which.empty()) {
+        B.start(which, warm, tol, maxit);
+        for (int p : which)
+            if (!B.is_running(p)) finished.push_back(p);  // met the tolerance at the start
+    }
+    if (B.running() > 0) {
+        const std::vector<int> f = B.step();
+        finished.insert(finished.end(), f.begin(), f.end());
+    }
+    for (int p : finished) {
+        const Grid &g = B.grid(p);
+        Res r;
+        r.phi.resize((size_t)g.nx * g.ny);
+        download_block(s, r.phi.data(), B.vec(p, BV_PHI), g, 0, 0, g.nx, g.ny);
+        r.iters = B.iterations(p);
+        r.outcome = B.outcome(p);
+        done.emplace_back(owner[p], std::move(r));
+        owner[p] = 0;
+    }
+    hip_check(hipStreamSynchronize(s), "server step");
+    return B.running();
+}
+
+int Server::result(long long ticket, double *phi, int *iters, int *outcome) {
+    for (size_t q = 0; q < done.size(); ++q)
+        if (done[q].first == ticket) {
+            const Res &r = done[q].second;
+            if (phi) std::copy(r.phi.begin(), r.phi.end(), phi);
+            if (iters) *iters = r.iters;
+            if (outcome) *outcome = r.outcome;
+            done.erase(done.begin() + (long)q);
+            return 1;
+        }
+    return 0;
 }
 
 }  // namespace mgp

@@ -108,6 +108,15 @@ struct mgp_batch {
     }
 };
 
+struct mgp_server {
+    hipStream_t stream = nullptr;
+    mgp::Server *srv = nullptr;
+    ~mgp_server() {
+        delete srv;
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 struct mgp_solver {
     hipStream_t stream = nullptr;
     mgp_solver_opts opts;
@@ -952,37 +961,45 @@ int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double
 extern "C++" {
 namespace {
 
+// The problem list of a batch or server (validated), and its options.
+void batch_problems(int n, const mgp_bc *bcs, const mgp_solver_opts *opts,
+                const std::function<void(int, BatchKMG::Problem &)> &fill, std::vector<BatchKMG::Problem> &probs,
+                mgp_solver_opts &o) {
+    if (n <= 0 || !bcs) {
+        g_last_error = "batch: n > 0 problems and their BCs";
+        throw Failure(MGP_ERR_ARG);
+    }
+    if (opts) o = *opts;
+    else mgp_solver_default_opts(&o);
+    if (o.method != MGP_METHOD_KRYLOV_MG || !o.homog) {
+        g_last_error = "batch: method KRYLOV_MG with homogeneous BCs (homog = 1)";
+        throw Failure(MGP_ERR_UNSUPPORTED);
+    }
+    probs.assign(n, BatchKMG::Problem());
+    for (int k = 0; k < n; ++k) {
+        check_bc(&bcs[k]);
+        BatchKMG::Problem &q = probs[k];
+        q.geo = nullptr;
+        q.map = nullptr;
+        for (int side = 0; side < 4; ++side) {
+            q.bc.type[side] = bcs[k].type[side];
+            q.bc.mode[side] = MODE_UNIFORM;
+            q.bc.val[side] = 0.0;
+            q.bc.data[side] = nullptr;
+        }
+        fill(k, q);
+    }
+}
+
 int create_batch(mgp_batch **out, int n, const mgp_bc *bcs, const mgp_solver_opts *opts,
                  const std::function<void(int, BatchKMG::Problem &)> &fill) {
     if (!out) return MGP_ERR_ARG;
     *out = nullptr;
     mgp_batch *b = new mgp_batch;
     const int st = guarded([&] {
-        if (n <= 0 || !bcs) {
-            g_last_error = "batch: n > 0 problems and their BCs";
-            throw Failure(MGP_ERR_ARG);
-        }
+        std::vector<BatchKMG::Problem> probs;
         mgp_solver_opts o;
-        if (opts) o = *opts;
-        else mgp_solver_default_opts(&o);
-        if (o.method != MGP_METHOD_KRYLOV_MG || !o.homog) {
-            g_last_error = "batch: method KRYLOV_MG with homogeneous BCs (homog = 1)";
-            throw Failure(MGP_ERR_UNSUPPORTED);
-        }
-        std::vector<BatchKMG::Problem> probs(n);
-        for (int k = 0; k < n; ++k) {
-            check_bc(&bcs[k]);
-            BatchKMG::Problem &q = probs[k];
-            q.geo = nullptr;
-            q.map = nullptr;
-            for (int side = 0; side < 4; ++side) {
-                q.bc.type[side] = bcs[k].type[side];
-                q.bc.mode[side] = MODE_UNIFORM;
-                q.bc.val[side] = 0.0;
-                q.bc.data[side] = nullptr;
-            }
-            fill(k, q);
-        }
+        batch_problems(n, bcs, opts, fill, probs, o);
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         b->b = new BatchKMG(b->stream, probs, o);
     });
@@ -1079,6 +1096,78 @@ int mgp_batch_solve(mgp_batch *b, double tol, int maxiters, int zerophi, int *it
 }
 
 int mgp_batch_last_cycles(const mgp_batch *b) { return b && b->b ? b->b->last_cycles : 0; }
+
+int mgp_server_create(mgp_server **out, int n, const mgp_mapping *maps, const int *nx, const int *ny,
+                      const double *dx, const double *dy, const mgp_bc *bcs, const mgp_solver_opts *opts,
+                      double hbm_budget) {
+    if (!out) return MGP_ERR_ARG;
+    *out = nullptr;
+    mgp_server *v = new mgp_server;
+    const int st = guarded([&] {
+        std::vector<BatchKMG::Problem> probs;
+        mgp_solver_opts o;
+        batch_problems(n, bcs, opts, [&](int k, BatchKMG::Problem &q) {
+            if (!maps || !nx || !ny || !dx || !dy) {
+                g_last_error = "server: null mapping or grid arrays";
+                throw Failure(MGP_ERR_ARG);
+            }
+            check_mapping(&maps[k]);
+            q.nx = nx[k];
+            q.ny = ny[k];
+            q.dx = dx[k];
+            q.dy = dy[k];
+            q.map = &maps[k];
+        }, probs, o);
+        hip_check(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking), "hipStreamCreate");
+        v->srv = new Server(v->stream, probs, o, hbm_budget);
+    });
+    if (st != MGP_OK) {
+        delete v;
+        return st;
+    }
+    *out = v;
+    return MGP_OK;
+}
+
+int mgp_server_destroy(mgp_server *v) {
+    delete v;
+    return MGP_OK;
+}
+
+double mgp_server_footprint(const mgp_server *v) { return v && v->srv ? (double)v->srv->footprint() : 0.0; }
+
+int mgp_server_submit(mgp_server *v, int slot, const double *rhs, double tol, int maxiters, int warm,
+                      long long *ticket) {
+    return guarded([&] {
+        if (!v || !v->srv || !ticket) {
+            g_last_error = "server submit: null server or ticket";
+            throw Failure(MGP_ERR_ARG);
+        }
+        *ticket = v->srv->submit(slot, rhs, tol, maxiters, warm);
+    });
+}
+
+int mgp_server_step(mgp_server *v, int *running, int *queued) {
+    return guarded([&] {
+        if (!v || !v->srv) {
+            g_last_error = "server step: null server";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const int r = v->srv->step();
+        if (running) *running = r;
+        if (queued) *queued = v->srv->queued();
+    });
+}
+
+int mgp_server_result(mgp_server *v, long long ticket, double *phi, int *iters, int *outcome, int *done) {
+    return guarded([&] {
+        if (!v || !v->srv || !done) {
+            g_last_error = "server result: null server or done flag";
+            throw Failure(MGP_ERR_ARG);
+        }
+        *done = v->srv->result(ticket, phi, iters, outcome);
+    });
+}
 
 int mgp_device_synchronize(void) {
     return guarded([&] { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
