@@ -64,42 +64,21 @@ __host__ __device__ inline size_t ix(int pitch, int i, int j) {
     return (size_t)(j + 1) * (size_t)pitch + (size_t)(OFFX + i);
 }
 
-// y-direction prolongation for 1x2 semicoarsening at fine cell (i, j), from
-// the coarse centre J = j/2 and its neighbour on j's side (J-1 for even j,
-// J+1 for odd j); ef += the interpolated coarse correction.  Beyond a y side
-// the neighbour is the ghost slo*c / shi*c (Neumann: sign +1, reflected;
-// Dirichlet: sign -1, negated).
-//
-// Jyy == nullptr: linear in the index, weights 3/4 and 1/4.  Else the weights
-// are linear in the y "resistance" of the fine level's column i (the
-// operator-induced interpolation of semicoarsening): face k between fine
-// rows k-1 and k has resistance R(k) = 1/Jyy(i, k) (its y conductance is
-// Jyy/dy^2, dy common to the level).  The coarse centre J lies on fine face
-// 2J+1, so from fine row j = 2J the distances are R(2J+1)/2 to centre J and
-// R(2J) + R(2J-1)/2 to centre J-1 (odd j: R(2J+2) + R(2J+3)/2 to J+1); a
-// neighbour beyond the side is the mirror image of centre J across the wall
-// face.  On a uniform column this is the linear rule; on strongly stretched
-// columns it follows the error that y-dominated smoothing leaves.
+// y-direction prolongation for 1x2 semicoarsening at fine cell (i, j):
+// linear in y between the coarse centre J = j/2 (weight 3/4) and its
+// neighbour on j's side (J-1 for even j, J+1 for odd j; weight 1/4); beyond a
+// y side the neighbour is the ghost slo*c / shi*c (Neumann: sign +1,
+// reflected; Dirichlet: sign -1, negated).  ef += the interpolated coarse
+// correction.
 template <typename T>
 __device__ __forceinline__ void prolong_y_cell(T *ef, const T *ec, const Grid &gf, const Grid &gc, T slo, T shi,
-                                               int i, int j, const T *Jyy = nullptr) {
+                                               int i, int j) {
     const int J = j >> 1;
     const T c = ec[ix(gc.pitch, i, J)];
-    const bool odd = j & 1;
-    const bool inside = odd ? J + 1 < gc.ny : J - 1 >= 0;
-    const T nb = inside ? ec[ix(gc.pitch, i, odd ? J + 1 : J - 1)] : (odd ? shi : slo) * c;
-    T wc = T(0.75), wn = T(0.25);
-    if (Jyy) {
-        auto R = [&](int k) { return T(1) / Jyy[ix(gf.pitch, i, k)]; };
-        const T ra = T(0.5) * R(2 * J + 1);
-        // faces towards the neighbour: 2J, 2J-1 (even j) or 2J+2, 2J+3 (odd j);
-        // past the wall face the mirror of face 2J+1
-        const int f1 = odd ? 2 * J + 2 : 2 * J, f2 = odd ? 2 * J + 3 : 2 * J - 1;
-        const T rb = R(f1) + T(0.5) * (inside ? R(f2) : R(2 * J + 1));
-        wc = rb / (ra + rb);
-        wn = ra / (ra + rb);
-    }
-    ef[ix(gf.pitch, i, j)] += wc * c + wn * nb;
+    T nb;
+    if (j & 1) nb = J + 1 < gc.ny ? ec[ix(gc.pitch, i, J + 1)] : shi * c;
+    else nb = J - 1 >= 0 ? ec[ix(gc.pitch, i, J - 1)] : slo * c;
+    ef[ix(gf.pitch, i, j)] += T(0.75) * c + T(0.25) * nb;
 }
 
 }  // namespace mgp

@@ -111,12 +111,11 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
                   int row0, int rowstep, int nrows, int extra0, int extra1);
 void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
                   int rowstep, int nrows, int extra0, int extra1);
-// ef += 1x2 (y) prolongation of ec; Jyy (the fine level's, or null): the
-// weights of prolong_y_cell
+// ef += 1x2 (y) prolongation of ec (prolong_y_cell)
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi, const double *Jyy);
+                      double shi);
 void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi, const float *Jyy);
+                      double shi);
 // Precision conversions over a level array.  ghosted: also the ghost ring.
 void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted);
 void launch_convert(hipStream_t s, float *out, const double *in, const Grid &g, int ghosted);

@@ -866,12 +866,11 @@ namespace mgp {
 
 // y-direction prolongation for 1x2 semicoarsening (prolong_y_cell).
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong_y(T *ef, const T *ec, Grid gf, Grid gc, T slo, T shi,
-                                                   const T *Jyy) {
+__global__ void __launch_bounds__(256) k_prolong_y(T *ef, const T *ec, Grid gf, Grid gc, T slo, T shi) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= gf.nx || j >= gf.ny) return;
-    prolong_y_cell(ef, ec, gf, gc, slo, shi, i, j, Jyy);
+    prolong_y_cell(ef, ec, gf, gc, slo, shi, i, j);
 }
 
 template <typename TO, typename TI>
@@ -1096,13 +1095,13 @@ void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sw
 
 namespace mgp {
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi, const double *Jyy) {
-    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, slo, shi, Jyy);
+                      double shi) {
+    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, slo, shi);
 }
 
 void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi, const float *Jyy) {
-    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, (float)slo, (float)shi, Jyy);
+                      double shi) {
+    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, (float)slo, (float)shi);
 }
 
 void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted) {

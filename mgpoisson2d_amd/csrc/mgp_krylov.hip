@@ -203,7 +203,7 @@ static void vcycle_t(hipStream_t s, KrylovMG::Impl &K, size_t d, T *e, const T *
     for (int k = 0; k < K.sd; ++k) zebra_sweep(s, l, e, f);
     launch_residual_restrict_y(s, cr, ce, e, f, table(l, e), l.g, c.g, metrics(l, e));
     vcycle_t(s, K, d + 1, ce, cr);
-    launch_prolong_y(s, e, ce, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI), metrics(l, e).Jyy);
+    launch_prolong_y(s, e, ce, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
     for (int k = 0; k < K.su; ++k) zebra_sweep(s, l, e, f);
 }
 

@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const
     const int i = t.tx * 64 + (threadIdx.x & 63), j = t.ty * 4 + (threadIdx.x >> 6);
     const Grid gf = F->g, gc = Cl->g;
     if (i >= gf.nx || j >= gf.ny) return;
-    prolong_y_cell(F->e[sel], (const double *)Cl->e[0], gf, gc, F->slo, F->shi, i, j, F->geo.Jyy);
+    prolong_y_cell(F->e[sel], (const double *)Cl->e[0], gf, gc, F->slo, F->shi, i, j);
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
