@@ -61,7 +61,7 @@ struct BTile {
 // the e/f pair of the finest level.
 // k >= 0: sweep k of a level's coarsest-level sweeps (levels with fewer return)
 void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
-                        int C, size_t lds, int k);
+                        int C, int threads, size_t lds, int k);
 // coarse (lev+1) r = 1x2 restriction of f - L e on lev, coarse e = 0; tiles over the coarse cells
 void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
                            int sel);
@@ -136,6 +136,7 @@ class BatchKMG {
         void *p = nullptr;
         int n = 0;
         int C = 0;        // x-line tables: chunk length
+        int threads = 0;  // x-line tables: threads per row
         size_t lds = 0;   // x-line tables: dynamic LDS
     };
     enum { SET_MID = 0, SET_BOTTOM = 1 };
@@ -160,7 +161,7 @@ class BatchKMG {
     TableSet full;               // every problem (built once)
     TableSet part;               // the problems still active, once they are at most half of `full`
     TableSet *T = &full;         // the set the launches use
-    std::vector<int> Cp;         // chunk length of each problem's line solves
+    std::vector<int> Cp, NTp;    // chunk length and threads per row of each problem's line solves
     bool use_graph = true;
 };
 

@@ -302,8 +302,9 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
     // each problem's line solves use its own chunk length, whatever the batch
     // (the partition of a row into chunks decides the rounding of its solve)
     Cp.resize(n);
+    NTp.resize(n);
     for (int p = 0; p < n; ++p) {
-        Cp[p] = xline_chunk_length(P[p].g.nx);
+        xline_batch_shape(P[p].g.nx, Cp[p], NTp[p]);
         for (int d = 0; d < nlev[p]; ++d) L[P[p].lev0 + d].nt = P[p].g.nx / Cp[p];
     }
 
@@ -345,20 +346,20 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
     ts.tp.assign(ts.depth, Table());
     ts.bottom.assign(ts.depth, 0);
     for (int d = 0; d < ts.depth; ++d) {
-        std::map<std::tuple<int, int, int>, std::vector<XlTask>> tabs;  // (set, colour, C)
-        std::map<std::tuple<int, int, int>, size_t> lds;
+        std::map<std::tuple<int, int, int, int>, std::vector<XlTask>> tabs;  // (set, colour, C, threads)
+        std::map<std::tuple<int, int, int, int>, size_t> lds;
         std::vector<BTile> trh, tph;
         for (int p : which) {
             if (d >= nlev[p]) continue;
             const int lev = P[p].lev0 + d;
             const int ny = L[lev].g.ny, nx = L[lev].g.nx;
             const int set = d == nlev[p] - 1 ? SET_BOTTOM : SET_MID;
-            const int C = Cp[p];
+            const int C = Cp[p], NT = NTp[p];
             if (set == SET_BOTTOM) ts.bottom[d] = std::max(ts.bottom[d], L[lev].sweeps);
             auto add = [&](int colour, XlTask t) {
-                const auto key = std::make_tuple(set, colour, C);
+                const auto key = std::make_tuple(set, colour, C, NT);
                 tabs[key].push_back(t);
-                lds[key] = std::max(lds[key], xline_lds_bytes_for(nx, C, sizeof(double)));
+                lds[key] = std::max(lds[key], xline_lds_bytes_for(nx, C, sizeof(double), NT));
             };
             // the order of xline_zebra: even interior rows, odd interior rows
             // (the first and last relaxing rows 0 and ny-1 after their own),
@@ -388,6 +389,7 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
             x.colour = std::get<1>(kv.first);
             x.t = upload(ts, kv.second.data(), kv.second.size() * sizeof(XlTask), (int)kv.second.size());
             x.t.C = std::get<2>(kv.first);
+            x.t.threads = std::get<3>(kv.first);
             x.t.lds = lds[kv.first];
             ts.xl.push_back(x);
         }
@@ -395,7 +397,8 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
         ts.tp[d] = upload(ts, tph.data(), tph.size() * sizeof(BTile), (int)tph.size());
     }
     std::sort(ts.xl.begin(), ts.xl.end(), [](const XlTable &a, const XlTable &b) {
-        return std::make_tuple(a.d, a.set, a.colour, a.t.C) < std::make_tuple(b.d, b.set, b.colour, b.t.C);
+        return std::make_tuple(a.d, a.set, a.colour, a.t.C, a.t.threads) <
+               std::make_tuple(b.d, b.set, b.colour, b.t.C, b.t.threads);
     });
     hip_check(hipStreamSynchronize(s), "batch tables");
     if (use_graph) {
@@ -441,7 +444,8 @@ void BatchKMG::zebra(int d, int set, int sel, int k) {
     for (int colour = 0; colour < 3; ++colour)
         for (const XlTable &x : T->xl)
             if (x.d == d && x.set == set && x.colour == colour)
-                launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.lds, k);
+                launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.threads, x.t.lds,
+                                   k);
 }
 
 // e = M f for the pair `sel` of every active problem (phat from p: 0, shat

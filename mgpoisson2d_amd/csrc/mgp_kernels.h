@@ -97,12 +97,13 @@ bool xline_supported(int nx);
 // Gauss-Seidel sweep in y, so a coarsest level of many rows needs more of
 // them (ny = 16 k with k odd leaves k rows).
 inline int coarsest_sweeps(int sb, int ny) { return sb > ny / 2 ? sb : ny / 2; }
-// chunk length C of the line solve of an nx-cell row (2, 4, 8 or 16; 0: too long)
-int xline_chunk_length(int nx);
+// Shape of the batched line solve of an nx-cell row: threads per row (64 up
+// to 512 cells, else 256) and chunk length C (2, 4, 8 or 16; 0: too long).
+void xline_batch_shape(int nx, int &C, int &threads);
 // dynamic LDS of one x-line workgroup for elements of `elem` bytes
 size_t xline_lds_bytes(int nx, size_t elem);
-// the same for the chunk length C (a batch may use a longer one than the row's own)
-size_t xline_lds_bytes_for(int nx, int C, size_t elem);
+// the same for chunk length C and `threads` threads per row
+size_t xline_lds_bytes_for(int nx, int C, size_t elem, int threads);
 // Relax rows j = row0 + k*rowstep, k < nrows (one workgroup each; no row of a
 // launch may read another: interior rows at least two apart, boundary rows
 // three).  The first workgroup then also relaxes row extra0, the last one row

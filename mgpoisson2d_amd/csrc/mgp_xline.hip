@@ -37,7 +37,12 @@ namespace mgp {
 
 namespace {
 
+// Threads per row of the line solve: 256, or 64 for rows of at most 512
+// cells in the batched launches (many short rows: a row then occupies one
+// wave, and four times as many rows are in flight).
 constexpr int XL_THREADS = 256;
+constexpr int XL_SHORT = 64;
+constexpr int XL_SHORT_MAX_NX = 512;
 constexpr int RING_PLANES = 9;
 
 // Impulse whose ghosts follow the homogeneous fill of the MG levels
@@ -204,7 +209,7 @@ __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
 // table.  Lanes past the row end evaluate the last cell and store into the
 // chunk layout's last slot, which no chunk uses (the last chunk has C cells):
 // the loop stays branch-free.
-template <int C, typename T, typename GM>
+template <int C, int NT, typename T, typename GM>
 __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const T *f_, const T *K_, const Grid &g,
                                          const GM &geo_, int j, T *Ld, T *La, T *Lb, T *Lc) {
     constexpr int U = C + 1;
@@ -216,7 +221,7 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
     if (j != 0 && j != ny - 1) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int iraw = tid + XL_THREADS * u;
+            const int iraw = tid + NT * u;
             const int i = iraw < nx ? iraw : nx - 1;
             const size_t c = ix(P, i, j);
             // interior stencil; for i = 0, nx-1 the reads stay inside the
@@ -239,7 +244,7 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
         // the two ring cells of the row (i = 0, nx-1): explicit stencil table
         for (int side = 0; side < 2; ++side) {
             const int iend = side ? nx - 1 : 0;
-            if (iend % XL_THREADS != tid) continue;
+            if (iend % NT != tid) continue;
             const int o = chunk_pos<C>(iend, X.rem);
             Ld[o] = f[ix(P, iend, j)] - ring_lap(e, K, X.nring, g, iend, j);
             ring_row_coef(K, X.nring, g, iend, j, La[o], Lb[o], Lc[o]);
@@ -248,7 +253,7 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
         // boundary row: every cell from the table
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int iraw = tid + XL_THREADS * u;
+            const int iraw = tid + NT * u;
             const int i = iraw < nx ? iraw : nx - 1;
             const T fv = f[ix(P, i, j)];
             const T lap = ring_lap(e, K, X.nring, g, i, j);
@@ -264,30 +269,33 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
 }
 
 // Coalesced update e(:, j) += x, x in the chunk layout (e re-read).
-template <int C, typename T>
+template <int C, int NT, typename T>
 __device__ __forceinline__ void row_update_add(const RowCtx<T> &X, T *e, int j, const T *Lx) {
 #pragma unroll
     for (int u = 0; u < C + 1; ++u) {
-        const int i = threadIdx.x + XL_THREADS * u;
+        const int i = threadIdx.x + NT * u;
         if (i < X.nx) e[ix(X.P, i, j)] += Lx[chunk_pos<C>(i, X.rem)];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Interface solve of the row system, one wave.  The 512 interface rows (2
-// per thread of the workgroup) sit in LDS lane-major: row q at
-// iface_pos(q) = (q % 8) * 64 + q / 8, so lane l reads its rows 8l .. 8l+7
-// without bank conflicts.  Lane l
-//  1. eliminates its 8 rows to 2 (the partition step of the chunks): its
-//     first row in terms of x(8l-1), x(8l), x(8l+7), its last in terms of
-//     x(8l), x(8l+7), x(8l+8);
+// Interface solve of the row system, one wave.  The 2 NT interface rows (2
+// per thread of the workgroup; NT = 256: 512, L = 8 per lane; NT = 64: 128,
+// L = 2) sit in LDS lane-major: row q at iface_pos(q) = (q % L) * 64 + q / L,
+// so lane l reads its rows L l .. L l + L-1 without bank conflicts.  Lane l
+//  1. eliminates its L rows to 2 (the partition step of the chunks): its
+//     first row in terms of x(Ll-1), x(Ll), x(Ll+L-1), its last in terms of
+//     x(Ll), x(Ll+L-1), x(Ll+L);
 //  2. takes part in parallel cyclic reduction of those 128 rows (7 steps,
 //     neighbours exchanged with cross-lane shuffles, no barrier);
-//  3. evaluates its 6 inner unknowns from the first and last ones (their
-//     linear forms were set up in step 1) and writes x(8l .. 8l+7) to X in
+//  3. evaluates its L-2 inner unknowns from the first and last ones (their
+//     linear forms were set up in step 1) and writes x(Ll .. Ll+L-1) to X in
 //     natural order.
 // ---------------------------------------------------------------------------
-constexpr int IFACE = 2 * XL_THREADS;  // interface rows, padded
+template <int NT>
+constexpr int iface_rows() {
+    return 2 * NT;  // interface rows, padded
+}
 
 // Quotients of the line solves: v_rcp_f64 refined by one Newton step (error
 // of order eps^2 relative, three dependent instructions instead of the
@@ -304,7 +312,11 @@ __device__ __forceinline__ T quot(T a, T b) {
     return a * rcp(b);
 }
 
-__device__ __forceinline__ int iface_pos(int q) { return (q & 7) * 64 + (q >> 3); }
+template <int NT>
+__device__ __forceinline__ int iface_pos(int q) {
+    constexpr int L = NT / 32;
+    return (q % L) * 64 + q / L;
+}
 
 template <typename T>
 struct Row4 {
@@ -334,9 +346,9 @@ __device__ __forceinline__ Row4<T> pcr_update(const Row4<T> &r, const Row4<T> &m
     return o;
 }
 
-template <typename T>
+template <int NT, typename T>
 __device__ __forceinline__ void interface_solve(T *IA, T *IB, const T *IC, T *ID, T *IX, int l) {
-    constexpr int L = 8;
+    constexpr int L = NT / 32;
     T a[L], b[L], c[L], d[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) {
@@ -421,14 +433,14 @@ __device__ __forceinline__ void interface_solve(T *IA, T *IB, const T *IC, T *ID
 // from the metrics, elimination, PCR and back substitution per call.
 // LDS: 4 nt (C+1) doubles.
 // ---------------------------------------------------------------------------
-template <int C, typename T, typename GM>
+template <int C, int NT, typename T, typename GM>
 __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
                           const Grid &g, const GM &geo, int j, T *lds) {
     constexpr int CP = C + 1;
     const int tid = threadIdx.x, nt = X.nt;
     const int NT1 = nt * CP;
     T *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
-    row_pass<C>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
+    row_pass<C, NT>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
     __syncthreads();
 
     // each thread takes its chunk into registers
@@ -499,9 +511,10 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
     // q = 2t, end at q = 2t+1; tridiagonal in that order), padded to 512 rows
     // with identity rows by the threads past nt.  Staged once through LDS,
     // solved by wave 0 (interface_solve), read back once: two barriers.
+    constexpr int IFACE = iface_rows<NT>();
     T *IA = lds + NT1, *IB = IA + IFACE, *IC = IB + IFACE, *ID = IC + IFACE, *IX = ID + IFACE;
     {
-        const int q0 = iface_pos(2 * tid), q1 = iface_pos(2 * tid + 1);
+        const int q0 = iface_pos<NT>(2 * tid), q1 = iface_pos<NT>(2 * tid + 1);
         IA[q0] = ea;
         IB[q0] = eb;
         IC[q0] = ec;
@@ -512,7 +525,7 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
         ID[q1] = od;
     }
     __syncthreads();
-    if (tid < 64) interface_solve(IA, IB, IC, ID, IX, tid);
+    if (tid < 64) interface_solve<NT>(IA, IB, IC, ID, IX, tid);
     __syncthreads();
     const T x_start = IX[2 * tid], x_end = IX[2 * tid + 1];
 
@@ -527,7 +540,7 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
         lds[tid * CP + (longc ? C : C - 1)] = xe;
     }
     __syncthreads();
-    row_update_add<C>(X, e, j, lds);
+    row_update_add<C, NT>(X, e, j, lds);
 }
 
 // Rows j = row0 + slot*rowstep; the workgroup of slot 0 then also relaxes
@@ -554,7 +567,7 @@ __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restric
         // ops) instead of being hoisted out of the loop and spilled
         RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
-        xline_row<C>(X, e, f, K, g, geo, jr, lds);
+        xline_row<C, XL_THREADS>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
@@ -635,7 +648,7 @@ __global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const
 // Batched forms: the workgroup's task or tile names the level (and through
 // it the problem); workgroups of inactive problems return at once.
 // ---------------------------------------------------------------------------
-template <int C>
+template <int C, int NT>
 __device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
     const XlTask t = tasks[launch_slot()];
     const BLev *L = levs + t.lev;
@@ -654,20 +667,27 @@ __device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs
         if (r) __syncthreads();
         RowCtx<double> X = X0;
         asm volatile("" : "+s"(X.rem));
-        xline_row<C>(X, e, f, K, g, geo, jr, lds);
+        xline_row<C, NT>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
 template <int C>
 __global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_xline_batch(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
-    xline_task<C>(tasks, levs, active, sel, k);
+    xline_task<C, XL_THREADS>(tasks, levs, active, sel, k);
 }
 
 template <int C>
 __global__ void __launch_bounds__(XL_THREADS) k_xline_batch_long(const XlTask *tasks, const BLev *levs,
                                                                  const int *active, int sel, int k) {
-    xline_task<C>(tasks, levs, active, sel, k);
+    xline_task<C, XL_THREADS>(tasks, levs, active, sel, k);
+}
+
+// Short rows (nx <= 512): one wave per row.
+template <int C>
+__global__ void __launch_bounds__(XL_SHORT) __attribute__((amdgpu_waves_per_eu(2)))
+k_xline_batch_short(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
+    xline_task<C, XL_SHORT>(tasks, levs, active, sel, k);
 }
 
 __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, const BLev *levs, const int *active,
@@ -696,9 +716,9 @@ __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
 // floor(nx / C) <= 256 chunks.  nx >= C*C then holds, so the remainder is
 // spread one cell per chunk (chunks of C or C+1 cells).
-int xline_chunk(int nx) {
+int xline_chunk(int nx, int nt_max = XL_THREADS) {
     for (int C = 2; C <= 16; C *= 2)
-        if (nx / C <= XL_THREADS) return C;
+        if (nx / C <= nt_max) return C;
     return 0;
 }
 
@@ -711,19 +731,22 @@ void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, c
     k_ring_coef<<<(n + 255) / 256, 256, 0, s>>>(K, g, geo, bc);
 }
 
-size_t xline_lds_bytes(int nx, size_t elem) { return xline_lds_bytes_for(nx, xline_chunk(nx), elem); }
+size_t xline_lds_bytes(int nx, size_t elem) { return xline_lds_bytes_for(nx, xline_chunk(nx), elem, XL_THREADS); }
 
-size_t xline_lds_bytes_for(int nx, int C, size_t elem) {
+size_t xline_lds_bytes_for(int nx, int C, size_t elem, int threads) {
     if (C == 0) return (size_t)-1;
     const size_t nt = (size_t)(nx / C);
     // chunk staging (4 arrays), or one chunk array plus the interface rows
-    const size_t tr = 4 * nt * (C + 1), iface = nt * (C + 1) + 5 * IFACE;
+    const size_t tr = 4 * nt * (C + 1), iface = nt * (C + 1) + 5 * 2 * (size_t)threads;
     return (tr > iface ? tr : iface) * elem;
 }
 
 bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
 
-int xline_chunk_length(int nx) { return xline_chunk(nx); }
+void xline_batch_shape(int nx, int &C, int &threads) {
+    threads = nx <= XL_SHORT_MAX_NX ? XL_SHORT : XL_THREADS;
+    C = xline_chunk(nx, threads);
+}
 
 template <typename T, typename G>
 static void residual_restrict_y(hipStream_t s, T *rc, T *ec, const T *e, const T *f, const T *K, const Grid &gf,
@@ -772,9 +795,18 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
 }
 
 void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
-                        int C, size_t lds, int k) {
+                        int C, int threads, size_t lds, int k) {
     if (ntasks <= 0) return;
-    switch (C) {
+    if (threads == XL_SHORT) {
+        switch (C) {
+            case 2: k_xline_batch_short<2><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 4: k_xline_batch_short<4><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 8: k_xline_batch_short<8><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
+            default:
+                g_last_error = "k_xline_batch_short: chunk length must be 2, 4 or 8";
+                throw Failure(MGP_ERR_ARG);
+        }
+    } else switch (C) {
         case 2: k_xline_batch<2><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
         case 4: k_xline_batch<4><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
         case 8: k_xline_batch<8><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
