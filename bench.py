@@ -71,7 +71,7 @@ def config4_mix(n=1024, seed=SEED):
     return probs, rhs
 
 
-def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2):
+def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2, use_graph=True):
     """Solves/s of the batched solver on the problem mix, against the plain FP64
     single-problem solver (KRYLOV_MG, one problem at a time) timed on a sample of
     the same mix; per-solve latencies of the plain path give p50/p99."""
@@ -82,7 +82,7 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2)
     maps = [capi.Mapping(capi.MAP_WARPED, amp, beta, 0.0, 0.0) for _, _, amp, beta, _ in probs]
     bcs = [G.BoundaryConditions.uniform(t) for *_, t in probs]
     t0 = time.perf_counter()
-    b = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb)
+    b = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb, use_graph=use_graph)
     for k in range(n):
         b.set_rhs(k, rhs[k])
     capi.device_synchronize()
@@ -107,7 +107,7 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2)
     diffs = []
     for k in range(min(sample, n)):
         s1 = capi.Solver(geos[k], bcs[k], method=capi.METHOD_KRYLOV_MG, mapping=maps[k], smooth_down=sd,
-                         smooth_up=su, smooth_bottom=sb)
+                         smooth_up=su, smooth_bottom=sb, use_graph=use_graph)
         s1.set_rhs(rhs[k])
         s1.solve_fast(tol, 400, zerophi=True)
         capi.device_synchronize()
@@ -310,7 +310,7 @@ def main():
 
     from mgpoisson2d_amd import capi
     if args.batch_only:
-        print(json.dumps(batched_bench(capi, args.batch_n)), flush=True)
+        print(json.dumps(batched_bench(capi, args.batch_n, use_graph=not args.no_graph)), flush=True)
         return
     if args.serving_only:
         print(json.dumps(serving_bench(capi)), flush=True)
@@ -404,7 +404,7 @@ def main():
 
     batched = None
     if not args.no_batched:
-        batched = batched_bench(capi, args.batch_n)
+        batched = batched_bench(capi, args.batch_n, use_graph=not args.no_graph)
     serving = None
     if not args.no_serving:
         serving = serving_bench(capi)

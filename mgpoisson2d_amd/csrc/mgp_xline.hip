@@ -37,12 +37,14 @@ namespace mgp {
 
 namespace {
 
-// Threads per row of the line solve: 256, or 64 for rows of at most 512
-// cells in the batched launches (many short rows: a row then occupies one
-// wave, and four times as many rows are in flight).
+// Threads per row of the line solve: 256; in the batched launches (many
+// rows) 64 for rows of at most 512 cells and 128 up to 1024 cells, so that
+// a CU holds more rows at once (a short row then occupies one or two waves).
 constexpr int XL_THREADS = 256;
 constexpr int XL_SHORT = 64;
 constexpr int XL_SHORT_MAX_NX = 512;
+constexpr int XL_MID = 128;
+constexpr int XL_MID_MAX_NX = 1024;
 constexpr int RING_PLANES = 9;
 
 // Impulse whose ghosts follow the homogeneous fill of the MG levels
@@ -690,6 +692,13 @@ k_xline_batch_short(const XlTask *tasks, const BLev *levs, const int *active, in
     xline_task<C, XL_SHORT>(tasks, levs, active, sel, k);
 }
 
+// Rows of 513..1024 cells: two waves per row.
+template <int C>
+__global__ void __launch_bounds__(XL_MID) __attribute__((amdgpu_waves_per_eu(2)))
+k_xline_batch_mid(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
+    xline_task<C, XL_MID>(tasks, levs, active, sel, k);
+}
+
 __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, const BLev *levs, const int *active,
                                                         int sel) {
     const BTile t = tiles[blockIdx.x];
@@ -744,7 +753,7 @@ size_t xline_lds_bytes_for(int nx, int C, size_t elem, int threads) {
 bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
 
 void xline_batch_shape(int nx, int &C, int &threads) {
-    threads = nx <= XL_SHORT_MAX_NX ? XL_SHORT : XL_THREADS;
+    threads = nx <= XL_SHORT_MAX_NX ? XL_SHORT : (nx <= XL_MID_MAX_NX ? XL_MID : XL_THREADS);
     C = xline_chunk(nx, threads);
 }
 
@@ -806,6 +815,12 @@ void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BL
                 g_last_error = "k_xline_batch_short: chunk length must be 2, 4 or 8";
                 throw Failure(MGP_ERR_ARG);
         }
+    } else if (threads == XL_MID) {
+        if (C != 8) {
+            g_last_error = "k_xline_batch_mid: chunk length must be 8";
+            throw Failure(MGP_ERR_ARG);
+        }
+        k_xline_batch_mid<8><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k);
     } else switch (C) {
         case 2: k_xline_batch<2><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
         case 4: k_xline_batch<4><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
