@@ -92,11 +92,15 @@ void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float
                                 const Grid &gf, const Grid &gc, const GeoF &geo);
 bool xline_supported(int nx);
 // Sweeps on the coarsest level of the semicoarsening hierarchy: at least
-// `sb`, and half the level's rows.  The hierarchy stops at an odd row count,
-// and an x-line zebra sweep reduces errors that are smooth in y like one
-// Gauss-Seidel sweep in y, so a coarsest level of many rows needs more of
-// them (ny = 16 k with k odd leaves k rows).
-inline int coarsest_sweeps(int sb, int ny) { return sb > ny / 2 ? sb : ny / 2; }
+// `sb`, and half the level's rows up to 64.  The hierarchy stops at an odd
+// row count, and an x-line zebra sweep reduces errors that are smooth in y
+// like one Gauss-Seidel sweep in y, so a coarsest level of many rows needs
+// more of them (ny = 16 k with k odd leaves k rows).  The cap bounds the cost
+// of a cycle where the count stays large (an odd ny keeps every row).
+inline int coarsest_sweeps(int sb, int ny) {
+    const int half = ny / 2 < 64 ? ny / 2 : 64;
+    return sb > half ? sb : half;
+}
 // Shape of the batched line solve of an nx-cell row: threads per row (64 up
 // to 512 cells, else 256) and chunk length C (2, 4, 8 or 16; 0: too long).
 void xline_batch_shape(int nx, int &C, int &threads);
