@@ -71,7 +71,7 @@ def config4_mix(n=1024, seed=SEED):
     return probs, rhs
 
 
-def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2, use_graph=True):
+def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2, use_graph=True, mixed=True):
     """Solves/s of the batched solver on the problem mix, against the plain FP64
     single-problem solver (KRYLOV_MG, one problem at a time) timed on a sample of
     the same mix; per-solve latencies of the plain path give p50/p99."""
@@ -100,6 +100,30 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2,
     if not np.all(outc == capi.CONVERGED):
         raise SystemExit("batched solve did not converge")
     cycles = b.last_cycles()
+    mixed_line = None
+    if mixed:
+        # the same mix with FP32 preconditioner cycles (FP64 Krylov, residual and answer)
+        b32 = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb,
+                         use_graph=use_graph, method=capi.METHOD_KRYLOV_MG32)
+        for k in range(n):
+            b32.set_rhs(k, rhs[k])
+        b32.solve(tol, 400)
+        times32 = []
+        for _ in range(steps):
+            capi.device_synchronize()
+            t1 = time.perf_counter()
+            it32, oc32 = b32.solve(tol, 400)
+            capi.device_synchronize()
+            times32.append(time.perf_counter() - t1)
+        if not np.all(oc32 == capi.CONVERGED):
+            raise SystemExit("batched mixed-precision solve did not converge")
+        d32 = max(float(np.max(np.abs(b32.get_phi(k) - b.get_phi(k))) / max(np.max(np.abs(b.get_phi(k))), 1e-300))
+                  for k in range(0, n, max(1, n // 64)))
+        mixed_line = {"solves_per_s": n / min(times32), "batch_s": min(times32),
+                      "v_cycles_mean": 2 * float(np.mean(it32)), "v_cycles_max": 2 * int(np.max(it32)),
+                      "max_rel_diff_vs_fp64_batch": d32}
+        b32.close()
+        print(f"# batched mixed: solve {min(times32):.3f} s", file=sys.stderr, flush=True)
     print(f"# batched: {n} problems, setup {setup_s:.2f} s, solve {min(times):.3f} s, "
           f"{2 * float(np.mean(iters)):.1f} V-cycles mean, {cycles} max", file=sys.stderr, flush=True)
     # plain FP64 path on a sample, one problem at a time (hierarchy set-up not timed)
@@ -141,6 +165,8 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2,
         "v_cycles_mean": 2 * float(np.mean(iters)),
         "v_cycles_max": cycles,
         "max_rel_diff_batched_vs_plain": max(diffs),
+        "batched_mixed": mixed_line,
+        "batched_mixed_over_plain": (mixed_line["solves_per_s"] / plain) if mixed_line else None,
     }
 
 

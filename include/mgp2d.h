@@ -271,7 +271,7 @@ int mgp_device_synchronize(void);
 typedef struct mgp_batch mgp_batch;
 
 /* n problems with host metrics (mgp_geo layouts) and BCs (Dirichlet or Neumann sides,
- * homogeneous).  opts: method MGP_METHOD_KRYLOV_MG, homog 1; smoothing counts, maxdepth and
+ * homogeneous).  opts: method MGP_METHOD_KRYLOV_MG or _KRYLOV_MG32, homog 1; smoothing counts, maxdepth and
  * use_graph as for mgp_solver_create.  Each problem keeps its own hierarchy in HBM. */
 int mgp_batch_create(mgp_batch **b, int n, const mgp_geo *geos, const mgp_bc *bcs, const mgp_solver_opts *opts);
 /* As mgp_batch_create, with problem k's metrics generated on the GPU from maps[k] on an

@@ -450,10 +450,11 @@ class Batch:
     Neumann sides; homogeneous)."""
 
     def __init__(self, geos, bcs, mappings=None, smooth_down=2, smooth_up=2, smooth_bottom=2, maxdepth=-1,
-                 use_graph=True):
+                 use_graph=True, method=METHOD_KRYLOV_MG):
+        """method: METHOD_KRYLOV_MG, or METHOD_KRYLOV_MG32 (FP32 preconditioner cycles)."""
         n = len(geos)
         self.shapes = [(g.ny, g.nx) for g in geos]
-        opts = SolverOpts(METHOD_KRYLOV_MG, ORDER_MC, 1, smooth_down, smooth_up, smooth_bottom, maxdepth, 1,
+        opts = SolverOpts(method, ORDER_MC, 1, smooth_down, smooth_up, smooth_bottom, maxdepth, 1,
                           int(use_graph))
         a = _Args()
         bc_arr = (_Bc * n)()
@@ -519,10 +520,10 @@ class Server:
     (phi, iterations, outcome) once finished, else None."""
 
     def __init__(self, geos, bcs, mappings, hbm_budget=0.0, smooth_down=2, smooth_up=2, smooth_bottom=2,
-                 use_graph=True):
+                 use_graph=True, method=METHOD_KRYLOV_MG):
         n = len(geos)
         self.shapes = [(g.ny, g.nx) for g in geos]
-        opts = SolverOpts(METHOD_KRYLOV_MG, ORDER_MC, 1, smooth_down, smooth_up, smooth_bottom, -1, 1,
+        opts = SolverOpts(method, ORDER_MC, 1, smooth_down, smooth_up, smooth_bottom, -1, 1,
                           int(use_graph))
         a = _Args()
         bc_arr = (_Bc * n)()

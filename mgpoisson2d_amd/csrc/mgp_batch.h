@@ -28,6 +28,12 @@ struct BLev {
     double slo, shi;    // ghost signs of the y sides in the prolongation
     int prob, nt;       // problem index; x-line chunks (nx / C)
     int sweeps;         // coarsest level of its problem: zebra sweeps there (else 0)
+    // FP32 copies for the mixed-precision cycle (null otherwise): metrics,
+    // stencil table, correction and right-hand side (one pair: the two
+    // applications of an iteration run one after the other)
+    GeoF geo32;
+    const float *K32;
+    float *e32, *f32;
 };
 
 // Krylov vectors of one problem (finest level, level layout).
@@ -60,14 +66,15 @@ struct BTile {
 // Level kernels (mgp_xline.hip).  `active` is indexed by problem; sel picks
 // the e/f pair of the finest level.
 // k >= 0: sweep k of a level's coarsest-level sweeps (levels with fewer return)
+// fp32: the FP32 fields of the levels (e32/f32, geo32, K32) instead of e[sel]/f[sel]
 void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
-                        int C, int threads, size_t lds, int k);
+                        int C, int threads, size_t lds, int k, bool fp32);
 // coarse (lev+1) r = 1x2 restriction of f - L e on lev, coarse e = 0; tiles over the coarse cells
 void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
-                           int sel);
+                           int sel, bool fp32);
 // e(lev) += linear-in-y prolongation of e(lev+1); tiles over the fine cells
 void launch_prolong_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
-                          int sel);
+                          int sel, bool fp32);
 
 // The batched solver.
 class BatchKMG {
@@ -162,6 +169,7 @@ class BatchKMG {
     TableSet part;               // the problems still active, once they are at most half of `full`
     TableSet *T = &full;         // the set the launches use
     std::vector<int> Cp, NTp;    // chunk length and threads per row of each problem's line solves
+    bool mixed = false;          // FP32 preconditioner cycle (method KRYLOV_MG32)
     bool use_graph = true;
 };
 

@@ -144,6 +144,34 @@ __global__ void __launch_bounds__(256) k_b_copy(const BTile *tiles, const BProb 
     p->v[dst][c] = src >= 0 ? p->v[src][c] : 0.0;
 }
 
+// Start of an FP32 cycle: the finest level's f32 = (float) v[P or S], e32 = 0.
+__global__ void __launch_bounds__(256) k_b_begin32(const BTile *tiles, const BProb *P, const BLev *L,
+                                                   const int *mask, int sel) {
+    const BTile t = tiles[blockIdx.x];
+    if (!mask[t.item]) return;
+    const BProb *p = P + t.item;
+    const Grid g = p->g;
+    int i, j;
+    if (!tile_cell(t, g, i, j)) return;
+    const size_t c = ix(g.pitch, i, j);
+    const BLev *l = L + p->lev0;
+    l->f32[c] = (float)p->v[sel ? BV_S : BV_P][c];
+    l->e32[c] = 0.0f;
+}
+
+// End of an FP32 cycle: v[PHAT or SHAT] = (double) e32 of the finest level.
+__global__ void __launch_bounds__(256) k_b_end32(const BTile *tiles, const BProb *P, const BLev *L, const int *mask,
+                                                 int sel) {
+    const BTile t = tiles[blockIdx.x];
+    if (!mask[t.item]) return;
+    const BProb *p = P + t.item;
+    const Grid g = p->g;
+    int i, j;
+    if (!tile_cell(t, g, i, j)) return;
+    const size_t c = ix(g.pitch, i, j);
+    p->v[sel ? BV_SHAT : BV_PHAT][c] = (double)L[p->lev0].e32[c];
+}
+
 size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 void tiles_of(std::vector<BTile> &out, int item, int nx, int ny) {
@@ -169,6 +197,7 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
         g_last_error = "batch: at least one problem";
         throw Failure(MGP_ERR_ARG);
     }
+    mixed = o.method == MGP_METHOD_KRYLOV_MG32;
     sd = o.smooth_down > 0 ? o.smooth_down : 1;
     su = o.smooth_up > 0 ? o.smooth_up : 1;
     sb = o.smooth_bottom > 0 ? o.smooth_bottom : 1;
@@ -192,14 +221,15 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
     // level counts and the arena
     nlev.resize(n);
     size_t bytes = 0;
-    auto take = [&](size_t elems) {
+    auto take = [&](size_t elems, size_t elem = sizeof(double)) {
         const size_t off = bytes;
-        bytes += round256(elems * sizeof(double));
+        bytes += round256(elems * elem);
         return off;
     };
     struct Off {
         size_t v[BV_COUNT], partial;
         std::vector<size_t> J, Jxx, Jxy, Jyx, Jyy, e, r, K;
+        std::vector<size_t> f32m[4], K32, e32, r32;  // FP32 copies (mixed precision)
     };
     std::vector<Off> off(n);
     for (int p = 0; p < n; ++p) {
@@ -221,6 +251,12 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             O.r.push_back(lev ? take(nl) : 0);
             const Grid gl = make_grid(q.nx, ny, q.dx, q.dy);
             O.K.push_back(take((size_t)ring_table_elems(gl)));
+            if (mixed) {
+                for (int m = 0; m < 4; ++m) O.f32m[m].push_back(take(nl, sizeof(float)));
+                O.K32.push_back(take((size_t)ring_table_elems(gl), sizeof(float)));
+                O.e32.push_back(take(nl, sizeof(float)));
+                O.r32.push_back(take(nl, sizeof(float)));
+            }
             ++lev;
             if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || lev >= maxlev) break;
             ny /= 2;
@@ -232,6 +268,7 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
     scal = static_cast<double *>(dev_alloc_zero((size_t)n * SC_COUNT * sizeof(double)));
     char *base = static_cast<char *>(arena);
     auto at = [&](size_t o) { return reinterpret_cast<double *>(base + o); };
+    auto atf = [&](size_t o) { return reinterpret_cast<float *>(base + o); };
 
     // problems, levels, metrics and stencil tables
     P.resize(n);
@@ -288,6 +325,20 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             }
             launch_ring_coef(s, at(O.K[d]), l.g, l.geo, lbc);
             l.K = at(O.K[d]);
+            l.geo32 = GeoF{nullptr, nullptr, nullptr, nullptr, nullptr};
+            l.K32 = nullptr;
+            l.e32 = l.f32 = nullptr;
+            if (mixed) {
+                // FP32 copies of the whole padded arrays (formed in FP64, rounded once)
+                const size_t nl = level_elems(q.nx, ny);
+                const double *src[4] = {Jxx, Jxy, Jyx, Jyy};
+                for (int m = 0; m < 4; ++m) launch_convert_flat(s, atf(O.f32m[m][d]), src[m], nl);
+                launch_convert_flat(s, atf(O.K32[d]), l.K, (size_t)ring_table_elems(l.g));
+                l.geo32 = GeoF{nullptr, atf(O.f32m[0][d]), atf(O.f32m[1][d]), atf(O.f32m[2][d]), atf(O.f32m[3][d])};
+                l.K32 = atf(O.K32[d]);
+                l.e32 = atf(O.e32[d]);
+                l.f32 = atf(O.r32[d]);
+            }
             l.slo = B.bc.type[YLO] == BC_DIRI ? -1.0 : 1.0;
             l.shi = B.bc.type[YHI] == BC_DIRI ? -1.0 : 1.0;
             l.prob = p;
@@ -359,7 +410,7 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
             auto add = [&](int colour, XlTask t) {
                 const auto key = std::make_tuple(set, colour, C, NT);
                 tabs[key].push_back(t);
-                lds[key] = std::max(lds[key], xline_lds_bytes_for(nx, C, sizeof(double), NT));
+                lds[key] = std::max(lds[key], xline_lds_bytes_for(nx, C, mixed ? sizeof(float) : sizeof(double), NT));
             };
             // the order of xline_zebra: even interior rows, odd interior rows
             // (the first and last relaxing rows 0 and ny-1 after their own),
@@ -445,7 +496,7 @@ void BatchKMG::zebra(int d, int set, int sel, int k) {
         for (const XlTable &x : T->xl)
             if (x.d == d && x.set == set && x.colour == colour)
                 launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.threads, x.t.lds,
-                                   k);
+                                   k, mixed);
 }
 
 // e = M f for the pair `sel` of every active problem (phat from p: 0, shat
@@ -453,18 +504,20 @@ void BatchKMG::zebra(int d, int set, int sel, int k) {
 // Problems whose coarsest level is d make their bottom sweeps there while
 // the others go on down.
 void BatchKMG::precondition(int sel) {
-    k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
+    if (mixed) k_b_begin32<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, dL, active, sel);
+    else k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
     const int depth = T->depth;
     for (int d = 0; d < depth; ++d) {
         for (int k = 0; k < sd; ++k) zebra(d, SET_MID, sel, -1);
         // each problem ending at level d makes its own number of sweeps there
         for (int k = 0; k < T->bottom[d]; ++k) zebra(d, SET_BOTTOM, sel, k);
-        if (d + 1 < depth) launch_restrict_batch(s, (const BTile *)T->tr[d].p, T->tr[d].n, dL, active, sel);
+        if (d + 1 < depth) launch_restrict_batch(s, (const BTile *)T->tr[d].p, T->tr[d].n, dL, active, sel, mixed);
     }
     for (int d = depth - 2; d >= 0; --d) {
-        launch_prolong_batch(s, (const BTile *)T->tp[d].p, T->tp[d].n, dL, active, sel);
+        launch_prolong_batch(s, (const BTile *)T->tp[d].p, T->tp[d].n, dL, active, sel, mixed);
         for (int k = 0; k < su; ++k) zebra(d, SET_MID, sel, -1);
     }
+    if (mixed) k_b_end32<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, dL, active, sel);
 }
 
 void BatchKMG::reduce(int a, int b, int slot, const int *m) {
@@ -573,8 +626,8 @@ void BatchKMG::start(const std::vector<int> &which, const std::vector<int> &warm
 // finished in this step.
 std::vector<int> BatchKMG::step() {
     const int n = count();
-    // iterative refinement factor of the FP64 cycle (as KrylovMG::solve)
-    const double refine = 1e-6;
+    // iterative refinement factors (as KrylovMG::solve)
+    const double refine = mixed ? 1e-3 : 1e-6;
     std::vector<int> done, msk(n, 0);
     if (T->graph) hip_check(hipGraphLaunch(T->graph, s), "graph launch");
     else body();

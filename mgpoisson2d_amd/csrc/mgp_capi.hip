@@ -971,8 +971,8 @@ void batch_problems(int n, const mgp_bc *bcs, const mgp_solver_opts *opts,
     }
     if (opts) o = *opts;
     else mgp_solver_default_opts(&o);
-    if (o.method != MGP_METHOD_KRYLOV_MG || !o.homog) {
-        g_last_error = "batch: method KRYLOV_MG with homogeneous BCs (homog = 1)";
+    if ((o.method != MGP_METHOD_KRYLOV_MG && o.method != MGP_METHOD_KRYLOV_MG32) || !o.homog) {
+        g_last_error = "batch: method KRYLOV_MG or KRYLOV_MG32 with homogeneous BCs (homog = 1)";
         throw Failure(MGP_ERR_UNSUPPORTED);
     }
     probs.assign(n, BatchKMG::Problem());

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 
 #include "mgp_batch.h"
 #include "mgp_host.h"
@@ -650,55 +651,73 @@ __global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const
 // Batched forms: the workgroup's task or tile names the level (and through
 // it the problem); workgroups of inactive problems return at once.
 // ---------------------------------------------------------------------------
-template <int C, int NT>
+// The level's fields of the cycle's element type.
+__device__ __forceinline__ void level_fields(const BLev *L, int sel, double *&e, const double *&f, const double *&K,
+                                             Geo &geo) {
+    e = L->e[sel];
+    f = L->f[sel];
+    K = L->K;
+    geo = L->geo;
+}
+__device__ __forceinline__ void level_fields(const BLev *L, int, float *&e, const float *&f, const float *&K,
+                                             GeoF &geo) {
+    e = L->e32;
+    f = L->f32;
+    K = L->K32;
+    geo = L->geo32;
+}
+
+template <int C, int NT, typename T>
 __device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
     const XlTask t = tasks[launch_slot()];
     const BLev *L = levs + t.lev;
     if (!active[L->prob] || (k >= 0 && k >= L->sweeps)) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_bytes[];
-    double *lds = reinterpret_cast<double *>(lds_bytes);
+    T *lds = reinterpret_cast<T *>(lds_bytes);
     const Grid g = L->g;
-    const Geo geo = L->geo;
-    double *e = L->e[sel];
-    const double *f = L->f[sel], *K = L->K;
-    const RowCtx<double> X0 = row_ctx<double>(g, L->nt, C);
+    T *e;
+    const T *f, *K;
+    typename std::conditional<sizeof(T) == 8, Geo, GeoF>::type geo;
+    level_fields(L, sel, e, f, K, geo);
+    const RowCtx<T> X0 = row_ctx<T>(g, L->nt, C);
 #pragma unroll 1
     for (int r = 0; r < 2; ++r) {
         const int jr = r == 0 ? t.row : t.extra;
         if (jr < 0) continue;  // uniform over the workgroup
         if (r) __syncthreads();
-        RowCtx<double> X = X0;
+        RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
         xline_row<C, NT>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
-template <int C>
-__global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+template <int C, typename T>
+__global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(xl_waves<T>(), xl_waves<T>())))
 k_xline_batch(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
-    xline_task<C, XL_THREADS>(tasks, levs, active, sel, k);
+    xline_task<C, XL_THREADS, T>(tasks, levs, active, sel, k);
 }
 
-template <int C>
+template <int C, typename T>
 __global__ void __launch_bounds__(XL_THREADS) k_xline_batch_long(const XlTask *tasks, const BLev *levs,
                                                                  const int *active, int sel, int k) {
-    xline_task<C, XL_THREADS>(tasks, levs, active, sel, k);
+    xline_task<C, XL_THREADS, T>(tasks, levs, active, sel, k);
 }
 
 // Short rows (nx <= 512): one wave per row.
-template <int C>
+template <int C, typename T>
 __global__ void __launch_bounds__(XL_SHORT) __attribute__((amdgpu_waves_per_eu(2)))
 k_xline_batch_short(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
-    xline_task<C, XL_SHORT>(tasks, levs, active, sel, k);
+    xline_task<C, XL_SHORT, T>(tasks, levs, active, sel, k);
 }
 
 // Rows of 513..1024 cells: two waves per row.
-template <int C>
+template <int C, typename T>
 __global__ void __launch_bounds__(XL_MID) __attribute__((amdgpu_waves_per_eu(2)))
 k_xline_batch_mid(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
-    xline_task<C, XL_MID>(tasks, levs, active, sel, k);
+    xline_task<C, XL_MID, T>(tasks, levs, active, sel, k);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, const BLev *levs, const int *active,
                                                         int sel) {
     const BTile t = tiles[blockIdx.x];
@@ -707,10 +726,16 @@ __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, cons
     const int i = t.tx * 64 + (threadIdx.x & 63), J = t.ty * 4 + (threadIdx.x >> 6);
     const Grid gc = Cl->g;
     if (i >= gc.nx || J >= gc.ny) return;
-    restrict_cell(Cl->f[0], Cl->e[0], (const double *)F->e[sel], (const double *)F->f[sel], F->K, F->g, gc, F->geo,
-                  i, J);
+    T *fe, *ce, *cf;
+    const T *ff, *K, *cff, *cK;
+    typename std::conditional<sizeof(T) == 8, Geo, GeoF>::type geo, cgeo;
+    level_fields(F, sel, fe, ff, K, geo);
+    level_fields(Cl, 0, ce, cff, cK, cgeo);
+    cf = const_cast<T *>(cff);
+    restrict_cell(cf, ce, (const T *)fe, ff, K, F->g, gc, geo, i, J);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const BLev *levs, const int *active,
                                                        int sel) {
     const BTile t = tiles[blockIdx.x];
@@ -719,7 +744,12 @@ __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const
     const int i = t.tx * 64 + (threadIdx.x & 63), j = t.ty * 4 + (threadIdx.x >> 6);
     const Grid gf = F->g, gc = Cl->g;
     if (i >= gf.nx || j >= gf.ny) return;
-    prolong_y_cell(F->e[sel], (const double *)Cl->e[0], gf, gc, F->slo, F->shi, i, j);
+    T *fe, *ce;
+    const T *ff, *K, *cf, *cK;
+    typename std::conditional<sizeof(T) == 8, Geo, GeoF>::type geo, cgeo;
+    level_fields(F, sel, fe, ff, K, geo);
+    level_fields(Cl, 0, ce, cf, cK, cgeo);
+    prolong_y_cell(fe, (const T *)ce, gf, gc, (T)F->slo, (T)F->shi, i, j);
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
@@ -803,14 +833,14 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
     xline(s, e, f, K, g, geo, row0, rowstep, nrows, extra0, extra1);
 }
 
-void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
+template <typename T>
+static void xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
                         int C, int threads, size_t lds, int k) {
-    if (ntasks <= 0) return;
     if (threads == XL_SHORT) {
         switch (C) {
-            case 2: k_xline_batch_short<2><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
-            case 4: k_xline_batch_short<4><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
-            case 8: k_xline_batch_short<8><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 2: k_xline_batch_short<2, T><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 4: k_xline_batch_short<4, T><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 8: k_xline_batch_short<8, T><<<ntasks, XL_SHORT, lds, s>>>(tasks, levs, active, sel, k); break;
             default:
                 g_last_error = "k_xline_batch_short: chunk length must be 2, 4 or 8";
                 throw Failure(MGP_ERR_ARG);
@@ -820,16 +850,23 @@ void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BL
             g_last_error = "k_xline_batch_mid: chunk length must be 8";
             throw Failure(MGP_ERR_ARG);
         }
-        k_xline_batch_mid<8><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k);
+        k_xline_batch_mid<8, T><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k);
     } else switch (C) {
-        case 2: k_xline_batch<2><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
-        case 4: k_xline_batch<4><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
-        case 8: k_xline_batch<8><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
-        case 16: k_xline_batch_long<16><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 2: k_xline_batch<2, T><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 4: k_xline_batch<4, T><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 8: k_xline_batch<8, T><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
+        case 16: k_xline_batch_long<16, T><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
         default:
             g_last_error = "k_xline_batch: chunk length must be 2, 4, 8 or 16";
             throw Failure(MGP_ERR_ARG);
     }
+}
+
+void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BLev *levs, const int *active, int sel,
+                        int C, int threads, size_t lds, int k, bool fp32) {
+    if (ntasks <= 0) return;
+    if (fp32) xline_batch<float>(s, tasks, ntasks, levs, active, sel, C, threads, lds, k);
+    else xline_batch<double>(s, tasks, ntasks, levs, active, sel, C, threads, lds, k);
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         g_last_error = std::string("k_xline_batch launch: ") + hipGetErrorString(err);
@@ -837,19 +874,23 @@ void launch_xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BL
     }
 }
 
-void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
-                           int sel) {
-    if (ntiles > 0) k_restrict_batch<<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
-}
-
-void launch_prolong_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
-                          int sel) {
-    if (ntiles > 0) k_prolong_batch<<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
-}
-
 void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
                   int rowstep, int nrows, int extra0, int extra1) {
     xline(s, e, f, K, g, geo, row0, rowstep, nrows, extra0, extra1);
+}
+
+void launch_restrict_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
+                           int sel, bool fp32) {
+    if (ntiles <= 0) return;
+    if (fp32) k_restrict_batch<float><<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
+    else k_restrict_batch<double><<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
+}
+
+void launch_prolong_batch(hipStream_t s, const BTile *tiles, int ntiles, const BLev *levs, const int *active,
+                          int sel, bool fp32) {
+    if (ntiles <= 0) return;
+    if (fp32) k_prolong_batch<float><<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
+    else k_prolong_batch<double><<<ntiles, 256, 0, s>>>(tiles, levs, active, sel);
 }
 
 }  // namespace mgp

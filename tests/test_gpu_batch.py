@@ -1,5 +1,7 @@
 """The batched solver (mgp_batch) on the GPU.
 
+* Both precisions of the preconditioner cycle (KRYLOV_MG, and KRYLOV_MG32
+  with FP32 cycles) meet the same bars below.
 * Every problem of a batch of uneven sizes, anisotropies and BCs reaches the
   requested relative residual of the reference operator, checked with the
   oracle's operator (bit-identical to the reference), and a small one equals
@@ -54,8 +56,9 @@ def problems(cases, seed0=40):
     return out
 
 
-def run_batch(probs, tol=1e-10, use_graph=True):
-    b = mg.Batch([p[0] for p in probs], [p[1] for p in probs], use_graph=use_graph)
+def run_batch(probs, tol=1e-10, use_graph=True, method=None):
+    b = mg.Batch([p[0] for p in probs], [p[1] for p in probs], use_graph=use_graph,
+                 method=mg.METHOD_KRYLOV_MG if method is None else method)
     for k, (_, _, rhs) in enumerate(probs):
         b.set_rhs(k, rhs)
     iters, outc = b.solve(tol, 300)
@@ -65,9 +68,10 @@ def run_batch(probs, tol=1e-10, use_graph=True):
     return phis, iters, outc, cycles
 
 
-def test_batch_reaches_tolerance():
+@pytest.mark.parametrize("method", [mg.METHOD_KRYLOV_MG, mg.METHOD_KRYLOV_MG32])
+def test_batch_reaches_tolerance(method):
     probs = problems(CASES)
-    phis, iters, outc, cycles = run_batch(probs)
+    phis, iters, outc, cycles = run_batch(probs, method=method)
     assert np.all(outc == mg.CONVERGED), (iters, outc)
     assert cycles == 2 * int(iters.max())
     for (geo, bc, rhs), phi in zip(probs, phis):
@@ -85,19 +89,21 @@ def test_batch_matches_direct_solve():
         np.testing.assert_allclose(phi, exact, rtol=0, atol=1e-10 * np.max(np.abs(exact)))
 
 
-def test_batch_alone_equals_in_mix():
+@pytest.mark.parametrize("method", [mg.METHOD_KRYLOV_MG, mg.METHOD_KRYLOV_MG32])
+def test_batch_alone_equals_in_mix(method):
     probs = problems(CASES)
-    phis, iters, outc, _ = run_batch(probs)
+    phis, iters, outc, _ = run_batch(probs, method=method)
     for k in (0, 2, 5, 7):
-        alone, it1, oc1, _ = run_batch([probs[k]])
+        alone, it1, oc1, _ = run_batch([probs[k]], method=method)
         assert (int(it1[0]), int(oc1[0])) == (int(iters[k]), int(outc[k]))
         np.testing.assert_array_equal(alone[0], phis[k])
 
 
-def test_batch_graph_matches_eager():
+@pytest.mark.parametrize("method", [mg.METHOD_KRYLOV_MG, mg.METHOD_KRYLOV_MG32])
+def test_batch_graph_matches_eager(method):
     probs = problems(CASES[:4], seed0=70)
-    g_phis, g_it, _, _ = run_batch(probs, use_graph=True)
-    e_phis, e_it, _, _ = run_batch(probs, use_graph=False)
+    g_phis, g_it, _, _ = run_batch(probs, use_graph=True, method=method)
+    e_phis, e_it, _, _ = run_batch(probs, use_graph=False, method=method)
     np.testing.assert_array_equal(g_it, e_it)
     for a, b in zip(g_phis, e_phis):
         np.testing.assert_array_equal(a, b)
