@@ -171,7 +171,7 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2,
 
 
 def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_requests=200, tol=1e-10,
-                  budget=200e9, seed=SEED + 5):
+                  budget=200e9, seed=SEED + 5, mixed=True):
     """Continuous batching: n_small resident geometries from the batched mix plus one
     big_n^2 (beta = 6) geometry that takes big_share of the requests; each request is the
     slot's base rhs plus a 5 % perturbation, warm-started from the slot's last solution.
@@ -188,7 +188,8 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
     geos = [SimpleNamespace(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny) for nx, ny, *_ in probs]
     maps = [capi.Mapping(capi.MAP_WARPED, amp, beta, 0.0, 0.0) for _, _, amp, beta, _ in probs]
     bcs = [G.BoundaryConditions.uniform(t) for *_, t in probs]
-    v = capi.Server(geos, bcs, maps, hbm_budget=budget)
+    v = capi.Server(geos, bcs, maps, hbm_budget=budget,
+                    method=capi.METHOD_KRYLOV_MG32 if mixed else capi.METHOD_KRYLOV_MG)
     big = len(probs) - 1
 
     def draw():
@@ -255,8 +256,8 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
     return {
         "workload": f"{n_small} resident geometries from the batched mix plus one {big_n}^2 beta=6 geometry "
                     f"({100 * big_share:.0f} % of requests); rhs = base + 5 % perturbation, warm start from the "
-                    f"slot's last solution, FP64 to {tol:g} relative residual; Poisson arrivals at {load} x the "
-                    f"closed-loop capacity",
+                    f"slot's last solution, {'FP32 cycles in FP64 BiCGStab' if mixed else 'FP64'} to {tol:g} "
+                    f"relative residual; Poisson arrivals at {load} x the closed-loop capacity",
         "capacity_solves_per_s": capacity,
         "offered_per_s": rate,
         "served_per_s": n_requests / span,
@@ -339,7 +340,7 @@ def main():
         print(json.dumps(batched_bench(capi, args.batch_n, use_graph=not args.no_graph)), flush=True)
         return
     if args.serving_only:
-        print(json.dumps(serving_bench(capi)), flush=True)
+        print(json.dumps({"mixed": serving_bench(capi), "fp64": serving_bench(capi, mixed=False)}), flush=True)
         return
     geo, bc, rhs = config2_problem(args.n, args.n)
     main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
