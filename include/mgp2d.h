@@ -310,8 +310,12 @@ double mgp_server_footprint(const mgp_server *v);
  * within maxiters iterations, from phi = 0 or (warm != 0) from the slot's last solution. */
 int mgp_server_submit(mgp_server *v, int slot, const double *rhs, double tol, int maxiters, int warm,
                       long long *ticket);
-/* Admits the oldest queued request of every idle slot and advances every running solve by one
- * iteration; *running / *queued (may be NULL) receive the solves on the device and waiting. */
+/* Slots are grouped by size (up to 2^22 cells, and larger), each group a batch on a stream of
+ * its own, so the groups' work overlaps and a group advances at its own pace.  A step takes the
+ * results of every group whose iteration has finished, admits the oldest queued request of each
+ * of its idle slots and launches its next iteration; it returns once at least one group has
+ * finished an iteration (or none has work).  *running / *queued (may be NULL) receive the solves
+ * on the device and waiting. */
 int mgp_server_step(mgp_server *v, int *running, int *queued);
 /* *done = 1 and phi (nx x ny), *iters, *outcome when the request has finished (its result is
  * then released), else *done = 0.  A result does not depend on the other requests in flight. */

@@ -105,6 +105,12 @@ class BatchKMG {
     void start(const std::vector<int> &which, const std::vector<int> &warm, const std::vector<double> &tol,
                const std::vector<int> &maxit);
     std::vector<int> step();
+    // step() in two halves, for callers that overlap several batches: the
+    // launch, then (once step_done()) the checks; finish_step syncs only for
+    // the true-residual checks it makes.
+    void launch_step();
+    bool step_done() const;
+    std::vector<int> finish_step();
     int running() const;
     bool is_running(int k) const { return act_[k] != 0; }
     int iterations(int k) const { return it_[k]; }
@@ -182,9 +188,10 @@ class BatchKMG {
 // else is running (the batch keeps problems independent).
 class Server {
   public:
-    Server(hipStream_t s, const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget);
-    int slots() const { return B.count(); }
-    size_t footprint() const { return B.footprint(); }
+    Server(const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget);
+    ~Server();
+    int slots() const { return (int)where.size(); }
+    size_t footprint() const;
     long long submit(int slot, const double *rhs, double tol, int maxiters, int warm);
     // admits and advances; returns the solves on the device after the step
     int step();
@@ -204,8 +211,19 @@ class Server {
         std::vector<double> phi;
         int iters, outcome;
     };
-    hipStream_t s;
-    BatchKMG B;
+    // Slots are grouped by size into classes, each a batch on a stream of
+    // its own: a class advances at its own pace, and the classes' work
+    // overlaps on the device.
+    struct Class {
+        hipStream_t s = nullptr;
+        BatchKMG *B = nullptr;
+        std::vector<int> slot;   // global slot of each local problem
+        bool in_flight = false;
+    };
+    void admit(Class &c, int cls);
+    void retire(Class &c, const std::vector<int> &finished);
+    std::vector<Class> cls;
+    std::vector<std::pair<int, int>> where;  // per slot: (class, local index)
     std::vector<Req> queue;
     std::vector<long long> owner;  // per slot: running ticket, or 0
     std::vector<std::pair<long long, Res>> done;

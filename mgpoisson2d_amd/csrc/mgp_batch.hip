@@ -19,6 +19,7 @@
 #include <cmath>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 
 #include "mgp_batch.h"
@@ -625,13 +626,27 @@ void BatchKMG::start(const std::vector<int> &which, const std::vector<int> &warm
 // the refinement factor: restart from it).  Returns the problems that
 // finished in this step.
 std::vector<int> BatchKMG::step() {
+    launch_step();
+    hip_check(hipStreamSynchronize(s), "sync");
+    return finish_step();
+}
+
+// The iteration and the copy of the scalars to the host, on the stream.
+void BatchKMG::launch_step() {
+    if (T->graph) hip_check(hipGraphLaunch(T->graph, s), "graph launch");
+    else body();
+    hip_check(hipMemcpyAsync(host_sc, scal, (size_t)count() * SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s),
+              "scalars");
+}
+
+bool BatchKMG::step_done() const { return hipStreamQuery(s) == hipSuccess; }
+
+// After launch_step has completed: the convergence checks (and restarts).
+std::vector<int> BatchKMG::finish_step() {
     const int n = count();
     // iterative refinement factors (as KrylovMG::solve)
     const double refine = mixed ? 1e-3 : 1e-6;
     std::vector<int> done, msk(n, 0);
-    if (T->graph) hip_check(hipGraphLaunch(T->graph, s), "graph launch");
-    else body();
-    fetch_scalars();
     bool check = false;
     for (int p = 0; p < n; ++p) {
         if (!act_[p]) continue;
@@ -712,22 +727,67 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
 // ---------------------------------------------------------------------------
 // Serving loop
 // ---------------------------------------------------------------------------
-Server::Server(hipStream_t s_, const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget)
-    : s(s_), B(s_, slots, o) {
-    if (budget > 0.0 && (double)B.footprint() > budget) {
-        g_last_error = "server: the resident hierarchies need " + std::to_string(B.footprint()) +
+// Size classes of the serving loop: slots of up to 2^22 cells, and larger ones.
+static int size_class(const BatchKMG::Problem &q) { return (long long)q.nx * q.ny > (1LL << 22) ? 1 : 0; }
+
+Server::Server(const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget) {
+    where.resize(slots.size());
+    std::vector<std::vector<BatchKMG::Problem>> per(2);
+    std::vector<std::vector<int>> ids(2);
+    for (size_t k = 0; k < slots.size(); ++k) {
+        const int c = size_class(slots[k]);
+        ids[c].push_back((int)k);
+        per[c].push_back(slots[k]);
+    }
+    try {
+        for (int c = 0; c < 2; ++c) {
+            if (per[c].empty()) continue;
+            Class C;
+            hip_check(hipStreamCreateWithFlags(&C.s, hipStreamNonBlocking), "hipStreamCreate");
+            cls.push_back(C);
+            cls.back().B = new BatchKMG(cls.back().s, per[c], o);
+            cls.back().slot = ids[c];
+            for (size_t q = 0; q < ids[c].size(); ++q) where[ids[c][q]] = {(int)cls.size() - 1, (int)q};
+        }
+    } catch (...) {
+        for (Class &C : cls) {
+            delete C.B;
+            if (C.s) (void)hipStreamDestroy(C.s);
+        }
+        throw;
+    }
+    if (budget > 0.0 && (double)footprint() > budget) {
+        g_last_error = "server: the resident hierarchies need " + std::to_string(footprint()) +
                        " bytes of HBM, over the budget of " + std::to_string((long long)budget);
+        for (Class &C : cls) {
+            delete C.B;
+            if (C.s) (void)hipStreamDestroy(C.s);
+        }
         throw Failure(MGP_ERR_NOMEM);
     }
     owner.assign(slots.size(), 0);
 }
 
+Server::~Server() {
+    for (Class &C : cls) {
+        if (C.s) (void)hipStreamSynchronize(C.s);
+        delete C.B;
+        if (C.s) (void)hipStreamDestroy(C.s);
+    }
+}
+
+size_t Server::footprint() const {
+    size_t b = 0;
+    for (const Class &C : cls) b += C.B->footprint();
+    return b;
+}
+
 long long Server::submit(int slot, const double *rhs, double tol, int maxiters, int warm) {
-    if (slot < 0 || slot >= B.count() || !rhs || !(tol >= 0.0) || maxiters < 0) {
+    if (slot < 0 || slot >= slots() || !rhs || !(tol >= 0.0) || maxiters < 0) {
         g_last_error = "server submit: bad slot, rhs, tol or maxiters";
         throw Failure(MGP_ERR_ARG);
     }
-    const Grid &g = B.grid(slot);
+    const Grid &g = cls[where[slot].first].B->grid(where[slot].second);
     Req r;
     r.ticket = next++;
     r.slot = slot;
@@ -739,47 +799,81 @@ long long Server::submit(int slot, const double *rhs, double tol, int maxiters, 
     return queue.back().ticket;
 }
 
-int Server::step() {
-    // admit: the oldest queued request of every idle slot
+// The oldest queued request of every idle slot of class c starts.
+void Server::admit(Class &c, int ci) {
     std::vector<int> which, warm, maxit;
     std::vector<double> tol;
     for (size_t q = 0; q < queue.size();) {
         Req &r = queue[q];
-        if (owner[r.slot] != 0) {
+        if (where[r.slot].first != ci || owner[r.slot] != 0) {
             ++q;
             continue;
         }
-        const Grid &g = B.grid(r.slot);
-        upload_block(s, B.vec(r.slot, BV_RHS), g, r.rhs.data(), 0, 0, g.nx, g.ny);
+        const int local = where[r.slot].second;
+        const Grid &g = c.B->grid(local);
+        upload_block(c.s, c.B->vec(local, BV_RHS), g, r.rhs.data(), 0, 0, g.nx, g.ny);
         owner[r.slot] = r.ticket;
-        which.push_back(r.slot);
+        which.push_back(local);
         warm.push_back(r.warm);
         tol.push_back(r.tol);
         maxit.push_back(r.maxiters);
         queue.erase(queue.begin() + (long)q);
     }
-    std::vector<int> finished;
-    if (!which.empty()) {
-        B.start(which, warm, tol, maxit);
-        for (int p : which)
-            if (!B.is_running(p)) finished.push_back(p);  // met the tolerance at the start
-    }
-    if (B.running() > 0) {
-        const std::vector<int> f = B.step();
-        finished.insert(finished.end(), f.begin(), f.end());
-    }
+    if (which.empty()) return;
+    c.B->start(which, warm, tol, maxit);
+    std::vector<int> at_once;
+    for (int p : which)
+        if (!c.B->is_running(p)) at_once.push_back(p);  // met the tolerance at the start
+    retire(c, at_once);
+}
+
+void Server::retire(Class &c, const std::vector<int> &finished) {
     for (int p : finished) {
-        const Grid &g = B.grid(p);
+        const Grid &g = c.B->grid(p);
         Res r;
         r.phi.resize((size_t)g.nx * g.ny);
-        download_block(s, r.phi.data(), B.vec(p, BV_PHI), g, 0, 0, g.nx, g.ny);
-        r.iters = B.iterations(p);
-        r.outcome = B.outcome(p);
-        done.emplace_back(owner[p], std::move(r));
-        owner[p] = 0;
+        download_block(c.s, r.phi.data(), c.B->vec(p, BV_PHI), g, 0, 0, g.nx, g.ny);
+        r.iters = c.B->iterations(p);
+        r.outcome = c.B->outcome(p);
+        const int slot = c.slot[p];
+        done.emplace_back(owner[slot], std::move(r));
+        owner[slot] = 0;
     }
-    hip_check(hipStreamSynchronize(s), "server step");
-    return B.running();
+    if (!finished.empty()) hip_check(hipStreamSynchronize(c.s), "server results");
+}
+
+int Server::step() {
+    // Every class whose iteration has finished takes its results, admits
+    // queued requests and launches its next iteration; a class whose
+    // iteration still runs is left alone.  Returns once at least one class
+    // has finished an iteration (or none has work).
+    bool progressed = false;
+    while (true) {
+        bool any_in_flight = false;
+        for (size_t ci = 0; ci < cls.size(); ++ci) {
+            Class &c = cls[ci];
+            if (c.in_flight) {
+                if (!c.B->step_done()) {
+                    any_in_flight = true;
+                    continue;
+                }
+                c.in_flight = false;
+                progressed = true;
+                retire(c, c.B->finish_step());
+            }
+            admit(c, (int)ci);
+            if (c.B->running() > 0) {
+                c.B->launch_step();
+                c.in_flight = true;
+                any_in_flight = true;
+            }
+        }
+        if (progressed || !any_in_flight) break;
+        std::this_thread::yield();
+    }
+    int running = 0;
+    for (const Class &c : cls) running += c.B->running();
+    return running;
 }
 
 int Server::result(long long ticket, double *phi, int *iters, int *outcome) {

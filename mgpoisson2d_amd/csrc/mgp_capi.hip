@@ -109,12 +109,8 @@ struct mgp_batch {
 };
 
 struct mgp_server {
-    hipStream_t stream = nullptr;
     mgp::Server *srv = nullptr;
-    ~mgp_server() {
-        delete srv;
-        if (stream) (void)hipStreamDestroy(stream);
-    }
+    ~mgp_server() { delete srv; }
 };
 
 struct mgp_solver {
@@ -1118,8 +1114,7 @@ int mgp_server_create(mgp_server **out, int n, const mgp_mapping *maps, const in
             q.dy = dy[k];
             q.map = &maps[k];
         }, probs, o);
-        hip_check(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking), "hipStreamCreate");
-        v->srv = new Server(v->stream, probs, o, hbm_budget);
+        v->srv = new Server(probs, o, hbm_budget);
     });
     if (st != MGP_OK) {
         delete v;

@@ -7,6 +7,8 @@
 * A cold request gives the bits of the batched solver on the same problem.
 * Warm starts use the slot's last solution: the same rhs again finishes at
   once, a perturbed one in fewer iterations.
+* Slots above 2^22 cells run in a batch of their own on a second stream;
+  the results of both size classes stay those of each slot alone.
 * The HBM budget is enforced at creation.
 """
 from types import SimpleNamespace
@@ -30,8 +32,8 @@ def slot_defs(slots):
     return geos, maps, bcs
 
 
-def rhs_for(slot, k):
-    nx, ny = SLOTS[slot][:2]
+def rhs_for(slot, k, slots=SLOTS):
+    nx, ny = slots[slot][:2]
     return np.random.default_rng(1000 * slot + k).standard_normal((ny, nx))
 
 
@@ -48,7 +50,7 @@ def run(schedule, slots=SLOTS):
     while todo or pending:
         while todo and todo[0][0] <= step:
             _, slot, k, warm = todo.pop(0)
-            pending[v.submit(slot, rhs_for(slot, k), 1e-10, 300, warm)] = (slot, k)
+            pending[v.submit(slot, rhs_for(slot, k, slots), 1e-10, 300, warm)] = (slot, k)
         v.step()
         step += 1
         for t in list(pending):
@@ -65,6 +67,22 @@ def test_server_results_independent_of_the_mix():
     assert len(mixed) == len(SCHEDULE)
     for slot in range(len(SLOTS)):
         alone = run([s for s in SCHEDULE if s[1] == slot])
+        for key, (phi, it, oc) in alone.items():
+            phi2, it2, oc2 = mixed[key]
+            assert oc == mg.CONVERGED and oc2 == mg.CONVERGED
+            assert it == it2
+            np.testing.assert_array_equal(phi, phi2)
+
+
+def test_server_size_classes_overlap_and_stay_independent():
+    """A slot above 2^22 cells runs in a second batch on its own stream; the
+    small slots' results and the large one's are those of each alone."""
+    slots = SLOTS[:2] + [(2064, 2048, 0.05, 3.0, (1, 1, 0, 0))]
+    sched = [(0, 2, 0, False), (0, 0, 0, False), (1, 1, 0, False), (2, 0, 1, True), (4, 1, 1, False)]
+    mixed = run(sched, slots)
+    assert len(mixed) == len(sched)
+    for slot in range(len(slots)):
+        alone = run([x for x in sched if x[1] == slot], slots)
         for key, (phi, it, oc) in alone.items():
             phi2, it2, oc2 = mixed[key]
             assert oc == mg.CONVERGED and oc2 == mg.CONVERGED
