@@ -52,3 +52,25 @@ def test_bad_arguments_rejected_without_device_work():
         capi.pnorm(8, 8, 0.125, 0.125, np.zeros((8, 8)), -1)
     with pytest.raises(capi.MgpError):
         capi.metric_fill(capi.Mapping(7, 0.0, 0.0, 0.0, 0.0), 8, 8, 0.125, 0.125)
+
+
+def test_batch_and_server_arguments_rejected_without_device_work():
+    import ctypes
+    import pytest
+    from mgpoisson2d_amd import capi
+    h = ctypes.c_void_p()
+    # no problems, or no BCs: rejected before any HIP call
+    with pytest.raises(capi.MgpError):
+        capi._check(capi.lib.mgp_batch_create(ctypes.byref(h), 0, None, None, None))
+    with pytest.raises(capi.MgpError):
+        capi._check(capi.lib.mgp_server_create(ctypes.byref(h), 0, None, None, None, None, None, None, None,
+                                               ctypes.c_double(0.0)))
+    with pytest.raises(capi.MgpError):
+        capi._check(capi.lib.mgp_batch_solve(None, ctypes.c_double(1e-10), 10, 1, None, None))
+    with pytest.raises(capi.MgpError):
+        capi._check(capi.lib.mgp_server_step(None, None, None))
+    done = ctypes.c_int(7)
+    with pytest.raises(capi.MgpError):
+        capi._check(capi.lib.mgp_server_result(None, ctypes.c_longlong(1), None, None, None, ctypes.byref(done)))
+    assert capi.lib.mgp_batch_count(None) == 0 and capi.lib.mgp_batch_last_cycles(None) == 0
+    assert capi.lib.mgp_server_footprint(None) == 0.0
