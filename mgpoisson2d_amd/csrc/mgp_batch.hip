@@ -603,13 +603,27 @@ void BatchKMG::start(const std::vector<int> &which, const std::vector<int> &warm
     upload_flags(mask, msk);
     reduce(BV_RHS, BV_RHS, SC_RN, mask);
     fetch_scalars();
-    for (int p : which) r0_[p] = std::sqrt(host_sc[p * SC_COUNT + SC_RN]);
+    std::vector<int> zero(n, 0);
+    bool any_zero = false;
+    for (int p : which) {
+        r0_[p] = std::sqrt(host_sc[p * SC_COUNT + SC_RN]);
+        if (r0_[p] == 0.0) zero[p] = any_zero = true;
+    }
+    if (any_zero) {
+        // rhs = 0: the solution is phi = 0 (a warm phi is not kept)
+        upload_flags(mask, zero);
+        k_b_copy<<<T->t0.n, 256, 0, s>>>(tl, dP, mask, BV_PHI, -1);
+        upload_flags(mask, msk);
+    }
     true_residual(mask);
     fetch_scalars();
     for (int p : which) {
         const double rel = r0_[p] > 0.0 ? std::sqrt(host_sc[p * SC_COUNT + SC_RN]) / r0_[p] : 0.0;
         if (rel <= tol_[p]) {
             oc_[p] = MGP_CONVERGED;
+            msk[p] = 0;
+        } else if (maxit_[p] <= 0) {
+            oc_[p] = MGP_MAXITERS;
             msk[p] = 0;
         } else {
             act_[p] = 1;
@@ -803,24 +817,29 @@ long long Server::submit(int slot, const double *rhs, double tol, int maxiters, 
 void Server::admit(Class &c, int ci) {
     std::vector<int> which, warm, maxit;
     std::vector<double> tol;
+    std::vector<Req> taken;  // kept until the uploads from their rhs have completed
     for (size_t q = 0; q < queue.size();) {
         Req &r = queue[q];
         if (where[r.slot].first != ci || owner[r.slot] != 0) {
             ++q;
             continue;
         }
+        owner[r.slot] = r.ticket;
+        taken.push_back(std::move(r));
+        queue.erase(queue.begin() + (long)q);
+    }
+    if (taken.empty()) return;
+    for (const Req &r : taken) {
         const int local = where[r.slot].second;
         const Grid &g = c.B->grid(local);
         upload_block(c.s, c.B->vec(local, BV_RHS), g, r.rhs.data(), 0, 0, g.nx, g.ny);
-        owner[r.slot] = r.ticket;
         which.push_back(local);
         warm.push_back(r.warm);
         tol.push_back(r.tol);
         maxit.push_back(r.maxiters);
-        queue.erase(queue.begin() + (long)q);
     }
-    if (which.empty()) return;
     c.B->start(which, warm, tol, maxit);
+    hip_check(hipStreamSynchronize(c.s), "server admit");
     std::vector<int> at_once;
     for (int p : which)
         if (!c.B->is_running(p)) at_once.push_back(p);  // met the tolerance at the start

@@ -124,6 +124,9 @@ def test_server_warm_start():
     np.testing.assert_array_equal(phi0, phi1)
     _, it2, oc2 = solve(rhs + 1e-3 * rhs_for(0, 5), True)
     assert oc2 == mg.CONVERGED and it2 < it0
+    # a zero rhs gives phi = 0 whatever the warm start
+    phi3, it3, oc3 = solve(np.zeros_like(rhs), True)
+    assert oc3 == mg.CONVERGED and it3 == 0 and not np.any(phi3)
     v.close()
 
 
