@@ -35,7 +35,8 @@ CASES = [
     (32, 32, 0.00, 0.0, (1, 1, 0, 0)),
     (200, 16, 0.10, 3.0, (1, 1, 0, 0)),
     (40, 24, 0.15, 6.0, (1, 1, 1, 1)),
-    (576, 32, 0.05, 1.0, (1, 1, 0, 0)),  # a longer chunk class (C = 4) in the same batch
+    (576, 32, 0.05, 1.0, (1, 1, 0, 0)),  # rows of 513-1024 cells (128 threads) in the same batch
+    (48, 90, 0.10, 5.0, (1, 1, 0, 0)),   # 45 rows on the coarsest level (its own sweep count)
 ]
 
 

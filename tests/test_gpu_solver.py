@@ -50,6 +50,19 @@ def test_krylov_mg_reaches_tolerance(case, method):
     np.testing.assert_array_equal(s.get_phi()[1:-1, 1:-1], phi[1:-1, 1:-1])
 
 
+@pytest.mark.parametrize("shape", [(48, 45), (40, 90), (36, 4)])
+def test_krylov_mg_odd_row_counts(shape):
+    """Row counts that stop the y-coarsening early (45: one level; 90: 45 rows
+    on the coarsest; 4: the smallest hierarchy) still reach the tolerance."""
+    nx, ny = shape
+    geo, bc, _, rhs = setup(nx, ny, 0.1, 4.0, (1, 1, 0, 0), seed=17)
+    s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
+    s.set_rhs(rhs)
+    hist, iters, outcome = s.solve(1e-10, 400, zerophi=True)
+    assert outcome == mg.CONVERGED, (hist[-3:], iters)
+    assert relres_oracle(geo, bc, rhs, s.get_phi()) <= 1.5e-10
+
+
 @pytest.mark.parametrize("method", [mg.METHOD_KRYLOV_MG, mg.METHOD_KRYLOV_MG32])
 def test_krylov_mg_matches_direct_solve(method):
     import scipy.sparse.linalg as spl
