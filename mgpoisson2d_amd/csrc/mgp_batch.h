@@ -143,6 +143,12 @@ class BatchKMG {
     BProb *dP = nullptr;
     BLev *dL = nullptr;
     int *active = nullptr, *mask = nullptr;
+    // The x-line launches of one phase (one per chunk class) cover different
+    // problems: they run side by side on streams forked from s and joined
+    // back (in the captured iteration, parallel branches).
+    std::vector<hipStream_t> aux;
+    hipEvent_t fork_ev = nullptr;
+    std::vector<hipEvent_t> join_ev;
     double *host_sc = nullptr;  // pinned: all problems' scalars
     // launch tables (device) with their lengths
     struct Table {

@@ -368,6 +368,21 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
     mask = static_cast<int *>(dev_alloc_zero(n * sizeof(int)));
     hip_check(hipHostMalloc((void **)&host_sc, (size_t)n * SC_COUNT * sizeof(double)), "hipHostMalloc");
     use_graph = o.use_graph != 0;
+    {
+        // streams for the chunk classes of a phase (default priority)
+        std::map<int, int> classes;
+        for (int p = 0; p < n; ++p) classes[Cp[p] * 1024 + NTp[p]] = 1;
+        const int nc = (int)classes.size();
+        hip_check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
+        for (int c = 0; c < nc; ++c) {
+            hipStream_t st = nullptr;
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+            aux.push_back(st);
+            hipEvent_t ev = nullptr;
+            hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+            join_ev.push_back(ev);
+        }
+    }
     act_.assign(n, 0);
     it_.assign(n, 0);
     oc_.assign(n, MGP_MAXITERS);
@@ -483,6 +498,12 @@ void BatchKMG::release(TableSet &ts) {
 BatchKMG::~BatchKMG() {
     release(part);
     release(full);
+    for (hipStream_t st : aux) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    for (hipEvent_t ev : join_ev) (void)hipEventDestroy(ev);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (dP) dev_free(dP);
     if (dL) dev_free(dL);
     if (active) dev_free(active);
@@ -493,11 +514,27 @@ BatchKMG::~BatchKMG() {
 }
 
 void BatchKMG::zebra(int d, int set, int sel, int k) {
-    for (int colour = 0; colour < 3; ++colour)
+    for (int colour = 0; colour < 3; ++colour) {
+        std::vector<const XlTable *> tabs;
         for (const XlTable &x : T->xl)
-            if (x.d == d && x.set == set && x.colour == colour)
-                launch_xline_batch(s, (const XlTask *)x.t.p, x.t.n, dL, active, sel, x.t.C, x.t.threads, x.t.lds,
-                                   k, mixed);
+            if (x.d == d && x.set == set && x.colour == colour) tabs.push_back(&x);
+        if (tabs.empty()) continue;
+        if (tabs.size() == 1 || tabs.size() > aux.size()) {
+            for (const XlTable *x : tabs)
+                launch_xline_batch(s, (const XlTask *)x->t.p, x->t.n, dL, active, sel, x->t.C, x->t.threads,
+                                   x->t.lds, k, mixed);
+            continue;
+        }
+        // one launch per chunk class, side by side
+        hip_check(hipEventRecord(fork_ev, s), "fork");
+        for (size_t i = 0; i < tabs.size(); ++i) {
+            hip_check(hipStreamWaitEvent(aux[i], fork_ev, 0), "fork wait");
+            launch_xline_batch(aux[i], (const XlTask *)tabs[i]->t.p, tabs[i]->t.n, dL, active, sel, tabs[i]->t.C,
+                               tabs[i]->t.threads, tabs[i]->t.lds, k, mixed);
+            hip_check(hipEventRecord(join_ev[i], aux[i]), "join");
+        }
+        for (size_t i = 0; i < tabs.size(); ++i) hip_check(hipStreamWaitEvent(s, join_ev[i], 0), "join wait");
+    }
 }
 
 // e = M f for the pair `sel` of every active problem (phat from p: 0, shat
