@@ -551,7 +551,7 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
 // row.  The boundary rows ride on the odd-row launch this way: row 0 after
 // row 1 and row ny-1 after the last odd row, the order of a separate third
 // phase, without its launch.
-template <int C, typename T, typename GM>
+template <int C, int NT, typename T, typename GM>
 __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
                                            const Grid &g, const GM &geo, int row0, int rowstep, int nt, int extra0,
                                            int extra1) {
@@ -570,7 +570,7 @@ __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restric
         // ops) instead of being hoisted out of the loop and spilled
         RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
-        xline_row<C, XL_THREADS>(X, e, f, K, g, geo, jr, lds);
+        xline_row<C, NT>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
@@ -589,7 +589,15 @@ template <int C, typename T, typename GM>
 __global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(xl_waves<T>(), xl_waves<T>())))
 k_xline(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K, Grid g, GM geo, int row0, int rowstep,
         int nt, int extra0, int extra1) {
-    xline_rows<C>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+    xline_rows<C, XL_THREADS>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+}
+
+// Rows of at most 512 cells: one 64-thread wave per row (as in the batch).
+template <int C, typename T, typename GM>
+__global__ void __launch_bounds__(XL_SHORT) __attribute__((amdgpu_waves_per_eu(2)))
+k_xline_short(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K, Grid g, GM geo, int row0,
+              int rowstep, int nt, int extra0, int extra1) {
+    xline_rows<C, XL_SHORT>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
 }
 
 // Longer rows (C = 16): one workgroup per CU in FP64 (139 KB of LDS), all registers.
@@ -597,7 +605,7 @@ template <int C, typename T, typename GM>
 __global__ void __launch_bounds__(XL_THREADS) k_xline_long(T *__restrict__ e, const T *__restrict__ f,
                                                            const T *__restrict__ K, Grid g, GM geo, int row0,
                                                            int rowstep, int nt, int extra0, int extra1) {
-    xline_rows<C>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+    xline_rows<C, XL_THREADS>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
 }
 
 // (L e)(i, j) at an interior cell from the face metrics: the stencil of
@@ -808,6 +816,21 @@ template <typename T, typename G>
 static void xline(hipStream_t s, T *e, const T *f, const T *K, const Grid &g, const G &geo, int row0, int rowstep,
                   int nrows, int extra0, int extra1) {
     if (nrows <= 0) return;
+    if (g.nx <= XL_SHORT_MAX_NX) {
+        const int C = xline_chunk(g.nx, XL_SHORT), nt = g.nx / C;
+        const size_t lds = xline_lds_bytes_for(g.nx, C, sizeof(T), XL_SHORT);
+        switch (C) {
+            case 2: k_xline_short<2><<<nrows, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
+            case 4: k_xline_short<4><<<nrows, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
+            default: k_xline_short<8><<<nrows, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
+        }
+        const hipError_t err = hipGetLastError();
+        if (err != hipSuccess) {
+            g_last_error = std::string("k_xline launch: ") + hipGetErrorString(err);
+            throw Failure(MGP_ERR_HIP);
+        }
+        return;
+    }
     const int C = xline_chunk(g.nx);
     const size_t lds = xline_lds_bytes(g.nx, sizeof(T));
     if (C == 0 || lds > 160 * 1024) {
