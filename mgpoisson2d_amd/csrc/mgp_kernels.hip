@@ -312,6 +312,47 @@ __global__ void __launch_bounds__(1024) k_gs_mc_ring(Fld phi, const double *rhs,
     block_gs_mc_ring(phi, rhs, invd, g, geo, bc, homog, omega);
 }
 
+// Multi-workgroup forms of the single-workgroup ghost fill and ring sweep
+// above, for large grids: the same cells with the same arithmetic, each
+// phase (x sides, y sides, corners; ring colours 0..3, corner cells) one
+// launch, in the same order.
+__global__ void __launch_bounds__(256) k_fill_x_ghosts(Fld f, Grid g, Geo geo, Bc bc, int homog, int do_neum) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= 2 * (g.ny + 2)) return;
+    fill_x_ghost(f, g, geo, bc, homog, do_neum, k < g.ny + 2 ? XLO : XHI, (k % (g.ny + 2)) - 1);
+}
+
+__global__ void __launch_bounds__(256) k_fill_y_ghosts(Fld f, Grid g, Geo geo, Bc bc, int homog, int do_neum) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= 2 * (g.nx + 2)) return;
+    fill_y_ghost(f, g, geo, bc, homog, do_neum, k < g.nx + 2 ? YLO : YHI, (k % (g.nx + 2)) - 1);
+}
+
+__global__ void k_fill_corner_ghosts(Fld f, Grid g, Bc bc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) fill_corners(f, g, bc);
+}
+
+__global__ void __launch_bounds__(256) k_gs_mc_ring_colour(Fld phi, const double *rhs, const double *invd, Grid g,
+                                                           Geo geo, Bc bc, int homog, double omega, int colour) {
+    const int nx = g.nx, ny = g.ny;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= 2 * (nx - 2) + 2 * (ny - 2)) return;
+    int i, j;
+    if (k < nx - 2) { i = 1 + k; j = 0; }
+    else if (k < 2 * (nx - 2)) { i = 1 + k - (nx - 2); j = ny - 1; }
+    else if (k < 2 * (nx - 2) + (ny - 2)) { i = 0; j = 1 + k - 2 * (nx - 2); }
+    else { i = nx - 1; j = 1 + k - 2 * (nx - 2) - (ny - 2); }
+    if (((i & 1) | ((j & 1) << 1)) == colour) gs_cell(phi, rhs, invd, g, geo, bc, homog, omega, i, j);
+}
+
+__global__ void k_gs_mc_corner_cells(Fld phi, const double *rhs, const double *invd, Grid g, Geo geo, Bc bc,
+                                     int homog, double omega) {
+    if (threadIdx.x < 4) {
+        const int i = (threadIdx.x & 1) ? g.nx - 1 : 0, j = (threadIdx.x & 2) ? g.ny - 1 : 0;
+        gs_cell(phi, rhs, invd, g, geo, bc, homog, omega, i, j);
+    }
+}
+
 // Exact lexicographic order (poisson2d.f90:1561-1959) by wavefronts t = i + 3j:
 // every value a cell reads from an earlier row (reach up to (i+2, j-1)) lies
 // on an earlier front and every later-row or later-column read on a later
@@ -970,9 +1011,20 @@ namespace mgp {
 
 static inline dim3 tiles(int nx, int ny) { return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4)); }
 
+// Grids whose perimeter one workgroup walks too slowly: the phases of the
+// ghost fill and of the ring sweep as multi-workgroup launches.
+static bool wide_perimeter(const Grid &g) { return g.nx + g.ny >= 512; }
+
 void launch_fill_ghosts(hipStream_t s, double *phi, const Grid &g, const Geo &geo, const Bc &bc, int homog,
                         int do_neum) {
-    k_fill_ghosts<<<1, 256, 0, s>>>(Fld{phi, g.pitch}, g, geo, bc, homog, do_neum);
+    const Fld f{phi, g.pitch};
+    if (!wide_perimeter(g)) {
+        k_fill_ghosts<<<1, 256, 0, s>>>(f, g, geo, bc, homog, do_neum);
+        return;
+    }
+    k_fill_x_ghosts<<<(2 * (g.ny + 2) + 255) / 256, 256, 0, s>>>(f, g, geo, bc, homog, do_neum);
+    k_fill_y_ghosts<<<(2 * (g.nx + 2) + 255) / 256, 256, 0, s>>>(f, g, geo, bc, homog, do_neum);
+    k_fill_corner_ghosts<<<1, 64, 0, s>>>(f, g, bc);
 }
 void launch_laplacian(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g,
                       const Geo &geo, const Bc &bc, int homog, int jscale) {
@@ -1004,7 +1056,16 @@ void launch_gs_mc_interior(hipStream_t s, double *phi, const double *rhs, const 
 }
 void launch_gs_mc_ring(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
                        const Geo &geo, const Bc &bc, int homog, double omega) {
-    k_gs_mc_ring<<<1, 1024, 0, s>>>(Fld{phi, g.pitch}, rhs, invd, g, geo, bc, homog, omega);
+    const Fld f{phi, g.pitch};
+    if (!wide_perimeter(g)) {
+        k_gs_mc_ring<<<1, 1024, 0, s>>>(f, rhs, invd, g, geo, bc, homog, omega);
+        return;
+    }
+    launch_fill_ghosts(s, phi, g, geo, bc, homog, 0);
+    const int nring = 2 * (g.nx - 2) + 2 * (g.ny - 2);
+    for (int colour = 0; colour < 4; ++colour)
+        k_gs_mc_ring_colour<<<(nring + 255) / 256, 256, 0, s>>>(f, rhs, invd, g, geo, bc, homog, omega, colour);
+    k_gs_mc_corner_cells<<<1, 64, 0, s>>>(f, rhs, invd, g, geo, bc, homog, omega);
 }
 void launch_gs_lex(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
                    const Geo &geo, const Bc &bc, int homog, double omega, int sweeps) {

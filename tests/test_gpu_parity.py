@@ -148,6 +148,26 @@ def test_relax_gs_lex_and_mc(case):
     same(mc, rc.oracle().relax_gs(geo, bc, 1, inv, 2.0 / 3.0, 3, rhs, phi, rc.MC))
 
 
+# perimeters long enough (nx + ny >= 512) for the multi-workgroup ghost fill and ring sweep
+WIDE_CASES = [(300, 240, 0.10, 3.0, (1, 1, 0, 0)), (288, 256, 0.08, 1.0, (0, 1, 1, 0))]
+
+
+@pytest.mark.parametrize("case", WIDE_CASES)
+def test_wide_fill_ghosts_and_mc_ring(case):
+    geo, bc, phi, rhs = setup(*case, seed=21)
+    o = rc.oracle()
+    for homog in (1, 0):
+        for do_neum in (1, 0):
+            same(mg.fill_ghosts(geo, bc, homog, do_neum, phi), o.fill_ghosts(geo, bc, homog, do_neum, phi))
+    inv = mg.compute_invdiags(geo, bc, 1)
+    mc = mg.relax_gs(geo, bc, 1, inv, 2.0 / 3.0, 2, rhs, phi, mg.ORDER_MC)
+    same(mc, o.relax_gs(geo, bc, 1, inv, 2.0 / 3.0, 2, rhs, phi, rc.MC))
+    lap, phig = mg.compute_laplacian(geo, bc, 1, phi)
+    l2, p2 = o.laplacian(geo, bc, 1, phi)
+    same(lap, l2)
+    same(phig, p2)
+
+
 @pytest.mark.parametrize("case", OPERATOR_CASES)
 def test_dc_operator_and_smoothers(case):
     from mgpoisson2d_amd import geometry as G
