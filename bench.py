@@ -272,6 +272,31 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
     }
 
 
+def dc_rb_bench(capi, geo, bc, reps=20):
+    """The DC red-black smoother (DeferredMGPoisson2D, RB ordering) on the
+    finest level: one call of 2 sweeps with LDS tiles (several sweeps per
+    load) against one launch per colour, and the tile kernel alone against
+    the HBM roofline (algorithmic bytes: phi, rhs, invd and the four face
+    metrics read once, phi written once)."""
+    s = capi.Solver(geo, bc, method=capi.METHOD_DC_VCYCLE, ordering=capi.ORDER_RB)
+    tiled_ms, nbytes = s.time_kernel(4, reps=reps)
+    colour_ms, _ = s.time_kernel(5, reps=reps)
+    kern_ms, _ = s.time_kernel(6, reps=reps)
+    s.close()
+    gbps = nbytes / (kern_ms * 1e-3) / 1e9
+    return {
+        "workload": f"DC red-black relax_gs, 2 sweeps (modrhs fused), {geo.nx}x{geo.ny} finest level",
+        "tiled_ms": tiled_ms,
+        "per_colour_ms": colour_ms,
+        "speedup": colour_ms / tiled_ms,
+        "tile_kernel_ms": kern_ms,
+        "tile_kernel_GBps": gbps,
+        "tile_kernel_frac": gbps / HBM_PEAK_GBS,
+        "alg_bytes_per_call": nbytes,
+        "timing": f"HIP events around {reps} back-to-back calls on the solver stream",
+    }
+
+
 def cpu_baseline(geo, bc, rhs, gpu_cycles, budget_s=20.0):
     """Time the reference Fortran (oracle/_ref/fast, -O3 -march=native, one
     core) on the same 2048^2 problem: DeferredMGPoisson2D::vcycle, one V(2,2)
@@ -323,6 +348,8 @@ def main():
     ap.add_argument("--batch-n", type=int, default=1024, help="problems in the batched mix")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving-loop measurement")
+    ap.add_argument("--no-dc-rb", action="store_true", help="skip the DC red-black smoother measurement")
+    ap.add_argument("--dc-rb-only", action="store_true", help="run only the DC red-black smoother measurement")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--kernels-only", action="store_true",
@@ -343,6 +370,9 @@ def main():
         print(json.dumps({"mixed": serving_bench(capi), "fp64": serving_bench(capi, mixed=False)}), flush=True)
         return
     geo, bc, rhs = config2_problem(args.n, args.n)
+    if args.dc_rb_only:
+        print(json.dumps(dc_rb_bench(capi, geo, bc)), flush=True)
+        return
     main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
     other_method = capi.METHOD_KRYLOV_MG if args.method == "mixed" else capi.METHOD_KRYLOV_MG32
 
@@ -435,6 +465,9 @@ def main():
     serving = None
     if not args.no_serving:
         serving = serving_bench(capi)
+    dc_rb = None
+    if not args.no_dc_rb:
+        dc_rb = dc_rb_bench(capi, geo, bc)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -483,6 +516,7 @@ def main():
         "mixed": compare,
         "batched": batched,
         "serving": serving,
+        "dc_rb_smoother": dc_rb,
     }
     print(json.dumps(line), flush=True)
     solver.close()

@@ -111,6 +111,17 @@ int mgp_dc_compute_invdiags(const mgp_geo *geo, double *invd);
 /* MGPoisson2D_DD.f90:153 DeferredPoisson2D::relax_gs; ordering LEX or RB */
 int mgp_dc_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *invd, double omega,
                     int maxiters, const double *rhs, double *phi, int ordering);
+/* Red-black DC sweeps (mgp_dc_relax_gs with RB, and the DC V-cycle with RB
+ * ordering) run several sweeps per LDS tile load when every side is a
+ * Neumann or Dirichlet side in BCMODE_LAH (the level BCs of the DC cycle;
+ * MGPoisson2D_DD.f90:262-285 is one launch per colour).  No counterpart in the
+ * reference: a tuning switch, process-wide.  choice -1: tile shape timed per
+ * level shape on first use (default); 0: one launch per colour; k >= 1: tile
+ * shape k.  Results are bit-identical for every choice.  *previous (if not
+ * null) receives the setting replaced. */
+int mgp_dc_rb_tiles(int choice, int *previous);
+/* Number of tile shapes the switch above accepts (1 .. n). */
+int mgp_dc_rb_tile_shapes(void);
 /* MGPoisson2D.f90:135 restrict(fine, crse); kind 0 cell, 1 x-face, 2 y-face */
 int mgp_restrict(int kind, int fnx, int fny, const double *fine, int cnx, int cny, double *crse);
 /* MGPoisson2D.f90:262 prolong(fine, crse, fine_geo, crse_geo, crse_bc, order), crse_bc
@@ -253,7 +264,11 @@ int mgp_solver_last_cycles(const mgp_solver *s);
  * kernel with HIP events on the solver's stream.  kernel 0: x-line
  * relaxation of the even interior rows; 1: residual; 2: plain copy of one
  * finest-level field (bandwidth reference); 3 (KRYLOV_MG32 only): the FP32
- * x-line relaxation of the even interior rows.  cold != 0: back-to-back launches
+ * x-line relaxation of the even interior rows.  DC_VCYCLE solvers with RB
+ * ordering: kernel 4: one red-black smoother call of 2 sweeps on the finest
+ * level with LDS tiles (tile kernel and copy back); 5: the same call with one
+ * launch per colour; 6: the tile kernel alone (cold is ignored; *bytes is 7
+ * fields read and 1 written).  cold != 0: back-to-back launches
  * that take four copies of the operands in turn, so that each launch finds its
  * operands evicted from the L2s and the MALL (as the kernels run inside a
  * solve); cold == 0: back-to-back launches on cache-resident operands.

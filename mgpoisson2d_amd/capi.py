@@ -68,7 +68,7 @@ lib = _load()
 EXPORTS = (
     "mgp_status_string", "mgp_last_error", "mgp_device_count", "mgp_fill_ghosts",
     "mgp_compute_laplacian", "mgp_compute_residual", "mgp_compute_invdiags", "mgp_relax_gs",
-    "mgp_dc_compute_laplacian", "mgp_dc_compute_invdiags", "mgp_dc_relax_gs", "mgp_restrict",
+    "mgp_dc_compute_laplacian", "mgp_dc_compute_invdiags", "mgp_dc_relax_gs", "mgp_dc_rb_tiles", "mgp_dc_rb_tile_shapes", "mgp_restrict",
     "mgp_prolong", "mgp_create_mgschedule", "mgp_pnorm", "mgp_solve_bicgstab2", "mgp_vcycle",
     "mgp_solver_default_opts", "mgp_solver_create", "mgp_solver_destroy", "mgp_solver_set_rhs",
     "mgp_solver_set_phi", "mgp_solver_get_phi", "mgp_solver_copy_rhs_from", "mgp_solver_solve",
@@ -195,6 +195,20 @@ def dc_relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_
     _check(lib.mgp_dc_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
                                _ptr(rhs), _ptr(phi), _i(ordering)))
     return phi
+
+
+def dc_rb_tiles(choice):
+    """Process-wide switch of the red-black DC sweeps: -1 tile shape timed per
+    level (default), 0 one launch per colour, k >= 1 tile shape k.  Returns the
+    previous setting; results are bit-identical for every choice."""
+    prev = ctypes.c_int(0)
+    _check(lib.mgp_dc_rb_tiles(_i(choice), ctypes.byref(prev)))
+    return prev.value
+
+
+def dc_rb_tile_shapes():
+    """Number of tile shapes dc_rb_tiles accepts (1 .. n)."""
+    return int(lib.mgp_dc_rb_tile_shapes())
 
 
 def compute_pd(nx, ny, dx, dy, phi, direction, nodedir):

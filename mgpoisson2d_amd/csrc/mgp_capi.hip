@@ -293,25 +293,35 @@ int mgp_dc_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const doubl
             throw Failure(MGP_ERR_ARG);
         }
         Call c(geo, bc);
-        DevBuf f = c.field(), r = c.field(), iv = c.field(), mod = c.field();
+        DevBuf f = c.field(), r = c.field(), iv = c.field(), mod = c.field(), tmp;
         up_phi(c.s, f, c.geo.g, phi);
         up_cc(c.s, r, c.geo.g, rhs);
         up_cc(c.s, iv, c.geo.g, invd);
         const Grid &g = c.geo.g;
-        launch_dc_modrhs(c.s, mod.p, r.p, f.p, g, c.geo.view());
         if (ordering == MGP_ORDER_RB) {
-            for (int k = 0; k < maxiters; ++k) {
-                launch_fill_ghosts(c.s, f.p, g, c.geo.view(), c.bc.bc, homog, 1);
-                launch_dc_rb(c.s, f.p, mod.p, iv.p, g, c.geo.view(), omega, 0);
-                launch_dc_rb(c.s, f.p, mod.p, iv.p, g, c.geo.view(), omega, 1);
-            }
+            tmp = c.field();
+            dc_rb_relax(c.s, f.p, tmp.p, mod.p, r.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega, maxiters);
         } else {
+            launch_dc_modrhs(c.s, mod.p, r.p, f.p, g, c.geo.view());
             launch_dc_lex(c.s, f.p, mod.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega, maxiters);
         }
         down_phi(c.s, phi, f, g);
         c.finish();
     });
 }
+
+int mgp_dc_rb_tiles(int choice, int *previous) {
+    return guarded([&] {
+        if (choice < -1 || choice > dc_rb_tile_shapes()) {
+            g_last_error = "dc_rb_tiles: choice -1 (timed), 0 (one launch per colour) or a tile shape 1..n";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const int prev = dc_rb_tile_choice(choice);
+        if (previous) *previous = prev;
+    });
+}
+
+int mgp_dc_rb_tile_shapes(void) { return dc_rb_tile_shapes(); }
 
 int mgp_restrict(int kind, int fnx, int fny, const double *fine, int cnx, int cny, double *crse) {
     return guarded([&] {
@@ -946,8 +956,13 @@ int mgp_solver_last_cycles(const mgp_solver *s) { return s ? s->last_cycles : 0;
 int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double *avg_ms, double *bytes) {
     return guarded([&] {
         const bool kmg = s->opts.method == MGP_METHOD_KRYLOV_MG || s->opts.method == MGP_METHOD_KRYLOV_MG32;
+        if (kernel >= 4 && s->opts.method == MGP_METHOD_DC_VCYCLE) {
+            *avg_ms = s->ref.time_relax(s->stream, kernel, reps, bytes);
+            return;
+        }
         if (!kmg || reps <= 0 || kernel < 0 || kernel > 3) {
-            g_last_error = "time_kernel: KRYLOV_MG / KRYLOV_MG32 solvers, kernel 0 to 3, reps > 0";
+            g_last_error = "time_kernel: KRYLOV_MG / KRYLOV_MG32 solvers with kernel 0 to 3, or DC_VCYCLE with "
+                           "kernel 4 to 6; reps > 0";
             throw Failure(MGP_ERR_ARG);
         }
         *avg_ms = s->kmg.time_kernel(s->stream, kernel, reps, cold, bytes);

@@ -115,6 +115,11 @@ struct RefHierarchy {
 
     void build(hipStream_t s, const DevGeo &top, const Bc &user_bc, int dc, int ordering, int maxdepth_user);
     void relax(hipStream_t s, int d, double *phi, const double *rhs, int sweeps, double omega);
+    // Measurement of the finest-level DC red-black smoother (2 sweeps):
+    // kernel 4 the relax with LDS tiles, 5 with one launch per colour, 6 the
+    // tile kernel alone.  Mean ms per call over reps; *bytes the algorithmic
+    // bytes of one call (7 fields read, 1 written).
+    double time_relax(hipStream_t s, int kernel, int reps, double *bytes);
     void bottom(hipStream_t s, int d);
     void vcycle_noinit(Workspace &W, hipStream_t s, int d, int numcycles, int sd, int su, int sb);
     void fmg_noinit(Workspace &W, hipStream_t s, int d, int numcycles, int sd, int su, int sb);

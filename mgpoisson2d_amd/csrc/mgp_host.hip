@@ -328,17 +328,50 @@ void RefHierarchy::relax(hipStream_t s, int d, double *phi, const double *rhs, i
             }
         }
     } else {
-        launch_dc_modrhs(s, l.mod.p, rhs, phi, l.g, l.geo);
         if (ordering == MGP_ORDER_RB) {
-            for (int k = 0; k < sweeps; ++k) {
-                launch_fill_ghosts(s, phi, l.g, l.geo, l.bc, 1, 1);
-                launch_dc_rb(s, phi, l.mod.p, l.invd.p, l.g, l.geo, omega, 0);
-                launch_dc_rb(s, phi, l.mod.p, l.invd.p, l.g, l.geo, omega, 1);
-            }
+            // w is free until the residual after the smoother
+            dc_rb_relax(s, phi, l.w.p, l.mod.p, rhs, l.invd.p, l.g, l.geo, l.bc, 1, omega, sweeps);
         } else {
+            launch_dc_modrhs(s, l.mod.p, rhs, phi, l.g, l.geo);
             launch_dc_lex(s, phi, l.mod.p, l.invd.p, l.g, l.geo, l.bc, 1, omega, sweeps);
         }
     }
+}
+
+double RefHierarchy::time_relax(hipStream_t s, int kernel, int reps, double *bytes) {
+    if (!dc || ordering != MGP_ORDER_RB || kernel < 4 || kernel > 6 || reps <= 0) {
+        g_last_error = "time_kernel: kernels 4-6 time the red-black smoother of a DC_VCYCLE solver (ordering RB)";
+        throw Failure(MGP_ERR_ARG);
+    }
+    HLevel &l = levels[0];
+    const double omega = 4.0 / 5.0;
+    const int prev = dc_rb_tile_choice(kernel == 5 ? 0 : -1);
+    auto once = [&] {
+        if (kernel == 6) dc_rb_tile_once(s, l.e.p, l.w.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, omega, 2);
+        else dc_rb_relax(s, l.e.p, l.w.p, l.mod.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, 1, omega, 2);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    try {
+        once();  // first use picks the tile shape
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) once();
+        hip_check(hipEventRecord(e1, s), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    } catch (...) {
+        dc_rb_tile_choice(prev);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    dc_rb_tile_choice(prev);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *bytes = 8.0 * 8.0 * (double)l.g.nx * (double)l.g.ny;
+    return ms / reps;
 }
 
 void RefHierarchy::bottom(hipStream_t s, int d) {

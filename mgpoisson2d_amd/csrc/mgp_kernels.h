@@ -32,6 +32,21 @@ void launch_dc_rb(hipStream_t s, double *phi, const double *modrhs, const double
                   const Geo &geo, double omega, int colour);
 void launch_dc_lex(hipStream_t s, double *phi, const double *modrhs, const double *invd, const Grid &g,
                    const Geo &geo, const Bc &bc, int homog, double omega, int sweeps);
+// DC relax_gs, red-black (MGPoisson2D_DD.f90:227-285): modrhs, then `sweeps`
+// times fill ghosts, colour 0, colour 1.  On LAH Neumann/Dirichlet sides the
+// sweeps run in LDS tiles, several per load (mgp_rbtile.hip), ping-ponging
+// through tmp; otherwise one launch per colour.  mod and tmp are level-sized
+// scratch; the result is bit-identical either way.
+void dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const double *rhs, const double *invd,
+                 const Grid &g, const Geo &geo, const Bc &bc, int homog, double omega, int sweeps);
+// -1: tile shape timed per level (default); 0: one launch per colour; k >= 1:
+// tile shape k - 1 of dc_rb_tile_shapes().  Returns the previous setting.
+int dc_rb_tile_choice(int choice);
+int dc_rb_tile_shapes();
+// One tile launch of S sweeps with the modified rhs formed in the kernel,
+// phi -> out (measurement); returns the tile shape used.
+int dc_rb_tile_once(hipStream_t s, const double *phi, double *out, const double *rhs, const double *invd,
+                    const Grid &g, const Geo &geo, const Bc &bc, double omega, int S);
 void launch_restrict_cc(hipStream_t s, double *crse, const double *fine, const Grid &cg, int fpitch, int rx,
                         int ry, double *e, const double *invd);
 void launch_restrict_xf(hipStream_t s, double *crse, const double *fine, const Grid &cg, int fpitch, int rx,

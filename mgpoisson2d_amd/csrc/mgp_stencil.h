@@ -250,34 +250,49 @@ __device__ __forceinline__ double dc_cross(const CFld &f, const Grid &g, const G
 
 // Lagged corner-difference cross terms of the DC smoother (scale half idx idy,
 // MGPoisson2D_DD.f90:227-250).
+// The same from the four diagonal values and the face metrics Jxy(i+1, j),
+// Jxy(i, j), Jyx(i, j+1), Jyx(i, j); xyscale = 0.5 idx idy.
+__device__ __forceinline__ double dc_lagged_cross_vals(double pne, double pnw, double psw, double pse, double jxye,
+                                                       double jxyw, double jyxn, double jyxs, double xyscale) {
+    const double gxye = jxye * (pne - pse);
+    const double gxyw = jxyw * (pnw - psw);
+    const double gyxn = jyxn * (pne - pnw);
+    const double gyxs = jyxs * (pse - psw);
+    return (gxye - gxyw + gyxn - gyxs) * xyscale;
+}
+
 __device__ __forceinline__ double dc_lagged_cross(const CFld &f, const Grid &g, const Geo &geo, int i, int j) {
     const int P = g.pitch;
     const size_t c = ix(P, i, j);
     const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, xyscale = 0.5 * idx * idy;
     const double pne = clamped(f, g, i + 1, j + 1), pnw = clamped(f, g, i - 1, j + 1);
     const double psw = clamped(f, g, i - 1, j - 1), pse = clamped(f, g, i + 1, j - 1);
-    const double gxye = geo.Jxy[c + 1] * (pne - pse);
-    const double gxyw = geo.Jxy[c] * (pnw - psw);
-    const double gyxn = geo.Jyx[c + P] * (pne - pnw);
-    const double gyxs = geo.Jyx[c] * (pse - psw);
-    return (gxye - gxyw + gyxn - gyxs) * xyscale;
+    return dc_lagged_cross_vals(pne, pnw, psw, pse, geo.Jxy[c + 1], geo.Jxy[c], geo.Jyx[c + P], geo.Jyx[c], xyscale);
 }
 
 // 5-point part, as its x and y halves (accumulated one at a time by the
 // operator, summed first by the smoother).
+// The same from the five values and the face metrics Jxx(i+1, j), Jxx(i, j),
+// Jyy(i, j+1), Jyy(i, j).
+__device__ __forceinline__ void dc_five_vals(double p, double pe, double pw, double pn, double ps, double jxe,
+                                             double jxw, double jyn, double jys, double idx, double idy, double &ax,
+                                             double &ay) {
+    const double xfe = jxe * (pe - p) * idx;
+    const double xfw = jxw * (p - pw) * idx;
+    const double yfn = jyn * (pn - p) * idy;
+    const double yfs = jys * (p - ps) * idy;
+    ax = (xfe - xfw) * idx;
+    ay = (yfn - yfs) * idy;
+}
+
 __device__ __forceinline__ void dc_five(const CFld &f, const Grid &g, const Geo &geo, int i, int j,
                                         double &ax, double &ay) {
     const int P = g.pitch;
     const size_t c = ix(P, i, j);
     const double *q = f.p + c;
     const double idx = 1.0 / g.dx, idy = 1.0 / g.dy;
-    const double p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
-    const double xfe = geo.Jxx[c + 1] * (pe - p) * idx;
-    const double xfw = geo.Jxx[c] * (p - pw) * idx;
-    const double yfn = geo.Jyy[c + P] * (pn - p) * idy;
-    const double yfs = geo.Jyy[c] * (p - ps) * idy;
-    ax = (xfe - xfw) * idx;
-    ay = (yfn - yfs) * idy;
+    dc_five_vals(q[0], q[1], q[-1], q[P], q[-P], geo.Jxx[c + 1], geo.Jxx[c], geo.Jyy[c + P], geo.Jyy[c], idx, idy,
+                 ax, ay);
 }
 
 // ---------------------------------------------------------------------------
