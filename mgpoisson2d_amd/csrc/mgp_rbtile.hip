@@ -75,6 +75,36 @@ __global__ void __launch_bounds__(64 * NW) k_dc_rb_tile(const double *__restrict
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int ly = wave + NW * q, gj = wy0 + ly;
+        const int gi0 = wx0 + lx0;
+        if (gj >= 0 && gj < ny && gi0 >= 0 && gi0 + 1 < nx) {
+            // both columns in the domain: 16-byte loads (the offset of cell i = 0,
+            // the pitch and gi0 are even), and Jxx east of column 0 is Jxx west of column 1
+            const size_t c = ix(P, gi0, gj);
+            const double2 v = *reinterpret_cast<const double2 *>(in + c);
+            const double2 x = *reinterpret_cast<const double2 *>(geo.Jxx + c);
+            const double2 ys = *reinterpret_cast<const double2 *>(geo.Jyy + c);
+            const double2 yn = *reinterpret_cast<const double2 *>(geo.Jyy + c + P);
+            const double2 sr = *reinterpret_cast<const double2 *>(src + c);
+            const double2 dv = *reinterpret_cast<const double2 *>(invd + c);
+            jw[q][0] = x.x;
+            jw[q][1] = x.y;
+            je[q][0] = x.y;
+            je[q][1] = geo.Jxx[c + 2];
+            js[q][0] = ys.x;
+            js[q][1] = ys.y;
+            jn[q][0] = yn.x;
+            jn[q][1] = yn.y;
+            md[q][0] = sr.x;
+            md[q][1] = sr.y;
+            iv[q][0] = dv.x;
+            iv[q][1] = dv.y;
+            w[ly][lx0] = v.x;
+            w[ly][lx0 + 1] = v.y;
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+                if (lx0 + m >= 1 && lx0 + m <= WW - 2 && ly >= 1 && ly <= H - 2) upd |= 1u << (2 * q + m);
+            continue;
+        }
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
             const int lx = lx0 + m, gi = wx0 + lx;
