@@ -272,7 +272,7 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
     }
 
 
-def dc_rb_bench(capi, geo, bc, reps=20):
+def dc_rb_bench(capi, geo, bc, rhs=None, reps=20, cycles=5):
     """The DC red-black smoother (DeferredMGPoisson2D, RB ordering) on the
     finest level: one call of 2 sweeps with LDS tiles (several sweeps per
     load) against one launch per colour, and the tile kernel alone against
@@ -282,6 +282,17 @@ def dc_rb_bench(capi, geo, bc, reps=20):
     tiled_ms, nbytes = s.time_kernel(4, reps=reps)
     colour_ms, _ = s.time_kernel(5, reps=reps)
     kern_ms, _ = s.time_kernel(6, reps=reps)
+    # whole DC V(2,2) cycles (every level's smoother, residuals, transfers, bottom solve)
+    cycle_ms = {}
+    if rhs is not None:
+        s.set_rhs(rhs)
+        for name, choice in (("tiled", -1), ("per_colour", 0)):
+            prev = capi.dc_rb_tiles(choice)
+            s.solve(1e-30, 1, zerophi=True)  # first use of each level shape picks its tile shape
+            t0 = time.perf_counter()
+            _, iters, _ = s.solve(1e-30, cycles, zerophi=True)
+            cycle_ms[name] = (time.perf_counter() - t0) * 1e3 / max(iters, 1)
+            capi.dc_rb_tiles(prev)
     s.close()
     gbps = nbytes / (kern_ms * 1e-3) / 1e9
     return {
@@ -294,6 +305,10 @@ def dc_rb_bench(capi, geo, bc, reps=20):
         "tile_kernel_frac": gbps / HBM_PEAK_GBS,
         "alg_bytes_per_call": nbytes,
         "timing": f"HIP events around {reps} back-to-back calls on the solver stream",
+        "dc_vcycle_ms": cycle_ms.get("tiled"),
+        "dc_vcycle_ms_per_colour": cycle_ms.get("per_colour"),
+        "dc_vcycle_timing": f"host clock around {cycles} DC V(2,2) cycles (MGPoisson2D_DD.f90 vcycle, RB ordering) "
+                            "with their residual checks",
     }
 
 
@@ -371,7 +386,7 @@ def main():
         return
     geo, bc, rhs = config2_problem(args.n, args.n)
     if args.dc_rb_only:
-        print(json.dumps(dc_rb_bench(capi, geo, bc)), flush=True)
+        print(json.dumps(dc_rb_bench(capi, geo, bc, rhs)), flush=True)
         return
     main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
     other_method = capi.METHOD_KRYLOV_MG if args.method == "mixed" else capi.METHOD_KRYLOV_MG32
@@ -467,7 +482,7 @@ def main():
         serving = serving_bench(capi)
     dc_rb = None
     if not args.no_dc_rb:
-        dc_rb = dc_rb_bench(capi, geo, bc)
+        dc_rb = dc_rb_bench(capi, geo, bc, rhs)
 
     cpu = None
     if not args.no_cpu_baseline:
