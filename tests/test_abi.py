@@ -31,6 +31,23 @@ def test_host_only_entry_points():
     assert (rx[:3], ry[:3]) == ([2, 2, 2], [1, 1, 2])
 
 
+def test_dc_rb_tile_switch_is_host_only():
+    import pytest
+    from mgpoisson2d_amd import capi
+    n = capi.dc_rb_tile_shapes()
+    assert n >= 2
+    prev = capi.dc_rb_tiles(0)
+    try:
+        assert capi.dc_rb_tiles(n) == 0
+        assert capi.dc_rb_tiles(-1) == n
+        for bad in (-2, n + 1):
+            with pytest.raises(capi.MgpError):
+                capi.dc_rb_tiles(bad)
+        assert capi.dc_rb_tiles(-1) == -1  # a rejected choice leaves the setting alone
+    finally:
+        capi.dc_rb_tiles(prev)
+
+
 def test_bad_arguments_rejected_without_device_work():
     import numpy as np
     import pytest
