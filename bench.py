@@ -295,6 +295,11 @@ def dc_rb_bench(capi, geo, bc, rhs=None, reps=20, cycles=5):
             capi.dc_rb_tiles(prev)
     s.close()
     gbps = nbytes / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    path = os.path.join(ROOT, "profiles", "traffic_dc_rb_2048.json")
+    if geo.nx == NX and geo.ny == NX and os.path.exists(path):
+        with open(path) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
     return {
         "workload": f"DC red-black relax_gs, 2 sweeps (modrhs fused), {geo.nx}x{geo.ny} finest level",
         "tiled_ms": tiled_ms,
@@ -304,6 +309,7 @@ def dc_rb_bench(capi, geo, bc, rhs=None, reps=20, cycles=5):
         "tile_kernel_GBps": gbps,
         "tile_kernel_frac": gbps / HBM_PEAK_GBS,
         "alg_bytes_per_call": nbytes,
+        "tile_kernel_traffic": traffic,
         "timing": f"HIP events around {reps} back-to-back calls on the solver stream",
         "dc_vcycle_ms": cycle_ms.get("tiled"),
         "dc_vcycle_ms_per_colour": cycle_ms.get("per_colour"),
