@@ -210,10 +210,12 @@ __global__ void __launch_bounds__(64 * NW) k_dc_rb_tile(const double *__restrict
     }
 }
 
-// dst = src over cells (-1 .. nx, -1 .. ny) of a level (n = nx + 2 per row)
+// dst = src over cells (-1 .. nx, -1 .. ny) of a level (n = nx + 2 per row,
+// nbx blocks of 256 per row, one 1-D grid over all rows)
 __global__ void __launch_bounds__(256) k_copy_level(double *__restrict__ dst, const double *__restrict__ src,
-                                                     int pitch, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x - 1, j = (int)blockIdx.y - 1;
+                                                     int pitch, int n, int nbx) {
+    const int bx = blockIdx.x % nbx, j = (int)(blockIdx.x / nbx) - 1;
+    const int i = bx * 256 + threadIdx.x - 1;
     if (i + 1 >= n) return;
     const size_t c = ix(pitch, i, j);
     dst[c] = src[c];
@@ -388,8 +390,8 @@ void dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const dou
         }
     }
     if (cur != phi) {
-        const int n = g.nx + 2;
-        k_copy_level<<<dim3((n + 255) / 256, g.ny + 2), 256, 0, s>>>(phi, cur, g.pitch, n);
+        const int n = g.nx + 2, nbx = (n + 255) / 256;
+        k_copy_level<<<nbx * (g.ny + 2), 256, 0, s>>>(phi, cur, g.pitch, n, nbx);
     }
 }
 
