@@ -20,8 +20,9 @@
 // reads: the result goes to a second array (ping-pong between phi and a
 // scratch level, with a final copy when the count of launches is odd).
 //
-// Tile shapes (rows per window, waves per workgroup) and the sweeps per load
-// are picked per level shape by timing the candidates on first use.
+// The tile shape (rows per window, waves per workgroup) is picked per level
+// shape by timing the candidates on first use.  Sweeps per load follow the
+// call: 2 per launch, 1 for an odd remainder.
 #include <hip/hip_runtime.h>
 
 #include <map>
