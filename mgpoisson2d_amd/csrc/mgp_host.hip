@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "mgp2d.h"
@@ -329,8 +330,12 @@ void RefHierarchy::relax(hipStream_t s, int d, double *phi, const double *rhs, i
         }
     } else {
         if (ordering == MGP_ORDER_RB) {
-            // w is free until the residual after the smoother
-            dc_rb_relax(s, phi, l.w.p, l.mod.p, rhs, l.invd.p, l.g, l.geo, l.bc, 1, omega, sweeps);
+            // w is free until the residual after the smoother.  phi is the
+            // level's e; when the result ends in w the two swap roles (every
+            // later use reads l.e.p afresh), which saves the copy back.
+            if (dc_rb_relax(s, phi, l.w.p, l.mod.p, rhs, l.invd.p, l.g, l.geo, l.bc, 1, omega, sweeps,
+                            phi == l.e.p))
+                std::swap(l.e, l.w);
         } else {
             launch_dc_modrhs(s, l.mod.p, rhs, phi, l.g, l.geo);
             launch_dc_lex(s, phi, l.mod.p, l.invd.p, l.g, l.geo, l.bc, 1, omega, sweeps);

@@ -36,9 +36,12 @@ void launch_dc_lex(hipStream_t s, double *phi, const double *modrhs, const doubl
 // times fill ghosts, colour 0, colour 1.  On LAH Neumann/Dirichlet sides the
 // sweeps run in LDS tiles, several per load (mgp_rbtile.hip), ping-ponging
 // through tmp; otherwise one launch per colour.  mod and tmp are level-sized
-// scratch; the result is bit-identical either way.
-void dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const double *rhs, const double *invd,
-                 const Grid &g, const Geo &geo, const Bc &bc, int homog, double omega, int sweeps);
+// scratch; the result is bit-identical either way.  may_end_in_tmp: when the
+// result lies in tmp, leave it there and return true (the caller swaps the
+// two arrays' roles) instead of copying it back to phi.
+bool dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const double *rhs, const double *invd,
+                 const Grid &g, const Geo &geo, const Bc &bc, int homog, double omega, int sweeps,
+                 bool may_end_in_tmp = false);
 // -1: tile shape timed per level (default); 0: one launch per colour; k >= 1:
 // tile shape k - 1 of dc_rb_tile_shapes().  Returns the previous setting.
 int dc_rb_tile_choice(int choice);

@@ -355,9 +355,9 @@ int dc_rb_tile_once(hipStream_t s, const double *phi, double *out, const double 
     return shape;
 }
 
-void dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const double *rhs, const double *invd,
-                 const Grid &g, const Geo &geo, const Bc &bc, int homog, double omega, int sweeps) {
-    if (sweeps <= 0) return;
+bool dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const double *rhs, const double *invd,
+                 const Grid &g, const Geo &geo, const Bc &bc, int homog, double omega, int sweeps, bool may_end_in_tmp) {
+    if (sweeps <= 0) return false;
     int choice;
     {
         std::lock_guard<std::mutex> lk(rb_mu);
@@ -370,7 +370,7 @@ void dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const dou
             launch_dc_rb(s, phi, mod, invd, g, geo, omega, 0);
             launch_dc_rb(s, phi, mod, invd, g, geo, omega, 1);
         }
-        return;
+        return false;
     }
     RbGhost gh;
     for (int k = 0; k < 4; ++k) gh.sgn[k] = bc.type[k] == BC_NEUM ? 1.0 : -1.0;
@@ -390,10 +390,11 @@ void dc_rb_relax(hipStream_t s, double *phi, double *tmp, double *mod, const dou
             left -= S;
         }
     }
-    if (cur != phi) {
-        const int n = g.nx + 2, nbx = (n + 255) / 256;
-        k_copy_level<<<nbx * (g.ny + 2), 256, 0, s>>>(phi, cur, g.pitch, n, nbx);
-    }
+    if (cur == phi) return false;
+    if (may_end_in_tmp) return true;
+    const int n = g.nx + 2, nbx = (n + 255) / 256;
+    k_copy_level<<<nbx * (g.ny + 2), 256, 0, s>>>(phi, cur, g.pitch, n, nbx);
+    return false;
 }
 
 }  // namespace mgp
