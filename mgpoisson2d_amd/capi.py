@@ -89,9 +89,22 @@ def _check(status):
         raise MgpError(f"{lib.mgp_status_string(status).decode()}: {msg}")
 
 
-def _f64(a):
+def _f64(a, shape=None, name="array"):
+    """a as a C-contiguous float64 array; with shape given, raises ValueError
+    unless a has exactly that shape (the C side sizes every copy from the
+    grid, so a wrong shape would read or write past the caller's buffer)."""
     a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {a.shape}")
     return a
+
+
+def _cc(nx, ny):
+    return (ny, nx)
+
+
+def _gh(nx, ny):
+    return (ny + 2, nx + 2)
 
 
 def _ptr(a):
@@ -105,12 +118,21 @@ class _Args:
         self.keep = []
 
     def geo(self, g: Geometry):
-        arrs = [_f64(a) for a in g.arrays()]
+        nx, ny = g.nx, g.ny
+        shapes = ((ny, nx), (ny, nx + 1), (ny, nx + 1), (ny + 1, nx), (ny + 1, nx))
+        names = ("J", "Jxx", "Jxy", "Jyx", "Jyy")
+        arrs = [_f64(a, sh, nm) for a, sh, nm in zip(g.arrays(), shapes, names)]
         self.keep += arrs
         return ctypes.byref(_Geo(g.nx, g.ny, g.dx, g.dy, *[_ptr(a) for a in arrs]))
 
-    def bc(self, b: BoundaryConditions):
-        data = [_f64(d) for d in b.data]
+    def bc(self, b: BoundaryConditions, nx=None, ny=None):
+        data = [_f64(d).ravel() for d in b.data]
+        for k in range(4):
+            need = 1
+            if int(b.modes[k]) == 1 and nx is not None:  # BCMODE_NONUNIFORM: one value per face cell
+                need = ny if k < 2 else nx
+            if data[k].size < need:
+                raise ValueError(f"BC data of side {k} needs {need} values, got {data[k].size}")
         self.keep += data
         s = _Bc()
         for k in range(4):
@@ -126,40 +148,40 @@ def device_count():
 
 def fill_ghosts(geo, bc, homog, do_neum, phi):
     a = _Args()
-    phi = _f64(phi).copy()
-    _check(lib.mgp_fill_ghosts(a.geo(geo), a.bc(bc), _i(homog), _i(do_neum), _ptr(phi)))
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    _check(lib.mgp_fill_ghosts(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _i(do_neum), _ptr(phi)))
     return phi
 
 
 def compute_laplacian(geo, bc, homog, phi, jscale=False):
     a = _Args()
-    phi = _f64(phi).copy()
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
     lap = np.zeros((geo.ny, geo.nx))
-    _check(lib.mgp_compute_laplacian(a.geo(geo), a.bc(bc), _i(homog), _i(jscale), _ptr(phi), _ptr(lap)))
+    _check(lib.mgp_compute_laplacian(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _i(jscale), _ptr(phi), _ptr(lap)))
     return lap, phi
 
 
 def compute_residual(geo, bc, homog, rhs, phi):
     a = _Args()
-    phi = _f64(phi).copy()
-    rhs = _f64(rhs)
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    rhs = _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
     res = np.zeros((geo.ny, geo.nx))
-    _check(lib.mgp_compute_residual(a.geo(geo), a.bc(bc), _i(homog), _ptr(rhs), _ptr(phi), _ptr(res)))
+    _check(lib.mgp_compute_residual(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(rhs), _ptr(phi), _ptr(res)))
     return res, phi
 
 
 def compute_invdiags(geo, bc, homog):
     a = _Args()
     out = np.zeros((geo.ny, geo.nx))
-    _check(lib.mgp_compute_invdiags(a.geo(geo), a.bc(bc), _i(homog), _ptr(out)))
+    _check(lib.mgp_compute_invdiags(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(out)))
     return out
 
 
 def relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_LEX):
     a = _Args()
-    phi = _f64(phi).copy()
-    invd, rhs = _f64(invd), _f64(rhs)
-    _check(lib.mgp_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    invd, rhs = _f64(invd, _cc(geo.nx, geo.ny), "invdiags"), _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
+    _check(lib.mgp_relax_gs(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
                             _ptr(rhs), _ptr(phi), _i(ordering)))
     return phi
 
@@ -167,17 +189,17 @@ def relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_LEX
 def xline_relax(geo, bc, rhs, phi, sweeps):
     """x-line zebra sweeps of the homogeneous operator (the fast solver's smoother)."""
     a = _Args()
-    phi = _f64(phi).copy()
-    rhs = _f64(rhs)
-    _check(lib.mgp_xline_relax(a.geo(geo), a.bc(bc), _ptr(rhs), _ptr(phi), _i(sweeps)))
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    rhs = _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
+    _check(lib.mgp_xline_relax(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _ptr(rhs), _ptr(phi), _i(sweeps)))
     return phi
 
 
 def dc_compute_laplacian(geo, bc, homog, phi):
     a = _Args()
-    phi = _f64(phi).copy()
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
     lap = np.zeros((geo.ny, geo.nx))
-    _check(lib.mgp_dc_compute_laplacian(a.geo(geo), a.bc(bc), _i(homog), _ptr(phi), _ptr(lap)))
+    _check(lib.mgp_dc_compute_laplacian(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(phi), _ptr(lap)))
     return lap, phi
 
 
@@ -190,9 +212,9 @@ def dc_compute_invdiags(geo):
 
 def dc_relax_gs(geo, bc, homog, invd, omega, maxiters, rhs, phi, ordering=ORDER_LEX):
     a = _Args()
-    phi = _f64(phi).copy()
-    invd, rhs = _f64(invd), _f64(rhs)
-    _check(lib.mgp_dc_relax_gs(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    invd, rhs = _f64(invd, _cc(geo.nx, geo.ny), "invdiags"), _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
+    _check(lib.mgp_dc_relax_gs(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(invd), _d(omega), _i(maxiters),
                                _ptr(rhs), _ptr(phi), _i(ordering)))
     return phi
 
@@ -213,7 +235,7 @@ def dc_rb_tile_shapes():
 
 def compute_pd(nx, ny, dx, dy, phi, direction, nodedir):
     """Poisson2D::compute_pd: pd is (ny, nx+1) for nodedir 1, (ny+1, nx) for nodedir 2."""
-    phi = _f64(phi)
+    phi = _f64(phi, _gh(nx, ny), "phi")
     pd = np.zeros((ny, nx + 1) if nodedir == 1 else (ny + 1, nx))
     _check(lib.mgp_compute_pd(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(phi), _i(direction), _i(nodedir), _ptr(pd)))
     return pd
@@ -222,15 +244,15 @@ def compute_pd(nx, ny, dx, dy, phi, direction, nodedir):
 def compute_grad(geo, bc, homog, phi):
     """Poisson2D::compute_grad: returns (phi with ghosts filled, xflux (ny, nx+1), yflux (ny+1, nx))."""
     a = _Args()
-    phi = _f64(phi).copy()
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
     xf = np.zeros((geo.ny, geo.nx + 1))
     yf = np.zeros((geo.ny + 1, geo.nx))
-    _check(lib.mgp_compute_grad(a.geo(geo), a.bc(bc), _i(homog), _ptr(phi), _ptr(xf), _ptr(yf)))
+    _check(lib.mgp_compute_grad(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(phi), _ptr(xf), _ptr(yf)))
     return phi, xf, yf
 
 
 def compute_div(nx, ny, dx, dy, xflux, yflux):
-    xflux, yflux = _f64(xflux), _f64(yflux)
+    xflux, yflux = _f64(xflux, (ny, nx + 1), "xflux"), _f64(yflux, (ny + 1, nx), "yflux")
     div = np.zeros((ny, nx))
     _check(lib.mgp_compute_div(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(xflux), _ptr(yflux), _ptr(div)))
     return div
@@ -238,8 +260,8 @@ def compute_div(nx, ny, dx, dy, xflux, yflux):
 
 def fill_boundary_fluxes(nx, ny, bc, homog, xflux, yflux):
     a = _Args()
-    xf, yf = _f64(xflux).copy(), _f64(yflux).copy()
-    _check(lib.mgp_fill_boundary_fluxes(_i(nx), _i(ny), a.bc(bc), _i(homog), _ptr(xf), _ptr(yf)))
+    xf, yf = _f64(xflux, (ny, nx + 1), "xflux").copy(), _f64(yflux, (ny + 1, nx), "yflux").copy()
+    _check(lib.mgp_fill_boundary_fluxes(_i(nx), _i(ny), a.bc(bc, nx, ny), _i(homog), _ptr(xf), _ptr(yf)))
     return xf, yf
 
 
@@ -254,15 +276,16 @@ def metric_fill(mapping, nx, ny, dx, dy):
 def restrict(kind, fnx, fny, fine, cnx, cny):
     shape = {0: (cny, cnx), 1: (cny, cnx + 1), 2: (cny + 1, cnx)}[kind]
     crse = np.zeros(shape)
-    fine = _f64(fine)
+    fshape = {0: (fny, fnx), 1: (fny, fnx + 1), 2: (fny + 1, fnx)}[kind]
+    fine = _f64(fine, fshape, "fine")
     _check(lib.mgp_restrict(_i(kind), _i(fnx), _i(fny), _ptr(fine), _i(cnx), _i(cny), _ptr(crse)))
     return crse
 
 
 def prolong(fgeo, cgeo, bc_types, bc_modes, order, fine, crse):
     a = _Args()
-    fine = _f64(fine).copy()
-    crse = _f64(crse).copy()
+    fine = _f64(fine, _gh(fgeo.nx, fgeo.ny), "fine").copy()
+    crse = _f64(crse, _gh(cgeo.nx, cgeo.ny), "crse").copy()
     t = (_i * 4)(*[int(x) for x in bc_types])
     m = (_i * 4)(*[int(x) for x in bc_modes])
     _check(lib.mgp_prolong(a.geo(fgeo), a.geo(cgeo), t, m, _i(order), _ptr(fine), _ptr(crse)))
@@ -279,7 +302,7 @@ def create_mgschedule(nx, ny, dx, dy, maxdepth=-1):
 
 def inner_prod(nx, ny, dx, dy, a, b):
     out = _d(0.0)
-    a, b = _f64(a), _f64(b)
+    a, b = _f64(a, (ny, nx), "a"), _f64(b, (ny, nx), "b")
     _check(lib.mgp_inner_prod(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), _ptr(b), ctypes.byref(out)))
     return out.value
 
@@ -287,22 +310,22 @@ def inner_prod(nx, ny, dx, dy, a, b):
 def integrate2d(nx, ny, dx, dy, a, J=None):
     """arrayutils integrate2d: sum dx dy J a (J given) or dx dy a."""
     out = _d(0.0)
-    a = _f64(a)
-    J = None if J is None else _f64(J)
+    a = _f64(a, (ny, nx), "a")
+    J = None if J is None else _f64(J, (ny, nx), "J")
     _check(lib.mgp_integrate2d(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), None if J is None else _ptr(J),
                                ctypes.byref(out)))
     return out.value
 
 
 def extrapolate_ghosts(nx, ny, order, phi):
-    phi = _f64(phi).copy()
+    phi = _f64(phi, _gh(nx, ny), "phi").copy()
     _check(lib.mgp_extrapolate_ghosts(_i(nx), _i(ny), _i(order), _ptr(phi)))
     return phi
 
 
 def pnorm(nx, ny, dx, dy, arr, p=2):
     out = _d(0.0)
-    arr = _f64(arr)
+    arr = _f64(arr, (ny, nx), "arr")
     _check(lib.mgp_pnorm(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(arr), _i(p), ctypes.byref(out)))
     return out.value
 
@@ -310,11 +333,11 @@ def pnorm(nx, ny, dx, dy, arr, p=2):
 def relax_jacobi(geo, bc, homog, invd, omega, tol, maxiters, zerophi, rhs, phi, ordering=ORDER_LEX):
     """Poisson2D::relax_jacobi: returns (phi, relres history (when tol > 0), sweeps)."""
     a = _Args()
-    phi = _f64(phi).copy()
-    invd, rhs = _f64(invd), _f64(rhs)
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    invd, rhs = _f64(invd, _cc(geo.nx, geo.ny), "invdiags"), _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
     relres = np.zeros(max(maxiters, 0) + 1)
     n = _i(0)
-    _check(lib.mgp_relax_jacobi(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(omega), _d(tol), _i(maxiters),
+    _check(lib.mgp_relax_jacobi(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(invd), _d(omega), _d(tol), _i(maxiters),
                                 _i(zerophi), _ptr(rhs), _ptr(phi), _i(ordering), _ptr(relres), ctypes.byref(n)))
     return phi, relres[: n.value + 1], n.value
 
@@ -322,11 +345,11 @@ def relax_jacobi(geo, bc, homog, invd, omega, tol, maxiters, zerophi, rhs, phi, 
 def solve_bicgstab(geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi, ordering=ORDER_LEX):
     """Poisson2D::solve_bicgstab (version 1): returns (phi, relres history)."""
     a = _Args()
-    phi = _f64(phi).copy()
-    invd, rhs = _f64(invd), _f64(rhs)
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    invd, rhs = _f64(invd, _cc(geo.nx, geo.ny), "invdiags"), _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
     relres = np.zeros(maxiters + maxrestarts + 1)
     last = _i(0)
-    _check(lib.mgp_solve_bicgstab(a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(tol), _i(maxiters),
+    _check(lib.mgp_solve_bicgstab(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(invd), _d(tol), _i(maxiters),
                                   _i(maxrestarts), _i(zerophi), _ptr(rhs), _ptr(phi), _i(ordering), _ptr(relres),
                                   ctypes.byref(last)))
     return phi, relres[: last.value + 1]
@@ -335,9 +358,9 @@ def solve_bicgstab(geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rh
 def solve_bicgstab2(kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi,
                     ordering=ORDER_LEX):
     a = _Args()
-    phi = _f64(phi).copy()
-    invd, rhs = _f64(invd), _f64(rhs)
-    _check(lib.mgp_solve_bicgstab2(_i(kind), a.geo(geo), a.bc(bc), _i(homog), _ptr(invd), _d(tol),
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    invd, rhs = _f64(invd, _cc(geo.nx, geo.ny), "invdiags"), _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
+    _check(lib.mgp_solve_bicgstab2(_i(kind), a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _ptr(invd), _d(tol),
                                    _i(maxiters), _i(maxrestarts), _i(zerophi), _ptr(rhs), _ptr(phi),
                                    _i(ordering)))
     return phi
@@ -348,11 +371,11 @@ def vcycle(kind, geo, bc, homog, tol, maxiters, phi, rhs, maxdepth=-1, numcycles
     """Reference vcycle (kind 0) / DC vcycle (kind 1) / fmg (kind 2) on the GPU.
     Returns (phi, rhs, relres history, iterations, outcome)."""
     a = _Args()
-    phi = _f64(phi).copy()
-    rhs = _f64(rhs).copy()
+    phi = _f64(phi, _gh(geo.nx, geo.ny), "phi").copy()
+    rhs = _f64(rhs, _cc(geo.nx, geo.ny), "rhs").copy()
     relres = np.zeros(maxiters + 1)
     iters, outcome = _i(0), _i(0)
-    _check(lib.mgp_vcycle(_i(kind), a.geo(geo), a.bc(bc), _i(homog), _d(tol), _i(maxiters), _i(maxdepth),
+    _check(lib.mgp_vcycle(_i(kind), a.geo(geo), a.bc(bc, geo.nx, geo.ny), _i(homog), _d(tol), _i(maxiters), _i(maxdepth),
                           _i(numcycles), _i(smooth_down), _i(smooth_up), _i(smooth_bottom), _i(zerophi),
                           _ptr(phi), _ptr(rhs), _i(ordering), _ptr(relres), ctypes.byref(iters),
                           ctypes.byref(outcome)))
@@ -373,10 +396,10 @@ class Solver:
         a = _Args()
         self._h = ctypes.c_void_p()
         if mapping is None:
-            _check(lib.mgp_solver_create(ctypes.byref(self._h), a.geo(geo), a.bc(bc), ctypes.byref(opts)))
+            _check(lib.mgp_solver_create(ctypes.byref(self._h), a.geo(geo), a.bc(bc, geo.nx, geo.ny), ctypes.byref(opts)))
         else:
             _check(lib.mgp_solver_create_mapped(ctypes.byref(self._h), ctypes.byref(mapping), _i(geo.nx),
-                                                _i(geo.ny), _d(geo.dx), _d(geo.dy), a.bc(bc), ctypes.byref(opts)))
+                                                _i(geo.ny), _d(geo.dx), _d(geo.dy), a.bc(bc, geo.nx, geo.ny), ctypes.byref(opts)))
 
     def close(self):
         if self._h:
@@ -390,11 +413,11 @@ class Solver:
             pass
 
     def set_rhs(self, rhs):
-        rhs = _f64(rhs)
+        rhs = _f64(rhs, _cc(self.geo.nx, self.geo.ny), "rhs")
         _check(lib.mgp_solver_set_rhs(self._h, _ptr(rhs)))
 
     def set_phi(self, phi):
-        phi = _f64(phi)
+        phi = _f64(phi, _gh(self.geo.nx, self.geo.ny), "phi")
         _check(lib.mgp_solver_set_phi(self._h, _ptr(phi)))
 
     def get_phi(self):
@@ -418,7 +441,8 @@ class Solver:
 
     def set_flux(self, xflux, yflux):
         """Resident face fluxes: xflux (ny, nx+1), yflux (ny+1, nx)."""
-        xflux, yflux = _f64(xflux), _f64(yflux)
+        nx, ny = self.geo.nx, self.geo.ny
+        xflux, yflux = _f64(xflux, (ny, nx + 1), "xflux"), _f64(yflux, (ny + 1, nx), "yflux")
         _check(lib.mgp_solver_set_flux(self._h, _ptr(xflux), _ptr(yflux)))
 
     def get_flux(self):
@@ -473,7 +497,7 @@ class Batch:
         a = _Args()
         bc_arr = (_Bc * n)()
         for k, b in enumerate(bcs):
-            bc_arr[k] = a.bc(b)._obj
+            bc_arr[k] = a.bc(b, geos[k].nx, geos[k].ny)._obj
         self._h = ctypes.c_void_p()
         if mappings is None:
             geo_arr = (_Geo * n)()
@@ -542,7 +566,7 @@ class Server:
         a = _Args()
         bc_arr = (_Bc * n)()
         for k, b in enumerate(bcs):
-            bc_arr[k] = a.bc(b)._obj
+            bc_arr[k] = a.bc(b, geos[k].nx, geos[k].ny)._obj
         maps = (Mapping * n)(*mappings)
         nx = (_i * n)(*[g.nx for g in geos])
         ny = (_i * n)(*[g.ny for g in geos])

@@ -289,19 +289,27 @@ int rb_fewest_cells(const Grid &g, int S) {
 
 // The shape for this level, sweeps per load and form: timed once per
 // (nx, ny, S, fuse) on the call's own operands (the timing launches write
-// the same values the real launch writes next).
+// the same values the real launch writes next).  The lock guards only the
+// setting and the cache: the timing runs without it, so other threads'
+// sweeps do not wait for it.  Two threads that time the same key at once
+// both insert; the first insertion stays (every shape gives the same bits).
 int rb_pick(hipStream_t s, int S, bool fuse, const double *in, double *out, const double *src, const double *invd,
             const Grid &g, const Geo &geo, const RbGhost &gh, double omega) {
-    std::lock_guard<std::mutex> lk(rb_mu);
-    if (rb_choice > 0) return rb_choice - 1;
     const auto key = std::make_tuple(g.nx, g.ny, S, (int)fuse);
-    auto it = rb_picked.find(key);
-    if (it != rb_picked.end()) return it->second;
+    {
+        std::lock_guard<std::mutex> lk(rb_mu);
+        if (rb_choice > 0) return rb_choice - 1;
+        auto it = rb_picked.find(key);
+        if (it != rb_picked.end()) return it->second;
+    }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
     if ((long long)g.nx * g.ny < 256LL * 256 || cs != hipStreamCaptureStatusNone) {
         const int k = rb_fewest_cells(g, S);
-        if (cs == hipStreamCaptureStatusNone) rb_picked[key] = k;
+        if (cs == hipStreamCaptureStatusNone) {
+            std::lock_guard<std::mutex> lk(rb_mu);
+            rb_picked.emplace(key, k);
+        }
         return k;
     }
     hipEvent_t e0, e1;
@@ -321,8 +329,8 @@ int rb_pick(hipStream_t s, int S, bool fuse, const double *in, double *out, cons
     }
     hip_check(hipEventDestroy(e0), "hipEventDestroy");
     hip_check(hipEventDestroy(e1), "hipEventDestroy");
-    rb_picked[key] = best;
-    return best;
+    std::lock_guard<std::mutex> lk(rb_mu);
+    return rb_picked.emplace(key, best).first->second;
 }
 
 bool rb_tiles_apply(const Bc &bc) {

@@ -1648,6 +1648,19 @@ void mgo_relax_gs(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, doub
     relax_gs(&G, &B, homog, invd, omega, maxiters, rhs, phi, order);
 }
 
+/* One point update of relax_gs (poisson2d.f90:1770-1771) at cell (i, j) on phi
+ * as it stands, ghosts included (no fill): the update every ordering of
+ * relax_gs is made of.  Exported so that a test can compose an ordering from
+ * it and the fill without going through relax_gs's own loops. */
+void mgo_gs_point(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double omega,
+                  const double *rhs, double *phi, int i, int j)
+{
+    MKG; MKB;
+    check_lah(&B);
+    Ring rg = mkring(&G, &B);
+    gs_point(&G, &B, &rg, homog, invd, omega, rhs, phi, i, j);
+}
+
 void mgo_dc_laplacian(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double *lap)
 {
     MKG; MKB;

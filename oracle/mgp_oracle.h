@@ -52,6 +52,9 @@ void mgo_residual(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *rhs, doubl
 void mgo_invdiags(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *invd);
 void mgo_relax_gs(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double omega,
                   int maxiters, const double *rhs, double *phi, int order);
+/* One relax_gs point update at (i, j) on phi as it stands (no ghost fill). */
+void mgo_gs_point(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double omega,
+                  const double *rhs, double *phi, int i, int j);
 void mgo_dc_laplacian(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, double *phi, double *lap);
 void mgo_dc_invdiags(MGO_GEO_ARGS, double *invd);
 void mgo_dc_relax_gs(MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double omega,
@@ -105,6 +108,15 @@ void mgo_metric_terrain(int nx, int ny, double dx, double dy, double amp, double
 void mgo_fill_boundary_fluxes(int nx, int ny, MGO_BC_ARGS, int homog, double *xflux, double *yflux);
 void mgo_compute_div(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
                      double *div);
+
+/* Fourth-order finite-volume operator (mgp_oracle4.c; no reference
+ * counterpart).  Face arrays: x faces (nx+1)*ny, y faces nx*(ny+1), i fastest;
+ * phi and lap are nx*ny cell averages (no ghosts); nx, ny >= 5. */
+void mgo_metric4(int nx, int ny, double dx, double dy, double amp, double beta, double L, double H, double *A4,
+                 double *Bx4, double *dA, double *dBx, double *C4, double *By4, double *dC, double *dBy);
+void mgo_lap4(int nx, int ny, double dx, double dy, const double *A4, const double *Bx4, const double *dA,
+              const double *dBx, const double *C4, const double *By4, const double *dC, const double *dBy,
+              const double *phi, double *lap);
 
 #ifdef __cplusplus
 }

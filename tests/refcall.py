@@ -143,6 +143,35 @@ class Checker:
         self._fn("relax_gs")(*args)
         return phi
 
+    def gs_point(self, geo, bc, homog, invd, omega, rhs, phi, i, j):
+        """One relax_gs point update at (i, j) in place on phi (ghosts as they
+        stand; oracle only)."""
+        assert self.is_oracle and phi.flags.c_contiguous
+        self._fn("gs_point")(*self._geo_args(geo), *self._bc_args(bc), _i(homog),
+                             _ptr(np.ascontiguousarray(invd)), _d(omega), _ptr(np.ascontiguousarray(rhs)),
+                             _ptr(phi), _i(i), _i(j))
+
+    # -- fourth-order operator (oracle only) ---------------------------------
+    MET4 = ("A4", "Bx4", "dA", "dBx", "C4", "By4", "dC", "dBy")
+
+    def metric4(self, nx, ny, dx, dy, amp, beta, L=0.0, H=0.0):
+        """Face-averaged metric set of the warped mapping: dict of x-face
+        arrays (ny, nx+1) A4, Bx4, dA, dBx and y-face arrays (ny+1, nx) C4,
+        By4, dC, dBy."""
+        out = {k: np.zeros((ny, nx + 1)) for k in self.MET4[:4]}
+        out.update({k: np.zeros((ny + 1, nx)) for k in self.MET4[4:]})
+        self._fn("metric4")(_i(nx), _i(ny), _d(dx), _d(dy), _d(amp), _d(beta), _d(L), _d(H),
+                            *[_ptr(out[k]) for k in self.MET4])
+        return out
+
+    def lap4(self, nx, ny, dx, dy, met, phi):
+        """Fourth-order operator on cell averages phi (ny, nx)."""
+        lap = np.zeros((ny, nx))
+        arrs = [np.ascontiguousarray(met[k], dtype=np.float64) for k in self.MET4]
+        self._fn("lap4")(_i(nx), _i(ny), _d(dx), _d(dy), *[_ptr(a) for a in arrs],
+                         _ptr(np.ascontiguousarray(phi, dtype=np.float64)), _ptr(lap))
+        return lap
+
     def dc_laplacian(self, geo, bc, homog, phi):
         phi = phi.copy()
         lap = np.zeros((geo.ny, geo.nx))

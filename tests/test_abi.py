@@ -91,3 +91,44 @@ def test_batch_and_server_arguments_rejected_without_device_work():
         capi._check(capi.lib.mgp_server_result(None, ctypes.c_longlong(1), None, None, None, ctypes.byref(done)))
     assert capi.lib.mgp_batch_count(None) == 0 and capi.lib.mgp_batch_last_cycles(None) == 0
     assert capi.lib.mgp_server_footprint(None) == 0.0
+
+
+def test_wrong_shapes_raise_before_the_library_is_called():
+    """Every wrapper checks its arrays against the grid: the C side sizes its
+    copies from nx, ny, so a wrong shape must never reach it."""
+    import numpy as np
+    import pytest
+    from mgpoisson2d_amd import capi, geometry as G
+    nx, ny = 12, 8
+    geo = G.mapped_geometry(nx, ny, 0.1, 2.0)
+    bc = G.BoundaryConditions.uniform((1, 1, 0, 0))
+    good_phi, good_cc = np.zeros((ny + 2, nx + 2)), np.zeros((ny, nx))
+    bad = [
+        lambda: capi.fill_ghosts(geo, bc, 1, 0, np.zeros((ny, nx))),
+        lambda: capi.compute_laplacian(geo, bc, 1, np.zeros((nx + 2, ny + 2))),
+        lambda: capi.compute_residual(geo, bc, 1, np.zeros((nx, ny)), good_phi),
+        lambda: capi.relax_gs(geo, bc, 1, good_cc, 1.0, 1, good_cc, np.zeros((ny + 1, nx + 2))),
+        lambda: capi.relax_gs(geo, bc, 1, np.zeros(nx * ny), 1.0, 1, good_cc, good_phi),
+        lambda: capi.dc_relax_gs(geo, bc, 1, good_cc, 1.0, 1, np.zeros((ny, nx + 1)), good_phi),
+        lambda: capi.xline_relax(geo, bc, good_cc, good_phi[:-1], 1),
+        lambda: capi.compute_grad(geo, bc, 1, good_cc),
+        lambda: capi.compute_div(nx, ny, 1.0, 1.0, np.zeros((ny + 1, nx)), np.zeros((ny + 1, nx))),
+        lambda: capi.fill_boundary_fluxes(nx, ny, bc, 1, np.zeros((ny, nx)), np.zeros((ny + 1, nx))),
+        lambda: capi.restrict(0, nx, ny, np.zeros((ny, nx + 1)), nx // 2, ny // 2),
+        lambda: capi.pnorm(nx, ny, 1.0, 1.0, np.zeros((nx, ny)), 2),
+        lambda: capi.inner_prod(nx, ny, 1.0, 1.0, good_cc, np.zeros((ny + 2, nx + 2))),
+        lambda: capi.integrate2d(nx, ny, 1.0, 1.0, good_cc, np.zeros((ny, nx + 1))),
+        lambda: capi.extrapolate_ghosts(nx, ny, 2, good_cc),
+        lambda: capi.vcycle(0, geo, bc, 1, 1e-10, 2, good_cc, good_cc),
+        lambda: capi.compute_pd(nx, ny, 1.0, 1.0, good_cc, 1, 1),
+    ]
+    for k, call in enumerate(bad):
+        with pytest.raises(ValueError):
+            call()
+    # a metric array of the wrong shape, and NONUNIFORM BC data that is too short
+    geo_bad = G.Geometry(nx, ny, geo.dx, geo.dy, geo.J, geo.Jxy[:, :-1], geo.Jxy, geo.Jyx, geo.Jyy)
+    with pytest.raises(ValueError):
+        capi.compute_invdiags(geo_bad, bc, 1)
+    bc_short = G.BoundaryConditions((1, 1, 0, 0), (1, 0, 0, 0), tuple(np.zeros(3) for _ in range(4)))
+    with pytest.raises(ValueError):
+        capi.compute_invdiags(geo, bc_short, 1)

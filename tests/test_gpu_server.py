@@ -10,6 +10,10 @@
 * Slots above 2^22 cells run in a batch of their own on a second stream;
   the results of both size classes stay those of each slot alone.
 * The HBM budget is enforced at creation.
+* FP32 V-cycles inside FP64 refinement (KRYLOV_MG32, the served precision):
+  results are still those of each slot alone, and each meets 1e-10 by the
+  oracle's operator (bit-identical to the reference's) on the GPU's own
+  metrics.
 """
 from types import SimpleNamespace
 
@@ -42,9 +46,9 @@ SCHEDULE = [(0, 0, 0, False), (0, 1, 0, False), (3, 2, 0, False), (3, 0, 1, True
             (9, 1, 1, True), (12, 2, 1, False)]
 
 
-def run(schedule, slots=SLOTS):
+def run(schedule, slots=SLOTS, method=mg.METHOD_KRYLOV_MG):
     geos, maps, bcs = slot_defs(slots)
-    v = mg.Server(geos, bcs, maps)
+    v = mg.Server(geos, bcs, maps, method=method)
     out, pending, step = {}, {}, 0
     todo = sorted(schedule)
     while todo or pending:
@@ -139,3 +143,28 @@ def test_server_budget():
     with pytest.raises(mg.MgpError):
         mg.Server(geos, bcs, maps, hbm_budget=need / 2)
     mg.Server(geos, bcs, maps, hbm_budget=need * 1.01).close()
+
+
+def test_server_mixed_precision_independent_and_accurate():
+    import refcall as rc
+    method = mg.METHOD_KRYLOV_MG32
+    mixed = run(SCHEDULE, method=method)
+    assert len(mixed) == len(SCHEDULE)
+    for slot in range(len(SLOTS)):
+        alone = run([s for s in SCHEDULE if s[1] == slot], method=method)
+        for key, (phi, it, oc) in alone.items():
+            phi2, it2, oc2 = mixed[key]
+            assert oc == mg.CONVERGED and oc2 == mg.CONVERGED
+            assert it == it2
+            np.testing.assert_array_equal(phi, phi2)
+    # accuracy against the oracle's operator on the metrics the server built
+    o = rc.oracle()
+    geos, maps, bcs = slot_defs(SLOTS)
+    for (slot, k), (phi, _, _) in mixed.items():
+        g = geos[slot]
+        geo = mg.metric_fill(maps[slot], g.nx, g.ny, g.dx, g.dy)
+        full = G.new_phi(g.nx, g.ny)
+        full[1:-1, 1:-1] = phi
+        rhs = rhs_for(slot, k)
+        r, _ = o.residual(geo, bcs[slot], 1, rhs, full)
+        assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(rhs), (slot, k)
