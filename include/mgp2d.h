@@ -140,6 +140,18 @@ int mgp_inner_prod(int nx, int ny, double dx, double dy, const double *a, const 
 /* arrayutils.f90:670 integrate2d(bd, bx, geo, opt_jscale): sum of dx dy J a (J given, jscale true)
  * or dx dy a (J == NULL). */
 int mgp_integrate2d(int nx, int ny, double dx, double dy, const double *a, const double *J, double *out);
+/* arrayutils.f90:721 integrate2d_bdry(xflux, yflux, bx): sum over j of dx (xflux(nx, j) - xflux(0, j)),
+ * then over i of dy (yflux(i, ny) - yflux(i, 0)), in the reference's order (bit-identical); xflux
+ * (nx+1) x ny, yflux nx x (ny+1) over the box's faces; dx, dy the box's spacing. */
+int mgp_integrate2d_bdry(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux,
+                         double *out);
+/* arrayutils.f90:802 gpnorm(bd, p) of a one-ghost cell field over the valid box ilo..ihi, jlo..jhi
+ * (phi holds data(ilo-1:ihi+1, jlo-1:jhi+1), column-major).  As the reference: p = 0 the max of |.|,
+ * else (sum dy phi^p + sum dx phi^p)^(1/p) without absolute values, over the columns i = jlo-1 and
+ * jhi+1 and the rows j = ilo-1 and ihi+1 (the ghost layer when ilo == jlo and ihi == jhi), corner
+ * ghosts skipped, summed in the reference's order.  MGP_ERR_ARG where that index pattern leaves the
+ * array (the reference reads out of bounds there). */
+int mgp_gpnorm(int ilo, int ihi, int jlo, int jhi, double dx, double dy, const double *phi, int p, double *out);
 /* arrayutils.f90:1497 extrapolate_ghosts(phi, order), order 0..3, on a one-ghost field
  * (nx+2) x (ny+2): x-side ghost columns first (over all rows), then y-side rows. */
 int mgp_extrapolate_ghosts(int nx, int ny, int order, double *phi);

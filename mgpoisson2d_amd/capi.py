@@ -79,7 +79,7 @@ EXPORTS = (
     "mgp_inner_prod", "mgp_integrate2d", "mgp_extrapolate_ghosts", "mgp_batch_create",
     "mgp_batch_create_mapped", "mgp_batch_destroy", "mgp_batch_count", "mgp_batch_set_rhs", "mgp_batch_get_phi",
     "mgp_batch_solve", "mgp_batch_last_cycles", "mgp_server_create", "mgp_server_destroy", "mgp_server_footprint",
-    "mgp_server_submit", "mgp_server_step", "mgp_server_result",
+    "mgp_server_submit", "mgp_server_step", "mgp_server_result", "mgp_integrate2d_bdry", "mgp_gpnorm",
 )
 
 
@@ -314,6 +314,22 @@ def integrate2d(nx, ny, dx, dy, a, J=None):
     J = None if J is None else _f64(J, (ny, nx), "J")
     _check(lib.mgp_integrate2d(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), None if J is None else _ptr(J),
                                ctypes.byref(out)))
+    return out.value
+
+
+def integrate2d_bdry(nx, ny, dx, dy, xflux, yflux):
+    """arrayutils integrate2d_bdry: the net outward flux, xflux (ny, nx+1), yflux (ny+1, nx)."""
+    out = _d(0.0)
+    xflux, yflux = _f64(xflux, (ny, nx + 1), "xflux"), _f64(yflux, (ny + 1, nx), "yflux")
+    _check(lib.mgp_integrate2d_bdry(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(xflux), _ptr(yflux), ctypes.byref(out)))
+    return out.value
+
+
+def gpnorm(ilo, ihi, jlo, jhi, dx, dy, phi, p=0):
+    """arrayutils gpnorm of a one-ghost field phi (jhi-jlo+3, ihi-ilo+3) on the box ilo..ihi x jlo..jhi."""
+    out = _d(0.0)
+    phi = _f64(phi, (jhi - jlo + 3, ihi - ilo + 3), "phi")
+    _check(lib.mgp_gpnorm(_i(ilo), _i(ihi), _i(jlo), _i(jhi), _d(dx), _d(dy), _ptr(phi), _i(p), ctypes.byref(out)))
     return out.value
 
 

@@ -479,6 +479,58 @@ int mgp_fill_boundary_fluxes(int nx, int ny, const mgp_bc *bc, int homog, double
     });
 }
 
+int mgp_integrate2d_bdry(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux, double *out) {
+    return guarded([&] {
+        if (nx <= 0 || ny <= 0 || !xflux || !yflux || !out) {
+            g_last_error = "integrate2d_bdry: bad shape or null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf xf, yf, sum;
+        xf.alloc(level_elems(nx, ny));
+        yf.alloc(level_elems(nx, ny));
+        sum.alloc(1);
+        upload_block(s, xf.p, g, xflux, 0, 0, nx + 1, ny);
+        upload_block(s, yf.p, g, yflux, 0, 0, nx, ny + 1);
+        launch_integrate_bdry(s, sum.p, xf.p, yf.p, g);
+        hip_check(hipMemcpyAsync(out, sum.p, sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int mgp_gpnorm(int ilo, int ihi, int jlo, int jhi, double dx, double dy, const double *phi, int p, double *out) {
+    return guarded([&] {
+        const int nx = ihi - ilo + 1, ny = jhi - jlo + 1;
+        if (nx <= 0 || ny <= 0 || !phi || !out || p < 0) {
+            g_last_error = "gpnorm: bad box, p < 0 or null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        // The reference reads the x-boundary ghost columns at i = jlo-1, jhi+1
+        // and the y-boundary ghost rows at j = ilo-1, ihi+1 (arrayutils.f90:820-
+        // 858); on a box with ilo == jlo and ihi == jhi these are the ghost
+        // layer.  Indices relative to the valid cells:
+        const int cx0 = (jlo - 1) - ilo, cx1 = (jhi + 1) - ilo;
+        const int ry0 = (ilo - 1) - jlo, ry1 = (ihi + 1) - jlo;
+        if (cx0 < -1 || cx1 > nx || cx1 < -1 || cx0 > nx || ry0 < -1 || ry1 > ny || ry1 < -1 || ry0 > ny) {
+            g_last_error = "gpnorm: the reference's index pattern (columns jlo-1, jhi+1; rows ilo-1, ihi+1) "
+                           "falls outside the array on this box";
+            throw Failure(MGP_ERR_ARG);
+        }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        DevBuf f, sum;
+        f.alloc(level_elems(nx, ny));
+        sum.alloc(1);
+        up_phi(s, f, g, phi);
+        launch_gpnorm_sum(s, sum.p, f.p, g, p, cx0, cx1, ry0, ry1);
+        double pn = 0.0;
+        hip_check(hipMemcpyAsync(&pn, sum.p, sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+        hip_check(hipStreamSynchronize(s), "sync");
+        *out = p == 0 ? pn : std::pow(pn, 1.0 / p);
+    });
+}
+
 int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out, int *refx,
                           int *refy) {
     return guarded([&] {

@@ -276,6 +276,57 @@ __global__ void k_boundary_fluxes(double *__restrict__ xflux, double *__restrict
     else yflux[ix(g.pitch, k, g.ny)] = v;
 }
 
+// integrate2d_bdry (arrayutils.f90:721): the net outward flux, summed in the
+// reference's order (x faces row by row, then y faces column by column) by
+// one thread, so the sum is the reference's bit for bit.
+__global__ void k_integrate_bdry(double *__restrict__ out, const double *__restrict__ xflux,
+                                 const double *__restrict__ yflux, Grid g)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double sum = 0.0;
+    for (int j = 0; j < g.ny; ++j) sum = sum + g.dx * (xflux[ix(g.pitch, g.nx, j)] - xflux[ix(g.pitch, 0, j)]);
+    for (int i = 0; i < g.nx; ++i) sum = sum + g.dy * (yflux[ix(g.pitch, i, g.ny)] - yflux[ix(g.pitch, i, 0)]);
+    *out = sum;
+}
+
+// a**p for integer p >= 1 by squaring from the lowest bit (the order of the
+// Fortran runtime's integer power).
+__device__ __forceinline__ double ipow(double a, int p)
+{
+    double r = 1.0;
+    while (true) {
+        if (p & 1) r *= a;
+        p >>= 1;
+        if (!p) break;
+        a *= a;
+    }
+    return r;
+}
+
+// gpnorm (arrayutils.f90:802) before the final root: over columns cx0, cx1
+// (rows 0 .. ny-1) and rows ry0, ry1 (columns 0 .. nx-1) of phi, in the
+// reference's order, one thread.  p == 0: max |phi|; else sum of w phi^p with
+// w = dy on the columns and dx on the rows.
+__global__ void k_gpnorm_sum(double *__restrict__ out, const double *__restrict__ phi, Grid g, int p, int cx0,
+                             int cx1, int ry0, int ry1)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double pn = 0.0;
+    const int cols[2] = {cx0, cx1}, rows[2] = {ry0, ry1};
+    if (p == 0) {
+        for (int s = 0; s < 2; ++s)
+            for (int j = 0; j < g.ny; ++j) pn = fmax(pn, fabs(phi[ix(g.pitch, cols[s], j)]));
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < g.nx; ++i) pn = fmax(pn, fabs(phi[ix(g.pitch, i, rows[s])]));
+    } else {
+        for (int s = 0; s < 2; ++s)
+            for (int j = 0; j < g.ny; ++j) pn = pn + g.dy * ipow(phi[ix(g.pitch, cols[s], j)], p);
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < g.nx; ++i) pn = pn + g.dx * ipow(phi[ix(g.pitch, i, rows[s])], p);
+    }
+    *out = pn;
+}
+
 dim3 face_grid(int w, int h) { return dim3((w + FLUX_TX - 1) / FLUX_TX, (h + FLUX_TY - 1) / FLUX_TY); }
 
 }  // namespace
@@ -308,6 +359,17 @@ void launch_boundary_fluxes(hipStream_t s, double *xflux, double *yflux, const G
 {
     const int n = g.nx > g.ny ? g.nx : g.ny;
     hipLaunchKernelGGL(k_boundary_fluxes, dim3((n + 255) / 256, 4), dim3(256), 0, s, xflux, yflux, g, bc, homog);
+}
+
+void launch_integrate_bdry(hipStream_t s, double *out, const double *xflux, const double *yflux, const Grid &g)
+{
+    hipLaunchKernelGGL(k_integrate_bdry, dim3(1), dim3(64), 0, s, out, xflux, yflux, g);
+}
+
+void launch_gpnorm_sum(hipStream_t s, double *out, const double *phi, const Grid &g, int p, int cx0, int cx1, int ry0,
+                       int ry1)
+{
+    hipLaunchKernelGGL(k_gpnorm_sum, dim3(1), dim3(64), 0, s, out, phi, g, p, cx0, cx1, ry0, ry1);
 }
 
 }  // namespace mgp

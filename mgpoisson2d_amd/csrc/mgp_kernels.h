@@ -164,5 +164,12 @@ void launch_grad_subtract(hipStream_t s, double *xflux, double *yflux, const dou
 void launch_pd(hipStream_t s, double *pd, const double *phi, const Grid &g, int dir, int nodedir);
 void launch_div(hipStream_t s, double *div, const double *xflux, const double *yflux, const Grid &g);
 void launch_boundary_fluxes(hipStream_t s, double *xflux, double *yflux, const Grid &g, const Bc &bc, int homog);
+// integrate2d_bdry: *out (device) = sum dx (xflux(nx, j) - xflux(0, j)) + sum dy (yflux(i, ny) - yflux(i, 0)),
+// summed in the reference's order.
+void launch_integrate_bdry(hipStream_t s, double *out, const double *xflux, const double *yflux, const Grid &g);
+// gpnorm before its root: p == 0 max |phi|, else sum dy phi^p over columns cx0, cx1 (rows 0 .. ny-1) then
+// sum dx phi^p over rows ry0, ry1 (columns 0 .. nx-1), in the reference's order; indices -1 .. n.
+void launch_gpnorm_sum(hipStream_t s, double *out, const double *phi, const Grid &g, int p, int cx0, int cx1, int ry0,
+                       int ry1);
 
 }  // namespace mgp

@@ -173,6 +173,21 @@ module ArrayUtils
             type(mgp_bc_c), intent(in) :: bc
             real(c_double)             :: xflux(*), yflux(*)
         end function
+        integer(c_int) function mgp_integrate2d_bdry_c(nx, ny, dx, dy, xflux, yflux, out) &
+                                                       bind(C, name="mgp_integrate2d_bdry")
+            import :: c_int, c_double
+            integer(c_int), value      :: nx, ny
+            real(c_double), value      :: dx, dy
+            real(c_double), intent(in) :: xflux(*), yflux(*)
+            real(c_double)             :: out
+        end function
+        integer(c_int) function mgp_gpnorm_c(ilo, ihi, jlo, jhi, dx, dy, phi, p, out) bind(C, name="mgp_gpnorm")
+            import :: c_int, c_double
+            integer(c_int), value      :: ilo, ihi, jlo, jhi, p
+            real(c_double), value      :: dx, dy
+            real(c_double), intent(in) :: phi(*)
+            real(c_double)             :: out
+        end function
         type(c_ptr) function mgp_last_error_c() bind(C, name="mgp_last_error")
             import :: c_ptr
         end function
@@ -399,6 +414,118 @@ contains
         real(dp), intent(in)          :: val
         bd%data(bd%valid%ilo:bd%valid%ihi, bd%valid%jlo:bd%valid%jhi) = val
     end subroutine setval_valid
+
+    ! arrayutils.f90:481 -- bd on the boundary faces of src (dir 1: x faces,
+    ! dir 2: y faces), no ghosts
+    subroutine define_box_data_bdry (bd, src, dir, side)
+        type(box_data), intent(inout) :: bd
+        type(box_data), intent(in)    :: src
+        integer, intent(in)           :: dir, side
+        integer :: ierr
+        if (dir .eq. 1) then
+            bd%valid = bdry_box (src%valid, src%offi, dir, side)
+            bd%offi = BD_NODE
+            bd%offx = zero
+            bd%offj = src%offj
+            bd%offy = src%offy
+        else if (dir .eq. 2) then
+            bd%valid = bdry_box (src%valid, src%offj, dir, side)
+            bd%offi = src%offi
+            bd%offx = src%offx
+            bd%offj = BD_NODE
+            bd%offy = zero
+        else
+            print*, 'define_box_data_bdry: Bad dir'
+        endif
+        bd%bx = bd%valid
+        bd%ngx = 0
+        bd%ngy = 0
+        bd%L = src%L
+        bd%H = src%H
+        if (allocated(bd%data)) deallocate (bd%data)
+        allocate (bd%data(bd%bx%ilo:bd%bx%ihi, bd%bx%jlo:bd%bx%jhi), stat=ierr)
+        if (ierr .ne. 0) then
+            print*, 'define_box_data: Out of memory'
+            stop
+        endif
+        bd%data = zero
+    end subroutine define_box_data_bdry
+
+    ! arrayutils.f90:530 -- crse over fine's valid box divided by the ratios.
+    ! The reference sets only the index bounds of the coarse box (its nx, ny,
+    ! dx and dy stay undefined); here they follow from the indices and ratios.
+    subroutine define_box_data_coarsened (crse, fine, refx, refy)
+        type(box_data), intent(out) :: crse
+        type(box_data), intent(in)  :: fine
+        integer, intent(in)         :: refx, refy
+        type(box) :: cv
+        cv%ilo = fine%valid%ilo / refx
+        cv%ihi = fine%valid%ihi / refx
+        cv%jlo = fine%valid%jlo / refy
+        cv%jhi = fine%valid%jhi / refy
+        cv%nx = cv%ihi - cv%ilo + 1
+        cv%ny = cv%jhi - cv%jlo + 1
+        cv%dx = fine%valid%dx * refx
+        cv%dy = fine%valid%dy * refy
+        call define_box_data_itemize (crse, cv, fine%ngx, fine%ngy, fine%offi, fine%offj)
+    end subroutine define_box_data_coarsened
+
+    ! arrayutils.f90:583
+    pure function compatible_boxes (bx1, bx2) result (is_same)
+        type(box), intent(in) :: bx1, bx2
+        logical :: is_same
+        is_same = bx1%ilo .eq. bx2%ilo .and. bx1%ihi .eq. bx2%ihi .and. &
+                  bx1%jlo .eq. bx2%jlo .and. bx1%jhi .eq. bx2%jhi .and. &
+                  bx1%dx .eq. bx2%dx .and. bx1%dy .eq. bx2%dy
+    end function compatible_boxes
+
+    ! arrayutils.f90:615 -- every ghost (corners included) to val
+    pure subroutine setval_ghosts (bd, val)
+        type(box_data), intent(inout) :: bd
+        real(dp), intent(in)          :: val
+        if (bd%ngx .ge. 1) then
+            bd%data(bd%bx%ilo:bd%valid%ilo-1, bd%bx%jlo:bd%bx%jhi) = val
+            bd%data(bd%valid%ihi+1:bd%bx%ihi, bd%bx%jlo:bd%bx%jhi) = val
+        endif
+        if (bd%ngy .ge. 1) then
+            bd%data(bd%bx%ilo:bd%bx%ihi, bd%bx%jlo:bd%valid%jlo-1) = val
+            bd%data(bd%bx%ilo:bd%bx%ihi, bd%valid%jhi+1:bd%bx%jhi) = val
+        endif
+    end subroutine setval_ghosts
+
+    ! arrayutils.f90:721 (runs on the GPU): net outward flux over bx
+    function integrate2d_bdry (xflux, yflux, bx) result (sum)
+        type(box_data), intent(in) :: xflux, yflux
+        type(box), intent(in)      :: bx
+        real(dp) :: sum
+        real(dp), allocatable :: xf(:,:), yf(:,:)
+        real(c_double) :: out
+        allocate (xf(bx%ilo:bx%ihi+1, bx%jlo:bx%jhi), yf(bx%ilo:bx%ihi, bx%jlo:bx%jhi+1))
+        xf = xflux%data(bx%ilo:bx%ihi+1, bx%jlo:bx%jhi)
+        yf = yflux%data(bx%ilo:bx%ihi, bx%jlo:bx%jhi+1)
+        call mgp_check (mgp_integrate2d_bdry_c(bx%nx, bx%ny, bx%dx, bx%dy, xf, yf, out), 'integrate2d_bdry')
+        sum = out
+    end function integrate2d_bdry
+
+    ! arrayutils.f90:802 (runs on the GPU): norm over the ghost layer, with the
+    ! reference's index pattern (see mgp_gpnorm)
+    function gpnorm (bd, p) result (pn)
+        type(box_data), intent(in) :: bd
+        integer, intent(in)        :: p
+        real(dp) :: pn
+        real(dp), allocatable :: a(:,:)
+        real(c_double) :: out
+        type(box) :: v
+        if ((bd%offi .ne. BD_CELL) .or. (bd%offj .ne. BD_CELL)) then
+            print*, 'gpnorm: Only works with cell-centered data for now.'
+            stop
+        endif
+        v = bd%valid
+        allocate (a(v%ilo-1:v%ihi+1, v%jlo-1:v%jhi+1))
+        a = bd%data(v%ilo-1:v%ihi+1, v%jlo-1:v%jhi+1)
+        call mgp_check (mgp_gpnorm_c(v%ilo, v%ihi, v%jlo, v%jhi, v%dx, v%dy, a, p, out), 'gpnorm')
+        pn = out
+    end function gpnorm
 
     ! arrayutils.f90:637 (runs on the GPU)
     function inner_prod (bd1, bd2) result (ip)

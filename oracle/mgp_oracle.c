@@ -1736,6 +1736,49 @@ double mgo_integrate2d(int nx, int ny, double dx, double dy, const double *a, co
     return sum;
 }
 
+/* arrayutils.f90 integrate2d_bdry (:721-748): x faces row by row, then y
+ * faces column by column; xflux (nx+1) x ny, yflux nx x (ny+1), i fastest */
+double mgo_integrate2d_bdry(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux)
+{
+    double sum = 0.0;
+    for (int j = 0; j < ny; ++j)
+        sum = sum + dx * (xflux[nx + (size_t)j * (nx + 1)] - xflux[(size_t)j * (nx + 1)]);
+    for (int i = 0; i < nx; ++i) sum = sum + dy * (yflux[i + (size_t)ny * nx] - yflux[i]);
+    return sum;
+}
+
+/* arrayutils.f90 gpnorm (:802-859) of a one-ghost field (nx+2) x (ny+2) on the
+ * box 0..nx-1 x 0..ny-1, where the reference's index pattern (columns jlo-1,
+ * jhi+1, rows ilo-1, ihi+1) is the ghost layer when nx == ny; the caller
+ * passes the columns cx0, cx1 and rows ry0, ry1 (-1 .. n) it reads. */
+double mgo_gpnorm(int nx, int ny, double dx, double dy, const double *phi, int p, int cx0, int cx1, int ry0,
+                  int ry1)
+{
+    const size_t w = (size_t)nx + 2;
+#define GP(i, j) phi[(size_t)((i) + 1) + (size_t)((j) + 1) * w]
+    const int cols[2] = {cx0, cx1}, rows[2] = {ry0, ry1};
+    double pn = 0.0;
+    if (p == 0) {
+        for (int s = 0; s < 2; ++s)
+            for (int j = 0; j < ny; ++j) {
+                const double v = fabs(GP(cols[s], j));
+                pn = pn > v ? pn : v;
+            }
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < nx; ++i) {
+                const double v = fabs(GP(i, rows[s]));
+                pn = pn > v ? pn : v;
+            }
+        return pn;
+    }
+    for (int s = 0; s < 2; ++s)
+        for (int j = 0; j < ny; ++j) pn = pn + dy * powi(GP(cols[s], j), p);
+    for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < nx; ++i) pn = pn + dx * powi(GP(i, rows[s]), p);
+#undef GP
+    return pow(pn, 1.0 / p);
+}
+
 void mgo_bicgstab2(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol,
                    int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
                    int order)

@@ -72,6 +72,12 @@ double mgo_pnorm(int nx, int ny, double dx, double dy, const double *a, int p);
  * over ghost-free nx x ny fields, in the reference's summation order. */
 double mgo_inner_prod(int nx, int ny, double dx, double dy, const double *a, const double *b);
 double mgo_integrate2d(int nx, int ny, double dx, double dy, const double *a, const double *J);
+/* integrate2d_bdry (arrayutils.f90:721-748) and gpnorm (:802-859, the columns
+ * cx0, cx1 and rows ry0, ry1 that the reference reads, -1 .. n), in the
+ * reference's summation order. */
+double mgo_integrate2d_bdry(int nx, int ny, double dx, double dy, const double *xflux, const double *yflux);
+double mgo_gpnorm(int nx, int ny, double dx, double dy, const double *phi, int p, int cx0, int cx1, int ry0,
+                  int ry1);
 void mgo_bicgstab2(int kind, MGO_GEO_ARGS, MGO_BC_ARGS, int homog, const double *invd, double tol,
                    int maxiters, int maxrestarts, int zerophi, const double *rhs, double *phi,
                    int order);

@@ -846,4 +846,38 @@ contains
         call undefine_box_data (geo%J)
     end function ref_integrate2d
 
+    ! --------------------------------------------------------------------------
+    ! ArrayUtils::integrate2d_bdry (arrayutils.f90:721) and gpnorm (:802) on the
+    ! box 0..nx-1 x 0..ny-1 (fluxes without ghosts, phi with one ghost layer).
+    ! --------------------------------------------------------------------------
+    function ref_integrate2d_bdry (nx, ny, dx, dy, xf, yf) result (sm) bind(C, name="ref_integrate2d_bdry")
+        integer(c_int), value      :: nx, ny
+        real(c_double), value      :: dx, dy
+        real(c_double), intent(in) :: xf(0:nx,0:ny-1), yf(0:nx-1,0:ny)
+        real(c_double)             :: sm
+        type(box) :: valid
+        type(box_data) :: xflux, yflux
+        call mk_valid (valid, nx, ny, dx, dy)
+        call define_box_data (xflux, valid, 0, 0, BD_NODE, BD_CELL)
+        call define_box_data (yflux, valid, 0, 0, BD_CELL, BD_NODE)
+        xflux%data = xf
+        yflux%data = yf
+        sm = integrate2d_bdry (xflux, yflux, valid)
+        call undefine_box_data (xflux)
+        call undefine_box_data (yflux)
+    end function ref_integrate2d_bdry
+
+    function ref_gpnorm (nx, ny, dx, dy, a, p) result (pn) bind(C, name="ref_gpnorm")
+        integer(c_int), value      :: nx, ny, p
+        real(c_double), value      :: dx, dy
+        real(c_double), intent(in) :: a(-1:nx, -1:ny)
+        real(c_double)             :: pn
+        type(box) :: valid
+        type(box_data) :: f
+        call mk_valid (valid, nx, ny, dx, dy)
+        call mk_cc1 (f, valid, a)
+        pn = gpnorm (f, p)
+        call undefine_box_data (f)
+    end function ref_gpnorm
+
 end module ref_bridge

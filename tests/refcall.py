@@ -232,6 +232,19 @@ class Checker:
         return self._fn("integrate2d", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(a), _ptr(Jarr),
                                            _i(0 if J is None else 1))
 
+    def integrate2d_bdry(self, nx, ny, dx, dy, xflux, yflux):
+        return self._fn("integrate2d_bdry", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(np.ascontiguousarray(xflux)),
+                                                _ptr(np.ascontiguousarray(yflux)))
+
+    def gpnorm(self, nx, ny, dx, dy, phi, p):
+        """gpnorm of a one-ghost field on the box 0..nx-1 x 0..ny-1 (square boxes: the ghost layer)."""
+        phi = np.ascontiguousarray(phi)
+        if self.is_oracle:
+            # the reference's columns jlo-1, jhi+1 and rows ilo-1, ihi+1 relative to the valid cells
+            return self._fn("gpnorm", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(phi), _i(p), _i(-1), _i(ny),
+                                          _i(-1), _i(nx))
+        return self._fn("gpnorm", _d)(_i(nx), _i(ny), _d(dx), _d(dy), _ptr(phi), _i(p))
+
     def bicgstab2(self, kind, geo, bc, homog, invd, tol, maxiters, maxrestarts, zerophi, rhs, phi, order=LEX):
         phi = phi.copy()
         args = [_i(kind), *self._geo_args(geo), *self._bc_args(bc), _i(homog), _ptr(np.ascontiguousarray(invd)),

@@ -132,3 +132,15 @@ def test_wrong_shapes_raise_before_the_library_is_called():
     bc_short = G.BoundaryConditions((1, 1, 0, 0), (1, 0, 0, 0), tuple(np.zeros(3) for _ in range(4)))
     with pytest.raises(ValueError):
         capi.compute_invdiags(geo, bc_short, 1)
+
+
+def test_gpnorm_rejects_out_of_array_index_pattern_without_device_work():
+    import numpy as np
+    import pytest
+    from mgpoisson2d_amd import capi
+    # columns jlo-1 and jhi+1 of a 10 x 4 box: jhi+1 = 4 lies inside, but the rows ilo-1 and
+    # ihi+1 = 10 lie past the 4 + 2 rows of the array
+    with pytest.raises(capi.MgpError):
+        capi.gpnorm(0, 9, 0, 3, 0.1, 0.25, np.zeros((6, 12)), 0)
+    with pytest.raises(ValueError):
+        capi.gpnorm(0, 9, 0, 9, 0.1, 0.1, np.zeros((11, 12)), 0)

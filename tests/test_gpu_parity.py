@@ -310,3 +310,24 @@ def test_vcycle_history(case, kind, ordering):
     if ordering == 0 and rc.reference_available():
         r = rc.reference().vcycle(kind, geo, bc, 1, 0.0, maxit, phi, rhs, zerophi=True)
         np.testing.assert_allclose(g[0][1:-1, 1:-1], r[0][1:-1, 1:-1], rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 8), (20, 7), (300, 260)])
+def test_integrate2d_bdry_and_gpnorm(nx, ny):
+    rng = np.random.default_rng(nx + ny)
+    xf, yf = rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx))
+    dx, dy = 1.0 / nx, 0.7 / ny
+    got = mg.integrate2d_bdry(nx, ny, dx, dy, xf, yf)
+    for c in checkers():
+        assert got == c.integrate2d_bdry(nx, ny, dx, dy, xf, yf)
+    if nx == ny:
+        phi = rng.standard_normal((ny + 2, nx + 2))
+        for p in (0, 2, 3):
+            arg = np.abs(phi) if p == 3 else phi
+            got = mg.gpnorm(0, nx - 1, 0, ny - 1, dx, dy, arg, p)
+            for c in checkers():
+                assert got == c.gpnorm(nx, ny, dx, dy, arg, p)
+    else:
+        # the reference's index pattern leaves the array on non-square boxes
+        with pytest.raises(mg.MgpError):
+            mg.gpnorm(0, nx - 1, 0, ny - 1, dx, dy, rng.standard_normal((ny + 2, nx + 2)), 0)

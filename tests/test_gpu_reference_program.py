@@ -1,0 +1,74 @@
+"""The reference's own test program (src/tests/test.f90, compiled unchanged by
+oracle/Makefile's testf90 target) linked against the drop-in layer runs on the
+GPU and prints what the same program linked against the reference modules
+prints.
+
+The program runs test_projection (test.f90:2817-3024) on the 512 x 512
+terrain geometry: flux integrals (integrate2d_bdry), the divergence and its
+integral, an all-Neumann FMG solve (fmg, 8/8/8 smoothing, 10 iterations,
+verbosity 3) and the divergence after the projection.
+
+Tolerances: the flux integrals run in the reference's order (bit-identical);
+the divergence integral and the FMG residual norms are parallel sums on the
+GPU, so values at the rounding floor (a few 1e-14) agree only in magnitude.
+Residual histories agree to 1e-9 relative while they are above 1e-10, the
+initial |div| to 1e-12, and both runs reduce |div| by more than 1e11.
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "oracle", "_ref", "test_f90")
+NUM = r"([-+]?\d+\.?\d*(?:[EeDd][-+]?\d+)?)"
+
+
+def run(kind):
+    exe = os.path.join(BUILD, kind, "test")
+    if not os.path.exists(exe):
+        pytest.skip("the reference test program was not built (no reference sources at build time)")
+    out = subprocess.run([exe], check=True, timeout=300, capture_output=True, text=True, cwd=ROOT).stdout
+    return out
+
+
+def parse(text):
+    val = lambda pat: float(re.search(pat + r"\s*" + NUM, text).group(1).replace("D", "E"))
+    hist = [float(m.group(2)) for m in re.finditer(r"FMG-Cycle iter\s+(\d+)\s*:\s*rel \|res\| =\s*" + NUM, text)]
+    m = re.search(r"\|div\| =\s*" + NUM + r"\s*-->\s*" + NUM, text)
+    return {
+        "flux_sum": val(r"  flux sum ="),
+        "new_flux_sum": val(r"new flux sum ="),
+        "sum_div": val(r"sum div ="),
+        "hist": hist,
+        "div0": float(m.group(1)),
+        "div1": float(m.group(2)),
+        "reduction": val(r"reduction ="),
+        "size": re.search(r"Projection test on\s+(\d+)\s+x\s+(\d+)", text).groups(),
+    }
+
+
+def test_reference_test_program_runs_on_the_dropin():
+    ref = parse(run("ref"))
+    gpu = parse(run("dropin"))
+    print("reference:", ref)
+    print("drop-in  :", gpu)
+    assert gpu["size"] == ref["size"] == ("512", "512")
+    # integrate2d_bdry sums in the reference's order on the device
+    assert gpu["flux_sum"] == ref["flux_sum"]
+    assert gpu["new_flux_sum"] == ref["new_flux_sum"]
+    # integrate2d of the divergence: rounding-level values
+    assert abs(gpu["sum_div"]) < 1e-11 and abs(ref["sum_div"]) < 1e-11
+    assert abs(gpu["div0"] - ref["div0"]) <= 1e-12 * ref["div0"]
+    # FMG residual history while above the rounding floor
+    n = 0
+    for a, b in zip(ref["hist"], gpu["hist"]):
+        if b < 1e-10:
+            break
+        assert abs(a - b) <= 1e-9 * b, (ref["hist"], gpu["hist"])
+        n += 1
+    assert n >= 3
+    assert ref["reduction"] < 1e-11 and gpu["reduction"] < 1e-11
