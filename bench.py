@@ -1,24 +1,30 @@
-"""Benchmark: FP64 solves/s to 1e-10 relative residual on one MI355X.
+"""Benchmark: solves/s at 1e-10 relative residual on one MI355X, mixed/batched
+against plain FP64.
 
-Workload (the FP64 drop-in single-solve configuration): 2048 x 2048 cells on
-(xi, eta) in [0,1]^2, mapping x = xi + 0.1 sin(2 pi xi) sin(2 pi eta),
-y = (exp(5 eta) - 1)/(exp(5) - 1), analytic J and J g^ij at faces, Dirichlet
-on the xi faces, Neumann on the eta faces, rhs = -J (f + 1e-2 N(0,1)) with f
-the manufactured source of u = sin(pi x) cos(pi y) (the operator is +Laplacian),
-zero initial guess, solved with the resident fast solver (FP64 BiCGStab on
-the reference operator, one V(2,2) semicoarsening x-line cycle per
-preconditioner application) to ||rhs - L phi|| <= tol ||rhs||.
+Headline workload (the batched uneven problem mix): 1024 independent mapped-
+grid Poisson problems, nx, ny = 16 round(exp(U[ln 32, ln 1024]) / 16),
+mapping x = xi + A sin(2 pi xi) sin(2 pi eta), y = (exp(beta eta) - 1)/(exp(beta) - 1)
+on [0,1]^2 with beta ~ U[0,6], A ~ U[0,0.15], all-Dirichlet or Dirichlet xi /
+Neumann eta BCs (50/50), rhs N(0,1) per cell, all from one seed.  Each problem
+is solved to ||rhs - L phi|| <= 1e-10 ||rhs|| from phi = 0 by the batched
+solver with FP32 V-cycles inside FP64 BiCGStab with iterative refinement (the
+operator, residual and solution stay FP64), one launch per level operation
+for all problems.  A step is one batched solve of the whole mix with the rhs
+resident in HBM; value = problems solved per second.
 
-tol defaults to 1e-9 at 2048^2: the FP64 evaluation of this operator has a
-rounding floor near 2e-10 relative at that size (the flux terms near the
-stretched wall are ~1e7 times the J-scaled rhs), so no FP64 solution vector
-reaches 1e-10 there; 1e-10 is reachable up to 1024^2 (--n 1024 --tol 1e-10).
-
-A step is one complete solve from phi = 0 with rhs already resident in HBM.
-The same solves with the preconditioner V-cycle in FP32 (KRYLOV_MG32; the
-Krylov vectors, operator, residual and solution stay FP64) are timed beside
-them and reported under "mixed", with their own relative residual and their
-distance from the FP64 solution.
+Beside it, in the same run:
+* plain FP64: the same 1024 problems solved one at a time by the
+  single-problem FP64 solver (hierarchy set-up untimed), with p50/p99
+  latency; the ratio value / plain is the metric's "mixed/batched vs plain";
+* the batched solver with FP64 cycles;
+* roofline: the batch's dominant kernel, the finest-level line relaxation of
+  every problem (one launch per chunk class, side by side), cache-cold;
+* single_2048: the FP64 drop-in single solve (2048^2 stretched, warped map);
+* serving: continuous batching of an open-loop request stream, against the
+  plain FP64 one-at-a-time path on the same arrivals;
+* dc_rb_smoother: the DC red-black smoother with LDS tiles;
+* cpu_baseline: the reference Fortran (one core) on a sample of the mix,
+  equal work.
 Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
 """
 from __future__ import annotations
@@ -71,67 +77,51 @@ def config4_mix(n=1024, seed=SEED):
     return probs, rhs
 
 
-def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2, use_graph=True, mixed=True):
-    """Solves/s of the batched solver on the problem mix, against the plain FP64
-    single-problem solver (KRYLOV_MG, one problem at a time) timed on a sample of
-    the same mix; per-solve latencies of the plain path give p50/p99."""
+def mix_problems(capi, n):
     from types import SimpleNamespace
     from mgpoisson2d_amd import geometry as G
     probs, rhs = config4_mix(n)
     geos = [SimpleNamespace(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny) for nx, ny, _, _, _ in probs]
     maps = [capi.Mapping(capi.MAP_WARPED, amp, beta, 0.0, 0.0) for _, _, amp, beta, _ in probs]
     bcs = [G.BoundaryConditions.uniform(t) for *_, t in probs]
-    t0 = time.perf_counter()
-    b = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb, use_graph=use_graph)
-    for k in range(n):
+    return probs, rhs, geos, maps, bcs
+
+
+def make_batch(capi, geos, bcs, maps, rhs, method, use_graph=True, sd=2, su=2, sb=2):
+    b = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb, use_graph=use_graph,
+                   method=method)
+    for k in range(len(geos)):
         b.set_rhs(k, rhs[k])
     capi.device_synchronize()
-    setup_s = time.perf_counter() - t0
-    iters, outc = b.solve(tol, 400)  # warm-up
-    if not np.all(outc == capi.CONVERGED):
-        raise SystemExit(f"batched warm-up: {int(np.sum(outc != capi.CONVERGED))} problems did not converge")
-    times = []
-    for _ in range(steps):
-        capi.device_synchronize()
-        t1 = time.perf_counter()
+    return b
+
+
+def timed_batch(capi, b, tol, steps, warmup, label):
+    """warmup untimed batched solves, then `steps` timed ones; returns (seconds, iterations, outcomes)."""
+    for _ in range(warmup):
         iters, outc = b.solve(tol, 400)
-        capi.device_synchronize()
-        times.append(time.perf_counter() - t1)
+        if not np.all(outc == capi.CONVERGED):
+            raise SystemExit(f"{label} warm-up: {int(np.sum(outc != capi.CONVERGED))} problems did not converge")
+    capi.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        iters, outc = b.solve(tol, 400)
+    capi.device_synchronize()
+    dt = time.perf_counter() - t0
     if not np.all(outc == capi.CONVERGED):
-        raise SystemExit("batched solve did not converge")
-    cycles = b.last_cycles()
-    mixed_line = None
-    if mixed:
-        # the same mix with FP32 preconditioner cycles (FP64 Krylov, residual and answer)
-        b32 = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb,
-                         use_graph=use_graph, method=capi.METHOD_KRYLOV_MG32)
-        for k in range(n):
-            b32.set_rhs(k, rhs[k])
-        b32.solve(tol, 400)
-        times32 = []
-        for _ in range(steps):
-            capi.device_synchronize()
-            t1 = time.perf_counter()
-            it32, oc32 = b32.solve(tol, 400)
-            capi.device_synchronize()
-            times32.append(time.perf_counter() - t1)
-        if not np.all(oc32 == capi.CONVERGED):
-            raise SystemExit("batched mixed-precision solve did not converge")
-        d32 = max(float(np.max(np.abs(b32.get_phi(k) - b.get_phi(k))) / max(np.max(np.abs(b.get_phi(k))), 1e-300))
-                  for k in range(0, n, max(1, n // 64)))
-        mixed_line = {"solves_per_s": n / min(times32), "batch_s": min(times32),
-                      "v_cycles_mean": 2 * float(np.mean(it32)), "v_cycles_max": 2 * int(np.max(it32)),
-                      "max_rel_diff_vs_fp64_batch": d32}
-        b32.close()
-        print(f"# batched mixed: solve {min(times32):.3f} s", file=sys.stderr, flush=True)
-    print(f"# batched: {n} problems, setup {setup_s:.2f} s, solve {min(times):.3f} s, "
-          f"{2 * float(np.mean(iters)):.1f} V-cycles mean, {cycles} max", file=sys.stderr, flush=True)
-    # plain FP64 path on a sample, one problem at a time (hierarchy set-up not timed)
-    lat = []
-    diffs = []
-    for k in range(min(sample, n)):
-        s1 = capi.Solver(geos[k], bcs[k], method=capi.METHOD_KRYLOV_MG, mapping=maps[k], smooth_down=sd,
-                         smooth_up=su, smooth_bottom=sb, use_graph=use_graph)
+        raise SystemExit(f"{label}: {int(np.sum(outc != capi.CONVERGED))} problems did not converge")
+    print(f"# {label}: {steps} batched solves of {len(iters)} problems in {dt:.3f} s, "
+          f"{2 * float(np.mean(iters)):.1f} V-cycles mean, {2 * int(np.max(iters))} max", file=sys.stderr, flush=True)
+    return dt, iters, outc
+
+
+def plain_fp64_mix(capi, geos, bcs, maps, rhs, tol, use_graph=True):
+    """The plain FP64 single-problem solver, one problem at a time over the whole
+    mix: per problem a resident solver (set-up untimed), one untimed solve (graph
+    capture), then one timed solve.  Returns (latencies in s, solutions)."""
+    lat, sols = [], []
+    for k in range(len(geos)):
+        s1 = capi.Solver(geos[k], bcs[k], method=capi.METHOD_KRYLOV_MG, mapping=maps[k], use_graph=use_graph)
         s1.set_rhs(rhs[k])
         s1.solve_fast(tol, 400, zerophi=True)
         capi.device_synchronize()
@@ -141,63 +131,49 @@ def batched_bench(capi, n=1024, tol=1e-10, steps=2, sample=48, sd=2, su=2, sb=2,
         lat.append(time.perf_counter() - t1)
         if oc != capi.CONVERGED:
             raise SystemExit(f"plain FP64 solve of problem {k} did not converge")
-        ref = s1.get_phi()[1:-1, 1:-1]
-        diffs.append(float(np.max(np.abs(b.get_phi(k) - ref)) / max(np.max(np.abs(ref)), 1e-300)))
+        sols.append(s1.get_phi()[1:-1, 1:-1])
         s1.close()
-    b.close()
-    lat = np.array(lat)
-    batched = n / min(times)
-    plain = len(lat) / float(np.sum(lat))
-    cells = int(sum(nx * ny for nx, ny, *_ in probs))
-    return {
-        "workload": f"{n} mapped-grid problems, nx, ny log-uniform in [32, 1024] (multiples of 16), "
-                    f"beta U[0,6], A U[0,0.15], BCs all-Dirichlet or Dirichlet xi / Neumann eta, rhs N(0,1), "
-                    f"FP64 to {tol:g} relative residual each",
-        "cells": cells,
-        "batched_solves_per_s": batched,
-        "batched_batch_s": min(times),
-        "batched_setup_s": setup_s,
-        "plain_fp64_solves_per_s": plain,
-        "plain_sample": len(lat),
-        "batched_over_plain": batched / plain,
-        "plain_latency_p50_ms": 1e3 * float(np.percentile(lat, 50)),
-        "plain_latency_p99_ms": 1e3 * float(np.percentile(lat, 99)),
-        "v_cycles_mean": 2 * float(np.mean(iters)),
-        "v_cycles_max": cycles,
-        "max_rel_diff_batched_vs_plain": max(diffs),
-        "batched_mixed": mixed_line,
-        "batched_mixed_over_plain": (mixed_line["solves_per_s"] / plain) if mixed_line else None,
-    }
+    print(f"# plain fp64: {len(lat)} problems one at a time in {sum(lat):.3f} s", file=sys.stderr, flush=True)
+    return np.array(lat), sols
+
+
+def rel_diff(a, b):
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
 
 
 def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_requests=200, tol=1e-10,
-                  budget=200e9, seed=SEED + 5, mixed=True):
+                  budget=200e9, seed=SEED + 5, pool=6):
     """Continuous batching: n_small resident geometries from the batched mix plus one
-    big_n^2 (beta = 6) geometry that takes big_share of the requests; each request is the
-    slot's base rhs plus a 5 % perturbation, warm-started from the slot's last solution.
-    Capacity is measured closed-loop (one request in flight per slot); then requests
-    arrive as a Poisson process at `load` x capacity and their latency (arrival to result)
-    is recorded."""
+    big_n^2 (beta = 6) geometry that takes big_share of the requests.  Every request
+    carries an N(0,1) rhs drawn for it from a seeded pool of `pool` independent draws
+    per geometry (a slot never receives the same rhs twice in a row), warm-started
+    from the slot's last solution; FP32 cycles in FP64 BiCGStab to tol.  Capacity is
+    measured closed-loop (one request in flight per slot); then requests arrive as a
+    Poisson process at `load` x capacity and their latency (arrival to result) is
+    recorded.  The same arrivals are then served by the plain FP64 single-problem
+    solver one request at a time (resident per-slot solvers, warm starts)."""
     from types import SimpleNamespace
     from mgpoisson2d_amd import geometry as G
-    probs, base = config4_mix(n_small, seed)
+    probs, _ = config4_mix(n_small, seed)
     probs.append((big_n, big_n, 0.1, 6.0, (1, 1, 0, 0)))
     rng = np.random.default_rng(seed + 1)
-    base.append(rng.standard_normal((big_n, big_n)))
-    pert = [[rng.standard_normal(b.shape) for _ in range(3)] for b in base]
+    pools = [[rng.standard_normal((ny, nx)) for _ in range(pool)] for nx, ny, *_ in probs]
     geos = [SimpleNamespace(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny) for nx, ny, *_ in probs]
     maps = [capi.Mapping(capi.MAP_WARPED, amp, beta, 0.0, 0.0) for _, _, amp, beta, _ in probs]
     bcs = [G.BoundaryConditions.uniform(t) for *_, t in probs]
-    v = capi.Server(geos, bcs, maps, hbm_budget=budget,
-                    method=capi.METHOD_KRYLOV_MG32 if mixed else capi.METHOD_KRYLOV_MG)
     big = len(probs) - 1
+    last = [-1] * len(probs)
 
     def draw():
         slot = big if rng.uniform() < big_share else int(rng.integers(n_small))
-        return slot, base[slot] + 0.05 * pert[slot][int(rng.integers(3))]
+        k = int(rng.integers(pool - 1))
+        k = k if k < last[slot] else k + 1  # never the slot's previous rhs
+        last[slot] = k
+        return slot, pools[slot][k]
 
+    v = capi.Server(geos, bcs, maps, hbm_budget=budget, method=capi.METHOD_KRYLOV_MG32)
     # cold solves of every slot (hierarchies and phi become resident)
-    pend = {v.submit(k, base[k], tol, 400, False): k for k in range(len(probs))}
+    pend = {v.submit(k, pools[k][0], tol, 400, False): k for k in range(len(probs))}
     while pend:
         v.step()
         for t in list(pend):
@@ -252,12 +228,41 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
     span = time.perf_counter() - t0
     footprint = v.footprint()
     v.close()
+    print(f"# serving: {n_requests} requests in {span:.2f} s", file=sys.stderr, flush=True)
+
+    # the plain FP64 path, one request at a time, on the same arrivals
+    solvers = [capi.Solver(geos[k], bcs[k], method=capi.METHOD_KRYLOV_MG, mapping=maps[k]) for k in range(len(probs))]
+    for k, s1 in enumerate(solvers):  # cold solves (resident phi, captured graphs)
+        s1.set_rhs(pools[k][0])
+        s1.solve_fast(tol, 400, zerophi=True)
+    capi.device_synchronize()
+    plat, pbig, service = [], [], []
+    t0 = time.perf_counter()
+    for (slot, r), arr in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if now < arr:
+            time.sleep(arr - now)
+        t1 = time.perf_counter()
+        solvers[slot].set_rhs(r)
+        _, oc = solvers[slot].solve_fast(tol, 400, zerophi=False)
+        solvers[slot].get_phi()
+        if oc != capi.CONVERGED:
+            raise SystemExit(f"plain FP64 served solve on slot {slot} did not converge")
+        end = time.perf_counter()
+        service.append(end - t1)
+        (pbig if slot == big else plat).append(end - t0 - arr)
+    for s1 in solvers:
+        s1.close()
+    print(f"# serving plain fp64: {n_requests} requests in {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+          flush=True)
     all_lat = np.array(lat + big_lat)
+    all_plain = np.array(plat + pbig)
     return {
         "workload": f"{n_small} resident geometries from the batched mix plus one {big_n}^2 beta=6 geometry "
-                    f"({100 * big_share:.0f} % of requests); rhs = base + 5 % perturbation, warm start from the "
-                    f"slot's last solution, {'FP32 cycles in FP64 BiCGStab' if mixed else 'FP64'} to {tol:g} "
-                    f"relative residual; Poisson arrivals at {load} x the closed-loop capacity",
+                    f"({100 * big_share:.0f} % of requests); each request an N(0,1) rhs (seeded pool of {pool} "
+                    f"per geometry, never the slot's previous one), warm start from the slot's last solution, "
+                    f"FP32 cycles in FP64 BiCGStab to {tol:g} relative residual; Poisson arrivals at {load} x the "
+                    f"closed-loop capacity",
         "capacity_solves_per_s": capacity,
         "offered_per_s": rate,
         "served_per_s": n_requests / span,
@@ -269,6 +274,13 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
         "big_latency_p50_ms": 1e3 * float(np.percentile(big_lat, 50)) if big_lat else None,
         "hbm_footprint_GB": footprint / 1e9,
         "hbm_budget_GB": budget / 1e9,
+        "plain_fp64_one_at_a_time": {
+            "capacity_solves_per_s": len(service) / float(np.sum(service)),
+            "latency_p50_ms": 1e3 * float(np.percentile(all_plain, 50)),
+            "latency_p99_ms": 1e3 * float(np.percentile(all_plain, 99)),
+            "small_latency_p50_ms": 1e3 * float(np.percentile(plat, 50)) if plat else None,
+            "big_latency_p50_ms": 1e3 * float(np.percentile(pbig, 50)) if pbig else None,
+        },
     }
 
 
@@ -318,6 +330,235 @@ def dc_rb_bench(capi, geo, bc, rhs=None, reps=20, cycles=5):
     }
 
 
+def single_bench(capi, args, steps=5, warmup=1):
+    """The FP64 drop-in single solve (2048^2 stretched, warped map; Dirichlet xi,
+    Neumann eta) with the resident solver, FP64 and FP32 cycles, to args.single_tol;
+    the finest-level x-line kernel against the HBM roofline; the reference on one
+    core for equal work.  With args.kernels_only: only the kernel timings."""
+    geo, bc, rhs = config2_problem(args.n, args.n)
+
+    def make(method):
+        s = capi.Solver(geo, bc, method=method, use_graph=not args.no_graph)
+        s.set_rhs(rhs)
+        return s
+
+    solver = make(capi.METHOD_KRYLOV_MG)
+    print(f"# single: set-up done, {args.n}x{args.n}", file=sys.stderr, flush=True)
+    if args.kernels_only:
+        mixed = make(capi.METHOD_KRYLOV_MG32)
+        out = {}
+        for cold in (True, False):
+            for s, k, name in ((solver, 0, "x-line even rows"), (solver, 1, "residual"), (solver, 2, "copy"),
+                               (mixed, 3, "x-line even rows FP32")):
+                ms, nbytes = s.time_kernel(k, reps=5, cold=cold)
+                key = f"{'cold' if cold else 'warm'} {name}"
+                out[key] = {"ms": ms, "GBps": nbytes / (ms * 1e-3) / 1e9}
+                print(f"# {key}: {ms:.4f} ms/launch, {nbytes / 1e6:.1f} MB algorithmic, "
+                      f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+        mixed.close()
+        solver.close()
+        return out
+
+    def timed(s, label):
+        for _ in range(warmup):
+            iters, outcome = s.solve_fast(args.single_tol, 400, zerophi=True)
+            if outcome != capi.CONVERGED:
+                raise SystemExit(f"{label} warmup solve did not converge (outcome {outcome}, {iters} iterations)")
+        capi.device_synchronize()
+        t0 = time.perf_counter()
+        cycles = 0
+        for _ in range(steps):
+            iters, outcome = s.solve_fast(args.single_tol, 400, zerophi=True)
+            cycles += s.last_cycles()
+            if outcome != capi.CONVERGED:
+                raise SystemExit(f"{label} solve did not converge (outcome {outcome})")
+        capi.device_synchronize()
+        dt = time.perf_counter() - t0
+        print(f"# single {label}: {steps} solves in {dt:.3f} s, {cycles / steps:.1f} V-cycles per solve",
+              file=sys.stderr, flush=True)
+        relres, _, _ = s.solve(args.single_tol, 400, zerophi=True)
+        return dt, cycles / steps, float(relres[-1])
+
+    dt, cyc, rr = timed(solver, "fp64")
+    mixed = make(capi.METHOD_KRYLOV_MG32)
+    mdt, mcyc, mrr = timed(mixed, "mixed")
+    diff = rel_diff(mixed.get_phi()[1:-1, 1:-1], solver.get_phi()[1:-1, 1:-1])
+    k_ms, k_bytes = solver.time_kernel(0, reps=20, cold=True)
+    kw_ms, _ = solver.time_kernel(0, reps=20, cold=False)
+    m_ms, m_bytes = mixed.time_kernel(3, reps=20, cold=True)
+    r_ms, r_bytes = solver.time_kernel(1, reps=20, cold=True)
+    c_ms, c_bytes = solver.time_kernel(2, reps=20, cold=True)
+    mixed.close()
+    solver.close()
+    traffic = None
+    if args.n == NX and os.path.exists(args.traffic):
+        with open(args.traffic) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "workload": f"FP64 single solve, {args.n}x{args.n} warped/stretched mapped grid (A=0.1, beta=5), "
+                    f"Dirichlet xi / Neumann eta, to {args.single_tol:g} relative residual",
+        "tol": args.single_tol,
+        "fp64_solves_per_s": steps / dt,
+        "fp64_v_cycles_per_solve": cyc,
+        "fp64_final_relres": rr,
+        "mixed_solves_per_s": steps / mdt,
+        "mixed_v_cycles_per_solve": mcyc,
+        "mixed_final_relres": mrr,
+        "max_rel_diff_mixed_vs_fp64_phi": diff,
+        "xline_fp64": {"kernel": "k_xline (finest level, even rows)", "avg_launch_ms": k_ms,
+                       "warm_avg_launch_ms": kw_ms, "alg_bytes_per_launch": k_bytes,
+                       "GBps": k_bytes / (k_ms * 1e-3) / 1e9, "frac": k_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "traffic": traffic,
+                       "timing": "HIP events around 20 back-to-back launches rotating over 4 copies of the "
+                                 "operands (cache-cold)"},
+        "xline_fp32_cold_ms": m_ms,
+        "xline_fp32_GBps": m_bytes / (m_ms * 1e-3) / 1e9,
+        "residual_kernel_GBps": r_bytes / (r_ms * 1e-3) / 1e9,
+        "copy_kernel_GBps": c_bytes / (c_ms * 1e-3) / 1e9,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_reference"] = cpu_baseline(geo, bc, rhs, int(round(cyc)))
+    return out
+
+
+NX3, NY3, AMP3, BETA3 = 8192, 2048, 0.12, 8.0
+
+
+def warped_points(xi, eta, amp, beta):
+    x = xi + amp * np.sin(2 * math.pi * xi) * np.sin(2 * math.pi * eta)
+    y = np.expm1(beta * eta) / math.expm1(beta)
+    return x, y
+
+
+def warped_jacobian(xi, eta, amp, beta):
+    x_xi = 1 + amp * 2 * math.pi * np.cos(2 * math.pi * xi) * np.sin(2 * math.pi * eta)
+    return x_xi * (beta * np.exp(beta * eta) / math.expm1(beta))
+
+
+def _rows_parallel(ny, fill):
+    """fill(j0, j1) for row blocks of 64 on a thread pool (numpy releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    blocks = [(j, min(j + 64, ny)) for j in range(0, ny, 64)]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(lambda b: fill(*b), blocks))
+
+
+def config3_rhs(nx, ny, amp, beta, seed, averages):
+    """-J (f + 1e-2 N(0,1)) for u = cos(pi x) cos(2 pi y), f = 5 pi^2 u (L = +J Laplacian), at cell
+    centres (second order) or, for the fourth-order operator, -(<J f> + 1e-2 J N(0,1)) with <J f>
+    the cell average by 2 x 2 Gauss-Legendre points (exact to cubics: an O(h^4) average);
+    float64, (ny, nx)."""
+    noise = np.random.default_rng(seed).standard_normal((ny, nx))
+    out = np.empty((ny, nx))
+    t, w = np.polynomial.legendre.leggauss(2)
+    t, w = (t + 1) / 2, w / 2
+
+    def fill(j0, j1):
+        Xc, Ec = np.meshgrid((np.arange(nx) + 0.5) / nx, (np.arange(j0, j1) + 0.5) / ny)
+        Jc = warped_jacobian(Xc, Ec, amp, beta)
+        if not averages:
+            x, y = warped_points(Xc, Ec, amp, beta)
+            out[j0:j1] = -Jc * (5 * math.pi ** 2 * np.cos(math.pi * x) * np.cos(2 * math.pi * y) + 1e-2 * noise[j0:j1])
+            return
+        acc = np.zeros((j1 - j0, nx))
+        for a, wa in zip(t, w):
+            for b, wb in zip(t, w):
+                X, E = np.meshgrid((np.arange(nx) + a) / nx, (np.arange(j0, j1) + b) / ny)
+                x, y = warped_points(X, E, amp, beta)
+                acc += wa * wb * warped_jacobian(X, E, amp, beta) * (5 * math.pi ** 2) * np.cos(math.pi * x) * np.cos(
+                    2 * math.pi * y)
+        out[j0:j1] = -(acc + 1e-2 * Jc * noise[j0:j1])
+
+    _rows_parallel(ny, fill)
+    return out
+
+
+def u_cells(nx, ny, amp, beta, averages):
+    """u at cell centres, or its cell averages (2 x 2 Gauss-Legendre points)."""
+    out = np.empty((ny, nx))
+    t, w = np.polynomial.legendre.leggauss(2)
+    t, w = (t + 1) / 2, w / 2
+
+    def fill(j0, j1):
+        if not averages:
+            x, y = warped_points(*np.meshgrid((np.arange(nx) + 0.5) / nx, (np.arange(j0, j1) + 0.5) / ny), amp, beta)
+            out[j0:j1] = np.cos(math.pi * x) * np.cos(2 * math.pi * y)
+            return
+        acc = np.zeros((j1 - j0, nx))
+        for a, wa in zip(t, w):
+            for b, wb in zip(t, w):
+                x, y = warped_points(*np.meshgrid((np.arange(nx) + a) / nx, (np.arange(j0, j1) + b) / ny), amp, beta)
+                acc += wa * wb * np.cos(math.pi * x) * np.cos(2 * math.pi * y)
+        out[j0:j1] = acc
+
+    _rows_parallel(ny, fill)
+    return out
+
+
+def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-10, seed=SEED + 3):
+    """Mixed precision + deferred correction on the strongly stretched all-Neumann
+    grid: 8192 x 2048 cells, x = xi + 0.12 sin(2 pi xi) sin(2 pi eta),
+    y = (exp(8 eta) - 1)/(exp(8) - 1), u = cos(pi x) cos(2 pi y), rhs with 1e-2
+    N(0,1) noise, projected to J-weighted zero mean in FP64 (compensated sums).
+    Rows of 8192 cells: the preconditioner relaxes y-lines and semicoarsens in x.
+
+    * second order: FP32 V(2,2) cycles inside FP64 BiCGStab refinement with the
+      solution and the refinement residual in double-double, to `tol`; the same
+      with FP64 cycles is the FP64 reference solution;
+    * fourth order: the fourth-order finite-volume operator outside, the same
+      second-order cycles inside (deferred correction), cell-average rhs, to
+      `tol4` (FP64 residual), FP32 and FP64 cycles.
+    Reported per solve: time (one untimed solve first), V-cycles, final relative
+    residual, max |phi - u| / max |u| (both with the J-weighted mean removed), and
+    the distance of the FP32-cycle solution from the FP64-cycle one."""
+    from types import SimpleNamespace
+    from mgpoisson2d_amd import geometry as G
+    geo = SimpleNamespace(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny)
+    mp = capi.Mapping(capi.MAP_WARPED, AMP3, BETA3, 0.0, 0.0)
+    bc = G.BoundaryConditions.uniform((0, 0, 0, 0))
+    J = warped_jacobian(*np.meshgrid((np.arange(nx) + 0.5) / nx, (np.arange(ny) + 0.5) / ny), AMP3, BETA3)
+
+    def mean_free(a):
+        return a - np.sum(J * a) / np.sum(J)
+
+    out = {"workload": f"{nx}x{ny} cells, warp {AMP3}, stretching beta={BETA3}, all-Neumann, "
+                       f"u = cos(pi x) cos(2 pi y) + 1e-2 N(0,1) rhs noise, J-weighted zero-mean projection"}
+    for order, tol_o in ((2, tol), (4, tol4)):
+        rhs = config3_rhs(nx, ny, AMP3, BETA3, seed, averages=order == 4)
+        u = mean_free(u_cells(nx, ny, AMP3, BETA3, averages=order == 4))
+        sols = {}
+        for method, label in ((capi.METHOD_KRYLOV_MG32, "mixed"), (capi.METHOD_KRYLOV_MG, "fp64")):
+            refine = capi.REFINE_DD if order == 2 else capi.REFINE_FP64
+            s = capi.Solver(geo, bc, method=method, mapping=mp, refine=refine, order=order)
+            s.set_rhs(rhs)
+            s.project_mean()
+            s.solve_fast(tol_o, 800)  # graph capture, first touch
+            capi.device_synchronize()
+            t0 = time.perf_counter()
+            hist, iters, oc = s.solve(tol_o, 800)
+            capi.device_synchronize()
+            dt = time.perf_counter() - t0
+            if oc != capi.CONVERGED:
+                raise SystemExit(f"cfg3 order {order} {label} did not converge ({iters} iterations, {hist[-3:]})")
+            phi = s.get_phi()[1:-1, 1:-1]
+            if order == 2:
+                phi = phi + s.get_phi_lo()[1:-1, 1:-1]
+            phi = mean_free(phi)
+            sols[label] = phi
+            out[f"order{order}_{label}"] = {
+                "solve_s": dt,
+                "v_cycles": s.last_cycles(),
+                "final_relres": float(hist[-1]),
+                "residual": "double-double" if order == 2 else "FP64",
+                "max_err_vs_u_rel": float(np.max(np.abs(phi - u)) / np.max(np.abs(u))),
+            }
+            s.close()
+            print(f"# cfg3 order {order} {label}: {dt:.3f} s, {out[f'order{order}_{label}']['v_cycles']} V-cycles, "
+                  f"relres {hist[-1]:.2e}", file=sys.stderr, flush=True)
+        out[f"order{order}_max_rel_diff_mixed_vs_fp64"] = rel_diff(sols["mixed"], sols["fp64"])
+    return out
+
+
 def cpu_baseline(geo, bc, rhs, gpu_cycles, budget_s=20.0):
     """Time the reference Fortran (oracle/_ref/fast, -O3 -march=native, one
     core) on the same 2048^2 problem: DeferredMGPoisson2D::vcycle, one V(2,2)
@@ -350,197 +591,208 @@ def cpu_baseline(geo, bc, rhs, gpu_cycles, budget_s=20.0):
     }
 
 
+def cpu_baseline_mix(probs, rhs, gpu_cycles, sample=24, cycles=2, budget_s=25.0):
+    """The reference Fortran (oracle/_ref/fast, one core) on a sample of the batched
+    mix: DeferredMGPoisson2D::vcycle V(2,2), `cycles` cycles per problem.  The mix's
+    grids are not square, and the reference's schedule then semicoarsens and its
+    prolongation stops (MGPoisson2D.f90:385-388), so each sampled problem is timed
+    with equal spacing in xi and eta (the same cells, the same mapping family on
+    [0, nx h] x [0, ny h]), which keeps the reference's 2x2 schedule.  Equal work:
+    seconds per reference V-cycle on each problem x the V-cycles the batched GPU
+    solve spent on it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refcall
+    from mgpoisson2d_amd import geometry as G
+    if not os.path.exists(refcall.REF_FAST_SO):
+        return None
+    ref = refcall.Checker(refcall.REF_FAST_SO, "ref_")
+    idx = np.random.default_rng(SEED + 9).choice(len(probs), size=min(sample, len(probs)), replace=False)
+    spent, work, used = 0.0, 0.0, 0
+    for k in idx:
+        nx, ny, amp, beta, types = probs[k]
+        h = 1.0 / max(nx, ny)
+        geo = G.mapped_geometry(nx, ny, amp, beta, L=nx * h, H=ny * h)
+        bc = G.BoundaryConditions.uniform(types)
+        phi = np.zeros((ny + 2, nx + 2))
+        t0 = time.perf_counter()
+        for _ in range(cycles):
+            phi, _, _, _, _ = ref.vcycle(1, geo, bc, 1, 0.0, 1, phi, rhs[k], sdown=2, sup=2, sbottom=2)
+        dt = time.perf_counter() - t0
+        spent += dt
+        work += dt / cycles * max(int(gpu_cycles[k]), 1)
+        used += 1
+        if spent > budget_s:
+            break
+    return {
+        "value": used / work,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": (f"reference DeferredMGPoisson2D::vcycle V(2,2) on {used} problems of the batched mix "
+                   f"({cycles} cycles each, {spent:.1f} s; equal spacing in xi and eta so the reference's "
+                   f"2x2 schedule runs); equal-work solves/s = problems / sum over them of (s per reference "
+                   f"V-cycle x V-cycles the batched GPU solve spent on that problem)"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed batched solves of the whole mix")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=NX)
-    ap.add_argument("--tol", type=float, default=TOL)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sd", type=int, default=2, help="pre-smoothing sweeps per level")
-    ap.add_argument("--su", type=int, default=2, help="post-smoothing sweeps per level")
-    ap.add_argument("--sb", type=int, default=2, help="sweeps on the coarsest level")
-    ap.add_argument("--method", choices=("fp64", "mixed"), default="fp64",
-                    help="solver of the headline value: FP64 cycles (default) or FP32 cycles in FP64 BiCGStab")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched problem-mix comparison")
-    ap.add_argument("--batch-only", action="store_true", help="run only the batched problem-mix comparison")
+    ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--batch-n", type=int, default=1024, help="problems in the batched mix")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 one-at-a-time comparison")
+    ap.add_argument("--no-single", action="store_true", help="skip the 2048^2 single-solve section")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--no-dc-rb", action="store_true", help="skip the DC red-black smoother measurement")
     ap.add_argument("--dc-rb-only", action="store_true", help="run only the DC red-black smoother measurement")
+    ap.add_argument("--single-only", action="store_true", help="run only the 2048^2 single-solve section")
+    ap.add_argument("--no-cfg3", action="store_true", help="skip the 8192x2048 mixed-precision / fourth-order section")
+    ap.add_argument("--cfg3-only", action="store_true", help="run only the 8192x2048 section")
+    ap.add_argument("--cfg3-n", type=int, default=NX3, help="nx of the cfg3 section (ny = nx / 4)")
+    ap.add_argument("--n", type=int, default=NX, help="grid of the single-solve section")
+    ap.add_argument("--single-tol", type=float, default=TOL, help="tolerance of the single-solve section")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each iteration's kernels eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="set up the solver and time only the x-line, residual and copy kernels "
-                         "(for hardware-counter passes)")
+                    help="time only the finest-level kernels (single solver: x-line, residual, copy; the "
+                         "batch: its line relaxation) for hardware-counter passes")
+    ap.add_argument("--batch-kernels-only", action="store_true",
+                    help="set up the mixed-precision batch, solve once and time only its finest-level line "
+                         "relaxation (for hardware-counter passes)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_xline_2048.json"),
-                    help="measured HBM bytes per x-line launch (from FETCH_SIZE/WRITE_SIZE passes)")
+                    help="measured HBM bytes per single-solver x-line launch (FETCH_SIZE/WRITE_SIZE passes)")
+    ap.add_argument("--batch-traffic", default=os.path.join(ROOT, "profiles", "traffic_batch_xline32.json"),
+                    help="measured HBM bytes per batched FP32 x-line launch set (FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     from mgpoisson2d_amd import capi
-    if args.batch_only:
-        print(json.dumps(batched_bench(capi, args.batch_n, use_graph=not args.no_graph)), flush=True)
-        return
+    use_graph = not args.no_graph
     if args.serving_only:
-        print(json.dumps({"mixed": serving_bench(capi), "fp64": serving_bench(capi, mixed=False)}), flush=True)
+        print(json.dumps(serving_bench(capi)), flush=True)
         return
-    geo, bc, rhs = config2_problem(args.n, args.n)
     if args.dc_rb_only:
+        geo, bc, rhs = config2_problem(args.n, args.n)
         print(json.dumps(dc_rb_bench(capi, geo, bc, rhs)), flush=True)
         return
-    main_method = capi.METHOD_KRYLOV_MG32 if args.method == "mixed" else capi.METHOD_KRYLOV_MG
-    other_method = capi.METHOD_KRYLOV_MG if args.method == "mixed" else capi.METHOD_KRYLOV_MG32
-
-    def make(method):
-        s = capi.Solver(geo, bc, method=method, smooth_down=args.sd, smooth_up=args.su, smooth_bottom=args.sb,
-                        use_graph=not args.no_graph)
-        s.set_rhs(rhs)
-        return s
-
-    solver = make(main_method)
-    print(f"# setup done: {args.n}x{args.n}", file=sys.stderr, flush=True)
-    if args.kernels_only:
-        mixed = make(capi.METHOD_KRYLOV_MG32) if not args.no_mixed else None
-        for cold in (True, False):
-            for s, k, name in ((solver, 0, "x-line even rows"), (solver, 1, "residual"), (solver, 2, "copy"),
-                               (mixed, 3, "x-line even rows FP32")):
-                if s is None:
-                    continue
-                ms, nbytes = s.time_kernel(k, reps=5, cold=cold)
-                print(f"# {'cold' if cold else 'warm'} {name}: {ms:.4f} ms/launch, {nbytes / 1e6:.1f} MB "
-                      f"algorithmic, {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
-        if mixed is not None:
-            mixed.close()
-        solver.close()
+    if args.cfg3_only:
+        print(json.dumps(cfg3_bench(capi, args.cfg3_n, args.cfg3_n // 4)), flush=True)
+        return
+    if args.single_only or args.kernels_only:
+        print(json.dumps(single_bench(capi, args)), flush=True)
         return
 
-    def timed(s, label):
-        for _ in range(args.warmup):
-            iters, outcome = s.solve_fast(args.tol, 400, zerophi=True)
-            if outcome != capi.CONVERGED:
-                raise SystemExit(f"{label} warmup solve did not converge (outcome {outcome}, {iters} iterations)")
-        capi.device_synchronize()
-        t0 = time.perf_counter()
-        cycles = 0
-        for _ in range(args.steps):
-            iters, outcome = s.solve_fast(args.tol, 400, zerophi=True)
-            cycles += s.last_cycles()
-            if outcome != capi.CONVERGED:
-                raise SystemExit(f"{label} solve did not converge (outcome {outcome})")
-        capi.device_synchronize()
-        dt = time.perf_counter() - t0
-        print(f"# {label}: {args.steps} solves in {dt:.3f} s, {cycles / args.steps:.1f} V-cycles per solve",
-              file=sys.stderr, flush=True)
-        # verify the last solution against the operator itself
-        relres, _, _ = s.solve(args.tol, 400, zerophi=True)
-        return dt, cycles / args.steps, float(relres[-1])
-
-    dt, cycles_per_solve, final_relres = timed(solver, args.method)
-    ms_per_step = 1e3 * dt / args.steps
-
-    compare = None
-    if not args.no_mixed:
-        other = make(other_method)
-        odt, ocycles, orelres = timed(other, "fp64" if args.method == "mixed" else "mixed")
-        a, b = solver.get_phi()[1:-1, 1:-1], other.get_phi()[1:-1, 1:-1]
-        diff = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-        mixed_s, fp64_s = (solver, other) if args.method == "mixed" else (other, solver)
-        mixed_dt, fp64_dt = (dt, odt) if args.method == "mixed" else (odt, dt)
-        mk_ms, mk_bytes = mixed_s.time_kernel(3, reps=20, cold=True)
-        compare = {
-            "mixed_solves_per_s": args.steps / mixed_dt,
-            "fp64_solves_per_s": args.steps / fp64_dt,
-            "mixed_over_fp64": fp64_dt / mixed_dt,
-            "mixed_v_cycles_per_solve": ocycles if args.method == "fp64" else cycles_per_solve,
-            "mixed_final_relres": orelres if args.method == "fp64" else final_relres,
-            "max_rel_diff_mixed_vs_fp64_phi": diff,
-            "xline_fp32_cold_ms": mk_ms,
-            "xline_fp32_GBps": mk_bytes / (mk_ms * 1e-3) / 1e9,
-        }
-        other.close()
-
-    # dominant kernel: finest-level x-line relaxation (even rows), HIP events
-    # around back-to-back launches whose operands are cache-cold (as inside a solve)
-    mixed_main = args.method == "mixed"
-    rk = 3 if mixed_main else 0
-    k_ms, k_bytes = solver.time_kernel(rk, reps=20, cold=True)
+    probs, rhs, geos, maps, bcs = mix_problems(capi, args.batch_n)
+    cells = int(sum(nx * ny for nx, ny, *_ in probs))
+    t0 = time.perf_counter()
+    b32 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG32, use_graph)
+    setup_s = time.perf_counter() - t0
+    print(f"# batch set-up: {len(probs)} problems, {cells} cells, {setup_s:.2f} s", file=sys.stderr, flush=True)
+    if args.batch_kernels_only:
+        b32.solve(args.tol, 400)
+        ms, nbytes = b32.time_kernel(0, reps=5)
+        print(f"# batched FP32 x-line set: {ms:.4f} ms, {nbytes / 1e6:.1f} MB algorithmic, "
+              f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+        b32.close()
+        return
+    dt, iters32, _ = timed_batch(capi, b32, args.tol, args.steps, args.warmup, "batched mixed")
+    value = len(probs) * args.steps / dt
+    # dominant kernel: finest-level line relaxation (even rows) of every problem, FP32
+    k_ms, k_bytes = b32.time_kernel(0, reps=10)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
-    kw_ms, _ = solver.time_kernel(rk, reps=20, cold=False)
-    r_ms, r_bytes = solver.time_kernel(1, reps=20, cold=True)
-    c_ms, c_bytes = solver.time_kernel(2, reps=20, cold=True)
-
     traffic = None
-    if args.n == NX and not mixed_main and os.path.exists(args.traffic):
-        with open(args.traffic) as fh:
+    if len(probs) == 1024 and os.path.exists(args.batch_traffic):
+        with open(args.batch_traffic) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
+    mixed_phi = [b32.get_phi(k) for k in range(len(probs))]
+    b32.close()
 
-    batched = None
-    if not args.no_batched:
-        batched = batched_bench(capi, args.batch_n, use_graph=not args.no_graph)
-    serving = None
-    if not args.no_serving:
-        serving = serving_bench(capi)
+    # the same mix with FP64 cycles
+    b64 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG, use_graph)
+    dt64, iters64, _ = timed_batch(capi, b64, args.tol, max(1, min(args.steps, 2)), 1, "batched fp64")
+    b64.close()
+
+    plain = None
+    if not args.no_plain:
+        lat, sols = plain_fp64_mix(capi, geos, bcs, maps, rhs, args.tol, use_graph)
+        plain_rate = len(lat) / float(np.sum(lat))
+        plain = {
+            "solves_per_s": plain_rate,
+            "problems": len(lat),
+            "latency_p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "latency_p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "max_rel_diff_mixed_batched_vs_plain": max(rel_diff(mixed_phi[k], sols[k]) for k in range(len(sols))),
+        }
+
+    single = None if args.no_single else single_bench(capi, args)
+    cfg3 = None if args.no_cfg3 else cfg3_bench(capi, args.cfg3_n, args.cfg3_n // 4)
+    serving = None if args.no_serving else serving_bench(capi)
     dc_rb = None
     if not args.no_dc_rb:
-        dc_rb = dc_rb_bench(capi, geo, bc, rhs)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(geo, bc, rhs, int(round(cycles_per_solve)))
+        geo, bc, rhs2 = config2_problem(NX, NX)
+        dc_rb = dc_rb_bench(capi, geo, bc, rhs2)
+    cpu = None if args.no_cpu_baseline else cpu_baseline_mix(probs, rhs, 2 * iters32)
 
     line = {
         "metric": "solves/s @1e-10 resid, mixed/batched vs plain FP64; p99 latency; HBM GB/s",
-        "value": args.steps / dt,
+        "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * dt / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32" if mixed_main else "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{'mixed-precision' if mixed_main else 'FP64'} single solve, {args.n}x{args.n} "
-                        f"warped/stretched mapped grid "
-                        f"(A=0.1, beta=5), Dirichlet xi / Neumann eta, to {args.tol:g} relative residual",
+            "workload": f"batched uneven problem mix: {len(probs)} mapped-grid Poisson problems, nx, ny "
+                        f"log-uniform in [32, 1024] (multiples of 16), beta U[0,6], A U[0,0.15], BCs all-Dirichlet "
+                        f"or Dirichlet xi / Neumann eta, rhs N(0,1); each to {args.tol:g} relative residual",
             "tol": args.tol,
-            "method": f"FP64 BiCGStab + {'FP32 ' if mixed_main else ''}V({args.sd},{args.su}) y-semicoarsening "
-                      f"x-line multigrid preconditioner ({args.sb} sweeps on the coarsest level)",
-            "v_cycles_per_solve": cycles_per_solve,
-            "final_relres": final_relres,
-            "residual_kernel_ms": r_ms,
-            "residual_kernel_GBps": r_bytes / (r_ms * 1e-3) / 1e9,
-            "copy_kernel_GBps": c_bytes / (c_ms * 1e-3) / 1e9,
+            "cells": cells,
+            "method": "batched: FP64 BiCGStab with iterative refinement on the reference 9-point operator, "
+                      "preconditioned by FP32 V(2,2) y-semicoarsening x-line cycles; one launch per level "
+                      "operation for all problems",
+            "step": "one batched solve of the whole mix from phi = 0, rhs resident in HBM",
+            "v_cycles_mean": 2 * float(np.mean(iters32)),
+            "v_cycles_max": 2 * int(np.max(iters32)),
+            "setup_s": setup_s,
+        },
+        "plain_fp64": plain,
+        "mixed_batched_over_plain_fp64": (value / plain["solves_per_s"]) if plain else None,
+        "batched_fp64": {
+            "solves_per_s": len(probs) * max(1, min(args.steps, 2)) / dt64,
+            "v_cycles_mean": 2 * float(np.mean(iters64)),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_xline{' FP32' if mixed_main else ''} (finest level, even rows)",
+            "kernel": "k_xline_batch_* FP32 (finest-level line relaxation, even rows, of all 1024 problems: "
+                      "one launch per chunk class, side by side)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "avg_launch_ms": k_ms,
-            "timing": "HIP events around 20 back-to-back launches rotating over 4 copies of the operands (cache-cold)",
-            "warm_avg_launch_ms": kw_ms,
             "alg_bytes_per_launch": k_bytes,
+            "timing": "HIP events on the batch stream around 10 back-to-back launch sets (1.7 GB of operands "
+                      "per set, so every set starts cache-cold)",
         },
         "cpu_baseline": cpu,
-        "mixed": compare,
-        "batched": batched,
+        "single_2048": single,
+        "cfg3_8192x2048": cfg3,
         "serving": serving,
         "dc_rb_smoother": dc_rb,
     }
     print(json.dumps(line), flush=True)
-    solver.close()
 
 
 if __name__ == "__main__":

@@ -128,6 +128,23 @@ int mgp_restrict(int kind, int fnx, int fny, const double *fine, int cnx, int cn
  * homogeneous with the given types/modes; fine and crse carry one ghost layer. */
 int mgp_prolong(const mgp_geo *fine_geo, const mgp_geo *crse_geo, const int *bc_types, const int *bc_modes,
                 int order, double *fine, double *crse);
+/* compute_residual in double-double (no counterpart in the reference): res = rhs - L (phi_hi + phi_lo)
+ * with every operation of poisson2d.f90 compute_laplacian's stencil (:886-1319) carried in
+ * double-double and the result rounded once; phi_hi, phi_lo (nx+2) x (ny+2) as phi, their homogeneous
+ * Dirichlet ghosts synthesised from the valid cells.  Dirichlet / Neumann sides, homogeneous, not LAH.
+ * The residual of a solution that FP64 cannot represent accurately enough (see refine in
+ * mgp_solver_opts). */
+int mgp_compute_residual_dd(const mgp_geo *geo, const mgp_bc *bc, const double *rhs, const double *phi_hi,
+                            const double *phi_lo, double *res);
+/* The fourth-order finite-volume operator (no counterpart in the reference; the operator of the
+ * fourth-order deferred correction): lap = L4 phi on cell averages phi (nx x ny), homogeneous Neumann
+ * sides, nx, ny >= 5.  metrics[0..3] on x faces ((nx+1) x ny): <J g^11>, <J g^12>, and (deta/12) times
+ * the differences of J g^11 and J g^12 across the face; metrics[4..7] on y faces (nx x (ny+1)):
+ * <J g^22>, <J g^21> and (dxi/12) times their differences along the face (mgp_metric4_fill). */
+int mgp_lap4(int nx, int ny, double dx, double dy, const double *const *metrics, const double *phi, double *lap);
+/* The eight face-averaged metric arrays of mgp_lap4 for an MGP_MAP_WARPED mapping, generated on the
+ * GPU (3-point Gauss-Legendre averages over each face). */
+int mgp_metric4_fill(const mgp_mapping *map, int nx, int ny, double dx, double dy, double *const *metrics);
 /* MGPoisson2D.f90:38,67 estimate_maxdepth + create_mgschedule; refx/refy hold >= 64 entries */
 int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out,
                           int *refx, int *refy);
@@ -235,7 +252,23 @@ typedef struct {
     int maxdepth;       /* -1: as deep as the schedule allows */
     int homog;          /* BCs of the top-level residual */
     int use_graph;      /* KRYLOV_MG: replay each iteration as a captured hipGraph */
+    /* KRYLOV_MG / KRYLOV_MG32 only (set by mgp_solver_default_opts): */
+    int lines;          /* MGP_LINES_AUTO: x-lines with y-semicoarsening while a row fits one
+                           workgroup's line solve (nx <= 4111), else y-lines; MGP_LINES_X; MGP_LINES_Y:
+                           the preconditioner relaxes columns and semicoarsens in x (it runs on the
+                           transposed problem: rows of ny <= 4111 cells) */
+    int refine;         /* MGP_REFINE_FP64: residual and solution in FP64; MGP_REFINE_DD: the solution
+                           is carried as phi + phi_lo (two FP64 fields) and the refinement residual
+                           rhs - L(phi + phi_lo) is evaluated in double-double, so the relative residual
+                           can fall below the FP64 operator's rounding (homog 1 only) */
+    int order;          /* 2: the reference 9-point operator; 4: the fourth-order finite-volume operator
+                           (all-Neumann sides, mapped solvers only): the outer iteration's residual is
+                           the fourth-order one, the preconditioner V-cycle stays second order (the
+                           deferred-correction pairing of MGPoisson2D_DD.f90:589-862, one order up) */
 } mgp_solver_opts;
+
+enum { MGP_LINES_AUTO = 0, MGP_LINES_X = 1, MGP_LINES_Y = 2 };
+enum { MGP_REFINE_FP64 = 0, MGP_REFINE_DD = 1 };
 
 void mgp_solver_default_opts(mgp_solver_opts *opts);
 int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts);
@@ -248,6 +281,14 @@ int mgp_solver_create_mapped(mgp_solver **s, const mgp_mapping *m, int nx, int n
 int mgp_solver_set_rhs(mgp_solver *s, const double *rhs);
 int mgp_solver_set_phi(mgp_solver *s, const double *phi);
 int mgp_solver_get_phi(mgp_solver *s, double *phi);
+/* Low part of the double-double solution (solvers with refine MGP_REFINE_DD): the solution is
+ * phi + phi_lo; layout of phi. */
+int mgp_solver_get_phi_lo(mgp_solver *s, double *phi_lo);
+/* The compatibility condition of an all-Neumann problem on the resident rhs: rhs -= J m with
+ * m = sum(rhs) / sum(J) (compensated sums in FP64), which makes sum(rhs) vanish -- the J-weighted
+ * zero-mean projection of f for rhs = -J f.  *removed (may be NULL) receives m.  KRYLOV_MG /
+ * KRYLOV_MG32 solvers. */
+int mgp_solver_project_mean(mgp_solver *s, double *removed);
 /* Device copies between resident fields of the same shape (no PCIe). */
 int mgp_solver_copy_rhs_from(mgp_solver *s, const mgp_solver *src);
 /* Solve in place on the resident phi/rhs; relres may be NULL (else maxiters+1 doubles). */
@@ -316,6 +357,13 @@ int mgp_batch_get_phi(mgp_batch *b, int k, double *phi);
 int mgp_batch_solve(mgp_batch *b, double tol, int maxiters, int zerophi, int *iters, int *outcomes);
 /* V-cycle applications of the longest-running problem in the last solve. */
 int mgp_batch_last_cycles(const mgp_batch *b);
+/* Measurement hook (between solves): times `reps` back-to-back launch sets of the finest-level
+ * line relaxation of the even interior rows of every problem (kernel 0; one launch per chunk
+ * class, side by side, as inside a solve) with HIP events on the batch's stream.  *avg_ms is the
+ * mean per set, *bytes the algorithmic bytes of one set (each array element it needs once, at
+ * the element size of the cycle: 8 bytes, or 4 with KRYLOV_MG32).  Changes the resident
+ * finest-level corrections, nothing else. */
+int mgp_batch_time_kernel(mgp_batch *b, int kernel, int reps, double *avg_ms, double *bytes);
 
 /* ---- 5. Serving loop (no counterpart in the reference): resident geometries
  *         ("slots") with their hierarchies and last solutions in HBM, requests

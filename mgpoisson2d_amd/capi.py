@@ -19,6 +19,8 @@ MGP_OK = 0
 CONVERGED, DIVERGED, MAXITERS = 0, 1, 2
 ORDER_LEX, ORDER_RB, ORDER_MC = 0, 1, 2
 METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG, METHOD_FMG, METHOD_KRYLOV_MG32 = 0, 1, 2, 3, 4
+LINES_AUTO, LINES_X, LINES_Y = 0, 1, 2
+REFINE_FP64, REFINE_DD = 0, 1
 MAP_TERRAIN, MAP_WARPED = 0, 1
 
 _d = ctypes.c_double
@@ -48,7 +50,13 @@ class Mapping(ctypes.Structure):
 
 class SolverOpts(ctypes.Structure):
     _fields_ = [("method", _i), ("ordering", _i), ("numcycles", _i), ("smooth_down", _i),
-                ("smooth_up", _i), ("smooth_bottom", _i), ("maxdepth", _i), ("homog", _i), ("use_graph", _i)]
+                ("smooth_up", _i), ("smooth_bottom", _i), ("maxdepth", _i), ("homog", _i), ("use_graph", _i),
+                ("lines", _i), ("refine", _i), ("order", _i)]
+
+    def __init__(self, method=2, ordering=2, numcycles=1, smooth_down=2, smooth_up=2, smooth_bottom=2, maxdepth=-1,
+                 homog=1, use_graph=1, lines=0, refine=0, order=2):
+        super().__init__(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth, homog,
+                         use_graph, lines, refine, order)
 
 
 def _load():
@@ -80,6 +88,8 @@ EXPORTS = (
     "mgp_batch_create_mapped", "mgp_batch_destroy", "mgp_batch_count", "mgp_batch_set_rhs", "mgp_batch_get_phi",
     "mgp_batch_solve", "mgp_batch_last_cycles", "mgp_server_create", "mgp_server_destroy", "mgp_server_footprint",
     "mgp_server_submit", "mgp_server_step", "mgp_server_result", "mgp_integrate2d_bdry", "mgp_gpnorm",
+    "mgp_batch_time_kernel", "mgp_compute_residual_dd", "mgp_lap4", "mgp_metric4_fill", "mgp_solver_get_phi_lo",
+    "mgp_solver_project_mean",
 )
 
 
@@ -333,6 +343,39 @@ def gpnorm(ilo, ihi, jlo, jhi, dx, dy, phi, p=0):
     return out.value
 
 
+def compute_residual_dd(geo, bc, rhs, phi_hi, phi_lo):
+    """rhs - L (phi_hi + phi_lo) in double-double, rounded once (homogeneous Dirichlet / Neumann)."""
+    a = _Args()
+    rhs = _f64(rhs, _cc(geo.nx, geo.ny), "rhs")
+    hi, lo = _f64(phi_hi, _gh(geo.nx, geo.ny), "phi_hi"), _f64(phi_lo, _gh(geo.nx, geo.ny), "phi_lo")
+    res = np.zeros((geo.ny, geo.nx))
+    _check(lib.mgp_compute_residual_dd(a.geo(geo), a.bc(bc, geo.nx, geo.ny), _ptr(rhs), _ptr(hi), _ptr(lo),
+                                       _ptr(res)))
+    return res
+
+
+def _metric4_shapes(nx, ny):
+    return [(ny, nx + 1)] * 4 + [(ny + 1, nx)] * 4
+
+
+def lap4(nx, ny, dx, dy, metrics, phi):
+    """Fourth-order operator on cell averages phi (ny, nx); metrics: the 8 arrays of metric4_fill."""
+    ms = [_f64(m, sh, f"metrics[{k}]") for k, (m, sh) in enumerate(zip(metrics, _metric4_shapes(nx, ny)))]
+    arr = (_dp * 8)(*[_ptr(m) for m in ms])
+    phi = _f64(phi, (ny, nx), "phi")
+    out = np.zeros((ny, nx))
+    _check(lib.mgp_lap4(_i(nx), _i(ny), _d(dx), _d(dy), arr, _ptr(phi), _ptr(out)))
+    return out
+
+
+def metric4_fill(mapping, nx, ny, dx, dy):
+    """The 8 face-averaged metric arrays of the fourth-order operator, generated on the GPU."""
+    ms = [np.zeros(sh) for sh in _metric4_shapes(nx, ny)]
+    arr = (_dp * 8)(*[_ptr(m) for m in ms])
+    _check(lib.mgp_metric4_fill(ctypes.byref(mapping), _i(nx), _i(ny), _d(dx), _d(dy), arr))
+    return ms
+
+
 def extrapolate_ghosts(nx, ny, order, phi):
     phi = _f64(phi, _gh(nx, ny), "phi").copy()
     _check(lib.mgp_extrapolate_ghosts(_i(nx), _i(ny), _i(order), _ptr(phi)))
@@ -403,12 +446,14 @@ class Solver:
     the device between solves."""
 
     def __init__(self, geo, bc, method=METHOD_KRYLOV_MG, ordering=ORDER_MC, numcycles=1, smooth_down=2,
-                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True, use_graph=True, mapping=None):
+                 smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True, use_graph=True, mapping=None,
+                 lines=LINES_AUTO, refine=REFINE_FP64, order=2):
         """geo: a Geometry (metrics uploaded), or with mapping given, any object
-        with nx, ny, dx, dy (metrics generated on the GPU)."""
+        with nx, ny, dx, dy (metrics generated on the GPU).  lines, refine,
+        order: the KRYLOV_MG options of mgp_solver_opts."""
         self.geo = geo
         opts = SolverOpts(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth,
-                          int(homog), int(use_graph))
+                          int(homog), int(use_graph), lines, refine, order)
         a = _Args()
         self._h = ctypes.c_void_p()
         if mapping is None:
@@ -440,6 +485,18 @@ class Solver:
         phi = np.zeros((self.geo.ny + 2, self.geo.nx + 2))
         _check(lib.mgp_solver_get_phi(self._h, _ptr(phi)))
         return phi
+
+    def get_phi_lo(self):
+        """Low part of a double-double solution (refine=REFINE_DD): the solution is get_phi() + get_phi_lo()."""
+        lo = np.zeros((self.geo.ny + 2, self.geo.nx + 2))
+        _check(lib.mgp_solver_get_phi_lo(self._h, _ptr(lo)))
+        return lo
+
+    def project_mean(self):
+        """rhs -= J sum(rhs)/sum(J) on the resident rhs (all-Neumann compatibility); returns the mean removed."""
+        m = _d(0.0)
+        _check(lib.mgp_solver_project_mean(self._h, ctypes.byref(m)))
+        return m.value
 
     def solve(self, tol=1e-10, maxiters=100, zerophi=True):
         relres = np.zeros(maxiters + 1)
@@ -565,6 +622,13 @@ class Batch:
 
     def last_cycles(self):
         return lib.mgp_batch_last_cycles(self._h)
+
+    def time_kernel(self, kernel=0, reps=10):
+        """Mean ms and algorithmic bytes of one launch set of the finest-level
+        line relaxation (even rows) of every problem (between solves only)."""
+        ms, nbytes = _d(0.0), _d(0.0)
+        _check(lib.mgp_batch_time_kernel(self._h, _i(kernel), _i(reps), ctypes.byref(ms), ctypes.byref(nbytes)))
+        return ms.value, nbytes.value
 
 
 class Server:

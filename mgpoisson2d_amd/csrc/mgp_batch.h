@@ -117,6 +117,13 @@ class BatchKMG {
     int outcome(int k) const { return oc_[k]; }
     // device bytes held by the batch (hierarchies, vectors, scalars, tables excluded)
     size_t footprint() const { return footprint_; }
+    // Measurement hook: mean ms of `reps` back-to-back launch sets of the
+    // finest-level line relaxation of the even interior rows of every problem
+    // (one launch per chunk class, side by side, as in a solve), timed with
+    // HIP events on the batch's stream; *bytes the algorithmic bytes of one
+    // set (each array element it needs once, the element size of the cycle).
+    // Between solves only: it relaxes the resident finest-level corrections.
+    double time_fine_xline(int reps, double *bytes);
 
   private:
     void body();

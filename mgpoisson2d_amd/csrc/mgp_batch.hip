@@ -735,6 +735,62 @@ std::vector<int> BatchKMG::finish_step() {
     return done;
 }
 
+double BatchKMG::time_fine_xline(int reps, double *bytes) {
+    if (T != &full) {
+        release(part);
+        T = &full;
+    }
+    const int n = count();
+    // the even interior rows of level 0 (colour 0 of the first zebra phase)
+    std::vector<const XlTable *> tabs;
+    for (const XlTable &x : T->xl)
+        if (x.d == 0 && x.set == SET_MID && x.colour == 0) tabs.push_back(&x);
+    const double es = mixed ? 4.0 : 8.0;
+    double b = 0.0;
+    for (int p = 0; p < n; ++p) {
+        if (nlev[p] < 2) continue;  // a single-level problem's rows are in the bottom set
+        const Grid &g = P[p].g;
+        const double R = g.ny >= 4 ? (g.ny - 4) / 2 + 1 : 0;
+        // e on rows 1 .. 2R+1 and written on R rows; f, Jxx, Jxy on R rows;
+        // Jyx, Jyy on 2R face rows; 9 table entries for the 2R ring cells
+        b += g.nx * es * ((2.0 * R + 1.0) + R + R + R + 2.0 * R + 2.0 * R + R) + R * 2.0 * 9.0 * es;
+    }
+    *bytes = b;
+    std::vector<int> all(n, 1);
+    upload_flags(active, all);
+    auto launch_set = [&]() {
+        if (tabs.size() == 1 || tabs.size() > aux.size()) {
+            for (const XlTable *x : tabs)
+                launch_xline_batch(s, (const XlTask *)x->t.p, x->t.n, dL, active, 0, x->t.C, x->t.threads, x->t.lds,
+                                   -1, mixed);
+            return;
+        }
+        hip_check(hipEventRecord(fork_ev, s), "fork");
+        for (size_t i = 0; i < tabs.size(); ++i) {
+            hip_check(hipStreamWaitEvent(aux[i], fork_ev, 0), "fork wait");
+            launch_xline_batch(aux[i], (const XlTask *)tabs[i]->t.p, tabs[i]->t.n, dL, active, 0, tabs[i]->t.C,
+                               tabs[i]->t.threads, tabs[i]->t.lds, -1, mixed);
+            hip_check(hipEventRecord(join_ev[i], aux[i]), "join");
+        }
+        for (size_t i = 0; i < tabs.size(); ++i) hip_check(hipStreamWaitEvent(s, join_ev[i], 0), "join wait");
+    };
+    hipEvent_t a = nullptr, e = nullptr;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&e), "event");
+    launch_set();  // warm
+    hip_check(hipEventRecord(a, s), "event");
+    for (int k = 0; k < reps; ++k) launch_set();
+    hip_check(hipEventRecord(e, s), "event");
+    hip_check(hipEventSynchronize(e), "event");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, a, e), "event");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(e);
+    upload_flags(active, act_);
+    hip_check(hipStreamSynchronize(s), "sync");
+    return ms / reps;
+}
+
 int BatchKMG::running() const {
     int r = 0;
     for (int a : act_) r += a;

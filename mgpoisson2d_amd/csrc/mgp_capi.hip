@@ -531,6 +531,85 @@ int mgp_gpnorm(int ilo, int ihi, int jlo, int jhi, double dx, double dy, const d
     });
 }
 
+int mgp_compute_residual_dd(const mgp_geo *geo, const mgp_bc *bc, const double *rhs, const double *phi_hi,
+                            const double *phi_lo, double *res) {
+    return guarded([&] {
+        if (!rhs || !phi_hi || !phi_lo || !res) {
+            g_last_error = "compute_residual_dd: null array";
+            throw Failure(MGP_ERR_ARG);
+        }
+        for (int k = 0; k < 4; ++k)
+            if (!bc || (bc->type[k] != BC_DIRI && bc->type[k] != BC_NEUM) || bc->mode[k] == MODE_LAH) {
+                g_last_error = "compute_residual_dd: Dirichlet or Neumann sides (homogeneous, not LAH)";
+                throw Failure(MGP_ERR_UNSUPPORTED);
+            }
+        Call c(geo, bc);
+        const Grid &g = c.geo.g;
+        DevBuf hi = c.field(), lo = c.field(), f = c.field(), r = c.field();
+        up_phi(c.s, hi, g, phi_hi);
+        up_phi(c.s, lo, g, phi_lo);
+        up_cc(c.s, f, g, rhs);
+        launch_residual_dd(c.s, r.p, hi.p, lo.p, f.p, g, c.geo.view(), c.bc.bc);
+        down_cc(c.s, res, r, g);
+        c.finish();
+    });
+}
+
+int mgp_lap4(int nx, int ny, double dx, double dy, const double *const *metrics, const double *phi, double *lap) {
+    return guarded([&] {
+        if (nx < 5 || ny < 5 || !(dx > 0) || !(dy > 0) || !metrics || !phi || !lap) {
+            g_last_error = "lap4: need nx, ny >= 5, positive spacing and arrays";
+            throw Failure(MGP_ERR_ARG);
+        }
+        for (int k = 0; k < 8; ++k)
+            if (!metrics[k]) {
+                g_last_error = "lap4: null metric array";
+                throw Failure(MGP_ERR_ARG);
+            }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        const size_t n = level_elems(nx, ny);
+        DevBuf m[8], f, out;
+        for (int k = 0; k < 8; ++k) {
+            m[k].alloc(n);
+            const bool xface = k < 4;
+            upload_block(s, m[k].p, g, metrics[k], 0, 0, xface ? nx + 1 : nx, xface ? ny : ny + 1);
+        }
+        f.alloc(n);
+        out.alloc(n);
+        upload_block(s, f.p, g, phi, 0, 0, nx, ny);
+        launch_lap4(s, out.p, f.p, nullptr, g, Geo4{m[0].p, m[1].p, m[2].p, m[3].p, m[4].p, m[5].p, m[6].p, m[7].p});
+        download_block(s, lap, out.p, g, 0, 0, nx, ny);
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int mgp_metric4_fill(const mgp_mapping *map, int nx, int ny, double dx, double dy, double *const *metrics) {
+    return guarded([&] {
+        check_mapping(map);
+        if (map->kind != MGP_MAP_WARPED || nx < 1 || ny < 1 || !(dx > 0) || !(dy > 0) || !metrics) {
+            g_last_error = "metric4_fill: MGP_MAP_WARPED, a positive grid and 8 arrays";
+            throw Failure(MGP_ERR_ARG);
+        }
+        for (int k = 0; k < 8; ++k)
+            if (!metrics[k]) {
+                g_last_error = "metric4_fill: null metric array";
+                throw Failure(MGP_ERR_ARG);
+            }
+        hipStream_t s = thread_stream();
+        const Grid g = make_grid(nx, ny, dx, dy);
+        const size_t n = level_elems(nx, ny);
+        DevBuf m[8];
+        for (int k = 0; k < 8; ++k) m[k].alloc(n);
+        launch_metric4(s, *map, g, Geo4W{m[0].p, m[1].p, m[2].p, m[3].p, m[4].p, m[5].p, m[6].p, m[7].p});
+        for (int k = 0; k < 8; ++k) {
+            const bool xface = k < 4;
+            download_block(s, metrics[k], m[k].p, g, 0, 0, xface ? nx + 1 : nx, xface ? ny : ny + 1);
+        }
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
 int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in, int *maxdepth_out, int *refx,
                           int *refy) {
     return guarded([&] {
@@ -768,6 +847,9 @@ void mgp_solver_default_opts(mgp_solver_opts *o) {
     o->maxdepth = -1;
     o->homog = 1;
     o->use_graph = 1;
+    o->lines = MGP_LINES_AUTO;
+    o->refine = MGP_REFINE_FP64;
+    o->order = 2;
 }
 
 extern "C++" {
@@ -775,7 +857,8 @@ namespace {
 
 // Shared body of the two create calls: make_geo(stream) fills s->geo.
 template <class MakeGeo>
-int create_solver(mgp_solver **out, int nx, int ny, const mgp_bc *bc, const mgp_solver_opts *opts, MakeGeo make_geo) {
+int create_solver(mgp_solver **out, int nx, int ny, const mgp_bc *bc, const mgp_solver_opts *opts, MakeGeo make_geo,
+                  const mgp_mapping *map = nullptr) {
     *out = nullptr;
     mgp_solver *s = new mgp_solver;
     const int st = guarded([&] {
@@ -803,7 +886,7 @@ int create_solver(mgp_solver **out, int nx, int ny, const mgp_bc *bc, const mgp_
             s->ref.build(s->stream, s->geo, s->ubc.bc, o.method == MGP_METHOD_DC_VCYCLE, o.ordering, o.maxdepth);
             s->ref.fmg = o.method == MGP_METHOD_FMG;
         } else if (o.method == MGP_METHOD_KRYLOV_MG || o.method == MGP_METHOD_KRYLOV_MG32)
-            s->kmg.build(s->stream, s->geo, s->ubc.bc, o);
+            s->kmg.build(s->stream, s->geo, s->ubc.bc, o, map);
         else {
             g_last_error = "unknown method";
             throw Failure(MGP_ERR_ARG);
@@ -846,8 +929,8 @@ int mgp_solver_create_mapped(mgp_solver **out, const mgp_mapping *m, int nx, int
         *out = nullptr;
         return st;
     }
-    return create_solver(out, nx, ny, bc, opts,
-                         [&](hipStream_t stream, DevGeo &dg) { dg.generate(stream, *m, nx, ny, dx, dy); });
+    return create_solver(
+        out, nx, ny, bc, opts, [&](hipStream_t stream, DevGeo &dg) { dg.generate(stream, *m, nx, ny, dx, dy); }, m);
 }
 
 int mgp_metric_fill(const mgp_mapping *m, int nx, int ny, double dx, double dy, double *J, double *Jxx, double *Jxy,
@@ -890,6 +973,7 @@ int mgp_solver_set_rhs(mgp_solver *s, const double *rhs) {
 int mgp_solver_set_phi(mgp_solver *s, const double *phi) {
     return guarded([&] {
         up_phi(s->stream, s->phi, s->geo.g, phi);
+        if (double *lo = s->kmg.phi_lo()) launch_setval(s->stream, lo, 0.0, s->geo.g, 1);
         hip_check(hipStreamSynchronize(s->stream), "sync");
     });
 }
@@ -898,6 +982,29 @@ int mgp_solver_get_phi(mgp_solver *s, double *phi) {
     return guarded([&] {
         down_phi(s->stream, phi, s->phi, s->geo.g);
         hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_get_phi_lo(mgp_solver *s, double *phi_lo) {
+    return guarded([&] {
+        double *lo = s->kmg.phi_lo();
+        if (!lo) {
+            g_last_error = "get_phi_lo: the solver does not carry a double-double solution (refine MGP_REFINE_DD)";
+            throw Failure(MGP_ERR_ARG);
+        }
+        download_block(s->stream, phi_lo, lo, s->geo.g, -1, -1, s->geo.g.nx + 2, s->geo.g.ny + 2);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_project_mean(mgp_solver *s, double *removed) {
+    return guarded([&] {
+        if (s->opts.method != MGP_METHOD_KRYLOV_MG && s->opts.method != MGP_METHOD_KRYLOV_MG32) {
+            g_last_error = "project_mean: KRYLOV_MG / KRYLOV_MG32 solvers only";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const double m = s->kmg.project_mean(s->stream, s->geo, s->rhs.p);
+        if (removed) *removed = m;
     });
 }
 
@@ -1155,6 +1262,16 @@ int mgp_batch_solve(mgp_batch *b, double tol, int maxiters, int zerophi, int *it
             throw Failure(MGP_ERR_ARG);
         }
         b->b->solve(tol, maxiters, zerophi, iters, outcomes);
+    });
+}
+
+int mgp_batch_time_kernel(mgp_batch *b, int kernel, int reps, double *avg_ms, double *bytes) {
+    return guarded([&] {
+        if (!b || !b->b || kernel != 0 || reps < 1 || !avg_ms || !bytes) {
+            g_last_error = "batch_time_kernel: kernel must be 0, reps >= 1";
+            throw Failure(MGP_ERR_ARG);
+        }
+        *avg_ms = b->b->time_fine_xline(reps, bytes);
     });
 }
 

@@ -50,6 +50,13 @@ struct GeoF {
     const float *J, *Jxx, *Jxy, *Jyx, *Jyy;
 };
 
+// Face-averaged metrics of the fourth-order operator (mgp_fourth.hip), level
+// layout: on x faces <J g^11>, <J g^12> and their tangential differences
+// (deta/12) d/deta, on y faces <J g^22>, <J g^21> and (dxi/12) d/dxi.
+struct Geo4 {
+    const double *A4, *Bx4, *dA, *dBx, *C4, *By4, *dC, *dBy;
+};
+
 // Boundary description as the kernels see it.  val[] is the UNIFORM value;
 // data[] points at device NONUNIFORM face data (ny values for x sides, nx for
 // y sides) or is null.

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mgp2d.h"
 #include "mgp_common.h"
 
 namespace mgp {
@@ -151,6 +152,29 @@ void launch_begin32(hipStream_t s, float *f32, float *e32, const double *f, cons
 void launch_axpbypcz(hipStream_t s, double *out, double a, const double *x, double b, const double *y, double c,
                      const double *z, const Grid &g);
 void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sweeps, int homog);
+
+// Fourth-order finite-volume operator (mgp_fourth.hip), homogeneous Neumann sides.
+// out = rhs - L4 phi (or L4 phi if rhs is null) over the valid cells; nx, ny >= 5.
+void launch_lap4(hipStream_t s, double *out, const double *phi, const double *rhs, const Grid &g, const Geo4 &m);
+// Face-averaged metrics of an MGP_MAP_WARPED mapping (3-point Gauss-Legendre over each face).
+struct Geo4W {
+    double *A4, *Bx4, *dA, *dBx, *C4, *By4, *dC, *dBy;
+};
+void launch_metric4(hipStream_t s, const mgp_mapping &m, const Grid &g, const Geo4W &out);
+
+// Refinement loop (mgp_refine.hip).
+// out = rhs - L (hi + lo) in double-double, rounded once (homogeneous Dirichlet / Neumann sides).
+void launch_residual_dd(hipStream_t s, double *out, const double *hi, const double *lo, const double *rhs,
+                        const Grid &g, const Geo &geo, const Bc &bc);
+// (hi, lo) += x in double-double, then x = 0, over the valid cells.
+void launch_fold_dd(hipStream_t s, double *hi, double *lo, double *x, const Grid &g);
+// rhs -= J sum(rhs) / sum(J) (compensated sums into sums[0..1]; partial: reduce_blocks(g) doubles).
+void launch_project_jmean(hipStream_t s, double *rhs, const double *J, double *sums, double *partial,
+                          const Grid &g);
+// dst(j, i) = src(i, j) for 0 <= i < ni, 0 <= j < nj (level layouts of pitch spitch / dpitch).
+void launch_transpose(hipStream_t s, double *dst, int dpitch, const double *src, int spitch, int ni, int nj);
+void launch_transpose(hipStream_t s, float *dst, int dpitch, const double *src, int spitch, int ni, int nj);
+void launch_transpose(hipStream_t s, double *dst, int dpitch, const float *src, int spitch, int ni, int nj);
 
 // Flux tools (mgp_flux.hip).  Face fields use the level layout: x-face (i, j)
 // and y-face (i, j) at ix(pitch, i, j).

@@ -18,7 +18,15 @@ struct KrylovMG {
     ~KrylovMG();
     KrylovMG(const KrylovMG &) = delete;
     KrylovMG &operator=(const KrylovMG &) = delete;
-    void build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o);
+    // map: the mapping the metrics came from (null for uploaded metrics); the
+    // fourth-order operator (o.order == 4) needs it for its face-averaged metrics.
+    void build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o,
+               const mgp_mapping *map = nullptr);
+    // Low part of the double-double solution (MGP_REFINE_DD; null otherwise).
+    double *phi_lo() const;
+    // rhs -= J sum(rhs) / sum(J) over the valid cells (the compatibility
+    // condition of an all-Neumann problem); returns sum(rhs) / sum(J) removed.
+    double project_mean(hipStream_t s, const DevGeo &top, double *rhs);
     int solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
               double *rhs, double tol, int maxiters, int zerophi, double *relres, int *iters);
     // Times `reps` launches of one fine-level kernel with HIP events on stream s.

@@ -22,6 +22,29 @@
 // approximate A^-1, so its rounding changes the iteration count, not the
 // accuracy of the converged solution, which is set by the FP64 residual
 // (with restarts from the FP64 true residual: iterative refinement).
+//
+// Orientation (MGP_LINES_Y): grids whose rows are too long for one
+// workgroup's line solve (nx > 4111) relax columns instead: the hierarchy is
+// built on the transposed problem (xi and eta exchanged, the face metrics
+// Jxx/Jyy and Jxy/Jyx exchanged and transposed, the sides permuted), whose
+// rows are the original columns, so its y-semicoarsening is x-semicoarsening
+// of the original grid.  A preconditioner application transposes its input
+// into the hierarchy and its output back (two extra passes over one field).
+//
+// Double-double refinement (MGP_REFINE_DD): the solution is phi + phi_lo,
+// each BiCGStab run accumulates its correction in a separate FP64 field x,
+// and at every true-residual check x is folded into phi + phi_lo and the
+// residual rhs - A (phi + phi_lo) is formed in double-double (mgp_dd.h).
+// Each refinement run then starts from an accurate residual, so the relative
+// residual can fall below the rounding of the FP64 operator (about 2e-10 of
+// the rhs at 2048^2 on the stretched map).
+//
+// Fourth order (o.order == 4): A is the fourth-order finite-volume operator
+// (mgp_fourth.hip) on face-averaged metrics of the mapping; the V-cycle stays
+// the second-order one on the point metrics.  As in the reference's deferred
+// correction (MGPoisson2D_DD.f90:589-862: outer residual with the full
+// operator, inner cycle with a low-order one), the converged solution is the
+// high-order operator's.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -51,7 +74,14 @@ struct KLevel {
 struct KrylovMG::Impl {
     std::vector<KLevel> L;
     bool mixed = false;
+    bool tr = false;     // y-lines: the hierarchy is the transposed problem's
+    bool dd = false;     // double-double refinement
+    int order = 2;       // outer operator: 2 reference, 4 fourth-order
+    Grid top_g;          // the caller's finest grid
     DevBuf r, rhat, p, v, s, t, phat, shat, scal, partial;
+    DevBuf lo, x;        // dd: low part of the solution, the run's correction
+    DevBuf m4[8];        // fourth order: face-averaged metrics (A4, Bx4, dA, dBx, C4, By4, dC, dBy)
+    DevBuf sums;         // project_mean: the two compensated sums
     double *host = nullptr;  // pinned copy of the device scalars
     int sd = 2, su = 2, sb = 2;
     bool use_graph = true;
@@ -67,7 +97,21 @@ KrylovMG::~KrylovMG() { delete impl; }
 
 static double side_sign(const Bc &bc, int side) { return bc.type[side] == BC_DIRI ? -1.0 : 1.0; }
 
-void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o) {
+// The BCs of the transposed problem: its x sides are the original y sides.
+static Bc transposed_bc(const Bc &b) {
+    Bc t = b;
+    const int perm[4] = {YLO, YHI, XLO, XHI};  // side k of the transposed problem is side perm[k]
+    for (int k = 0; k < 4; ++k) {
+        t.type[k] = b.type[perm[k]];
+        t.mode[k] = b.mode[perm[k]];
+        t.val[k] = b.val[perm[k]];
+        t.data[k] = b.data[perm[k]];
+    }
+    return t;
+}
+
+void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o,
+                     const mgp_mapping *map) {
     if (top.g.ny < 4 || top.g.nx < 4) {
         g_last_error = "KRYLOV_MG needs nx, ny >= 4";
         throw Failure(MGP_ERR_ARG);
@@ -77,25 +121,59 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
             g_last_error = "KRYLOV_MG supports Dirichlet and Neumann sides";
             throw Failure(MGP_ERR_UNSUPPORTED);
         }
-    if (!xline_supported(top.g.nx)) {
-        g_last_error = "KRYLOV_MG: row too long for one workgroup's line solve (nx <= 4111 in FP64)";
+    if (o.lines < MGP_LINES_AUTO || o.lines > MGP_LINES_Y || (o.refine != MGP_REFINE_FP64 && o.refine != MGP_REFINE_DD) ||
+        (o.order != 2 && o.order != 4)) {
+        g_last_error = "KRYLOV_MG: lines must be AUTO, X or Y, refine FP64 or DD, order 2 or 4";
+        throw Failure(MGP_ERR_ARG);
+    }
+    const bool tr = o.lines == MGP_LINES_Y || (o.lines == MGP_LINES_AUTO && !xline_supported(top.g.nx));
+    if (!xline_supported(tr ? top.g.ny : top.g.nx)) {
+        g_last_error = "KRYLOV_MG: lines too long for one workgroup's line solve (rows of at most 4111 cells; "
+                       "with y-lines, columns)";
         throw Failure(MGP_ERR_UNSUPPORTED);
+    }
+    if (o.refine == MGP_REFINE_DD && !o.homog) {
+        g_last_error = "KRYLOV_MG: the double-double residual needs homogeneous BCs (homog 1)";
+        throw Failure(MGP_ERR_UNSUPPORTED);
+    }
+    if (o.order == 4) {
+        for (int k = 0; k < 4; ++k)
+            if (user_bc.type[k] != BC_NEUM) {
+                g_last_error = "KRYLOV_MG: the fourth-order operator needs Neumann sides";
+                throw Failure(MGP_ERR_UNSUPPORTED);
+            }
+        if (!map || map->kind != MGP_MAP_WARPED || top.g.nx < 5 || top.g.ny < 5) {
+            g_last_error = "KRYLOV_MG: the fourth-order operator needs a solver made from an MGP_MAP_WARPED "
+                           "mapping (mgp_solver_create_mapped), nx, ny >= 5";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        if (o.refine == MGP_REFINE_DD) {
+            g_last_error = "KRYLOV_MG: the double-double residual is built for the second-order operator only";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
     }
     const bool mixed = o.method == MGP_METHOD_KRYLOV_MG32;
     delete impl;
     impl = new Impl;
     impl->mixed = mixed;
+    impl->tr = tr;
+    impl->dd = o.refine == MGP_REFINE_DD;
+    impl->order = o.order;
+    impl->top_g = top.g;
     impl->sd = o.smooth_down > 0 ? o.smooth_down : 1;
     impl->su = o.smooth_up > 0 ? o.smooth_up : 1;
     impl->sb = o.smooth_bottom > 0 ? o.smooth_bottom : 1;
     std::vector<KLevel> &L = impl->L;
-    int ny = top.g.ny;
-    double dy = top.g.dy;
+    // the hierarchy's finest grid and BCs: the caller's, or the transposed ones
+    const Grid g0 = tr ? make_grid(top.g.ny, top.g.nx, top.g.dy, top.g.dx) : top.g;
+    const Bc bc0 = tr ? transposed_bc(user_bc) : user_bc;
+    int ny = g0.ny;
+    double dy = g0.dy;
     const int maxlev = o.maxdepth < 0 ? 64 : o.maxdepth + 1;
     while (true) {
         L.emplace_back();
         KLevel &l = L.back();
-        l.g = make_grid(top.g.nx, ny, top.g.dx, dy);
+        l.g = make_grid(g0.nx, ny, g0.dx, dy);
         const size_t n = level_elems(l.g.nx, l.g.ny);
         for (DevBuf *b : {&l.J, &l.Jxx, &l.Jxy, &l.Jyx, &l.Jyy}) b->alloc(n);
         // FP64 level fields: the FP64 cycle, and on the finest level the
@@ -103,9 +181,18 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         if (!mixed || L.size() == 1)
             for (DevBuf *b : {&l.e, &l.r, &l.w}) b->alloc(n);
         l.K.alloc((size_t)ring_table_elems(l.g));
-        l.bc = mg_bc(user_bc, MODE_UNIFORM);
+        l.bc = mg_bc(bc0, MODE_UNIFORM);
         const size_t bytes = n * sizeof(double);
-        if (L.size() == 1) {
+        if (L.size() == 1 && tr) {
+            // transposed metrics: x faces of the transposed problem are the
+            // original y faces (Jxx' = Jyy^T, Jxy' = Jyx^T), y faces the x faces
+            const int P = top.g.pitch, nx0 = top.g.nx, ny0 = top.g.ny;
+            launch_transpose(s, l.J.p, l.g.pitch, top.J.p, P, nx0, ny0);
+            launch_transpose(s, l.Jxx.p, l.g.pitch, top.Jyy.p, P, nx0, ny0 + 1);
+            launch_transpose(s, l.Jxy.p, l.g.pitch, top.Jyx.p, P, nx0, ny0 + 1);
+            launch_transpose(s, l.Jyx.p, l.g.pitch, top.Jxy.p, P, nx0 + 1, ny0);
+            launch_transpose(s, l.Jyy.p, l.g.pitch, top.Jxx.p, P, nx0 + 1, ny0);
+        } else if (L.size() == 1) {
             hip_check(hipMemcpyAsync(l.J.p, top.J.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
             hip_check(hipMemcpyAsync(l.Jxx.p, top.Jxx.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
             hip_check(hipMemcpyAsync(l.Jxy.p, top.Jxy.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
@@ -143,6 +230,16 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
     const size_t n0 = level_elems(top.g.nx, top.g.ny);
     for (DevBuf *b : {&impl->r, &impl->rhat, &impl->p, &impl->v, &impl->s, &impl->t, &impl->phat, &impl->shat})
         b->alloc(n0);
+    if (impl->dd) {
+        impl->lo.alloc(n0);
+        impl->x.alloc(n0);
+    }
+    if (impl->order == 4) {
+        for (DevBuf &b : impl->m4) b.alloc(n0);
+        DevBuf *m = impl->m4;
+        launch_metric4(s, *map, top.g, Geo4W{m[0].p, m[1].p, m[2].p, m[3].p, m[4].p, m[5].p, m[6].p, m[7].p});
+    }
+    impl->sums.alloc(2);
     impl->scal.alloc(SC_COUNT);
     impl->partial.alloc((size_t)reduce_blocks(top.g) + 8);
     hip_check(hipHostMalloc((void **)&impl->host, SC_COUNT * sizeof(double)), "hipHostMalloc");
@@ -208,9 +305,26 @@ static void vcycle_t(hipStream_t s, KrylovMG::Impl &K, size_t d, T *e, const T *
 }
 
 // Preconditioner application e = M f (one V-cycle from e = 0), FP64 in and
-// out; in mixed precision the cycle runs on FP32 copies.
+// out; in mixed precision the cycle runs on FP32 copies.  With y-lines the
+// input is transposed into the hierarchy's finest level and the result
+// transposed back (the valid cells of e; its ghosts are not read).
 static void precondition(hipStream_t s, KrylovMG::Impl &K, double *e, const double *f) {
     KLevel &l = K.L[0];
+    const Grid &tg = K.top_g;
+    if (K.tr) {
+        if (!K.mixed) {
+            launch_transpose(s, l.r.p, l.g.pitch, f, tg.pitch, tg.nx, tg.ny);
+            launch_setval(s, l.e.p, 0.0, l.g, 1);
+            vcycle_t(s, K, 0, l.e.p, (const double *)l.r.p);
+            launch_transpose(s, e, tg.pitch, l.e.p, l.g.pitch, l.g.nx, l.g.ny);
+        } else {
+            launch_transpose(s, l.r32.p, l.g.pitch, f, tg.pitch, tg.nx, tg.ny);
+            launch_setval(s, l.e32.p, 0.0f, l.g, 1);
+            vcycle_t(s, K, 0, l.e32.p, (const float *)l.r32.p);
+            launch_transpose(s, e, tg.pitch, l.e32.p, l.g.pitch, l.g.nx, l.g.ny);
+        }
+        return;
+    }
     if (!K.mixed) {
         launch_setval(s, e, 0.0, l.g, 1);
         vcycle_t(s, K, 0, e, f);
@@ -221,9 +335,31 @@ static void precondition(hipStream_t s, KrylovMG::Impl &K, double *e, const doub
     launch_convert(s, e, l.e32.p, l.g, 0);
 }
 
-// out = A_h v (homogeneous user BCs); v's ghosts are refilled.
-static void apply_h(hipStream_t s, const DevGeo &top, const Bc &ubc, double *v, double *out) {
-    launch_residual_synth(s, out, v, nullptr, top.g, top.view(), ubc);
+static Geo4 metrics4(const KrylovMG::Impl &K) {
+    const DevBuf *m = K.m4;
+    return Geo4{m[0].p, m[1].p, m[2].p, m[3].p, m[4].p, m[5].p, m[6].p, m[7].p};
+}
+
+// out = A_h v (homogeneous user BCs): the reference operator with its
+// homogeneous Dirichlet ghosts synthesised from v, or the fourth-order one.
+static void apply_h(hipStream_t s, const KrylovMG::Impl &K, const DevGeo &top, const Bc &ubc, double *v,
+                    double *out) {
+    if (K.order == 4) launch_lap4(s, out, v, nullptr, top.g, metrics4(K));
+    else launch_residual_synth(s, out, v, nullptr, top.g, top.view(), ubc);
+}
+
+double *KrylovMG::phi_lo() const { return impl && impl->dd ? impl->lo.p : nullptr; }
+
+double KrylovMG::project_mean(hipStream_t s, const DevGeo &top, double *rhs) {
+    if (!built) {
+        g_last_error = "solver not built";
+        throw Failure(MGP_ERR_ARG);
+    }
+    launch_project_jmean(s, rhs, top.J.p, impl->sums.p, impl->partial.p, top.g);
+    double h[2];
+    hip_check(hipMemcpyAsync(h, impl->sums.p, sizeof h, hipMemcpyDeviceToHost, s), "copy");
+    hip_check(hipStreamSynchronize(s), "sync");
+    return h[0] / h[1];
 }
 
 int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
@@ -238,9 +374,19 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     const double vol = g.dx * g.dy;
     double *sc = K.scal.p;
     double *partial = K.partial.p;
+    // the solution the BiCGStab runs update: phi itself, or (double-double)
+    // the correction x of the current run, folded into phi + lo at each check
+    double *xs = K.dd ? K.x.p : phi;
     auto true_residual = [&]() {
-        launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
-        launch_laplacian(s, K.r.p, phi, rhs, g, geo, user_bc, homog, 0);
+        if (K.dd) {
+            launch_fold_dd(s, phi, K.lo.p, K.x.p, g);
+            launch_residual_dd(s, K.r.p, phi, K.lo.p, rhs, g, geo, user_bc);
+        } else if (K.order == 4) {
+            launch_lap4(s, K.r.p, phi, rhs, g, metrics4(K));
+        } else {
+            launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
+            launch_laplacian(s, K.r.p, phi, rhs, g, geo, user_bc, homog, 0);
+        }
         return pnorm2(W, s, K.r.p, g);
     };
     // One BiCGStab iteration with every scalar on the device (right
@@ -250,17 +396,17 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
         launch_bicg_scalars(s, sc, 0);
         launch_bicg_update(s, K.p.p, K.r.p, K.v.p, sc, g, 0);
         precondition(s, K, K.phat.p, K.p.p);
-        apply_h(s, top, user_bc, K.phat.p, K.v.p);
+        apply_h(s, K, top, user_bc, K.phat.p, K.v.p);
         launch_reduce(s, sc + SC_RV, partial, K.rhat.p, K.v.p, nullptr, vol, g, 0);
         launch_bicg_scalars(s, sc, 1);
         launch_bicg_update(s, K.s.p, K.r.p, K.v.p, sc, g, 1);
-        launch_bicg_update(s, phi, K.phat.p, nullptr, sc, g, 2);
+        launch_bicg_update(s, xs, K.phat.p, nullptr, sc, g, 2);
         precondition(s, K, K.shat.p, K.s.p);
-        apply_h(s, top, user_bc, K.shat.p, K.t.p);
+        apply_h(s, K, top, user_bc, K.shat.p, K.t.p);
         launch_reduce(s, sc + SC_TS, partial, K.t.p, K.s.p, nullptr, vol, g, 0);
         launch_reduce(s, sc + SC_TT, partial, K.t.p, K.t.p, nullptr, vol, g, 0);
         launch_bicg_scalars(s, sc, 2);
-        launch_bicg_update(s, phi, K.shat.p, nullptr, sc, g, 3);
+        launch_bicg_update(s, xs, K.shat.p, nullptr, sc, g, 3);
         launch_bicg_update(s, K.r.p, K.s.p, K.t.p, sc, g, 4);
         launch_reduce(s, sc + SC_RN, partial, K.r.p, K.r.p, nullptr, vol, g, 0);
     };
@@ -280,6 +426,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     }
 
     if (zerophi) launch_setval(s, phi, 0.0, g, 1);
+    if (zerophi && K.dd) launch_setval(s, K.lo.p, 0.0, g, 1);
     const double r0 = true_residual();
     std::vector<double> hist;
     hist.push_back(r0 == 0.0 ? 0.0 : 1.0);
@@ -333,6 +480,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
             hist.push_back(rn);
         }
     }
+    if (K.dd) launch_fold_dd(s, phi, K.lo.p, K.x.p, g);  // a run cut off by maxiters
     hip_check(hipStreamSynchronize(s), "sync");
     if (relres)
         for (size_t k = 0; k < hist.size() && (int)k <= maxiters; ++k) relres[k] = hist[k];

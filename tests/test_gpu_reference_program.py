@@ -10,9 +10,13 @@ verbosity 3) and the divergence after the projection.
 
 Tolerances: the flux integrals run in the reference's order (bit-identical);
 the divergence integral and the FMG residual norms are parallel sums on the
-GPU, so values at the rounding floor (a few 1e-14) agree only in magnitude.
-Residual histories agree to 1e-9 relative while they are above 1e-10, the
-initial |div| to 1e-12, and both runs reduce |div| by more than 1e11.
+GPU, and the all-Neumann cycle's mean removal on the 512^2 level is a
+parallel sum too, whose last-bit differences the singular bottom solve
+amplifies (measured: 1.6e-9 relative at the second FMG iteration, 6e-6 at the
+third, where the residual is 3.5e-12).  So: the initial |div| to 1e-12, the
+residual history to 1e-4 relative while above 1e-12 (the per-cycle factors
+then agree well within 5 %), values at the rounding floor (a few 1e-14) only
+in magnitude, and both runs reduce |div| by more than 1e11.
 """
 import os
 import re
@@ -66,9 +70,9 @@ def test_reference_test_program_runs_on_the_dropin():
     # FMG residual history while above the rounding floor
     n = 0
     for a, b in zip(ref["hist"], gpu["hist"]):
-        if b < 1e-10:
+        if b < 1e-12:
             break
-        assert abs(a - b) <= 1e-9 * b, (ref["hist"], gpu["hist"])
+        assert abs(a - b) <= 1e-4 * b, (ref["hist"], gpu["hist"])
         n += 1
-    assert n >= 3
+    assert n >= 4
     assert ref["reduction"] < 1e-11 and gpu["reduction"] < 1e-11
