@@ -152,6 +152,17 @@ int mgp_create_mgschedule(int nx, int ny, double dx, double dy, int maxdepth_in,
  * (sum dx dy |a|^p)^(1/p).  Sums are parallel and compensated (within an ulp or two of the
  * reference's sequential sums); the max norm is exact. */
 int mgp_pnorm(int nx, int ny, double dx, double dy, const double *a, int p, double *out);
+/* No counterpart in the reference: mgp_pnorm on a field already in device memory, for callers that
+ * cannot allocate (code under HIP graph capture, serving loops that forbid allocation).  It allocates
+ * nothing: the reduction's per-block partial sums live in a static 8 KiB device array (1024
+ * doubles, one per GPU, held until the process ends); every grid size fits, its rows being strided
+ * over at most 1024 row blocks.  a: device pointer in the resident layout of mgp_solver_field
+ * (element (i, j) at a[(j + 1) * pitch + 32 + i], pitch >= nx + 34); p = 0 (max |a|), 1 or 2, the
+ * sums of mgp_pnorm (the same kernels and row blocks, so the same result); *out: device pointer,
+ * written on `stream` (a hipStream_t; null: the null stream).  Calls that overlap on one GPU
+ * (different streams or threads) race on the shared array. */
+int mgp_pnorm_noalloc(int nx, int ny, double dx, double dy, const double *a, int pitch, int p, double *out,
+                      void *stream);
 /* arrayutils.f90:637 inner_prod(bd1, bd2): sum of dx dy a b over the nx x ny cells. */
 int mgp_inner_prod(int nx, int ny, double dx, double dy, const double *a, const double *b, double *out);
 /* arrayutils.f90:670 integrate2d(bd, bx, geo, opt_jscale): sum of dx dy J a (J given, jscale true)

@@ -89,7 +89,7 @@ EXPORTS = (
     "mgp_batch_solve", "mgp_batch_last_cycles", "mgp_server_create", "mgp_server_destroy", "mgp_server_footprint",
     "mgp_server_submit", "mgp_server_step", "mgp_server_result", "mgp_integrate2d_bdry", "mgp_gpnorm",
     "mgp_batch_time_kernel", "mgp_compute_residual_dd", "mgp_lap4", "mgp_metric4_fill", "mgp_solver_get_phi_lo",
-    "mgp_solver_project_mean",
+    "mgp_solver_project_mean", "mgp_pnorm_noalloc",
 )
 
 
