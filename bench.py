@@ -704,6 +704,12 @@ def main():
         print(f"# batched FP32 x-line set: {ms:.4f} ms, {nbytes / 1e6:.1f} MB algorithmic, "
               f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
         b32.close()
+        # the counters' calibration: a plain copy of 32 MiB (2048^2 doubles) read and written
+        geo, bc, _ = config2_problem(NX, NX)
+        s = capi.Solver(geo, bc, method=capi.METHOD_KRYLOV_MG)
+        cms, cbytes = s.time_kernel(2, reps=5, cold=True)
+        print(f"# copy (calibration): {cms:.4f} ms, {cbytes / 1e6:.1f} MB", flush=True)
+        s.close()
         return
     dt, iters32, _ = timed_batch(capi, b32, args.tol, args.steps, args.warmup, "batched mixed")
     value = len(probs) * args.steps / dt

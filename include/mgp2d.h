@@ -392,6 +392,15 @@ int mgp_server_create(mgp_server **v, int n, const mgp_mapping *maps, const int 
 int mgp_server_destroy(mgp_server *v);
 /* Device bytes the resident slots hold. */
 double mgp_server_footprint(const mgp_server *v);
+/* Admits a geometry while serving (as the slots of mgp_server_create, its metrics generated on the
+ * GPU): its hierarchy is built, in a batch of its own, if the resident footprint stays within the
+ * server's HBM budget; otherwise MGP_ERR_NOMEM and nothing changes.  *slot receives its index. */
+int mgp_server_add_slot(mgp_server *v, const mgp_mapping *map, int nx, int ny, double dx, double dy,
+                        const mgp_bc *bc, int *slot);
+/* Frees an idle slot's hierarchy and warm start (no request of it queued or running, nor of a slot
+ * sharing its batch: else MGP_ERR_ARG, retry later); the slot's other batch members keep their warm
+ * starts.  Submits to an evicted slot fail with MGP_ERR_ARG. */
+int mgp_server_evict(mgp_server *v, int slot);
 /* Queues a solve of slot `slot` with this rhs (nx x ny, copied): ||rhs - L phi|| <= tol ||rhs||
  * within maxiters iterations, from phi = 0 or (warm != 0) from the slot's last solution. */
 int mgp_server_submit(mgp_server *v, int slot, const double *rhs, double tol, int maxiters, int warm,

@@ -89,7 +89,7 @@ EXPORTS = (
     "mgp_batch_solve", "mgp_batch_last_cycles", "mgp_server_create", "mgp_server_destroy", "mgp_server_footprint",
     "mgp_server_submit", "mgp_server_step", "mgp_server_result", "mgp_integrate2d_bdry", "mgp_gpnorm",
     "mgp_batch_time_kernel", "mgp_compute_residual_dd", "mgp_lap4", "mgp_metric4_fill", "mgp_solver_get_phi_lo",
-    "mgp_solver_project_mean", "mgp_pnorm_noalloc",
+    "mgp_solver_project_mean", "mgp_pnorm_noalloc", "mgp_server_add_slot", "mgp_server_evict",
 )
 
 
@@ -669,6 +669,20 @@ class Server:
 
     def footprint(self):
         return lib.mgp_server_footprint(self._h)
+
+    def add_slot(self, geo, bc, mapping):
+        """Admits a geometry while serving; raises MgpError (out of device memory) if its
+        hierarchy does not fit the budget.  Returns the new slot's index."""
+        a = _Args()
+        slot = _i(0)
+        _check(lib.mgp_server_add_slot(self._h, ctypes.byref(mapping), _i(geo.nx), _i(geo.ny), _d(geo.dx),
+                                       _d(geo.dy), a.bc(bc, geo.nx, geo.ny), ctypes.byref(slot)))
+        self.shapes.append((geo.ny, geo.nx))
+        return slot.value
+
+    def evict(self, slot):
+        """Frees an idle slot's hierarchy and warm start."""
+        _check(lib.mgp_server_evict(self._h, _i(slot)))
 
     def submit(self, slot, rhs, tol=1e-10, maxiters=400, warm=False):
         rhs = _f64(rhs)

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 #include "mgp_common.h"
@@ -117,6 +118,8 @@ class BatchKMG {
     int outcome(int k) const { return oc_[k]; }
     // device bytes held by the batch (hierarchies, vectors, scalars, tables excluded)
     size_t footprint() const { return footprint_; }
+    // the bytes one problem adds to a batch (its share of footprint())
+    static size_t problem_bytes(const Problem &q, const mgp_solver_opts &o);
     // Measurement hook: mean ms of `reps` back-to-back launch sets of the
     // finest-level line relaxation of the even interior rows of every problem
     // (one launch per chunk class, side by side, as in a solve), timed with
@@ -205,6 +208,15 @@ class Server {
     ~Server();
     int slots() const { return (int)where.size(); }
     size_t footprint() const;
+    // Admits a geometry at run time: its hierarchy is built (a batch of its
+    // own) if the resident footprint stays within the budget, else
+    // MGP_ERR_NOMEM and nothing changes.  Returns the new slot's index.
+    int add_slot(const BatchKMG::Problem &q, const mgp_mapping *map);
+    // Drops an idle slot (no request queued or running): its hierarchy and
+    // warm start are freed (a batch shared with other slots is rebuilt
+    // without it, keeping their warm starts).  Later submits to it fail.
+    void evict(int slot);
+    bool resident(int slot) const { return slot >= 0 && slot < slots() && where[slot].first >= 0; }
     long long submit(int slot, const double *rhs, double tol, int maxiters, int warm);
     // admits and advances; returns the solves on the device after the step
     int step();
@@ -235,6 +247,11 @@ class Server {
     };
     void admit(Class &c, int cls);
     void retire(Class &c, const std::vector<int> &finished);
+    void free_class(Class &c);
+    mgp_solver_opts opts_;
+    double budget_ = 0.0;
+    std::vector<BatchKMG::Problem> desc_;    // per slot: the geometry (map points into maps_)
+    std::vector<std::unique_ptr<mgp_mapping>> maps_;
     std::vector<Class> cls;
     std::vector<std::pair<int, int>> where;  // per slot: (class, local index)
     std::vector<Req> queue;

@@ -192,6 +192,37 @@ BatchKMG::Table BatchKMG::upload(TableSet &ts, const void *host, size_t bytes, i
     return t;
 }
 
+// Device bytes one problem adds to a batch (its part of the arena, in the
+// constructor's layout, plus its scalars and descriptors).
+size_t BatchKMG::problem_bytes(const Problem &q, const mgp_solver_opts &o) {
+    const bool mx = o.method == MGP_METHOD_KRYLOV_MG32;
+    const int maxlev = o.maxdepth < 0 ? 64 : o.maxdepth + 1;
+    size_t b = 0;
+    const size_t n0 = level_elems(q.nx, q.ny);
+    b += BV_COUNT * round256(n0 * sizeof(double));
+    const int nb = std::max(1, std::min(q.ny, (q.nx * q.ny + 8191) / 8192));
+    b += round256((size_t)nb * sizeof(double));
+    int ny = q.ny, lev = 0;
+    size_t levels = 0;
+    while (true) {
+        const size_t nl = level_elems(q.nx, ny);
+        b += 5 * round256(nl * sizeof(double));
+        if (lev) b += 2 * round256(nl * sizeof(double));
+        const Grid gl = make_grid(q.nx, ny, q.dx, q.dy);
+        b += round256((size_t)ring_table_elems(gl) * sizeof(double));
+        if (mx) {
+            b += 4 * round256(nl * sizeof(float));
+            b += round256((size_t)ring_table_elems(gl) * sizeof(float));
+            b += 2 * round256(nl * sizeof(float));
+        }
+        ++lev;
+        ++levels;
+        if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || lev >= maxlev) break;
+        ny /= 2;
+    }
+    return b + SC_COUNT * sizeof(double) + sizeof(BProb) + 2 * sizeof(int) + levels * sizeof(BLev);
+}
+
 BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_solver_opts &o) : s(s_) {
     const int n = (int)probs.size();
     if (n <= 0) {
@@ -837,7 +868,15 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
 // Size classes of the serving loop: slots of up to 2^22 cells, and larger ones.
 static int size_class(const BatchKMG::Problem &q) { return (long long)q.nx * q.ny > (1LL << 22) ? 1 : 0; }
 
-Server::Server(const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget) {
+Server::Server(const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opts &o, double budget)
+    : opts_(o), budget_(budget) {
+    for (const BatchKMG::Problem &q : slots) {
+        desc_.push_back(q);
+        if (q.map) {
+            maps_.push_back(std::make_unique<mgp_mapping>(*q.map));
+            desc_.back().map = maps_.back().get();
+        }
+    }
     where.resize(slots.size());
     std::vector<std::vector<BatchKMG::Problem>> per(2);
     std::vector<std::vector<int>> ids(2);
@@ -875,23 +914,110 @@ Server::Server(const std::vector<BatchKMG::Problem> &slots, const mgp_solver_opt
     owner.assign(slots.size(), 0);
 }
 
+void Server::free_class(Class &C) {
+    if (C.s) (void)hipStreamSynchronize(C.s);
+    delete C.B;
+    C.B = nullptr;
+    if (C.s) (void)hipStreamDestroy(C.s);
+    C.s = nullptr;
+    C.slot.clear();
+    C.in_flight = false;
+}
+
 Server::~Server() {
-    for (Class &C : cls) {
-        if (C.s) (void)hipStreamSynchronize(C.s);
-        delete C.B;
-        if (C.s) (void)hipStreamDestroy(C.s);
-    }
+    for (Class &C : cls) free_class(C);
 }
 
 size_t Server::footprint() const {
     size_t b = 0;
-    for (const Class &C : cls) b += C.B->footprint();
+    for (const Class &C : cls)
+        if (C.B) b += C.B->footprint();
     return b;
+}
+
+int Server::add_slot(const BatchKMG::Problem &q, const mgp_mapping *map) {
+    if (!map) {
+        g_last_error = "server add_slot: the geometry needs a mapping";
+        throw Failure(MGP_ERR_ARG);
+    }
+    BatchKMG::Problem p = q;
+    auto m = std::make_unique<mgp_mapping>(*map);
+    p.map = m.get();
+    const size_t need = BatchKMG::problem_bytes(p, opts_);
+    if (budget_ > 0.0 && (double)(footprint() + need) > budget_) {
+        g_last_error = "server add_slot: the geometry's hierarchy needs " + std::to_string(need) + " bytes; with the " +
+                       std::to_string(footprint()) + " resident that exceeds the budget of " +
+                       std::to_string((long long)budget_);
+        throw Failure(MGP_ERR_NOMEM);
+    }
+    Class C;
+    hip_check(hipStreamCreateWithFlags(&C.s, hipStreamNonBlocking), "hipStreamCreate");
+    try {
+        C.B = new BatchKMG(C.s, std::vector<BatchKMG::Problem>{p}, opts_);
+    } catch (...) {
+        (void)hipStreamDestroy(C.s);
+        throw;
+    }
+    const int slot = slots();
+    C.slot = {slot};
+    cls.push_back(C);
+    maps_.push_back(std::move(m));
+    desc_.push_back(p);
+    where.push_back({(int)cls.size() - 1, 0});
+    owner.push_back(0);
+    return slot;
+}
+
+void Server::evict(int slot) {
+    if (!resident(slot)) {
+        g_last_error = "server evict: the slot is not resident";
+        throw Failure(MGP_ERR_ARG);
+    }
+    bool queued_for = false;
+    for (const Req &r : queue) queued_for = queued_for || r.slot == slot;
+    Class &c = cls[where[slot].first];
+    if (owner[slot] != 0 || queued_for || c.in_flight || c.B->running() > 0) {
+        g_last_error = "server evict: the slot (or its batch) has requests queued or running";
+        throw Failure(MGP_ERR_ARG);
+    }
+    if (c.slot.size() == 1) {
+        free_class(c);
+        where[slot] = {-1, -1};
+        return;
+    }
+    // rebuild the batch over the other slots, keeping their warm starts
+    std::vector<int> keep;
+    std::vector<BatchKMG::Problem> probs;
+    std::vector<std::vector<double>> phi;
+    for (size_t q = 0; q < c.slot.size(); ++q) {
+        if (c.slot[q] == slot) continue;
+        keep.push_back(c.slot[q]);
+        probs.push_back(desc_[c.slot[q]]);
+        const Grid &g = c.B->grid((int)q);
+        phi.emplace_back((size_t)g.nx * g.ny);
+        download_block(c.s, phi.back().data(), c.B->vec((int)q, BV_PHI), g, 0, 0, g.nx, g.ny);
+    }
+    hip_check(hipStreamSynchronize(c.s), "server evict");
+    BatchKMG *nb = new BatchKMG(c.s, probs, opts_);
+    for (size_t q = 0; q < keep.size(); ++q) {
+        const Grid &g = nb->grid((int)q);
+        upload_block(c.s, nb->vec((int)q, BV_PHI), g, phi[q].data(), 0, 0, g.nx, g.ny);
+    }
+    hip_check(hipStreamSynchronize(c.s), "server evict");
+    delete c.B;
+    c.B = nb;
+    c.slot = keep;
+    for (size_t q = 0; q < keep.size(); ++q) where[keep[q]].second = (int)q;
+    where[slot] = {-1, -1};
 }
 
 long long Server::submit(int slot, const double *rhs, double tol, int maxiters, int warm) {
     if (slot < 0 || slot >= slots() || !rhs || !(tol >= 0.0) || maxiters < 0) {
         g_last_error = "server submit: bad slot, rhs, tol or maxiters";
+        throw Failure(MGP_ERR_ARG);
+    }
+    if (!resident(slot)) {
+        g_last_error = "server submit: the slot was evicted";
         throw Failure(MGP_ERR_ARG);
     }
     const Grid &g = cls[where[slot].first].B->grid(where[slot].second);
@@ -964,6 +1090,7 @@ int Server::step() {
         bool any_in_flight = false;
         for (size_t ci = 0; ci < cls.size(); ++ci) {
             Class &c = cls[ci];
+            if (!c.B) continue;  // every slot of it evicted
             if (c.in_flight) {
                 if (!c.B->step_done()) {
                     any_in_flight = true;
@@ -984,7 +1111,8 @@ int Server::step() {
         std::this_thread::yield();
     }
     int running = 0;
-    for (const Class &c : cls) running += c.B->running();
+    for (const Class &c : cls)
+        if (c.B) running += c.B->running();
     return running;
 }
 

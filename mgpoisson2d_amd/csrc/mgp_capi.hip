@@ -1315,6 +1315,37 @@ int mgp_server_destroy(mgp_server *v) {
 
 double mgp_server_footprint(const mgp_server *v) { return v && v->srv ? (double)v->srv->footprint() : 0.0; }
 
+int mgp_server_add_slot(mgp_server *v, const mgp_mapping *map, int nx, int ny, double dx, double dy, const mgp_bc *bc,
+                        int *slot) {
+    return guarded([&] {
+        if (!v || !v->srv || !slot) {
+            g_last_error = "server add_slot: null server or slot";
+            throw Failure(MGP_ERR_ARG);
+        }
+        check_mapping(map);
+        std::vector<BatchKMG::Problem> probs;
+        mgp_solver_opts o;
+        batch_problems(1, bc, nullptr, [&](int, BatchKMG::Problem &q) {
+            q.nx = nx;
+            q.ny = ny;
+            q.dx = dx;
+            q.dy = dy;
+            q.map = map;
+        }, probs, o);
+        *slot = v->srv->add_slot(probs[0], map);
+    });
+}
+
+int mgp_server_evict(mgp_server *v, int slot) {
+    return guarded([&] {
+        if (!v || !v->srv) {
+            g_last_error = "server evict: null server";
+            throw Failure(MGP_ERR_ARG);
+        }
+        v->srv->evict(slot);
+    });
+}
+
 int mgp_server_submit(mgp_server *v, int slot, const double *rhs, double tol, int maxiters, int warm,
                       long long *ticket) {
     return guarded([&] {

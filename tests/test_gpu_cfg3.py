@@ -107,14 +107,15 @@ def jmean_free(phi, J):
 
 @pytest.mark.parametrize("method", ["KRYLOV_MG", "KRYLOV_MG32"])
 def test_dd_refinement_below_the_fp64_floor(method):
-    # the configuration's family at 512 x 128: beta = 8, warp 0.12, all Neumann, y-lines
+    # the configuration's family at 512 x 128: beta = 8, warp 0.12, all Neumann; rows of 512
+    # cells take x-lines (AUTO; the y-line orientation is checked above)
     nx, ny, amp, beta = 512, 128, 0.12, 8.0
     geo = G.mapped_geometry(nx, ny, amp, beta)
     bc = G.BoundaryConditions.uniform((0, 0, 0, 0))
     x, y = G.cell_centres(geo, amp, beta)
     f = 5 * math.pi ** 2 * np.cos(math.pi * x) * np.cos(2 * math.pi * y)
     rhs = -geo.J * (f + 1e-2 * np.random.default_rng(3).standard_normal((ny, nx)))
-    s = mg.Solver(geo, bc, method=getattr(mg, "METHOD_" + method), lines=mg.LINES_Y, refine=mg.REFINE_DD)
+    s = mg.Solver(geo, bc, method=getattr(mg, "METHOD_" + method), refine=mg.REFINE_DD)
     s.set_rhs(rhs)
     m = s.project_mean()
     rhs_p = rhs - geo.J * m

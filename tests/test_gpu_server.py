@@ -9,7 +9,8 @@
   once, a perturbed one in fewer iterations.
 * Slots above 2^22 cells run in a batch of their own on a second stream;
   the results of both size classes stay those of each slot alone.
-* The HBM budget is enforced at creation.
+* The HBM budget is enforced at creation and when geometries are admitted
+  while serving; evicted geometries free their bytes.
 * FP32 V-cycles inside FP64 refinement (KRYLOV_MG32, the served precision):
   results are still those of each slot alone, and each meets 1e-10 by the
   oracle's operator (bit-identical to the reference's) on the GPU's own
@@ -168,3 +169,77 @@ def test_server_mixed_precision_independent_and_accurate():
         rhs = rhs_for(slot, k)
         r, _ = o.residual(geo, bcs[slot], 1, rhs, full)
         assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(rhs), (slot, k)
+
+
+def drain(v, pending):
+    out = {}
+    steps = 0
+    while pending:
+        v.step()
+        steps += 1
+        for t in list(pending):
+            r = v.result(t, pending[t])
+            if r is not None:
+                out[t] = r
+                del pending[t]
+        assert steps < 5000
+    return out
+
+
+def test_server_admits_and_evicts_geometries_at_run_time():
+    """Geometries join and leave while serving: a geometry whose hierarchy fits the HBM budget
+    is admitted and solves what it solves alone; one that does not fit is refused with
+    MGP_ERR_NOMEM and the server goes on; an evicted slot frees its bytes (its batch partners
+    keep their warm starts) and refuses further requests; the freed room admits the refused
+    geometry."""
+    geos, maps, bcs = slot_defs(SLOTS[:2])
+    probe = mg.Server(geos, bcs, maps)
+    base = probe.footprint()
+    probe.close()
+    extra_small = [(72, 56, 0.08, 3.0, (1, 1, 0, 0))]
+    big = [(512, 512, 0.1, 5.0, (1, 1, 0, 0))]
+    gs, ms, bs = slot_defs(extra_small)
+    gb, mb, bb = slot_defs(big)
+    # the budget holds the two slots, the small newcomer and a margin, not the big one
+    small_alone = mg.Server(gs, bs, ms)
+    small_bytes = small_alone.footprint()
+    small_alone.close()
+    budget = base + small_bytes + 0.5e6
+    v = mg.Server(geos, bcs, maps, hbm_budget=budget)
+    t0 = v.submit(0, rhs_for(0, 0), 1e-10, 300, False)
+    s_new = v.add_slot(gs[0], bs[0], ms[0])
+    assert s_new == 2
+    assert v.footprint() <= budget
+    t1 = v.submit(s_new, rhs_for(0, 0, extra_small), 1e-10, 300, False)
+    with pytest.raises(mg.MgpError, match="out of device memory"):
+        v.add_slot(gb[0], bb[0], mb[0])
+    # a copy of slot 1's geometry does not fit beside it either
+    with pytest.raises(mg.MgpError, match="out of device memory"):
+        v.add_slot(geos[1], bcs[1], maps[1])
+    res = drain(v, {t0: 0, t1: s_new})
+    assert all(r[2] == mg.CONVERGED for r in res.values())
+    # the newcomer's result is what it gives alone
+    alone = mg.Server(gs, bs, ms)
+    ta = alone.submit(0, rhs_for(0, 0, extra_small), 1e-10, 300, False)
+    ra = drain(alone, {ta: 0})[ta]
+    alone.close()
+    np.testing.assert_array_equal(res[t1][0], ra[0])
+    assert res[t1][1] == ra[1]
+    # eviction frees the slot's bytes and refuses its requests; the other slots keep working
+    before = v.footprint()
+    v.evict(1)
+    assert v.footprint() < before
+    with pytest.raises(mg.MgpError):
+        v.submit(1, rhs_for(1, 0), 1e-10, 300, False)
+    # slot 0 shared its batch with slot 1: its warm start survived the rebuild
+    t2 = v.submit(0, rhs_for(0, 0), 1e-10, 300, True)
+    r2 = drain(v, {t2: 0})[t2]
+    assert r2[2] == mg.CONVERGED and r2[1] == 0  # the same rhs again: already converged
+    # the room slot 1 freed admits its copy, which then solves
+    s_copy = v.add_slot(geos[1], bcs[1], maps[1])
+    t3 = v.submit(s_copy, rhs_for(1, 0), 1e-10, 300, False)
+    assert drain(v, {t3: s_copy})[t3][2] == mg.CONVERGED
+    for slot in (0, s_new, s_copy):
+        v.evict(slot)
+    assert v.footprint() == 0
+    v.close()
