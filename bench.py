@@ -274,6 +274,7 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
         "small_latency_p50_ms": 1e3 * float(np.percentile(lat, 50)) if lat else None,
         "small_latency_p99_ms": 1e3 * float(np.percentile(lat, 99)) if lat else None,
         "big_latency_p50_ms": 1e3 * float(np.percentile(big_lat, 50)) if big_lat else None,
+        "big_latency_p99_ms": 1e3 * float(np.percentile(big_lat, 99)) if big_lat else None,
         "hbm_footprint_GB": footprint / 1e9,
         "hbm_budget_GB": budget / 1e9,
         "plain_fp64_one_at_a_time": {
@@ -281,7 +282,9 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
             "latency_p50_ms": 1e3 * float(np.percentile(all_plain, 50)),
             "latency_p99_ms": 1e3 * float(np.percentile(all_plain, 99)),
             "small_latency_p50_ms": 1e3 * float(np.percentile(plat, 50)) if plat else None,
+            "small_latency_p99_ms": 1e3 * float(np.percentile(plat, 99)) if plat else None,
             "big_latency_p50_ms": 1e3 * float(np.percentile(pbig, 50)) if pbig else None,
+            "big_latency_p99_ms": 1e3 * float(np.percentile(pbig, 99)) if pbig else None,
         },
     }
 
@@ -499,7 +502,7 @@ def u_cells(nx, ny, amp, beta, averages):
     return out
 
 
-def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-10, seed=SEED + 3):
+def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-8, seed=SEED + 3):
     """Mixed precision + deferred correction on the strongly stretched all-Neumann
     grid: 8192 x 2048 cells, x = xi + 0.12 sin(2 pi xi) sin(2 pi eta),
     y = (exp(8 eta) - 1)/(exp(8) - 1), u = cos(pi x) cos(2 pi y), rhs with 1e-2
@@ -511,7 +514,9 @@ def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-10, seed=SEED + 3):
       with FP64 cycles is the FP64 reference solution;
     * fourth order: the fourth-order finite-volume operator outside, the same
       second-order cycles inside (deferred correction), cell-average rhs, to
-      `tol4` (FP64 residual), FP32 and FP64 cycles.
+      `tol4`: its residual is evaluated in FP64, whose rounding on this grid is
+      a few 1e-9 of the rhs (about 4x per halving of h from 1.2e-11 measured at
+      512 x 128), so 1e-8; FP32 and FP64 cycles.
     Reported per solve: time (one untimed solve first), V-cycles, final relative
     residual, max |phi - u| / max |u| (both with the J-weighted mean removed), and
     the distance of the FP32-cycle solution from the FP64-cycle one."""
@@ -531,25 +536,24 @@ def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-10, seed=SEED + 3):
         rhs = config3_rhs(nx, ny, AMP3, BETA3, seed, averages=order == 4)
         u = mean_free(u_cells(nx, ny, AMP3, BETA3, averages=order == 4))
         sols = {}
-        for method, label in ((capi.METHOD_KRYLOV_MG32, "mixed"), (capi.METHOD_KRYLOV_MG, "fp64")):
+        for method, label in ((capi.METHOD_KRYLOV_MG, "fp64"), (capi.METHOD_KRYLOV_MG32, "mixed")):
             refine = capi.REFINE_DD if order == 2 else capi.REFINE_FP64
             s = capi.Solver(geo, bc, method=method, mapping=mp, refine=refine, order=order)
             s.set_rhs(rhs)
             s.project_mean()
-            s.solve_fast(tol_o, 800)  # graph capture, first touch
+            s.solve_fast(tol_o, 1000)  # graph capture, first touch
             capi.device_synchronize()
             t0 = time.perf_counter()
-            hist, iters, oc = s.solve(tol_o, 800)
+            hist, iters, oc = s.solve(tol_o, 1000)
             capi.device_synchronize()
             dt = time.perf_counter() - t0
-            if oc != capi.CONVERGED:
-                raise SystemExit(f"cfg3 order {order} {label} did not converge ({iters} iterations, {hist[-3:]})")
             phi = s.get_phi()[1:-1, 1:-1]
             if order == 2:
                 phi = phi + s.get_phi_lo()[1:-1, 1:-1]
             phi = mean_free(phi)
             sols[label] = phi
             out[f"order{order}_{label}"] = {
+                "outcome": {capi.CONVERGED: "converged", capi.DIVERGED: "diverged"}.get(oc, "maxiters"),
                 "solve_s": dt,
                 "v_cycles": s.last_cycles(),
                 "final_relres": float(hist[-1]),
@@ -560,6 +564,8 @@ def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-10, seed=SEED + 3):
             print(f"# cfg3 order {order} {label}: {dt:.3f} s, {out[f'order{order}_{label}']['v_cycles']} V-cycles, "
                   f"relres {hist[-1]:.2e}", file=sys.stderr, flush=True)
         out[f"order{order}_max_rel_diff_mixed_vs_fp64"] = rel_diff(sols["mixed"], sols["fp64"])
+        if out[f"order{order}_fp64"]["outcome"] != "converged":
+            raise SystemExit(f"cfg3 order {order} with FP64 cycles did not converge")
     return out
 
 
@@ -665,8 +671,10 @@ def main():
                     help="time only the finest-level kernels (single solver: x-line, residual, copy; the "
                          "batch: its line relaxation) for hardware-counter passes")
     ap.add_argument("--batch-kernels-only", action="store_true",
-                    help="set up the mixed-precision batch, solve once and time only its finest-level line "
-                         "relaxation (for hardware-counter passes)")
+                    help="set up the mixed-precision batch and time only its finest-level line relaxation "
+                         "(and the calibration copy), for hardware-counter passes")
+    ap.add_argument("--with-solve", action="store_true",
+                    help="with --batch-kernels-only: one batched solve (captured graphs) before the timings")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_xline_2048.json"),
                     help="measured HBM bytes per single-solver x-line launch (FETCH_SIZE/WRITE_SIZE passes)")
     ap.add_argument("--batch-traffic", default=os.path.join(ROOT, "profiles", "traffic_batch_xline32.json"),
@@ -699,7 +707,9 @@ def main():
     setup_s = time.perf_counter() - t0
     print(f"# batch set-up: {len(probs)} problems, {cells} cells, {setup_s:.2f} s", file=sys.stderr, flush=True)
     if args.batch_kernels_only:
-        b32.solve(args.tol, 400)
+        if args.with_solve:
+            iters, _ = b32.solve(args.tol, 400)
+            print(f"# batched solve: {2 * int(np.max(iters))} V-cycles for the slowest problem", flush=True)
         ms, nbytes = b32.time_kernel(0, reps=5)
         print(f"# batched FP32 x-line set: {ms:.4f} ms, {nbytes / 1e6:.1f} MB algorithmic, "
               f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)

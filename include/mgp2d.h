@@ -102,7 +102,10 @@ int mgp_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *
  * reference): each sweep solves, row by row, T_j delta = rhs - L phi and adds delta, where L is
  * the reference operator with homogeneous BCs and T_j its tridiagonal part along row j; even
  * interior rows, then odd ones, then rows 0 and ny-1.  Dirichlet/Neumann sides, nx, ny >= 4;
- * phi's ghost cells are neither read nor written. */
+ * phi's ghost cells are neither read nor written.  Rows over 4111 cells are cut into 2, 4, ..
+ * equal segments (the fewest that fit) and each colour relaxes the even segments of its rows,
+ * then the odd ones, each segment with the tridiagonal part of its own block (block
+ * Gauss-Seidel along the row); MGP_ERR_ARG if nx has no such cut. */
 int mgp_xline_relax(const mgp_geo *geo, const mgp_bc *bc, const double *rhs, double *phi, int sweeps);
 /* MGPoisson2D_DD.f90:45 DeferredPoisson2D::compute_laplacian */
 int mgp_dc_compute_laplacian(const mgp_geo *geo, const mgp_bc *bc, int homog, double *phi, double *lap);

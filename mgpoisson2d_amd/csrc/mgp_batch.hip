@@ -239,7 +239,7 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             g_last_error = "batch: every problem needs nx, ny >= 4 and dx, dy > 0";
             throw Failure(MGP_ERR_ARG);
         }
-        if (!xline_supported(q.nx)) {
+        if (!xline_single_line(q.nx)) {
             g_last_error = "batch: row too long for one workgroup's line solve (nx <= 4111)";
             throw Failure(MGP_ERR_UNSUPPORTED);
         }

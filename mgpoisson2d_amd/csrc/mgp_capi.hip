@@ -246,7 +246,8 @@ int mgp_xline_relax(const mgp_geo *geo, const mgp_bc *bc, const double *rhs, dou
                 throw Failure(MGP_ERR_UNSUPPORTED);
             }
         if (g.nx < 4 || g.ny < 4 || !xline_supported(g.nx)) {
-            g_last_error = "xline_relax needs 4 <= nx <= 4111 and ny >= 4";
+            g_last_error = "xline_relax needs ny >= 4 and nx >= 4 with nx <= 4111 or nx divisible into <= 64 "
+                           "segments (a power of two) of at most 4111 cells";
             throw Failure(MGP_ERR_ARG);
         }
         const Bc hb = mg_bc(c.bc.bc, MODE_UNIFORM);

@@ -109,7 +109,18 @@ void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const dou
 // FP32 form (mixed-precision preconditioner): same layout and formulas.
 void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
                                 const Grid &gf, const Grid &gc, const GeoF &geo);
+// Rows the line relaxation supports: up to 4111 cells solved whole (8207 in
+// FP32), longer ones cut into 2, 4, .. equal segments (xline_segments, the
+// fewest that fit), each solved as a line, the even segments of a colour
+// before the odd ones.  xline_supported: some form exists in FP64.
 bool xline_supported(int nx);
+// Rows a 256-thread workgroup solves whole (the batched kernels' only form).
+bool xline_single_line(int nx);
+// Rows solved whole for elements of `elem` bytes.
+bool xline_whole(int nx, size_t elem);
+// Segments per row for elements of `elem` bytes: 1 for a whole row, 0 if nx
+// has no supported cut.
+int xline_segments(int nx, size_t elem);
 // Sweeps on the coarsest level of the semicoarsening hierarchy: at least
 // `sb`, and half the level's rows up to 64.  The hierarchy stops at an odd
 // row count, and an x-line zebra sweep reduces errors that are smooth in y

@@ -80,6 +80,7 @@ struct KrylovMG::Impl {
     Grid top_g;          // the caller's finest grid
     DevBuf r, rhat, p, v, s, t, phat, shat, scal, partial;
     DevBuf lo, x;        // dd: low part of the solution, the run's correction
+    DevBuf bk, bk_lo;    // the solution where the current run started (undo of a failed run)
     DevBuf m4[8];        // fourth order: face-averaged metrics (A4, Bx4, dA, dBx, C4, By4, dC, dBy)
     DevBuf sums;         // project_mean: the two compensated sums
     double *host = nullptr;  // pinned copy of the device scalars
@@ -126,10 +127,14 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         g_last_error = "KRYLOV_MG: lines must be AUTO, X or Y, refine FP64 or DD, order 2 or 4";
         throw Failure(MGP_ERR_ARG);
     }
-    const bool tr = o.lines == MGP_LINES_Y || (o.lines == MGP_LINES_AUTO && !xline_supported(top.g.nx));
-    if (!xline_supported(tr ? top.g.ny : top.g.nx)) {
-        g_last_error = "KRYLOV_MG: lines too long for one workgroup's line solve (rows of at most 4111 cells; "
-                       "with y-lines, columns)";
+    // AUTO: x-lines where the cycle's element type solves the rows whole,
+    // else y-lines where it solves the columns whole, else rows in segments
+    const size_t celem = o.method == MGP_METHOD_KRYLOV_MG32 ? sizeof(float) : sizeof(double);
+    const bool tr = o.lines == MGP_LINES_Y ||
+                    (o.lines == MGP_LINES_AUTO && !xline_whole(top.g.nx, celem) && xline_whole(top.g.ny, celem));
+    if (xline_segments(tr ? top.g.ny : top.g.nx, celem) == 0) {
+        g_last_error = "KRYLOV_MG: no line solve for this size (lines of at most 4111 cells, 8207 in the FP32 "
+                       "cycle, or cut into at most 64 equal segments of that length; with y-lines, columns)";
         throw Failure(MGP_ERR_UNSUPPORTED);
     }
     if (o.refine == MGP_REFINE_DD && !o.homog) {
@@ -230,9 +235,11 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
     const size_t n0 = level_elems(top.g.nx, top.g.ny);
     for (DevBuf *b : {&impl->r, &impl->rhat, &impl->p, &impl->v, &impl->s, &impl->t, &impl->phat, &impl->shat})
         b->alloc(n0);
+    impl->bk.alloc(n0);
     if (impl->dd) {
         impl->lo.alloc(n0);
         impl->x.alloc(n0);
+        impl->bk_lo.alloc(n0);
     }
     if (impl->order == 4) {
         for (DevBuf &b : impl->m4) b.alloc(n0);
@@ -444,11 +451,19 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
         // the 2048^2 stretched-grid problem (74 V-cycles to 1e-9, against 168
         // without restarts).  The FP64 cycle restarts after 1e-6 (62 V-cycles
         // against 64).
+        //
+        // A run whose recursive residual grows 1e3 above where it started is
+        // checked at once.  A run that ends without lowering the true residual
+        // is undone (the solution returns to where the run started) and the
+        // solve stops as diverged: the preconditioner is then too inaccurate
+        // for this operator, and a repeated run would repeat the failure.
         const double refine = K.mixed ? 1e-3 : 1e-6;
         double base = 1.0;  // relres at the last restart
         bool restart = true;
+        const size_t nbytes = K.r.n * sizeof(double);
         for (int it = 1; it <= maxiters; ++it) {
             if (restart) {
+                if (!K.dd) hip_check(hipMemcpyAsync(K.bk.p, phi, nbytes, hipMemcpyDeviceToDevice, s), "backup");
                 hip_check(hipMemcpyAsync(K.rhat.p, K.r.p, K.r.n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
                 launch_setval(s, K.p.p, 0.0, g, 0);
                 launch_setval(s, K.v.p, 0.0, g, 0);
@@ -462,14 +477,21 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
             hip_check(hipStreamSynchronize(s), "sync");
             const double rn = std::sqrt(K.host[SC_RN]) / r0;
             const double omega = K.host[SC_OMEGA];
-            if (rn <= tol || omega == 0.0 || !std::isfinite(rn) || rn <= refine * base) {
+            if (rn <= tol || omega == 0.0 || !std::isfinite(rn) || rn <= refine * base || rn > 1e3 * base) {
+                if (K.dd) {  // the fold below is undone if the run made no progress
+                    hip_check(hipMemcpyAsync(K.bk.p, phi, nbytes, hipMemcpyDeviceToDevice, s), "backup");
+                    hip_check(hipMemcpyAsync(K.bk_lo.p, K.lo.p, nbytes, hipMemcpyDeviceToDevice, s), "backup");
+                }
                 const double tr = true_residual() / r0;
                 hist.push_back(tr);
                 if (tr <= tol) {
                     outcome = MGP_CONVERGED;
                     break;
                 }
-                if (!std::isfinite(tr)) {
+                if (!std::isfinite(tr) || tr >= base) {
+                    hip_check(hipMemcpyAsync(phi, K.bk.p, nbytes, hipMemcpyDeviceToDevice, s), "undo");
+                    if (K.dd) hip_check(hipMemcpyAsync(K.lo.p, K.bk_lo.p, nbytes, hipMemcpyDeviceToDevice, s), "undo");
+                    hist.back() = base;  // the solution returned is the run's starting one
                     outcome = MGP_DIVERGED;
                     break;
                 }

@@ -46,6 +46,9 @@ constexpr int XL_SHORT = 64;
 constexpr int XL_SHORT_MAX_NX = 512;
 constexpr int XL_MID = 128;
 constexpr int XL_MID_MAX_NX = 1024;
+// FP32 rows of 4112 .. 8207 cells: 512 threads, chunks of 16 (139 KB of LDS),
+// so the mixed-precision cycle solves rows of 8192 cells whole.
+constexpr int XL_WIDE = 512;
 constexpr int RING_PLANES = 9;
 
 // Impulse whose ghosts follow the homogeneous fill of the MG levels
@@ -184,6 +187,7 @@ __device__ __forceinline__ int launch_slot() {
 template <typename T>
 struct RowCtx {
     int nx, ny, P, nt, rem, nring;
+    int i0, nxg;  // the cells solved: i0 .. i0+nx-1 of a row of nxg cells (a whole row: 0, nx)
     T idx, idy, idx2, idy2, ixy4;
 };
 
@@ -192,6 +196,8 @@ template <typename T>
 __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
     RowCtx<T> r;
     r.nx = g.nx;
+    r.i0 = 0;
+    r.nxg = g.nx;
     r.ny = g.ny;
     r.P = g.pitch;
     r.nt = nt;
@@ -212,20 +218,28 @@ __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
 // table.  Lanes past the row end evaluate the last cell and store into the
 // chunk layout's last slot, which no chunk uses (the last chunk has C cells):
 // the loop stays branch-free.
-template <int C, int NT, typename T, typename GM>
+template <int C, int NT, bool SEG = false, typename T, typename GM>
 __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const T *f_, const T *K_, const Grid &g,
                                          const GM &geo_, int j, T *Ld, T *La, T *Lb, T *Lc) {
     constexpr int U = C + 1;
     gconst<T> *e = gptr(e_), *f = gptr(f_), *K = gptr(K_);
     gconst<T> *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
-    const int tid = threadIdx.x, nx = X.nx, ny = X.ny, P = X.P;
+    // SEG: the cells solved may be a segment of the row (xline_rows); whole
+    // rows compile without the segment terms
+    const int tid = threadIdx.x, nx = X.nx, ny = X.ny, P = X.P, i0 = SEG ? X.i0 : 0, nxg = SEG ? X.nxg : X.nx;
     const T fourth = T(0.25);
     const int dump = X.nt * (C + 1) - 1;
+    // A segment of a longer row is solved as a line of its own: its couplings
+    // to the cells beyond its ends (in the neighbouring segments) stay in the
+    // residual and leave its tridiagonal system (a = 0 at its first cell,
+    // c = 0 at its last), as in block Gauss-Seidel over the segments.
+    const bool cut_w = SEG && i0 > 0, cut_e = SEG && i0 + nx < nxg;
     if (j != 0 && j != ny - 1) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int iraw = tid + NT * u;
-            const int i = iraw < nx ? iraw : nx - 1;
+            const int il = iraw < nx ? iraw : nx - 1;
+            const int i = i0 + il;
             const size_t c = ix(P, i, j);
             // interior stencil; for i = 0, nx-1 the reads stay inside the
             // padded level and the values are replaced below
@@ -238,17 +252,18 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
             const T xfw = Jxx_w * (p - pw) * X.idx + Jxy_w * fourth * (pn - ps + enw - esw) * X.idy;
             const T yfn = Jyy_n * (pn - p) * X.idy + Jyx_n * fourth * (ene - enw + pe - pw) * X.idx;
             const T yfs = Jyy_s * (p - ps) * X.idy + Jyx_s * fourth * (pe - pw + ese - esw) * X.idx;
-            const int o = iraw < nx ? chunk_pos<C>(i, X.rem) : dump;
+            const int o = iraw < nx ? chunk_pos<C>(il, X.rem) : dump;
             Ld[o] = f[c] - ((xfe - xfw) * X.idx + (yfn - yfs) * X.idy);
-            La[o] = Jxx_w * X.idx2 - (Jyx_n - Jyx_s) * X.ixy4;
+            La[o] = cut_w && il == 0 ? T(0) : Jxx_w * X.idx2 - (Jyx_n - Jyx_s) * X.ixy4;
             Lb[o] = -(Jxx_e + Jxx_w) * X.idx2 - (Jyy_n + Jyy_s) * X.idy2;
-            Lc[o] = Jxx_e * X.idx2 + (Jyx_n - Jyx_s) * X.ixy4;
+            Lc[o] = cut_e && il == nx - 1 ? T(0) : Jxx_e * X.idx2 + (Jyx_n - Jyx_s) * X.ixy4;
         }
-        // the two ring cells of the row (i = 0, nx-1): explicit stencil table
+        // the two ring cells of the row (i = 0, nxg-1), where they lie in
+        // the cells solved: explicit stencil table
         for (int side = 0; side < 2; ++side) {
-            const int iend = side ? nx - 1 : 0;
-            if (iend % NT != tid) continue;
-            const int o = chunk_pos<C>(iend, X.rem);
+            const int iend = side ? nxg - 1 : 0, il = iend - i0;
+            if (il < 0 || il >= nx || il % NT != tid) continue;
+            const int o = chunk_pos<C>(il, X.rem);
             Ld[o] = f[ix(P, iend, j)] - ring_lap(e, K, X.nring, g, iend, j);
             ring_row_coef(K, X.nring, g, iend, j, La[o], Lb[o], Lc[o]);
         }
@@ -257,27 +272,28 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int iraw = tid + NT * u;
-            const int i = iraw < nx ? iraw : nx - 1;
+            const int il = iraw < nx ? iraw : nx - 1;
+            const int i = i0 + il;
             const T fv = f[ix(P, i, j)];
             const T lap = ring_lap(e, K, X.nring, g, i, j);
             T a, b, c;
             ring_row_coef(K, X.nring, g, i, j, a, b, c);
-            const int o = iraw < nx ? chunk_pos<C>(i, X.rem) : dump;
+            const int o = iraw < nx ? chunk_pos<C>(il, X.rem) : dump;
             Ld[o] = fv - lap;
-            La[o] = a;
+            La[o] = cut_w && il == 0 ? T(0) : a;
             Lb[o] = b;
-            Lc[o] = c;
+            Lc[o] = cut_e && il == nx - 1 ? T(0) : c;
         }
     }
 }
 
 // Coalesced update e(:, j) += x, x in the chunk layout (e re-read).
-template <int C, int NT, typename T>
+template <int C, int NT, bool SEG = false, typename T>
 __device__ __forceinline__ void row_update_add(const RowCtx<T> &X, T *e, int j, const T *Lx) {
 #pragma unroll
     for (int u = 0; u < C + 1; ++u) {
         const int i = threadIdx.x + NT * u;
-        if (i < X.nx) e[ix(X.P, i, j)] += Lx[chunk_pos<C>(i, X.rem)];
+        if (i < X.nx) e[ix(X.P, (SEG ? X.i0 : 0) + i, j)] += Lx[chunk_pos<C>(i, X.rem)];
     }
 }
 
@@ -436,14 +452,14 @@ __device__ __forceinline__ void interface_solve(T *IA, T *IB, const T *IC, T *ID
 // from the metrics, elimination, PCR and back substitution per call.
 // LDS: 4 nt (C+1) doubles.
 // ---------------------------------------------------------------------------
-template <int C, int NT, typename T, typename GM>
+template <int C, int NT, bool SEG = false, typename T, typename GM>
 __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
                           const Grid &g, const GM &geo, int j, T *lds) {
     constexpr int CP = C + 1;
     const int tid = threadIdx.x, nt = X.nt;
     const int NT1 = nt * CP;
     T *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
-    row_pass<C, NT>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
+    row_pass<C, NT, SEG>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
     __syncthreads();
 
     // each thread takes its chunk into registers
@@ -543,7 +559,7 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
         lds[tid * CP + (longc ? C : C - 1)] = xe;
     }
     __syncthreads();
-    row_update_add<C, NT>(X, e, j, lds);
+    row_update_add<C, NT, SEG>(X, e, j, lds);
 }
 
 // Rows j = row0 + slot*rowstep; the workgroup of slot 0 then also relaxes
@@ -551,15 +567,31 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
 // row.  The boundary rows ride on the odd-row launch this way: row 0 after
 // row 1 and row ny-1 after the last odd row, the order of a separate third
 // phase, without its launch.
-template <int C, int NT, typename T, typename GM>
+//
+// Rows longer than one workgroup's line solve are cut into segments of
+// seglen cells, each solved as a line (row_pass).  A launch relaxes segments
+// seg0, seg0 + segstep, .. of its rows, nsegl of them per row: launch slot s
+// takes row slot s / nsegl, segment seg0 + segstep (s % nsegl).  With
+// segstep 2 the even segments of a colour run in one launch and the odd ones
+// in the next, so no workgroup reads a cell that another one of the same
+// launch writes: a segment's residual reaches one cell beyond each end, into
+// a segment of the other parity.  A whole row: seglen = nx, nsegl = 1.
+template <int C, int NT, bool SEG, typename T, typename GM>
 __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
                                            const Grid &g, const GM &geo, int row0, int rowstep, int nt, int extra0,
-                                           int extra1) {
+                                           int extra1, int seglen, int seg0, int segstep, int nsegl) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_bytes[];
     T *lds = reinterpret_cast<T *>(lds_bytes);
-    const RowCtx<T> X0 = row_ctx<T>(g, nt, C);
-    const int slot = launch_slot();
-    const int ex0 = slot == 0 ? extra0 : -1, ex1 = slot == (int)gridDim.x - 1 ? extra1 : -1;
+    RowCtx<T> X0 = row_ctx<T>(g, nt, C);
+    int slot = launch_slot(), nslots = (int)gridDim.x;
+    if (SEG) {
+        X0.nx = seglen;
+        X0.rem = seglen - nt * C;
+        X0.i0 = (seg0 + segstep * (slot % nsegl)) * seglen;
+        slot /= nsegl;
+        nslots /= nsegl;
+    }
+    const int ex0 = slot == 0 ? extra0 : -1, ex1 = slot == nslots - 1 ? extra1 : -1;
 #pragma unroll 1
     for (int r = 0; r < 3; ++r) {
         const int jr = r == 0 ? row0 + slot * rowstep : (r == 1 ? ex0 : ex1);
@@ -570,7 +602,7 @@ __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restric
         // ops) instead of being hoisted out of the loop and spilled
         RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
-        xline_row<C, NT>(X, e, f, K, g, geo, jr, lds);
+        xline_row<C, NT, SEG>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
@@ -585,27 +617,37 @@ constexpr int xl_waves() {
 // Rows up to 2055 cells: the fine levels are bandwidth-bound and need the
 // other workgroups of the CU to cover the elimination and interface phases
 // of each one.
-template <int C, typename T, typename GM>
+template <int C, bool SEG, typename T, typename GM>
 __global__ void __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(xl_waves<T>(), xl_waves<T>())))
 k_xline(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K, Grid g, GM geo, int row0, int rowstep,
-        int nt, int extra0, int extra1) {
-    xline_rows<C, XL_THREADS>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+        int nt, int extra0, int extra1, int seglen, int seg0, int segstep, int nsegl) {
+    xline_rows<C, XL_THREADS, SEG>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl);
 }
 
 // Rows of at most 512 cells: one 64-thread wave per row (as in the batch).
-template <int C, typename T, typename GM>
+template <int C, bool SEG, typename T, typename GM>
 __global__ void __launch_bounds__(XL_SHORT) __attribute__((amdgpu_waves_per_eu(2)))
 k_xline_short(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K, Grid g, GM geo, int row0,
-              int rowstep, int nt, int extra0, int extra1) {
-    xline_rows<C, XL_SHORT>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+              int rowstep, int nt, int extra0, int extra1, int seglen, int seg0, int segstep, int nsegl) {
+    xline_rows<C, XL_SHORT, SEG>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl);
+}
+
+// FP32 rows past 4111 cells (C = 16, 512 threads): one workgroup per CU.
+template <int C, typename T, typename GM>
+__global__ void __launch_bounds__(XL_WIDE) k_xline_wide(T *__restrict__ e, const T *__restrict__ f,
+                                                        const T *__restrict__ K, Grid g, GM geo, int row0,
+                                                        int rowstep, int nt, int extra0, int extra1, int seglen,
+                                                        int seg0, int segstep, int nsegl) {
+    xline_rows<C, XL_WIDE, false>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl);
 }
 
 // Longer rows (C = 16): one workgroup per CU in FP64 (139 KB of LDS), all registers.
-template <int C, typename T, typename GM>
+template <int C, bool SEG, typename T, typename GM>
 __global__ void __launch_bounds__(XL_THREADS) k_xline_long(T *__restrict__ e, const T *__restrict__ f,
                                                            const T *__restrict__ K, Grid g, GM geo, int row0,
-                                                           int rowstep, int nt, int extra0, int extra1) {
-    xline_rows<C, XL_THREADS>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1);
+                                                           int rowstep, int nt, int extra0, int extra1,
+                                                           int seglen, int seg0, int segstep, int nsegl) {
+    xline_rows<C, XL_THREADS, SEG>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl);
 }
 
 // (L e)(i, j) at an interior cell from the face metrics: the stencil of
@@ -788,7 +830,20 @@ size_t xline_lds_bytes_for(int nx, int C, size_t elem, int threads) {
     return (tr > iface ? tr : iface) * elem;
 }
 
-bool xline_supported(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
+bool xline_supported(int nx) { return xline_segments(nx, sizeof(double)) != 0; }
+
+bool xline_single_line(int nx) { return nx >= 4 && xline_chunk(nx) != 0; }
+
+bool xline_whole(int nx, size_t elem) { return xline_segments(nx, elem) == 1; }
+
+int xline_segments(int nx, size_t elem) {
+    if (nx < 4) return 0;
+    if (xline_chunk(nx) != 0) return 1;
+    if (elem == sizeof(float) && xline_chunk(nx, XL_WIDE) == 16) return 1;
+    for (int n = 2; n <= 64; n *= 2)
+        if (nx % n == 0 && nx / n >= 4 && xline_chunk(nx / n) != 0) return n;
+    return 0;
+}
 
 void xline_batch_shape(int nx, int &C, int &threads) {
     threads = nx <= XL_SHORT_MAX_NX ? XL_SHORT : (nx <= XL_MID_MAX_NX ? XL_MID : XL_THREADS);
@@ -812,42 +867,64 @@ void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float
     residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo);
 }
 
+// One launch of the line kernel for rows (SEG = false) or segments of seglen cells.
+template <bool SEG, typename T, typename G>
+static void xline_launch(hipStream_t s, T *e, const T *f, const T *K, const Grid &g, const G &geo, int row0,
+                         int rowstep, int extra0, int extra1, int seglen, int seg0, int segstep, int nsegl,
+                         unsigned grid) {
+    if (xline_chunk(seglen) == 0) {
+        // only whole FP32 rows get here (xline_segments): the 512-thread form
+        if constexpr (sizeof(T) == 4 && !SEG) {
+            const int nt = seglen / 16;
+            const size_t lds = xline_lds_bytes_for(seglen, 16, sizeof(T), XL_WIDE);
+            k_xline_wide<16><<<grid, XL_WIDE, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl);
+        }
+    } else if (seglen <= XL_SHORT_MAX_NX) {
+        const int C = xline_chunk(seglen, XL_SHORT), nt = seglen / C;
+        const size_t lds = xline_lds_bytes_for(seglen, C, sizeof(T), XL_SHORT);
+        switch (C) {
+            case 2: k_xline_short<2, SEG><<<grid, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+            case 4: k_xline_short<4, SEG><<<grid, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+            default: k_xline_short<8, SEG><<<grid, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+        }
+    } else {
+        const int C = xline_chunk(seglen);
+        const size_t lds = xline_lds_bytes(seglen, sizeof(T));
+        if (lds > 160 * 1024) {
+            g_last_error = "k_xline: row length not supported";
+            throw Failure(MGP_ERR_UNSUPPORTED);
+        }
+        const int nt = seglen / C;
+        switch (C) {
+            case 2: k_xline<2, SEG><<<grid, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+            case 4: k_xline<4, SEG><<<grid, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+            case 8: k_xline<8, SEG><<<grid, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+            default: k_xline_long<16, SEG><<<grid, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1, seglen, seg0, segstep, nsegl); break;
+        }
+    }
+}
+
 template <typename T, typename G>
 static void xline(hipStream_t s, T *e, const T *f, const T *K, const Grid &g, const G &geo, int row0, int rowstep,
                   int nrows, int extra0, int extra1) {
     if (nrows <= 0) return;
-    if (g.nx <= XL_SHORT_MAX_NX) {
-        const int C = xline_chunk(g.nx, XL_SHORT), nt = g.nx / C;
-        const size_t lds = xline_lds_bytes_for(g.nx, C, sizeof(T), XL_SHORT);
-        switch (C) {
-            case 2: k_xline_short<2><<<nrows, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-            case 4: k_xline_short<4><<<nrows, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-            default: k_xline_short<8><<<nrows, XL_SHORT, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-        }
+    const int nseg = xline_segments(g.nx, sizeof(T));
+    if (nseg == 0) {
+        g_last_error = "k_xline: row length not supported";
+        throw Failure(MGP_ERR_UNSUPPORTED);
+    }
+    const int seglen = g.nx / nseg;
+    // whole rows in one launch; segmented rows: the even segments, then the odd ones
+    const int nlaunch = nseg == 1 ? 1 : 2, segstep = nseg == 1 ? 1 : 2, nsegl = nseg == 1 ? 1 : nseg / 2;
+    const unsigned grid = (unsigned)(nrows * nsegl);
+    for (int seg0 = 0; seg0 < nlaunch; ++seg0) {
+        if (nseg > 1) xline_launch<true>(s, e, f, K, g, geo, row0, rowstep, extra0, extra1, seglen, seg0, segstep, nsegl, grid);
+        else xline_launch<false>(s, e, f, K, g, geo, row0, rowstep, extra0, extra1, seglen, seg0, segstep, nsegl, grid);
         const hipError_t err = hipGetLastError();
         if (err != hipSuccess) {
             g_last_error = std::string("k_xline launch: ") + hipGetErrorString(err);
             throw Failure(MGP_ERR_HIP);
         }
-        return;
-    }
-    const int C = xline_chunk(g.nx);
-    const size_t lds = xline_lds_bytes(g.nx, sizeof(T));
-    if (C == 0 || lds > 160 * 1024) {
-        g_last_error = "k_xline: row length not supported";
-        throw Failure(MGP_ERR_UNSUPPORTED);
-    }
-    const int nt = g.nx / C;
-    switch (C) {
-        case 2: k_xline<2><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-        case 4: k_xline<4><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-        case 8: k_xline<8><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-        default: k_xline_long<16><<<nrows, XL_THREADS, lds, s>>>(e, f, K, g, geo, row0, rowstep, nt, extra0, extra1); break;
-    }
-    const hipError_t err = hipGetLastError();
-    if (err != hipSuccess) {
-        g_last_error = std::string("k_xline launch: ") + hipGetErrorString(err);
-        throw Failure(MGP_ERR_HIP);
     }
 }
 

@@ -8,7 +8,8 @@ seconds:
   metric arrays, and the GPU-generated metric set equals the oracle's to
   rounding;
 * y-line relaxation (the transposed hierarchy) solves what x-line relaxation
-  solves;
+  solves, and rows of 8192 cells are relaxed whole in the FP32 cycle and in
+  segments in the FP64 one;
 * double-double refinement takes a strongly stretched all-Neumann problem to
   a relative residual below what its FP64 operator can resolve, and the
   solution agrees with a sparse direct solve;
@@ -90,6 +91,46 @@ def test_y_lines_solve_what_x_lines_solve(method):
         sols.append(phi[1:-1, 1:-1])
         s.close()
     assert np.max(np.abs(sols[0] - sols[1])) <= 1e-7 * np.max(np.abs(sols[0]))
+
+
+def test_rows_of_8192_cells():
+    """Rows past one 256-thread workgroup (the 8192-cell rows of the
+    fourth-order workload): the FP32 cycle solves them whole (512 threads,
+    AUTO keeps x-lines), the FP64 cycle with x-lines in two segments of 4096.
+    Both reach the reference operator's solution; on a grid whose x couplings
+    dominate (dx = 1/8192, dy = 1/16), where the x-line sweep is nearly a
+    direct solve when rows are solved whole, the whole-row cycle needs fewer
+    iterations than the segmented one."""
+    nx, ny, types = 8192, 16, (1, 1, 0, 0)
+    geo = G.mapped_geometry(nx, ny, 0.0, 0.0)
+    bc = G.BoundaryConditions.uniform(types)
+    rhs = np.random.default_rng(5).standard_normal((ny, nx)) * geo.J
+    sols, its = [], []
+    for method, lines in ((mg.METHOD_KRYLOV_MG32, mg.LINES_AUTO), (mg.METHOD_KRYLOV_MG, mg.LINES_X)):
+        s = mg.Solver(geo, bc, method=method, lines=lines)
+        s.set_rhs(rhs)
+        hist, iters, oc = s.solve(1e-10, 400)
+        assert oc == mg.CONVERGED, (method, iters, hist[-3:])
+        phi = s.get_phi()
+        r, _ = rc.oracle().residual(geo, bc, 1, rhs, phi)
+        assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(rhs)
+        sols.append(phi[1:-1, 1:-1])
+        its.append(s.last_cycles())
+        s.close()
+    print("V-cycles: whole FP32 rows", its[0], "FP64 segments", its[1])
+    assert its[0] < its[1]
+    assert np.max(np.abs(sols[0] - sols[1])) <= 1e-7 * np.max(np.abs(sols[1]))
+    # the mapped family with stretching: both forms converge
+    geo = G.mapped_geometry(nx, 32, 0.1, 3.0)
+    rhs = np.random.default_rng(6).standard_normal((32, nx)) * geo.J
+    for method in (mg.METHOD_KRYLOV_MG32, mg.METHOD_KRYLOV_MG):
+        s = mg.Solver(geo, bc, method=method, lines=mg.LINES_X)
+        s.set_rhs(rhs)
+        hist, iters, oc = s.solve(1e-10, 400)
+        assert oc == mg.CONVERGED, (method, iters, hist[-3:])
+        r, _ = rc.oracle().residual(geo, bc, 1, rhs, s.get_phi())
+        assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(rhs)
+        s.close()
 
 
 def neumann_direct(A, b):

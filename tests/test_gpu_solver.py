@@ -120,29 +120,43 @@ def test_graph_replay_matches_eager(method):
         eager.solve(1e-10, 5, zerophi=True)  # eager work between replays
 
 
-def zebra_reference(geo, bc, rhs, phi, sweeps):
+def zebra_reference(geo, bc, rhs, phi, sweeps, nseg=1):
     """x-line zebra relaxation with scipy on the probed operator: per row j,
     T_j delta = (f - A e)_j with T_j the tridiagonal part of A's (j, j) block,
     rows in the order even interior, odd interior, then 0 and ny-1 (rows of
-    one colour do not couple, so a sequential pass equals the parallel one)."""
+    one colour do not couple, so a sequential pass equals the parallel one).
+
+    nseg > 1: each row is cut into nseg equal segments, each solved with the
+    tridiagonal part of its own diagonal block (the couplings across a cut
+    stay in the residual); per colour the even segments of every row go
+    first, then the odd ones -- the two launches of the segmented kernel, in
+    each of which no segment reads a cell that another one writes."""
     from scipy.linalg import solve_banded
     A = probed_matrix(geo, bc).tocsr()
     nx, ny = geo.nx, geo.ny
+    L = nx // nseg
     e = phi[1:-1, 1:-1].copy().ravel()
     f = rhs.ravel()
-    order = list(range(2, ny - 1, 2)) + list(range(1, ny - 1, 2)) + [0, ny - 1]
+    colours = [list(range(2, ny - 1, 2)), list(range(1, ny - 1, 2)) + [0, ny - 1]]
     blocks = {}
-    for j in order:
-        blk = A[j * nx:(j + 1) * nx, j * nx:(j + 1) * nx].toarray()
-        ab = np.zeros((3, nx))
-        ab[0, 1:] = np.diag(blk, 1)
-        ab[1, :] = np.diag(blk)
-        ab[2, :-1] = np.diag(blk, -1)
-        blocks[j] = ab
+    for j in range(ny):
+        for s in range(nseg):
+            a0 = j * nx + s * L
+            blk = A[a0:a0 + L, a0:a0 + L]
+            ab = np.zeros((3, L))
+            ab[0, 1:] = blk.diagonal(1)
+            ab[1, :] = blk.diagonal(0)
+            ab[2, :-1] = blk.diagonal(-1)
+            blocks[j, s] = ab
+    parities = [0, 1] if nseg > 1 else [0]
     for _ in range(sweeps):
-        for j in order:
-            r = (f - A @ e)[j * nx:(j + 1) * nx]
-            e[j * nx:(j + 1) * nx] += solve_banded((1, 1), blocks[j], r)
+        for rows in colours:
+            for par in parities:
+                for j in rows:
+                    for s in range(par, nseg, 2 if nseg > 1 else 1):
+                        a0 = j * nx + s * L
+                        r = f[a0:a0 + L] - A[a0:a0 + L] @ e
+                        e[a0:a0 + L] += solve_banded((1, 1), blocks[j, s], r)
     return e.reshape(ny, nx)
 
 
@@ -165,6 +179,31 @@ def test_xline_relax_matches_dense(case):
     np.testing.assert_allclose(got[1:-1, 1:-1], want, rtol=0, atol=1e-10 * scale)
     # ghost cells are left as they were
     np.testing.assert_array_equal(got[0, 1:-1], phi[0, 1:-1])
+
+
+@pytest.mark.parametrize("case,nseg", [
+    ((4112, 5, 0.10, 3.0, (1, 1, 0, 0)), 2),   # two segments of 2056 cells
+    ((8192, 4, 0.10, 3.0, (0, 1, 1, 0)), 2),   # two of 4096 (the fourth-order workload's rows)
+    ((12288, 6, 0.10, 3.0, (1, 0, 0, 1)), 4),  # four of 3072: two per launch
+])
+def test_xline_relax_segmented_rows(case, nseg):
+    """Rows longer than 4111 cells are relaxed in segments (block
+    Gauss-Seidel over the segments of a row, even ones first): the result
+    matches the dense segmented reference.
+
+    Tolerance: a segment's tridiagonal system has condition number about
+    kappa = 2 (dy/dx)^2 here (x couplings 1/dx^2 against the 2/dy^2 the
+    y couplings add to the diagonal: 8e6 for 8192 x 4), and both solves
+    round at eps kappa relative; so 4 eps kappa of the scale (measured
+    3.6e-9 relative at 8192 x 4, where 4 eps kappa = 7.5e-9)."""
+    geo, bc, phi, rhs = setup(*case, seed=22)
+    got = mg.xline_relax(geo, bc, rhs, phi, 2)
+    want = zebra_reference(geo, bc, rhs, phi, 2, nseg=nseg)
+    scale = np.max(np.abs(want))
+    kappa = 2.0 * (geo.dy / geo.dx) ** 2
+    np.testing.assert_allclose(got[1:-1, 1:-1], want, rtol=0, atol=max(1e-10, 4 * np.finfo(float).eps * kappa) * scale)
+    # a plain whole-row relaxation differs: the cut is real
+    assert np.max(np.abs(zebra_reference(geo, bc, rhs, phi, 2, nseg=1) - want)) > 1e-6 * scale
 
 
 def projection_fluxes(geo):
