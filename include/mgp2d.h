@@ -261,16 +261,20 @@ enum {
 typedef struct {
     int method;
     int ordering;       /* smoother ordering for methods 0/1 */
-    int numcycles;      /* 1 = V, 2 = W */
+    int numcycles;      /* methods 0-3: 1 = V, 2 = W; KRYLOV_MG: V-cycles per preconditioner application
+                           (the cycle iterated from e = 0), 1 by default */
     int smooth_down, smooth_up, smooth_bottom;
     int maxdepth;       /* -1: as deep as the schedule allows */
     int homog;          /* BCs of the top-level residual */
     int use_graph;      /* KRYLOV_MG: replay each iteration as a captured hipGraph */
     /* KRYLOV_MG / KRYLOV_MG32 only (set by mgp_solver_default_opts): */
-    int lines;          /* MGP_LINES_AUTO: x-lines with y-semicoarsening while a row fits one
-                           workgroup's line solve (nx <= 4111), else y-lines; MGP_LINES_X; MGP_LINES_Y:
-                           the preconditioner relaxes columns and semicoarsens in x (it runs on the
-                           transposed problem: rows of ny <= 4111 cells) */
+    int lines;          /* MGP_LINES_AUTO: x-lines with y-semicoarsening while one workgroup solves a
+                           row whole in the cycle's precision (nx <= 4111 in FP64, 8207 in FP32), else
+                           y-lines where it solves a column whole, else x-lines in segments;
+                           MGP_LINES_X; MGP_LINES_Y: the preconditioner relaxes columns and semicoarsens
+                           in x (it runs on the transposed problem).  Lines longer than a whole-line
+                           solve are cut into 2, 4, .. equal segments (block Gauss-Seidel along the
+                           line) */
     int refine;         /* MGP_REFINE_FP64: residual and solution in FP64; MGP_REFINE_DD: the solution
                            is carried as phi + phi_lo (two FP64 fields) and the refinement residual
                            rhs - L(phi + phi_lo) is evaluated in double-double, so the relative residual
@@ -279,10 +283,14 @@ typedef struct {
                            (all-Neumann sides, mapped solvers only): the outer iteration's residual is
                            the fourth-order one, the preconditioner V-cycle stays second order (the
                            deferred-correction pairing of MGPoisson2D_DD.f90:589-862, one order up) */
+    int krylov;         /* outer iteration: MGP_KRYLOV_AUTO (BiCGStab for order 2, flexible GMRES for
+                           order 4), MGP_KRYLOV_BICGSTAB, MGP_KRYLOV_FGMRES */
+    int restart;        /* FGMRES: basis vectors per cycle (restart length); 0: 30 */
 } mgp_solver_opts;
 
 enum { MGP_LINES_AUTO = 0, MGP_LINES_X = 1, MGP_LINES_Y = 2 };
 enum { MGP_REFINE_FP64 = 0, MGP_REFINE_DD = 1 };
+enum { MGP_KRYLOV_AUTO = 0, MGP_KRYLOV_BICGSTAB = 1, MGP_KRYLOV_FGMRES = 2 };
 
 void mgp_solver_default_opts(mgp_solver_opts *opts);
 int mgp_solver_create(mgp_solver **out, const mgp_geo *geo, const mgp_bc *bc, const mgp_solver_opts *opts);
@@ -305,6 +313,17 @@ int mgp_solver_get_phi_lo(mgp_solver *s, double *phi_lo);
 int mgp_solver_project_mean(mgp_solver *s, double *removed);
 /* Device copies between resident fields of the same shape (no PCIe). */
 int mgp_solver_copy_rhs_from(mgp_solver *s, const mgp_solver *src);
+/* The resident solution of src copied into s: phi, and phi_lo when both carry a double-double
+ * solution (else s's phi_lo is cleared; a double-double phi is its value rounded to FP64).  The
+ * warm start of one discretisation from another: the fourth-order solve from the second-order
+ * solution (deferred correction).  No counterpart in the reference. */
+int mgp_solver_copy_phi_from(mgp_solver *s, const mgp_solver *src);
+/* KRYLOV_MG / KRYLOV_MG32: ||rhs - A phi|| and ||rhs|| on the resident fields (either pointer may
+ * be NULL), with the operator and precision of the solver's refinement residual (fourth order
+ * for order 4; double-double on phi + phi_lo for MGP_REFINE_DD); norms as pnorm p = 2.  Turns a
+ * target relative to ||rhs|| into solve's tolerance, which is relative to the initial residual.
+ * No counterpart in the reference. */
+int mgp_solver_residual(mgp_solver *s, double *rnorm, double *rhsnorm);
 /* Solve in place on the resident phi/rhs; relres may be NULL (else maxiters+1 doubles). */
 int mgp_solver_solve(mgp_solver *s, double tol, int maxiters, int zerophi, double *relres, int *iters,
                      int *outcome);

@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmgp2d.so")
-SOURCES = ["mgp_kernels.hip", "mgp_xline.hip", "mgp_flux.hip", "mgp_single.hip", "mgp_geometry.hip", "mgp_host.hip", "mgp_krylov.hip", "mgp_batch.hip", "mgp_rbtile.hip", "mgp_capi.hip", "mgp_noalloc.hip", "mgp_refine.hip", "mgp_fourth.hip"]
+SOURCES = ["mgp_kernels.hip", "mgp_xline.hip", "mgp_flux.hip", "mgp_single.hip", "mgp_geometry.hip", "mgp_host.hip", "mgp_krylov.hip", "mgp_batch.hip", "mgp_rbtile.hip", "mgp_capi.hip", "mgp_noalloc.hip", "mgp_refine.hip", "mgp_fourth.hip", "mgp_gmres.hip"]
 ARCH = os.environ.get("MGP_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

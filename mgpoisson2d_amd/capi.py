@@ -21,6 +21,7 @@ ORDER_LEX, ORDER_RB, ORDER_MC = 0, 1, 2
 METHOD_VCYCLE, METHOD_DC_VCYCLE, METHOD_KRYLOV_MG, METHOD_FMG, METHOD_KRYLOV_MG32 = 0, 1, 2, 3, 4
 LINES_AUTO, LINES_X, LINES_Y = 0, 1, 2
 REFINE_FP64, REFINE_DD = 0, 1
+KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_FGMRES = 0, 1, 2
 MAP_TERRAIN, MAP_WARPED = 0, 1
 
 _d = ctypes.c_double
@@ -51,12 +52,12 @@ class Mapping(ctypes.Structure):
 class SolverOpts(ctypes.Structure):
     _fields_ = [("method", _i), ("ordering", _i), ("numcycles", _i), ("smooth_down", _i),
                 ("smooth_up", _i), ("smooth_bottom", _i), ("maxdepth", _i), ("homog", _i), ("use_graph", _i),
-                ("lines", _i), ("refine", _i), ("order", _i)]
+                ("lines", _i), ("refine", _i), ("order", _i), ("krylov", _i), ("restart", _i)]
 
     def __init__(self, method=2, ordering=2, numcycles=1, smooth_down=2, smooth_up=2, smooth_bottom=2, maxdepth=-1,
-                 homog=1, use_graph=1, lines=0, refine=0, order=2):
+                 homog=1, use_graph=1, lines=0, refine=0, order=2, krylov=0, restart=0):
         super().__init__(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth, homog,
-                         use_graph, lines, refine, order)
+                         use_graph, lines, refine, order, krylov, restart)
 
 
 def _load():
@@ -90,6 +91,7 @@ EXPORTS = (
     "mgp_server_submit", "mgp_server_step", "mgp_server_result", "mgp_integrate2d_bdry", "mgp_gpnorm",
     "mgp_batch_time_kernel", "mgp_compute_residual_dd", "mgp_lap4", "mgp_metric4_fill", "mgp_solver_get_phi_lo",
     "mgp_solver_project_mean", "mgp_pnorm_noalloc", "mgp_server_add_slot", "mgp_server_evict",
+    "mgp_solver_copy_phi_from", "mgp_solver_residual",
 )
 
 
@@ -447,13 +449,14 @@ class Solver:
 
     def __init__(self, geo, bc, method=METHOD_KRYLOV_MG, ordering=ORDER_MC, numcycles=1, smooth_down=2,
                  smooth_up=2, smooth_bottom=2, maxdepth=-1, homog=True, use_graph=True, mapping=None,
-                 lines=LINES_AUTO, refine=REFINE_FP64, order=2):
+                 lines=LINES_AUTO, refine=REFINE_FP64, order=2, krylov=KRYLOV_AUTO, restart=0):
         """geo: a Geometry (metrics uploaded), or with mapping given, any object
         with nx, ny, dx, dy (metrics generated on the GPU).  lines, refine,
-        order: the KRYLOV_MG options of mgp_solver_opts."""
+        order, krylov, restart: the KRYLOV_MG options of mgp_solver_opts
+        (numcycles: V-cycles per preconditioner application there)."""
         self.geo = geo
         opts = SolverOpts(method, ordering, numcycles, smooth_down, smooth_up, smooth_bottom, maxdepth,
-                          int(homog), int(use_graph), lines, refine, order)
+                          int(homog), int(use_graph), lines, refine, order, krylov, restart)
         a = _Args()
         self._h = ctypes.c_void_p()
         if mapping is None:
@@ -497,6 +500,16 @@ class Solver:
         m = _d(0.0)
         _check(lib.mgp_solver_project_mean(self._h, ctypes.byref(m)))
         return m.value
+
+    def copy_phi_from(self, src):
+        """The resident solution of another solver of the same shape (device copy)."""
+        _check(lib.mgp_solver_copy_phi_from(self._h, src._h))
+
+    def residual(self):
+        """(||rhs - A phi||, ||rhs||) on the resident fields, with the solver's refinement residual."""
+        rn, rh = _d(0.0), _d(0.0)
+        _check(lib.mgp_solver_residual(self._h, ctypes.byref(rn), ctypes.byref(rh)))
+        return rn.value, rh.value
 
     def solve(self, tol=1e-10, maxiters=100, zerophi=True):
         relres = np.zeros(maxiters + 1)

@@ -851,6 +851,8 @@ void mgp_solver_default_opts(mgp_solver_opts *o) {
     o->lines = MGP_LINES_AUTO;
     o->refine = MGP_REFINE_FP64;
     o->order = 2;
+    o->krylov = MGP_KRYLOV_AUTO;
+    o->restart = 0;
 }
 
 extern "C++" {
@@ -995,6 +997,39 @@ int mgp_solver_get_phi_lo(mgp_solver *s, double *phi_lo) {
         }
         download_block(s->stream, phi_lo, lo, s->geo.g, -1, -1, s->geo.g.nx + 2, s->geo.g.ny + 2);
         hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_copy_phi_from(mgp_solver *s, const mgp_solver *src) {
+    return guarded([&] {
+        if (src->geo.g.nx != s->geo.g.nx || src->geo.g.ny != s->geo.g.ny) {
+            g_last_error = "copy_phi_from: shape mismatch";
+            throw Failure(MGP_ERR_ARG);
+        }
+        const size_t bytes = s->phi.n * sizeof(double);
+        hip_check(hipMemcpyAsync(s->phi.p, src->phi.p, bytes, hipMemcpyDeviceToDevice, s->stream), "copy");
+        if (double *lo = s->kmg.phi_lo()) {
+            if (const double *slo = src->kmg.phi_lo())
+                hip_check(hipMemcpyAsync(lo, slo, bytes, hipMemcpyDeviceToDevice, s->stream), "copy");
+            else
+                launch_setval(s->stream, lo, 0.0, s->geo.g, 1);
+        }
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int mgp_solver_residual(mgp_solver *s, double *rnorm, double *rhsnorm) {
+    return guarded([&] {
+        if (s->opts.method != MGP_METHOD_KRYLOV_MG && s->opts.method != MGP_METHOD_KRYLOV_MG32) {
+            g_last_error = "residual: KRYLOV_MG / KRYLOV_MG32 solvers only";
+            throw Failure(MGP_ERR_ARG);
+        }
+        double rh = 0.0;
+        const double rn = s->kmg.residual_norm(s->W, s->stream, s->geo, s->ubc.bc, s->opts.homog, s->phi.p, s->rhs.p,
+                                               &rh);
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+        if (rnorm) *rnorm = rn;
+        if (rhsnorm) *rhsnorm = rh;
     });
 }
 

@@ -109,6 +109,17 @@ void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const dou
 // FP32 form (mixed-precision preconditioner): same layout and formulas.
 void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
                                 const Grid &gf, const Grid &gc, const GeoF &geo);
+// Flexible GMRES vector kernels (mgp_gmres.hip); V holds vectors i < nv at
+// V + i * stride (padded level layout, valid cells only).
+// out[i] = V_i . w (plain sum over the valid cells); partial: mdot_blocks(g) * nv doubles.
+int mdot_blocks(const Grid &g);
+void launch_mdot(hipStream_t s, double *out, double *partial, const double *V, size_t stride, const double *w, int nv,
+                 const Grid &g);
+// y += sign * sum_i h[i] V_i (h on the device).
+void launch_maxpy(hipStream_t s, double *y, const double *V, size_t stride, const double *h, int nv, double sign,
+                  const Grid &g);
+// v = src / sqrt(*nrm2) (nrm2 on the device).
+void launch_scale_rsqrt(hipStream_t s, double *v, const double *src, const double *nrm2, const Grid &g);
 // Rows the line relaxation supports: up to 4111 cells solved whole (8207 in
 // FP32), longer ones cut into 2, 4, .. equal segments (xline_segments, the
 // fewest that fit), each solved as a line, the even segments of a colour

@@ -29,6 +29,17 @@ struct KrylovMG {
     double project_mean(hipStream_t s, const DevGeo &top, double *rhs);
     int solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
               double *rhs, double tol, int maxiters, int zerophi, double *relres, int *iters);
+    // ||rhs - A phi|| (the operator and precision of the refinement residual:
+    // fourth order, double-double with phi + lo) and, if rhsnorm, ||rhs||; the
+    // residual is left in the solver's residual field.
+    double residual_norm(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
+                         double *rhs, double *rhsnorm);
+    void ensure_fgmres(const Grid &g);
+    // The flexible GMRES outer iteration of solve (opts.krylov); true_residual()
+    // leaves rhs - A phi in the solver's residual field and returns its norm.
+    template <typename TR>
+    int solve_fgmres(hipStream_t s, const DevGeo &top, const Bc &user_bc, double *phi, double tol, int maxiters,
+                     int zerophi, double *relres, int *iters, TR &true_residual);
     // Times `reps` launches of one fine-level kernel with HIP events on stream s.
     // kernel 0: x-line relaxation of the even interior rows; 1: residual
     // (9-point operator); 2: plain copy of one field.  Returns the mean ms per launch; *bytes receives the

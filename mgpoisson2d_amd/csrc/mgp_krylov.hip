@@ -47,6 +47,7 @@
 // high-order operator's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -85,11 +86,21 @@ struct KrylovMG::Impl {
     DevBuf sums;         // project_mean: the two compensated sums
     double *host = nullptr;  // pinned copy of the device scalars
     int sd = 2, su = 2, sb = 2;
+    int ncyc = 1;        // V-cycles per preconditioner application
+    // flexible GMRES (the outer iteration of the fourth-order operator): basis
+    // V_0..V_m and preconditioned vectors Z_0..Z_{m-1} (level layout, stride
+    // r.n), Gram-Schmidt coefficients, dot partials, the solution coefficients
+    bool fgmres = false;
+    bool fallback = false;  // krylov AUTO with BiCGStab: FGMRES takes over after a breakdown
+    int m = 30;
+    DevBuf gv, gz, gh, gpart, gy;
+    double *ghost = nullptr;  // pinned: 3 (m + 2) coefficients, then m solution coefficients
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;
     ~Impl() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (host) (void)hipHostFree(host);
+        if (ghost) (void)hipHostFree(ghost);
     }
 };
 
@@ -111,6 +122,22 @@ static Bc transposed_bc(const Bc &b) {
     return t;
 }
 
+// The FGMRES basis (m + 1 vectors), preconditioned vectors (m) and
+// coefficient buffers, allocated once (at build for krylov FGMRES, at the
+// first breakdown for AUTO).
+void KrylovMG::ensure_fgmres(const Grid &g) {
+    Impl &K = *impl;
+    if (K.ghost) return;
+    const int m = K.m;
+    const size_t n0 = level_elems(g.nx, g.ny);
+    K.gv.alloc((size_t)(m + 1) * n0);
+    K.gz.alloc((size_t)m * n0);
+    K.gh.alloc((size_t)3 * (m + 2));
+    K.gpart.alloc((size_t)mdot_blocks(g) * (m + 1));
+    K.gy.alloc((size_t)m);
+    hip_check(hipHostMalloc((void **)&K.ghost, (size_t)(4 * m + 6) * sizeof(double)), "hipHostMalloc");
+}
+
 void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const mgp_solver_opts &o,
                      const mgp_mapping *map) {
     if (top.g.ny < 4 || top.g.nx < 4) {
@@ -123,8 +150,10 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
             throw Failure(MGP_ERR_UNSUPPORTED);
         }
     if (o.lines < MGP_LINES_AUTO || o.lines > MGP_LINES_Y || (o.refine != MGP_REFINE_FP64 && o.refine != MGP_REFINE_DD) ||
-        (o.order != 2 && o.order != 4)) {
-        g_last_error = "KRYLOV_MG: lines must be AUTO, X or Y, refine FP64 or DD, order 2 or 4";
+        (o.order != 2 && o.order != 4) || o.krylov < MGP_KRYLOV_AUTO || o.krylov > MGP_KRYLOV_FGMRES ||
+        o.restart < 0 || o.restart > 200) {
+        g_last_error = "KRYLOV_MG: lines must be AUTO, X or Y, refine FP64 or DD, order 2 or 4, krylov AUTO, "
+                       "BICGSTAB or FGMRES, restart 0..200";
         throw Failure(MGP_ERR_ARG);
     }
     // AUTO: x-lines where the cycle's element type solves the rows whole,
@@ -166,6 +195,10 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
     impl->order = o.order;
     impl->top_g = top.g;
     impl->sd = o.smooth_down > 0 ? o.smooth_down : 1;
+    impl->ncyc = o.numcycles > 1 ? o.numcycles : 1;
+    impl->fgmres = o.krylov == MGP_KRYLOV_FGMRES || (o.krylov == MGP_KRYLOV_AUTO && o.order == 4);
+    impl->m = o.restart > 0 ? o.restart : 30;
+    impl->fallback = o.krylov == MGP_KRYLOV_AUTO && !impl->fgmres;
     impl->su = o.smooth_up > 0 ? o.smooth_up : 1;
     impl->sb = o.smooth_bottom > 0 ? o.smooth_bottom : 1;
     std::vector<KLevel> &L = impl->L;
@@ -246,6 +279,7 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
         DevBuf *m = impl->m4;
         launch_metric4(s, *map, top.g, Geo4W{m[0].p, m[1].p, m[2].p, m[3].p, m[4].p, m[5].p, m[6].p, m[7].p});
     }
+    if (impl->fgmres) ensure_fgmres(top.g);
     impl->sums.alloc(2);
     impl->scal.alloc(SC_COUNT);
     impl->partial.alloc((size_t)reduce_blocks(top.g) + 8);
@@ -294,7 +328,9 @@ static const double *table(KLevel &l, double *) { return l.K.p; }
 static const GeoF &metrics(KLevel &l, float *) { return l.geo32; }
 static const Geo &metrics(KLevel &l, double *) { return l.geo; }
 
-// V-cycle from level d down, e = 0 on entry (set by the caller).
+// V-cycle from level d down.  Coarse levels start from e = 0 (set by the
+// restriction); the finest level's e may hold a previous cycle's result, so
+// consecutive calls iterate the cycle.
 template <typename T>
 static void vcycle_t(hipStream_t s, KrylovMG::Impl &K, size_t d, T *e, const T *f) {
     KLevel &l = K.L[d];
@@ -322,23 +358,23 @@ static void precondition(hipStream_t s, KrylovMG::Impl &K, double *e, const doub
         if (!K.mixed) {
             launch_transpose(s, l.r.p, l.g.pitch, f, tg.pitch, tg.nx, tg.ny);
             launch_setval(s, l.e.p, 0.0, l.g, 1);
-            vcycle_t(s, K, 0, l.e.p, (const double *)l.r.p);
+            for (int c = 0; c < K.ncyc; ++c) vcycle_t(s, K, 0, l.e.p, (const double *)l.r.p);
             launch_transpose(s, e, tg.pitch, l.e.p, l.g.pitch, l.g.nx, l.g.ny);
         } else {
             launch_transpose(s, l.r32.p, l.g.pitch, f, tg.pitch, tg.nx, tg.ny);
             launch_setval(s, l.e32.p, 0.0f, l.g, 1);
-            vcycle_t(s, K, 0, l.e32.p, (const float *)l.r32.p);
+            for (int c = 0; c < K.ncyc; ++c) vcycle_t(s, K, 0, l.e32.p, (const float *)l.r32.p);
             launch_transpose(s, e, tg.pitch, l.e32.p, l.g.pitch, l.g.nx, l.g.ny);
         }
         return;
     }
     if (!K.mixed) {
         launch_setval(s, e, 0.0, l.g, 1);
-        vcycle_t(s, K, 0, e, f);
+        for (int c = 0; c < K.ncyc; ++c) vcycle_t(s, K, 0, e, f);
         return;
     }
     launch_begin32(s, l.r32.p, l.e32.p, f, l.g);
-    vcycle_t(s, K, 0, l.e32.p, (const float *)l.r32.p);
+    for (int c = 0; c < K.ncyc; ++c) vcycle_t(s, K, 0, l.e32.p, (const float *)l.r32.p);
     launch_convert(s, e, l.e32.p, l.g, 0);
 }
 
@@ -369,6 +405,137 @@ double KrylovMG::project_mean(hipStream_t s, const DevGeo &top, double *rhs) {
     return h[0] / h[1];
 }
 
+double KrylovMG::residual_norm(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog,
+                               double *phi, double *rhs, double *rhsnorm) {
+    if (!built) {
+        g_last_error = "solver not built";
+        throw Failure(MGP_ERR_ARG);
+    }
+    Impl &K = *impl;
+    const Grid &g = top.g;
+    const Geo geo = top.view();
+    if (K.dd) {
+        launch_fold_dd(s, phi, K.lo.p, K.x.p, g);
+        launch_residual_dd(s, K.r.p, phi, K.lo.p, rhs, g, geo, user_bc);
+    } else if (K.order == 4) {
+        launch_lap4(s, K.r.p, phi, rhs, g, metrics4(K));
+    } else {
+        launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
+        launch_laplacian(s, K.r.p, phi, rhs, g, geo, user_bc, homog, 0);
+    }
+    if (rhsnorm) *rhsnorm = pnorm2(W, s, rhs, g);
+    return pnorm2(W, s, K.r.p, g);
+}
+
+// Flexible GMRES(m), right preconditioned (Z_k = M V_k kept, so the FP32
+// V-cycle need not be an exact linear map), classical Gram-Schmidt applied
+// twice (two block passes: two reductions per step instead of k + 1), Givens
+// rotations on the host.  Each cycle starts from the true residual
+// (true_residual: FP64, or double-double with the correction folded in) and
+// adds Z y to the solution; a cycle that does not lower the true residual is
+// undone and the solve stops.  GMRES minimises the residual over the Krylov
+// space, where BiCGStab's short recurrences break down on the strongly
+// non-normal pairing of the fourth-order operator with the second-order
+// preconditioner.
+template <typename TR>
+int KrylovMG::solve_fgmres(hipStream_t s, const DevGeo &top, const Bc &user_bc, double *phi, double tol,
+                           int maxiters, int zerophi, double *relres, int *iters, TR &true_residual) {
+    Impl &K = *impl;
+    const Grid &g = top.g;
+    const int m = K.m;
+    const size_t n = K.r.n;
+    double *V = K.gv.p, *Z = K.gz.p, *dh = K.gh.p, *hh = K.ghost;
+    double *h1 = dh, *h2 = dh + (m + 2), *nw = dh + 2 * (m + 2);
+    double *xc = K.dd ? K.x.p : phi;  // where a cycle's correction is added
+    const size_t nbytes = n * sizeof(double);
+    if (zerophi) launch_setval(s, phi, 0.0, g, 1);
+    if (zerophi && K.dd) launch_setval(s, K.lo.p, 0.0, g, 1);
+    const double r0 = true_residual();
+    std::vector<double> hist{r0 == 0.0 ? 0.0 : 1.0};
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gg(m + 1), y(m);
+    auto Hc = [&](int i, int k) -> double & { return H[(size_t)k * (m + 1) + i]; };
+    last_cycles = 0;
+    int outcome = r0 == 0.0 ? MGP_CONVERGED : MGP_MAXITERS;
+    double base = 1.0;  // relres where the current cycle started
+    int it = 0;
+    while (outcome == MGP_MAXITERS && it < maxiters) {
+        // V_0 = r / ||r||
+        launch_mdot(s, nw, K.gpart.p, K.r.p, n, K.r.p, 1, g);
+        launch_scale_rsqrt(s, V, K.r.p, nw, g);
+        hip_check(hipMemcpyAsync(hh, nw, sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+        hip_check(hipStreamSynchronize(s), "sync");
+        const double beta = std::sqrt(hh[0]);
+        if (!(beta > 0.0) || !std::isfinite(beta)) break;
+        std::fill(gg.begin(), gg.end(), 0.0);
+        gg[0] = beta;
+        int k = 0;
+        while (k < m && it < maxiters) {
+            double *vk = V + (size_t)k * n, *zk = Z + (size_t)k * n, *w = V + (size_t)(k + 1) * n;
+            precondition(s, K, zk, vk);
+            apply_h(s, K, top, user_bc, zk, w);
+            launch_mdot(s, h1, K.gpart.p, V, n, w, k + 1, g);
+            launch_maxpy(s, w, V, n, h1, k + 1, -1.0, g);
+            launch_mdot(s, h2, K.gpart.p, V, n, w, k + 1, g);
+            launch_maxpy(s, w, V, n, h2, k + 1, -1.0, g);
+            launch_mdot(s, nw, K.gpart.p, w, n, w, 1, g);
+            launch_scale_rsqrt(s, w, w, nw, g);
+            hip_check(hipMemcpyAsync(hh, dh, (size_t)3 * (m + 2) * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+            hip_check(hipStreamSynchronize(s), "sync");
+            for (int i = 0; i <= k; ++i) Hc(i, k) = hh[i] + hh[(m + 2) + i];
+            const double hn = std::sqrt(hh[2 * (m + 2)]);
+            Hc(k + 1, k) = hn;
+            for (int i = 0; i < k; ++i) {
+                const double a = Hc(i, k), b = Hc(i + 1, k);
+                Hc(i, k) = cs[i] * a + sn[i] * b;
+                Hc(i + 1, k) = -sn[i] * a + cs[i] * b;
+            }
+            const double rr = std::hypot(Hc(k, k), Hc(k + 1, k));
+            cs[k] = rr > 0.0 ? Hc(k, k) / rr : 1.0;
+            sn[k] = rr > 0.0 ? Hc(k + 1, k) / rr : 0.0;
+            Hc(k, k) = rr;
+            Hc(k + 1, k) = 0.0;
+            gg[k + 1] = -sn[k] * gg[k];
+            gg[k] = cs[k] * gg[k];
+            ++k;
+            ++it;
+            last_cycles += K.ncyc;
+            const double est = base * std::fabs(gg[k]) / beta;
+            hist.push_back(est);
+            if (!std::isfinite(est) || est <= tol || !(hn > 1e-300)) break;
+        }
+        // y = R^-1 g, then x += Z y
+        for (int i = k - 1; i >= 0; --i) {
+            double t = gg[i];
+            for (int l = i + 1; l < k; ++l) t -= Hc(i, l) * y[l];
+            y[i] = Hc(i, i) != 0.0 ? t / Hc(i, i) : 0.0;
+        }
+        double *hy = hh + 3 * (m + 2);
+        for (int i = 0; i < k; ++i) hy[i] = y[i];
+        hip_check(hipMemcpyAsync(K.gy.p, hy, (size_t)k * sizeof(double), hipMemcpyHostToDevice, s), "copy");
+        hip_check(hipMemcpyAsync(K.bk.p, phi, nbytes, hipMemcpyDeviceToDevice, s), "backup");
+        if (K.dd) hip_check(hipMemcpyAsync(K.bk_lo.p, K.lo.p, nbytes, hipMemcpyDeviceToDevice, s), "backup");
+        launch_maxpy(s, xc, Z, n, K.gy.p, k, 1.0, g);
+        const double tr = true_residual() / r0;
+        hist.back() = tr;
+        if (tr <= tol) {
+            outcome = MGP_CONVERGED;
+        } else if (!std::isfinite(tr) || tr >= base) {
+            // no progress over a whole cycle: return the cycle's starting solution
+            hip_check(hipMemcpyAsync(phi, K.bk.p, nbytes, hipMemcpyDeviceToDevice, s), "undo");
+            if (K.dd) hip_check(hipMemcpyAsync(K.lo.p, K.bk_lo.p, nbytes, hipMemcpyDeviceToDevice, s), "undo");
+            outcome = (!std::isfinite(tr) || tr > 10.0 * base) ? MGP_DIVERGED : MGP_MAXITERS;
+            hist.back() = base;
+            break;
+        }
+        base = tr;
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (relres)
+        for (size_t k = 0; k < hist.size() && (int)k <= maxiters; ++k) relres[k] = hist[k];
+    if (iters) *iters = (int)hist.size() - 1 < maxiters ? (int)hist.size() - 1 : maxiters;
+    return outcome;
+}
+
 int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &user_bc, int homog, double *phi,
                     double *rhs, double tol, int maxiters, int zerophi, double *relres, int *iters) {
     if (!built) {
@@ -384,18 +551,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     // the solution the BiCGStab runs update: phi itself, or (double-double)
     // the correction x of the current run, folded into phi + lo at each check
     double *xs = K.dd ? K.x.p : phi;
-    auto true_residual = [&]() {
-        if (K.dd) {
-            launch_fold_dd(s, phi, K.lo.p, K.x.p, g);
-            launch_residual_dd(s, K.r.p, phi, K.lo.p, rhs, g, geo, user_bc);
-        } else if (K.order == 4) {
-            launch_lap4(s, K.r.p, phi, rhs, g, metrics4(K));
-        } else {
-            launch_fill_ghosts(s, phi, g, geo, user_bc, homog, 0);
-            launch_laplacian(s, K.r.p, phi, rhs, g, geo, user_bc, homog, 0);
-        }
-        return pnorm2(W, s, K.r.p, g);
-    };
+    auto true_residual = [&]() { return residual_norm(W, s, top, user_bc, homog, phi, rhs, nullptr); };
     // One BiCGStab iteration with every scalar on the device (right
     // preconditioning, M = one V-cycle).  Ends with ||r||^2 * vol in sc[SC_RN].
     auto body = [&]() {
@@ -417,6 +573,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
         launch_bicg_update(s, K.r.p, K.s.p, K.t.p, sc, g, 4);
         launch_reduce(s, sc + SC_RN, partial, K.r.p, K.r.p, nullptr, vol, g, 0);
     };
+    if (K.fgmres) return solve_fgmres(s, top, user_bc, phi, tol, maxiters, zerophi, relres, iters, true_residual);
     if (K.use_graph && !K.graph) {
         hipGraph_t graph = nullptr;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
@@ -472,7 +629,7 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
             }
             if (K.graph) hip_check(hipGraphLaunch(K.graph, s), "graph launch");
             else body();
-            last_cycles += 2;
+            last_cycles += 2 * K.ncyc;
             hip_check(hipMemcpyAsync(K.host, sc, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s), "scalars");
             hip_check(hipStreamSynchronize(s), "sync");
             const double rn = std::sqrt(K.host[SC_RN]) / r0;
@@ -503,6 +660,20 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
         }
     }
     if (K.dd) launch_fold_dd(s, phi, K.lo.p, K.x.p, g);  // a run cut off by maxiters
+    const int used = (int)hist.size() - 1;
+    if (outcome == MGP_DIVERGED && K.fallback && used < maxiters) {
+        // BiCGStab broke down (the solution is the last good iterate): flexible
+        // GMRES carries on from it with the iterations left, to the same
+        // tolerance relative to the initial residual
+        const double at = hist.back();
+        ensure_fgmres(top.g);
+        std::vector<double> h2((size_t)(maxiters - used) + 1);
+        int it2 = 0;
+        const int cyc = last_cycles;
+        outcome = solve_fgmres(s, top, user_bc, phi, tol / at, maxiters - used, 0, h2.data(), &it2, true_residual);
+        last_cycles += cyc;
+        for (int k = 1; k <= it2; ++k) hist.push_back(at * h2[k]);
+    }
     hip_check(hipStreamSynchronize(s), "sync");
     if (relres)
         for (size_t k = 0; k < hist.size() && (int)k <= maxiters; ++k) relres[k] = hist[k];

@@ -133,6 +133,95 @@ def test_rows_of_8192_cells():
         s.close()
 
 
+@pytest.mark.parametrize("method", ["KRYLOV_MG", "KRYLOV_MG32"])
+def test_fgmres_solves_what_bicgstab_solves(method):
+    """The flexible GMRES outer iteration reaches the reference operator's
+    solution, as BiCGStab does, and its residual history never rises."""
+    nx, ny, amp, beta, types = 128, 96, 0.1, 4.0, (1, 1, 0, 0)
+    geo = G.mapped_geometry(nx, ny, amp, beta)
+    bc = G.BoundaryConditions.uniform(types)
+    rhs = np.random.default_rng(11).standard_normal((ny, nx)) * geo.J
+    sols = []
+    for krylov, restart in ((mg.KRYLOV_BICGSTAB, 0), (mg.KRYLOV_FGMRES, 0), (mg.KRYLOV_FGMRES, 7)):
+        s = mg.Solver(geo, bc, method=getattr(mg, "METHOD_" + method), krylov=krylov, restart=restart)
+        s.set_rhs(rhs)
+        hist, iters, oc = s.solve(1e-10, 400)
+        assert oc == mg.CONVERGED, (krylov, restart, iters, hist[-3:])
+        if krylov == mg.KRYLOV_FGMRES:
+            assert np.all(np.diff(hist) <= 1e-12), hist
+            assert s.last_cycles() == iters
+        phi = s.get_phi()
+        r, _ = rc.oracle().residual(geo, bc, 1, rhs, phi)
+        assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(rhs)
+        rn, rh = s.residual()
+        # the solver's own residual norm (pnorm 2) is the oracle's
+        want = math.sqrt(geo.dx * geo.dy * np.sum(r * r))
+        assert abs(rn - want) <= 1e-6 * want + 1e-300
+        assert abs(rh - math.sqrt(geo.dx * geo.dy * np.sum(rhs * rhs))) <= 1e-12 * rh
+        sols.append(phi[1:-1, 1:-1])
+        s.close()
+    for other in sols[1:]:
+        assert np.max(np.abs(other - sols[0])) <= 1e-7 * np.max(np.abs(sols[0]))
+
+
+def test_fourth_order_from_the_second_order_solution():
+    """Deferred correction: the second-order solution (double-double, FGMRES)
+    copied into a fourth-order solver (device copy) leaves a fourth-order
+    residual of the size of the truncation-error difference; the fourth-order
+    solve from there reaches a target relative to ||rhs|| set through
+    residual(), and the solution is the direct solution of the oracle's
+    fourth-order operator."""
+    nx, ny, amp, beta = 48, 40, 0.12, 4.0
+    geo, mapping = mapped(nx, ny, amp, beta)
+    bc = G.BoundaryConditions.uniform((0, 0, 0, 0))
+    rhs = mms4.rhs_mms(nx, ny, amp, beta)
+    s2 = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG32, mapping=mapping, refine=mg.REFINE_DD,
+                   krylov=mg.KRYLOV_FGMRES)
+    s2.set_rhs(rhs)
+    s2.project_mean()
+    hist, iters, oc = s2.solve(1e-12, 400)
+    assert oc == mg.CONVERGED, (iters, hist[-3:])
+    s4 = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG32, mapping=mapping, order=4)
+    s4.set_rhs(rhs)
+    s4.project_mean()
+    s4.copy_phi_from(s2)
+    np.testing.assert_array_equal(s4.get_phi()[1:-1, 1:-1], s2.get_phi()[1:-1, 1:-1])
+    rn, rh = s4.residual()
+    assert 1e-8 * rh < rn < 1e-1 * rh, (rn, rh)
+    hist, iters, oc = s4.solve(1e-11 * rh / rn, 400, zerophi=False)
+    assert oc == mg.CONVERGED, (iters, hist[-3:])
+    rn2, _ = s4.residual()
+    assert rn2 <= 1.01e-11 * rh
+    phi = s4.get_phi()[1:-1, 1:-1]
+    m = rc.oracle().metric4(nx, ny, 1.0 / nx, 1.0 / ny, amp, beta)
+    A = mms4.probed_lap4(nx, ny, m)
+    ref = mms4.solve_neumann(A, rhs - rhs.mean())
+    d = (phi - phi.mean()) - (ref - ref.mean())
+    assert np.max(np.abs(d)) <= 1e-8 * np.max(np.abs(ref))
+    s2.close()
+    s4.close()
+
+
+def test_second_order_solve_of_the_8192_workload_converges():
+    """The 8192 x 2048 all-Neumann workload (beta = 8, warp 0.12) with its
+    cell-average rhs, mixed precision, double-double refinement, krylov AUTO:
+    BiCGStab breaks down on this rhs and FGMRES carries on from its last good
+    iterate; the solve reaches 1e-12."""
+    import bench
+    nx, ny = 8192, 2048
+    geo, mapping = mapped(nx, ny, bench.AMP3, bench.BETA3)
+    bc = G.BoundaryConditions.uniform((0, 0, 0, 0))
+    rhs = bench.config3_rhs(nx, ny, bench.AMP3, bench.BETA3, bench.SEED + 3, averages=True)
+    s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG32, mapping=mapping, refine=mg.REFINE_DD)
+    s.set_rhs(rhs)
+    s.project_mean()
+    hist, iters, oc = s.solve(1e-12, 1000)
+    print("iterations", iters, "V-cycles", s.last_cycles(), "final", hist[-1])
+    assert oc == mg.CONVERGED, (iters, hist[-5:])
+    assert hist[-1] <= 1e-12
+    s.close()
+
+
 def neumann_direct(A, b):
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
