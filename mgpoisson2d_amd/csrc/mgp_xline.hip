@@ -218,7 +218,13 @@ __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
 // table.  Lanes past the row end evaluate the last cell and store into the
 // chunk layout's last slot, which no chunk uses (the last chunk has C cells):
 // the loop stays branch-free.
-template <int C, int NT, bool SEG = false, typename T, typename GM>
+//
+// RG cells' loads are issued together before their stencils are evaluated,
+// so RG memory round trips overlap instead of following one another (one
+// cell at a time, a coarse-level row pass took 5.4 us of a 10.6 us row
+// solve).  The batched kernels keep RG = 1: the registers of more would cut
+// the workgroups per CU they rely on.
+template <int C, int NT, bool SEG = false, int RG = 1, typename T, typename GM>
 __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const T *f_, const T *K_, const Grid &g,
                                          const GM &geo_, int j, T *Ld, T *La, T *Lb, T *Lc) {
     constexpr int U = C + 1;
@@ -235,28 +241,58 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
     // c = 0 at its last), as in block Gauss-Seidel over the segments.
     const bool cut_w = SEG && i0 > 0, cut_e = SEG && i0 + nx < nxg;
     if (j != 0 && j != ny - 1) {
+        struct CellIn {
+            T p, pe, pw, pn, ps, ene, enw, ese, esw, Jxx_e, Jxx_w, Jxy_e, Jxy_w, Jyy_n, Jyy_s, Jyx_n, Jyx_s, fv;
+        };
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int iraw = tid + NT * u;
-            const int il = iraw < nx ? iraw : nx - 1;
-            const int i = i0 + il;
-            const size_t c = ix(P, i, j);
-            // interior stencil; for i = 0, nx-1 the reads stay inside the
-            // padded level and the values are replaced below
-            gconst<T> *q = e + c;
-            const T p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
-            const T ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
-            const T Jxx_e = Jxx[c + 1], Jxx_w = Jxx[c], Jxy_e = Jxy[c + 1], Jxy_w = Jxy[c];
-            const T Jyy_n = Jyy[c + P], Jyy_s = Jyy[c], Jyx_n = Jyx[c + P], Jyx_s = Jyx[c];
-            const T xfe = Jxx_e * (pe - p) * X.idx + Jxy_e * fourth * (ene - ese + pn - ps) * X.idy;
-            const T xfw = Jxx_w * (p - pw) * X.idx + Jxy_w * fourth * (pn - ps + enw - esw) * X.idy;
-            const T yfn = Jyy_n * (pn - p) * X.idy + Jyx_n * fourth * (ene - enw + pe - pw) * X.idx;
-            const T yfs = Jyy_s * (p - ps) * X.idy + Jyx_s * fourth * (pe - pw + ese - esw) * X.idx;
-            const int o = iraw < nx ? chunk_pos<C>(il, X.rem) : dump;
-            Ld[o] = f[c] - ((xfe - xfw) * X.idx + (yfn - yfs) * X.idy);
-            La[o] = cut_w && il == 0 ? T(0) : Jxx_w * X.idx2 - (Jyx_n - Jyx_s) * X.ixy4;
-            Lb[o] = -(Jxx_e + Jxx_w) * X.idx2 - (Jyy_n + Jyy_s) * X.idy2;
-            Lc[o] = cut_e && il == nx - 1 ? T(0) : Jxx_e * X.idx2 + (Jyx_n - Jyx_s) * X.ixy4;
+        for (int u0 = 0; u0 < U; u0 += RG) {
+            CellIn in[RG];
+#pragma unroll
+            for (int r = 0; r < RG; ++r) {
+                const int u = u0 + r;
+                if (u >= U) break;
+                const int iraw = tid + NT * u;
+                const int il = iraw < nx ? iraw : nx - 1;
+                const size_t c = ix(P, i0 + il, j);
+                // interior stencil; for i = 0, nx-1 the reads stay inside the
+                // padded level and the values are replaced below
+                gconst<T> *q = e + c;
+                in[r].p = q[0];
+                in[r].pe = q[1];
+                in[r].pw = q[-1];
+                in[r].pn = q[P];
+                in[r].ps = q[-P];
+                in[r].ene = q[P + 1];
+                in[r].enw = q[P - 1];
+                in[r].ese = q[-P + 1];
+                in[r].esw = q[-P - 1];
+                in[r].Jxx_e = Jxx[c + 1];
+                in[r].Jxx_w = Jxx[c];
+                in[r].Jxy_e = Jxy[c + 1];
+                in[r].Jxy_w = Jxy[c];
+                in[r].Jyy_n = Jyy[c + P];
+                in[r].Jyy_s = Jyy[c];
+                in[r].Jyx_n = Jyx[c + P];
+                in[r].Jyx_s = Jyx[c];
+                in[r].fv = f[c];
+            }
+#pragma unroll
+            for (int r = 0; r < RG; ++r) {
+                const int u = u0 + r;
+                if (u >= U) break;
+                const int iraw = tid + NT * u;
+                const int il = iraw < nx ? iraw : nx - 1;
+                const CellIn &v = in[r];
+                const T xfe = v.Jxx_e * (v.pe - v.p) * X.idx + v.Jxy_e * fourth * (v.ene - v.ese + v.pn - v.ps) * X.idy;
+                const T xfw = v.Jxx_w * (v.p - v.pw) * X.idx + v.Jxy_w * fourth * (v.pn - v.ps + v.enw - v.esw) * X.idy;
+                const T yfn = v.Jyy_n * (v.pn - v.p) * X.idy + v.Jyx_n * fourth * (v.ene - v.enw + v.pe - v.pw) * X.idx;
+                const T yfs = v.Jyy_s * (v.p - v.ps) * X.idy + v.Jyx_s * fourth * (v.pe - v.pw + v.ese - v.esw) * X.idx;
+                const int o = iraw < nx ? chunk_pos<C>(il, X.rem) : dump;
+                Ld[o] = v.fv - ((xfe - xfw) * X.idx + (yfn - yfs) * X.idy);
+                La[o] = cut_w && il == 0 ? T(0) : v.Jxx_w * X.idx2 - (v.Jyx_n - v.Jyx_s) * X.ixy4;
+                Lb[o] = -(v.Jxx_e + v.Jxx_w) * X.idx2 - (v.Jyy_n + v.Jyy_s) * X.idy2;
+                Lc[o] = cut_e && il == nx - 1 ? T(0) : v.Jxx_e * X.idx2 + (v.Jyx_n - v.Jyx_s) * X.ixy4;
+            }
         }
         // the two ring cells of the row (i = 0, nxg-1), where they lie in
         // the cells solved: explicit stencil table
@@ -452,14 +488,14 @@ __device__ __forceinline__ void interface_solve(T *IA, T *IB, const T *IC, T *ID
 // from the metrics, elimination, PCR and back substitution per call.
 // LDS: 4 nt (C+1) doubles.
 // ---------------------------------------------------------------------------
-template <int C, int NT, bool SEG = false, typename T, typename GM>
+template <int C, int NT, bool SEG = false, int RG = 1, typename T, typename GM>
 __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
                           const Grid &g, const GM &geo, int j, T *lds) {
     constexpr int CP = C + 1;
     const int tid = threadIdx.x, nt = X.nt;
     const int NT1 = nt * CP;
     T *Ld = lds, *Lb = lds + NT1, *La = lds + 2 * NT1, *Lc = lds + 3 * NT1;
-    row_pass<C, NT, SEG>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
+    row_pass<C, NT, SEG, RG>(X, e, f, K, g, geo, j, Ld, La, Lb, Lc);
     __syncthreads();
 
     // each thread takes its chunk into registers
@@ -576,7 +612,7 @@ __device__ void xline_row(const RowCtx<T> &X, T *__restrict__ e, const T *__rest
 // in the next, so no workgroup reads a cell that another one of the same
 // launch writes: a segment's residual reaches one cell beyond each end, into
 // a segment of the other parity.  A whole row: seglen = nx, nsegl = 1.
-template <int C, int NT, bool SEG, typename T, typename GM>
+template <int C, int NT, bool SEG, int RG = 3, typename T, typename GM>
 __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restrict__ f, const T *__restrict__ K,
                                            const Grid &g, const GM &geo, int row0, int rowstep, int nt, int extra0,
                                            int extra1, int seglen, int seg0, int segstep, int nsegl) {
@@ -602,7 +638,7 @@ __device__ __forceinline__ void xline_rows(T *__restrict__ e, const T *__restric
         // ops) instead of being hoisted out of the loop and spilled
         RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
-        xline_row<C, NT, SEG>(X, e, f, K, g, geo, jr, lds);
+        xline_row<C, NT, SEG, RG>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
