@@ -87,9 +87,12 @@ def mix_problems(capi, n):
     return probs, rhs, geos, maps, bcs
 
 
-def make_batch(capi, geos, bcs, maps, rhs, method, use_graph=True, sd=2, su=2, sb=2):
+def make_batch(capi, geos, bcs, maps, rhs, method, use_graph=True, sd=2, su=2, sb=2, tuning=None):
+    """tuning: autotuner entries (nx, bc_class, fp32, threads) to apply, or None for the default shapes."""
     b = capi.Batch(geos, bcs, mappings=maps, smooth_down=sd, smooth_up=su, smooth_bottom=sb, use_graph=use_graph,
                    method=method)
+    if tuning:
+        b.set_tuning(tuning)
     for k in range(len(geos)):
         b.set_rhs(k, rhs[k])
     capi.device_synchronize()
@@ -607,6 +610,36 @@ def cfg3_bench(capi, nx=NX3, ny=NY3, tol=1e-12, tol4=1e-7, seed=SEED + 3):
     return out
 
 
+def tune_bench(capi, args, probs, rhs, geos, maps, bcs, reps=5):
+    """The batch autotuner on the mix: per precision, timed batched solves with the
+    default line-solve shapes, the tuner (per (nx, BC class) key: `reps` launches of
+    the finest-level even-row relaxation of the key's problems per candidate width),
+    timed solves with its choices.  The choices of both precisions go to
+    gpurun_out/tuning_cache.json, the cache format of capi.save_tuning."""
+    out = {"workload": f"batched mix, {len(probs)} problems", "tol": args.tol}
+    entries = []
+    for method, label in ((capi.METHOD_KRYLOV_MG32, "mixed"), (capi.METHOD_KRYLOV_MG, "fp64")):
+        b = make_batch(capi, geos, bcs, maps, rhs, method, not args.no_graph)
+        dt0, _, _ = timed_batch(capi, b, args.tol, 2, 1, f"{label} default shapes")
+        t0 = time.perf_counter()
+        chosen = b.tune(reps)
+        tune_s = time.perf_counter() - t0
+        for k in range(len(geos)):
+            b.set_rhs(k, rhs[k])
+        dt1, _, _ = timed_batch(capi, b, args.tol, 2, 1, f"{label} tuned shapes")
+        b.close()
+        entries += chosen
+        counts = {}
+        for e in chosen:
+            counts[e[3]] = counts.get(e[3], 0) + 1
+        out[label] = {"default_solves_per_s": 2 * len(probs) / dt0, "tuned_solves_per_s": 2 * len(probs) / dt1,
+                      "tune_s": tune_s, "keys": len(chosen), "keys_per_width": counts}
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    capi.save_tuning(entries, os.path.join(ROOT, "gpurun_out", "tuning_cache.json"),
+                     note=f"bench.py --tune on the {len(probs)}-problem mix, {reps} launches per candidate")
+    return out
+
+
 def cpu_baseline(geo, bc, rhs, gpu_cycles, budget_s=20.0):
     """Time the reference Fortran (oracle/_ref/fast, -O3 -march=native, one
     core) on the same 2048^2 problem: DeferredMGPoisson2D::vcycle, one V(2,2)
@@ -713,6 +746,11 @@ def main():
                          "(and the calibration copy), for hardware-counter passes")
     ap.add_argument("--with-solve", action="store_true",
                     help="with --batch-kernels-only: one batched solve (captured graphs) before the timings")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the batched line-solve shapes on the mix (both precisions), report the "
+                         "batched solves/s before and after, and write the choices to gpurun_out/tuning_cache.json")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="batched sections with the default line-solve shapes instead of the committed cache")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_xline_2048.json"),
                     help="measured HBM bytes per single-solver x-line launch (FETCH_SIZE/WRITE_SIZE passes)")
     ap.add_argument("--batch-traffic", default=os.path.join(ROOT, "profiles", "traffic_batch_xline32.json"),
@@ -740,8 +778,12 @@ def main():
 
     probs, rhs, geos, maps, bcs = mix_problems(capi, args.batch_n)
     cells = int(sum(nx * ny for nx, ny, *_ in probs))
+    if args.tune:
+        print(json.dumps(tune_bench(capi, args, probs, rhs, geos, maps, bcs)), flush=True)
+        return
+    tuning = [] if args.no_tuning else capi.load_tuning()
     t0 = time.perf_counter()
-    b32 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG32, use_graph)
+    b32 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG32, use_graph, tuning=tuning)
     setup_s = time.perf_counter() - t0
     print(f"# batch set-up: {len(probs)} problems, {cells} cells, {setup_s:.2f} s", file=sys.stderr, flush=True)
     if args.batch_kernels_only:
@@ -772,7 +814,7 @@ def main():
     b32.close()
 
     # the same mix with FP64 cycles
-    b64 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG, use_graph)
+    b64 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG, use_graph, tuning=tuning)
     dt64, iters64, _ = timed_batch(capi, b64, args.tol, max(1, min(args.steps, 2)), 1, "batched fp64")
     b64.close()
 

@@ -397,6 +397,23 @@ int mgp_batch_last_cycles(const mgp_batch *b);
  * the element size of the cycle: 8 bytes, or 4 with KRYLOV_MG32).  Changes the resident
  * finest-level corrections, nothing else. */
 int mgp_batch_time_kernel(mgp_batch *b, int kernel, int reps, double *avg_ms, double *bytes);
+/* Autotuner of the batched line solve (no counterpart in the reference).  The shape of a row's
+ * line solve -- threads per row (64, 128 or 256) and with it the chunk length -- is chosen per key
+ * (nx, BC class, precision); BC class: bit k set when side k is Dirichlet; precision: 1 for
+ * KRYLOV_MG32.  Tables hold 4 ints per entry: nx, BC class, precision, threads.
+ * mgp_batch_tuning exports the batch's current choice per key present (at most max_entries are
+ * written; *n_entries receives the count).  mgp_batch_set_tuning applies the entries that match
+ * problems of the batch (*changed, may be NULL: problems whose shape changed) and rebuilds the
+ * launch tables; MGP_ERR_ARG for a shape with no kernel.  mgp_batch_tune searches one band of
+ * row lengths at a time (up to 128, 256, 512, 1024, 4111 cells): each width the band's rows all
+ * have a kernel for is timed with `reps` launch sets of the whole batch's finest-level even-row
+ * relaxation (HIP events, the other bands as chosen so far) and kept when 2 % faster; it applies
+ * the result and exports the choice per key.  A shape moves a solution by rounding
+ * only; a committed table of choices keeps results reproducible.  Between solves only (tune
+ * relaxes the resident finest-level corrections). */
+int mgp_batch_tuning(const mgp_batch *b, int *table, int max_entries, int *n_entries);
+int mgp_batch_set_tuning(mgp_batch *b, const int *table, int n_entries, int *changed);
+int mgp_batch_tune(mgp_batch *b, int reps, int *table, int max_entries, int *n_entries);
 
 /* ---- 5. Serving loop (no counterpart in the reference): resident geometries
  *         ("slots") with their hierarchies and last solutions in HBM, requests

@@ -7,6 +7,7 @@ module raises if the library has not been built.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -91,7 +92,8 @@ EXPORTS = (
     "mgp_server_submit", "mgp_server_step", "mgp_server_result", "mgp_integrate2d_bdry", "mgp_gpnorm",
     "mgp_batch_time_kernel", "mgp_compute_residual_dd", "mgp_lap4", "mgp_metric4_fill", "mgp_solver_get_phi_lo",
     "mgp_solver_project_mean", "mgp_pnorm_noalloc", "mgp_server_add_slot", "mgp_server_evict",
-    "mgp_solver_copy_phi_from", "mgp_solver_residual",
+    "mgp_solver_copy_phi_from", "mgp_solver_residual", "mgp_batch_tuning", "mgp_batch_set_tuning",
+    "mgp_batch_tune",
 )
 
 
@@ -564,6 +566,28 @@ def device_synchronize():
     _check(lib.mgp_device_synchronize())
 
 
+TUNING_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
+
+
+def load_tuning(path=TUNING_CACHE):
+    """The committed autotuner choices: [(nx, bc_class, fp32, threads), ...] ([] if absent)."""
+    if not os.path.exists(path):
+        return []
+    with open(path) as fh:
+        return [tuple(e) for e in json.load(fh)["entries"]]
+
+
+def save_tuning(entries, path=TUNING_CACHE, note=""):
+    """Merges entries into the cache at path (a new entry replaces one of the same key)."""
+    merged = {(e[0], e[1], e[2]): e[3] for e in load_tuning(path)}
+    for e in entries:
+        merged[(int(e[0]), int(e[1]), int(e[2]))] = int(e[3])
+    out = {"key": ["nx", "bc_class (bit k: side k Dirichlet)", "fp32", "threads"], "note": note,
+           "entries": [[k[0], k[1], k[2], v] for k, v in sorted(merged.items())]}
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=0)
+
+
 class Batch:
     """Many independent problems solved together (mgp_batch): each is solved as
     KRYLOV_MG solves it, with one GPU launch per level operation for all.
@@ -642,6 +666,32 @@ class Batch:
         ms, nbytes = _d(0.0), _d(0.0)
         _check(lib.mgp_batch_time_kernel(self._h, _i(kernel), _i(reps), ctypes.byref(ms), ctypes.byref(nbytes)))
         return ms.value, nbytes.value
+
+    def tuning(self):
+        """The line-solve shape per key present: [(nx, bc_class, fp32, threads), ...]."""
+        n = _i(0)
+        _check(lib.mgp_batch_tuning(self._h, None, _i(0), ctypes.byref(n)))
+        tab = (_i * (4 * n.value))()
+        _check(lib.mgp_batch_tuning(self._h, tab, n, ctypes.byref(n)))
+        return [tuple(tab[4 * k:4 * k + 4]) for k in range(n.value)]
+
+    def set_tuning(self, entries):
+        """Applies (nx, bc_class, fp32, threads) entries matching problems of the
+        batch; returns the number of problems whose shape changed."""
+        entries = [tuple(int(v) for v in e) for e in entries]
+        tab = (_i * (4 * len(entries)))(*[v for e in entries for v in e])
+        changed = _i(0)
+        _check(lib.mgp_batch_set_tuning(self._h, tab, _i(len(entries)), ctypes.byref(changed)))
+        return changed.value
+
+    def tune(self, reps=10):
+        """Times the candidate line-solve shapes per key, applies the fastest,
+        returns the choices (between solves only)."""
+        n = len(self)
+        tab = (_i * (4 * n))()
+        cnt = _i(0)
+        _check(lib.mgp_batch_tune(self._h, _i(reps), tab, _i(n), ctypes.byref(cnt)))
+        return [tuple(tab[4 * k:4 * k + 4]) for k in range(min(cnt.value, n))]
 
 
 class Server:

@@ -128,7 +128,29 @@ class BatchKMG {
     // Between solves only: it relaxes the resident finest-level corrections.
     double time_fine_xline(int reps, double *bytes);
 
+    // Autotuner of the batched line solve's shape: the threads per row (64,
+    // 128 or 256), and with them the chunk length, per key (nx, BC class, the
+    // batch's precision).  The default is xline_batch_shape's rule.  The shape
+    // decides how a row is partitioned into chunks, so a change moves the
+    // solution by rounding; choices keyed this way (not timed per batch) keep
+    // a problem's result the same alone and in any mix.
+    struct TuneEntry {
+        int nx, bc, fp32, threads;
+    };
+    static int bc_class(const Bc &bc);  // bit k: side k Dirichlet
+    std::vector<TuneEntry> tuning() const;
+    // Applies the entries whose key matches problems of the batch; returns the
+    // number of problems whose shape changed.  Rebuilds tables and graphs.
+    int set_tuning(const std::vector<TuneEntry> &t);
+    // Picks the widths by timing the finest-level even-row line relaxation of
+    // the whole batch (`reps` launch sets per candidate, HIP events), one band
+    // of row lengths at a time; applies and returns the choice per key.
+    // Between solves only: it relaxes the resident finest-level corrections.
+    std::vector<TuneEntry> tune(int reps);
+
   private:
+    void apply_shapes();
+    double time_shapes(const std::vector<int> &C, const std::vector<int> &NT, int reps);
     void body();
     void precondition(int sel);
     void zebra(int d, int set, int sel, int k);

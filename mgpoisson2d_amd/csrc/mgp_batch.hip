@@ -383,7 +383,8 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
     }
 
     // each problem's line solves use its own chunk length, whatever the batch
-    // (the partition of a row into chunks decides the rounding of its solve)
+    // (the partition of a row into chunks decides the rounding of its solve);
+    // set_tuning / tune change it per (nx, BC class)
     Cp.resize(n);
     NTp.resize(n);
     for (int p = 0; p < n; ++p) {
@@ -820,6 +821,195 @@ double BatchKMG::time_fine_xline(int reps, double *bytes) {
     upload_flags(active, act_);
     hip_check(hipStreamSynchronize(s), "sync");
     return ms / reps;
+}
+
+int BatchKMG::bc_class(const Bc &bc) {
+    int c = 0;
+    for (int k = 0; k < 4; ++k)
+        if (bc.type[k] == BC_DIRI) c |= 1 << k;
+    return c;
+}
+
+std::vector<BatchKMG::TuneEntry> BatchKMG::tuning() const {
+    std::map<std::pair<int, int>, int> keys;
+    for (int p = 0; p < count(); ++p) keys[{P[p].g.nx, bc_class(P[p].bc)}] = NTp[p];
+    std::vector<TuneEntry> out;
+    for (const auto &kv : keys) out.push_back(TuneEntry{kv.first.first, kv.first.second, mixed ? 1 : 0, kv.second});
+    return out;
+}
+
+// After a change of shapes: the level table's chunk counts, enough streams
+// for the chunk classes of a phase, the launch tables and the captured graph.
+void BatchKMG::apply_shapes() {
+    const int n = count();
+    for (int p = 0; p < n; ++p)
+        for (int d = 0; d < nlev[p]; ++d) L[P[p].lev0 + d].nt = P[p].g.nx / Cp[p];
+    hip_check(hipMemcpyAsync(dL, L.data(), L.size() * sizeof(BLev), hipMemcpyHostToDevice, s), "level table");
+    std::map<int, int> classes;
+    for (int p = 0; p < n; ++p) classes[Cp[p] * 1024 + NTp[p]] = 1;
+    while (aux.size() < classes.size()) {
+        hipStream_t st = nullptr;
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        aux.push_back(st);
+        hipEvent_t ev = nullptr;
+        hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        join_ev.push_back(ev);
+    }
+    release(part);
+    release(full);
+    T = &full;
+    std::vector<int> all(n);
+    for (int p = 0; p < n; ++p) all[p] = p;
+    build_tables(full, all);
+}
+
+int BatchKMG::set_tuning(const std::vector<TuneEntry> &t) {
+    const size_t elem = mixed ? sizeof(float) : sizeof(double);
+    std::map<std::pair<int, int>, int> want;
+    for (const TuneEntry &e : t)
+        if (e.fp32 == (mixed ? 1 : 0)) want[{e.nx, e.bc}] = e.threads;
+    int changed = 0;
+    for (int p = 0; p < count(); ++p) {
+        const auto it = want.find({P[p].g.nx, bc_class(P[p].bc)});
+        if (it == want.end() || it->second == NTp[p]) continue;
+        const int C = xline_batch_chunk(P[p].g.nx, it->second, elem);
+        if (C == 0) {
+            g_last_error = "batch set_tuning: no line kernel with " + std::to_string(it->second) +
+                           " threads for rows of " + std::to_string(P[p].g.nx) + " cells";
+            throw Failure(MGP_ERR_ARG);
+        }
+        Cp[p] = C;
+        NTp[p] = it->second;
+        ++changed;
+    }
+    if (changed) apply_shapes();
+    return changed;
+}
+
+// Mean ms of `reps` launch sets of the finest-level even-row line relaxation
+// of every problem with shapes (C[p], NT[p]): one launch per chunk class, side
+// by side on the forked streams, as inside a solve.  Sets the level table's
+// finest-level chunk counts to those shapes (apply_shapes restores them).
+double BatchKMG::time_shapes(const std::vector<int> &C, const std::vector<int> &NT, int reps) {
+    const int n = count();
+    const size_t elem = mixed ? sizeof(float) : sizeof(double);
+    std::map<std::pair<int, int>, std::vector<XlTask>> cls;
+    std::map<std::pair<int, int>, size_t> lds;
+    for (int p = 0; p < n; ++p) {
+        const int nx = P[p].g.nx, ny = P[p].g.ny, lev = P[p].lev0;
+        L[lev].nt = nx / C[p];
+        if (ny < 4) continue;
+        auto &t = cls[{C[p], NT[p]}];
+        for (int k = 0; k < (ny - 4) / 2 + 1; ++k) t.push_back(XlTask{lev, 2 + 2 * k, -1, 0});
+        size_t &l = lds[{C[p], NT[p]}];
+        l = std::max(l, xline_lds_bytes_for(nx, C[p], elem, NT[p]));
+    }
+    hip_check(hipMemcpyAsync(dL, L.data(), L.size() * sizeof(BLev), hipMemcpyHostToDevice, s), "level table");
+    while (aux.size() < cls.size()) {
+        hipStream_t st = nullptr;
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        aux.push_back(st);
+        hipEvent_t ev = nullptr;
+        hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        join_ev.push_back(ev);
+    }
+    struct Cls {
+        void *t;
+        int n, C, NT;
+        size_t lds;
+    };
+    std::vector<Cls> v;
+    for (auto &kv : cls) {
+        void *d = dev_alloc_zero(kv.second.size() * sizeof(XlTask));
+        v.push_back(Cls{d, (int)kv.second.size(), kv.first.first, kv.first.second, lds[kv.first]});
+        hip_check(hipMemcpyAsync(d, kv.second.data(), kv.second.size() * sizeof(XlTask), hipMemcpyHostToDevice, s),
+                  "tasks");
+    }
+    auto launch_set = [&]() {
+        hip_check(hipEventRecord(fork_ev, s), "fork");
+        for (size_t i = 0; i < v.size(); ++i) {
+            hip_check(hipStreamWaitEvent(aux[i], fork_ev, 0), "fork wait");
+            launch_xline_batch(aux[i], (const XlTask *)v[i].t, v[i].n, dL, active, 0, v[i].C, v[i].NT, v[i].lds, -1,
+                               mixed);
+            hip_check(hipEventRecord(join_ev[i], aux[i]), "join");
+        }
+        for (size_t i = 0; i < v.size(); ++i) hip_check(hipStreamWaitEvent(s, join_ev[i], 0), "join wait");
+    };
+    hipEvent_t a = nullptr, e = nullptr;
+    float ms = 0.0f;
+    try {
+        hip_check(hipEventCreate(&a), "event");
+        hip_check(hipEventCreate(&e), "event");
+        launch_set();  // warm
+        hip_check(hipEventRecord(a, s), "event");
+        for (int r = 0; r < reps; ++r) launch_set();
+        hip_check(hipEventRecord(e, s), "event");
+        hip_check(hipEventSynchronize(e), "event");
+        hip_check(hipEventElapsedTime(&ms, a, e), "event");
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        for (Cls &c : v) dev_free(c.t);
+        if (a) (void)hipEventDestroy(a);
+        if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    for (Cls &c : v) dev_free(c.t);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(e);
+    return ms / reps;
+}
+
+// Coordinate search over row-length bands (rows of up to 128, 256, 512, 1024
+// and 4111 cells): for each band in turn, each width its rows all have a
+// kernel for is timed with the whole batch's finest-level set (the other
+// bands as chosen so far), and a width replaces the current one when it is at
+// least 2 % faster.  Timing the whole set, not a band alone, weighs a width by
+// what it does to the launches that run beside it.
+std::vector<BatchKMG::TuneEntry> BatchKMG::tune(int reps) {
+    if (reps < 1) {
+        g_last_error = "batch tune: reps >= 1";
+        throw Failure(MGP_ERR_ARG);
+    }
+    if (T != &full) {
+        release(part);
+        T = &full;
+    }
+    const int n = count();
+    const size_t elem = mixed ? sizeof(float) : sizeof(double);
+    std::vector<int> all(n, 1);
+    upload_flags(active, all);
+    std::vector<int> C = Cp, NT = NTp;
+    const int edges[6] = {0, 128, 256, 512, 1024, 1 << 30};
+    for (int b = 0; b < 5; ++b) {
+        std::vector<int> mem;
+        for (int p = 0; p < n; ++p)
+            if (P[p].g.nx > edges[b] && P[p].g.nx <= edges[b + 1]) mem.push_back(p);
+        if (mem.empty()) continue;
+        double best = time_shapes(C, NT, reps);
+        for (int w : {64, 128, 256}) {  // the batched line kernels' widths
+            std::vector<int> c2 = C, t2 = NT;
+            bool ok = true, same = true;
+            for (int p : mem) {
+                c2[p] = xline_batch_chunk(P[p].g.nx, w, elem);
+                t2[p] = w;
+                ok = ok && c2[p] != 0;
+                same = same && NT[p] == w;
+            }
+            if (!ok || same) continue;
+            const double ms = time_shapes(c2, t2, reps);
+            if (ms < 0.98 * best) {
+                best = ms;
+                C = c2;
+                NT = t2;
+            }
+        }
+    }
+    Cp = C;
+    NTp = NT;
+    upload_flags(active, act_);
+    apply_shapes();
+    return tuning();
 }
 
 int BatchKMG::running() const {

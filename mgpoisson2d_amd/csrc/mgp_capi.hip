@@ -1313,6 +1313,53 @@ int mgp_batch_time_kernel(mgp_batch *b, int kernel, int reps, double *avg_ms, do
 
 int mgp_batch_last_cycles(const mgp_batch *b) { return b && b->b ? b->b->last_cycles : 0; }
 
+namespace {
+void export_tuning(const std::vector<mgp::BatchKMG::TuneEntry> &t, int *table, int max_entries, int *n_entries) {
+    if (n_entries) *n_entries = (int)t.size();
+    if (!table) return;
+    for (size_t k = 0; k < t.size() && (int)k < max_entries; ++k) {
+        table[4 * k] = t[k].nx;
+        table[4 * k + 1] = t[k].bc;
+        table[4 * k + 2] = t[k].fp32;
+        table[4 * k + 3] = t[k].threads;
+    }
+}
+}  // namespace
+
+int mgp_batch_tuning(const mgp_batch *b, int *table, int max_entries, int *n_entries) {
+    return guarded([&] {
+        if (!b || !b->b || max_entries < 0) {
+            g_last_error = "batch_tuning: no batch";
+            throw Failure(MGP_ERR_ARG);
+        }
+        export_tuning(b->b->tuning(), table, max_entries, n_entries);
+    });
+}
+
+int mgp_batch_set_tuning(mgp_batch *b, const int *table, int n_entries, int *changed) {
+    return guarded([&] {
+        if (!b || !b->b || n_entries < 0 || (n_entries > 0 && !table)) {
+            g_last_error = "batch_set_tuning: no batch or table";
+            throw Failure(MGP_ERR_ARG);
+        }
+        std::vector<mgp::BatchKMG::TuneEntry> t;
+        for (int k = 0; k < n_entries; ++k)
+            t.push_back(mgp::BatchKMG::TuneEntry{table[4 * k], table[4 * k + 1], table[4 * k + 2], table[4 * k + 3]});
+        const int c = b->b->set_tuning(t);
+        if (changed) *changed = c;
+    });
+}
+
+int mgp_batch_tune(mgp_batch *b, int reps, int *table, int max_entries, int *n_entries) {
+    return guarded([&] {
+        if (!b || !b->b || reps < 1 || max_entries < 0) {
+            g_last_error = "batch_tune: reps >= 1";
+            throw Failure(MGP_ERR_ARG);
+        }
+        export_tuning(b->b->tune(reps), table, max_entries, n_entries);
+    });
+}
+
 int mgp_server_create(mgp_server **out, int n, const mgp_mapping *maps, const int *nx, const int *ny,
                       const double *dx, const double *dy, const mgp_bc *bcs, const mgp_solver_opts *opts,
                       double hbm_budget) {

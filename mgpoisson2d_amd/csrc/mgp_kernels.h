@@ -143,8 +143,13 @@ inline int coarsest_sweeps(int sb, int ny) {
     return sb > half ? sb : half;
 }
 // Shape of the batched line solve of an nx-cell row: threads per row (64 up
-// to 512 cells, else 256) and chunk length C (2, 4, 8 or 16; 0: too long).
+// to 512 cells, 128 up to 1024, else 256) and chunk length C (2, 4, 8 or 16;
+// 0: too long).  The default rule; the batch autotuner may pick another.
 void xline_batch_shape(int nx, int &C, int &threads);
+// Chunk length of the batched line solve of an nx-cell row with `threads`
+// threads per row (64, 128 or 256), 0 if that shape has no kernel or its LDS
+// exceeds 160 KB for elements of `elem` bytes.
+int xline_batch_chunk(int nx, int threads, size_t elem);
 // dynamic LDS of one x-line workgroup for elements of `elem` bytes
 size_t xline_lds_bytes(int nx, size_t elem);
 // the same for chunk length C and `threads` threads per row

@@ -881,6 +881,13 @@ int xline_segments(int nx, size_t elem) {
     return 0;
 }
 
+int xline_batch_chunk(int nx, int threads, size_t elem) {
+    if (nx < 4 || (threads != XL_SHORT && threads != XL_MID && threads != XL_THREADS)) return 0;
+    const int C = xline_chunk(nx, threads);
+    if (C == 0 || (C == 16 && threads != XL_THREADS)) return 0;
+    return xline_lds_bytes_for(nx, C, elem, threads) <= 160 * 1024 ? C : 0;
+}
+
 void xline_batch_shape(int nx, int &C, int &threads) {
     threads = nx <= XL_SHORT_MAX_NX ? XL_SHORT : (nx <= XL_MID_MAX_NX ? XL_MID : XL_THREADS);
     C = xline_chunk(nx, threads);
@@ -982,11 +989,14 @@ static void xline_batch(hipStream_t s, const XlTask *tasks, int ntasks, const BL
                 throw Failure(MGP_ERR_ARG);
         }
     } else if (threads == XL_MID) {
-        if (C != 8) {
-            g_last_error = "k_xline_batch_mid: chunk length must be 8";
-            throw Failure(MGP_ERR_ARG);
+        switch (C) {
+            case 2: k_xline_batch_mid<2, T><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 4: k_xline_batch_mid<4, T><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k); break;
+            case 8: k_xline_batch_mid<8, T><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k); break;
+            default:
+                g_last_error = "k_xline_batch_mid: chunk length must be 2, 4 or 8";
+                throw Failure(MGP_ERR_ARG);
         }
-        k_xline_batch_mid<8, T><<<ntasks, XL_MID, lds, s>>>(tasks, levs, active, sel, k);
     } else switch (C) {
         case 2: k_xline_batch<2, T><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
         case 4: k_xline_batch<4, T><<<ntasks, XL_THREADS, lds, s>>>(tasks, levs, active, sel, k); break;
