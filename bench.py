@@ -292,6 +292,29 @@ def serving_bench(capi, n_small=16, big_n=4096, big_share=0.1, load=0.7, n_reque
     }
 
 
+def mc_bench(capi, geo, bc, reps=20):
+    """The multi-colour GS sweep of the plain operator (Poisson2D relax_gs, MC
+    ordering) on the finest level: the four interior colours in one LDS-tiled
+    launch against one launch per colour, each followed by the ring phase
+    (algorithmic bytes: phi, rhs, invd and the four face metrics read once,
+    phi written once)."""
+    s = capi.Solver(geo, bc, method=capi.METHOD_VCYCLE, ordering=capi.ORDER_MC)
+    tiled_ms, nbytes = s.time_kernel(7, reps=reps)
+    colour_ms, _ = s.time_kernel(8, reps=reps)
+    s.close()
+    gbps = nbytes / (tiled_ms * 1e-3) / 1e9
+    return {
+        "workload": f"Poisson2D relax_gs, MC ordering, 1 sweep, {geo.nx}x{geo.ny} finest level",
+        "tiled_ms": tiled_ms,
+        "per_colour_ms": colour_ms,
+        "speedup": colour_ms / tiled_ms,
+        "tiled_GBps": gbps,
+        "tiled_frac": gbps / HBM_PEAK_GBS,
+        "alg_bytes_per_sweep": nbytes,
+        "timing": f"HIP events around {reps} back-to-back sweeps on the solver stream (interior and ring phase)",
+    }
+
+
 def dc_rb_bench(capi, geo, bc, rhs=None, reps=20, cycles=5):
     """The DC red-black smoother (DeferredMGPoisson2D, RB ordering) on the
     finest level: one call of 2 sweeps with LDS tiles (several sweeps per
@@ -767,7 +790,9 @@ def main():
         return
     if args.dc_rb_only:
         geo, bc, rhs = config2_problem(args.n, args.n)
-        print(json.dumps(dc_rb_bench(capi, geo, bc, rhs)), flush=True)
+        out = dc_rb_bench(capi, geo, bc, rhs)
+        out["mc_sweep"] = mc_bench(capi, geo, bc)
+        print(json.dumps(out), flush=True)
         return
     if args.cfg3_only:
         print(json.dumps(cfg3_bench(capi, args.cfg3_n, args.cfg3_n // 4)), flush=True)
@@ -837,6 +862,7 @@ def main():
     if not args.no_dc_rb:
         geo, bc, rhs2 = config2_problem(NX, NX)
         dc_rb = dc_rb_bench(capi, geo, bc, rhs2)
+        dc_rb["mc_sweep"] = mc_bench(capi, geo, bc)
     cpu = None if args.no_cpu_baseline else cpu_baseline_mix(probs, rhs, 2 * iters32)
 
     line = {

@@ -354,7 +354,10 @@ int mgp_solver_last_cycles(const mgp_solver *s);
  * ordering: kernel 4: one red-black smoother call of 2 sweeps on the finest
  * level with LDS tiles (tile kernel and copy back); 5: the same call with one
  * launch per colour; 6: the tile kernel alone (cold is ignored; *bytes is 7
- * fields read and 1 written).  cold != 0: back-to-back launches
+ * fields read and 1 written).  VCYCLE solvers with MC ordering: kernel 7: one
+ * multi-colour sweep of the finest level (the four interior colours in one
+ * LDS-tiled launch, then the ring phase); 8: the same sweep with one launch per
+ * interior colour (cold is ignored; *bytes as for 6).  cold != 0: back-to-back launches
  * that take four copies of the operands in turn, so that each launch finds its
  * operands evicted from the L2s and the MALL (as the kernels run inside a
  * solve); cold == 0: back-to-back launches on cache-resident operands.

@@ -225,8 +225,7 @@ int mgp_relax_gs(const mgp_geo *geo, const mgp_bc *bc, int homog, const double *
             launch_gs_lex(c.s, f.p, r.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega, maxiters);
         } else {
             for (int k = 0; k < maxiters; ++k) {
-                for (int col = 0; col < 4; ++col)
-                    launch_gs_mc_interior(c.s, f.p, r.p, iv.p, g, c.geo.view(), omega, col & 1, col >> 1);
+                launch_gs_mc_interior_tiled(c.s, f.p, r.p, iv.p, g, c.geo.view(), omega);
                 launch_gs_mc_ring(c.s, f.p, r.p, iv.p, g, c.geo.view(), c.bc.bc, homog, omega);
             }
         }
@@ -1151,7 +1150,8 @@ int mgp_solver_last_cycles(const mgp_solver *s) { return s ? s->last_cycles : 0;
 int mgp_solver_time_kernel(mgp_solver *s, int kernel, int reps, int cold, double *avg_ms, double *bytes) {
     return guarded([&] {
         const bool kmg = s->opts.method == MGP_METHOD_KRYLOV_MG || s->opts.method == MGP_METHOD_KRYLOV_MG32;
-        if (kernel >= 4 && s->opts.method == MGP_METHOD_DC_VCYCLE) {
+        if ((kernel >= 4 && kernel <= 6 && s->opts.method == MGP_METHOD_DC_VCYCLE) ||
+            ((kernel == 7 || kernel == 8) && s->opts.method == MGP_METHOD_VCYCLE)) {
             *avg_ms = s->ref.time_relax(s->stream, kernel, reps, bytes);
             return;
         }

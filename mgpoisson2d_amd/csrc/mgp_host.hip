@@ -324,7 +324,7 @@ void RefHierarchy::relax(hipStream_t s, int d, double *phi, const double *rhs, i
             launch_gs_lex(s, phi, rhs, l.invd.p, l.g, l.geo, l.bc, 1, omega, sweeps);
         } else {
             for (int k = 0; k < sweeps; ++k) {
-                for (int c = 0; c < 4; ++c) launch_gs_mc_interior(s, phi, rhs, l.invd.p, l.g, l.geo, omega, c & 1, c >> 1);
+                launch_gs_mc_interior_tiled(s, phi, rhs, l.invd.p, l.g, l.geo, omega);
                 launch_gs_mc_ring(s, phi, rhs, l.invd.p, l.g, l.geo, l.bc, 1, omega);
             }
         }
@@ -344,16 +344,28 @@ void RefHierarchy::relax(hipStream_t s, int d, double *phi, const double *rhs, i
 }
 
 double RefHierarchy::time_relax(hipStream_t s, int kernel, int reps, double *bytes) {
-    if (!dc || ordering != MGP_ORDER_RB || kernel < 4 || kernel > 6 || reps <= 0) {
-        g_last_error = "time_kernel: kernels 4-6 time the red-black smoother of a DC_VCYCLE solver (ordering RB)";
+    const bool rb = dc && ordering == MGP_ORDER_RB && kernel >= 4 && kernel <= 6;
+    const bool mc = !dc && ordering == MGP_ORDER_MC && (kernel == 7 || kernel == 8);
+    if (!(rb || mc) || reps <= 0) {
+        g_last_error = "time_kernel: kernels 4-6 time the red-black smoother of a DC_VCYCLE solver (ordering RB), "
+                       "7-8 the multi-colour sweep of a VCYCLE solver (ordering MC)";
         throw Failure(MGP_ERR_ARG);
     }
     HLevel &l = levels[0];
-    const double omega = 4.0 / 5.0;
+    const double omega = rb ? 4.0 / 5.0 : 1.0;
     const int prev = dc_rb_tile_choice(kernel == 5 ? 0 : -1);
     auto once = [&] {
-        if (kernel == 6) dc_rb_tile_once(s, l.e.p, l.w.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, omega, 2);
-        else dc_rb_relax(s, l.e.p, l.w.p, l.mod.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, 1, omega, 2);
+        if (kernel == 6) {
+            dc_rb_tile_once(s, l.e.p, l.w.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, omega, 2);
+        } else if (kernel == 7) {  // one multi-colour sweep: the tiled interior colours, then the ring
+            launch_gs_mc_interior_tiled(s, l.e.p, l.r.p, l.invd.p, l.g, l.geo, omega);
+            launch_gs_mc_ring(s, l.e.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, 1, omega);
+        } else if (kernel == 8) {  // the same sweep with one launch per interior colour
+            for (int c = 0; c < 4; ++c) launch_gs_mc_interior(s, l.e.p, l.r.p, l.invd.p, l.g, l.geo, omega, c & 1, c >> 1);
+            launch_gs_mc_ring(s, l.e.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, 1, omega);
+        } else {
+            dc_rb_relax(s, l.e.p, l.w.p, l.mod.p, l.r.p, l.invd.p, l.g, l.geo, l.bc, 1, omega, 2);
+        }
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms = 0.0f;
@@ -375,6 +387,9 @@ double RefHierarchy::time_relax(hipStream_t s, int kernel, int reps, double *byt
     dc_rb_tile_choice(prev);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // RB: two sweeps of phi, modified rhs, invdiags and four face metrics read
+    // once per call and phi written (64 B/cell); MC: one sweep of the same
+    // operands (64 B/cell)
     *bytes = 8.0 * 8.0 * (double)l.g.nx * (double)l.g.ny;
     return ms / reps;
 }

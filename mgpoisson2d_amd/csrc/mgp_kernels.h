@@ -25,6 +25,10 @@ void launch_dc_modrhs(hipStream_t s, double *modrhs, const double *rhs, const do
                       const Geo &geo);
 void launch_gs_mc_interior(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
                            const Geo &geo, double omega, int ci, int cj);
+// The four interior colours of one multi-colour sweep in one launch over LDS
+// tiles (bit-identical to the four launch_gs_mc_interior calls in colour order).
+void launch_gs_mc_interior_tiled(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
+                                 const Geo &geo, double omega);
 void launch_gs_mc_ring(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
                        const Geo &geo, const Bc &bc, int homog, double omega);
 void launch_gs_lex(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,

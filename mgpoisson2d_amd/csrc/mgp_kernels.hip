@@ -275,6 +275,61 @@ __global__ void __launch_bounds__(256) k_gs_mc_interior(Fld phi, const double *r
     phi.p[c] = gs_update(phi.p[c], rhs[c], lphi, invd[c], omega);
 }
 
+// The four interior colours of a multi-colour sweep in one pass over LDS
+// tiles of MC_TX x MC_TY interior cells.  Colour c (in the order of the
+// per-colour launches: (i, j) parity (even, even), (odd, even), (even, odd),
+// (odd, odd), even meaning 2, 4, ..) reads its neighbours as left by colours
+// < c, so the tile also updates colour c on a halo of 3 - c cells (the
+// neighbouring tiles' cells, recomputed from the same values): its window
+// holds the tile and a halo of MC_H = 4 cells of phi.  Every update is the
+// per-colour kernel's expression on the same operands, so the sweep gives its
+// bits.  phi is read once per tile window and written once; the metrics, rhs
+// and invdiags of the halo updates come from the L2.  Ring cells and ghosts
+// stay as they are (the ring phase follows).
+constexpr int MC_TX = 64, MC_TY = 16, MC_H = 4;
+constexpr int MC_WX = MC_TX + 2 * MC_H, MC_WY = MC_TY + 2 * MC_H;
+
+__global__ void __launch_bounds__(256) k_gs_mc_tile(Fld phi, const double *rhs, const double *invd, Grid g, Geo geo,
+                                                    double omega) {
+    __shared__ double w[MC_WY][MC_WX + 1];
+    const int P = g.pitch, nx = g.nx, ny = g.ny;
+    const int ti0 = 1 + blockIdx.x * MC_TX, tj0 = 1 + blockIdx.y * MC_TY;  // tile origin (first interior cell 1)
+    for (int k = threadIdx.x; k < MC_WX * MC_WY; k += 256) {
+        const int wi = k % MC_WX, wj = k / MC_WX;
+        const int i = ti0 - MC_H + wi, j = tj0 - MC_H + wj;
+        // only valid cells are ever read by an interior update
+        w[wj][wi] = (i >= 0 && i < nx && j >= 0 && j < ny) ? phi.p[ix(P, i, j)] : 0.0;
+    }
+    __syncthreads();
+    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy;
+    for (int colour = 0; colour < 4; ++colour) {
+        const int pi = (colour & 1) ? 1 : 0, pj = (colour >> 1) ? 1 : 0, h = 3 - colour;
+        // the colour's cells in the tile plus halo h, within the interior 1..n-2
+        int x0 = ti0 - h, x1 = ti0 + MC_TX + h, y0 = tj0 - h, y1 = tj0 + MC_TY + h;
+        x0 = x0 < 1 ? 1 : x0;
+        y0 = y0 < 1 ? 1 : y0;
+        x1 = x1 > nx - 1 ? nx - 1 : x1;
+        y1 = y1 > ny - 1 ? ny - 1 : y1;
+        x0 += (x0 & 1) != pi;
+        y0 += (y0 & 1) != pj;
+        const int nxr = x0 < x1 ? (x1 - 1 - x0) / 2 + 1 : 0, nyr = y0 < y1 ? (y1 - 1 - y0) / 2 + 1 : 0;
+        for (int k = threadIdx.x; k < nxr * nyr; k += 256) {
+            const int i = x0 + 2 * (k % nxr), j = y0 + 2 * (k / nxr);
+            const int a = i - ti0 + MC_H, b = j - tj0 + MC_H;
+            const size_t c = ix(P, i, j);
+            const double lphi = interior_lap_values(geo, c, P, idx, idy, w[b][a], w[b][a + 1], w[b][a - 1],
+                                                    w[b + 1][a], w[b - 1][a], w[b + 1][a + 1], w[b + 1][a - 1],
+                                                    w[b - 1][a + 1], w[b - 1][a - 1]);
+            w[b][a] = gs_update(w[b][a], rhs[c], lphi, invd[c], omega);
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < MC_TX * MC_TY; k += 256) {
+        const int i = ti0 + k % MC_TX, j = tj0 + k / MC_TX;
+        if (i <= nx - 2 && j <= ny - 2) phi.p[ix(P, i, j)] = w[j - tj0 + MC_H][i - ti0 + MC_H];
+    }
+}
+
 __device__ __forceinline__ void gs_cell(Fld phi, const double *rhs, const double *invd, const Grid &g,
                                         const Geo &geo, const Bc &bc, int homog, double omega, int i, int j) {
     const size_t c = ix(g.pitch, i, j);
@@ -1053,6 +1108,12 @@ void launch_gs_mc_interior(hipStream_t s, double *phi, const double *rhs, const 
     const int na = (g.nx - 2 - (ci ? 1 : 2)) / 2 + 1, nb = (g.ny - 2 - (cj ? 1 : 2)) / 2 + 1;
     if (na <= 0 || nb <= 0) return;
     k_gs_mc_interior<<<tiles(na, nb), 256, 0, s>>>(Fld{phi, g.pitch}, rhs, invd, g, geo, omega, ci, cj);
+}
+void launch_gs_mc_interior_tiled(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
+                                 const Geo &geo, double omega) {
+    if (g.nx < 3 || g.ny < 3) return;
+    const dim3 grid((unsigned)((g.nx - 2 + MC_TX - 1) / MC_TX), (unsigned)((g.ny - 2 + MC_TY - 1) / MC_TY));
+    k_gs_mc_tile<<<grid, 256, 0, s>>>(Fld{phi, g.pitch}, rhs, invd, g, geo, omega);
 }
 void launch_gs_mc_ring(hipStream_t s, double *phi, const double *rhs, const double *invd, const Grid &g,
                        const Geo &geo, const Bc &bc, int homog, double omega) {

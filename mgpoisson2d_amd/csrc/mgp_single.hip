@@ -137,7 +137,7 @@ void SingleLevel::relax(double *phi, const double *rhs, const double *invd, doub
         return;
     }
     for (int k = 0; k < sweeps; ++k) {
-        for (int col = 0; col < 4; ++col) launch_gs_mc_interior(s, phi, rhs, invd, g, geo, omega, col & 1, col >> 1);
+        launch_gs_mc_interior_tiled(s, phi, rhs, invd, g, geo, omega);
         launch_gs_mc_ring(s, phi, rhs, invd, g, geo, bc, homog, omega);
     }
 }

@@ -197,20 +197,30 @@ __device__ __forceinline__ double cell_lap(const Get &get, const Grid &g, const 
 }
 
 // Interior fast path: no extrapolation, no BC replacement.
-__device__ __forceinline__ double cell_lap_interior(const CFld &f, const Grid &g, const Geo &geo,
-                                                    int i, int j) {
-    const int P = g.pitch;
-    const size_t c = ix(P, i, j);
-    const double *q = f.p + c;
-    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, fourth = 0.25;
-    const double p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
-    const double ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
+// The interior stencil on the 3x3 neighbourhood values (centre p, east,
+// west, north, south, then the corners), the cell's metrics at c taken from
+// geo: one expression for every caller, so that values read from global
+// memory or from an LDS tile give the same bits.
+__device__ __forceinline__ double interior_lap_values(const Geo &geo, size_t c, int P, double idx, double idy,
+                                                      double p, double pe, double pw, double pn, double ps,
+                                                      double ene, double enw, double ese, double esw) {
+    const double fourth = 0.25;
     const double ee = pe, ew = pw, en = pn, es = ps;
     const double xfe = geo.Jxx[c + 1] * (pe - p) * idx + geo.Jxy[c + 1] * fourth * (ene - ese + en - es) * idy;
     const double xfw = geo.Jxx[c] * (p - pw) * idx + geo.Jxy[c] * fourth * (en - es + enw - esw) * idy;
     const double yfn = geo.Jyy[c + P] * (pn - p) * idy + geo.Jyx[c + P] * fourth * (ene - enw + ee - ew) * idx;
     const double yfs = geo.Jyy[c] * (p - ps) * idy + geo.Jyx[c] * fourth * (ee - ew + ese - esw) * idx;
     return (xfe - xfw) * idx + (yfn - yfs) * idy;
+}
+
+__device__ __forceinline__ double cell_lap_interior(const CFld &f, const Grid &g, const Geo &geo,
+                                                    int i, int j) {
+    const int P = g.pitch;
+    const size_t c = ix(P, i, j);
+    const double *q = f.p + c;
+    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy;
+    return interior_lap_values(geo, c, P, idx, idy, q[0], q[1], q[-1], q[P], q[-P], q[P + 1], q[P - 1], q[-P + 1],
+                               q[-P - 1]);
 }
 
 // GS/SOR point update (poisson2d.f90:1770-1771, MGPoisson2D_DD.f90:303-304).
