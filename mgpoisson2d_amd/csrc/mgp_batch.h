@@ -28,6 +28,7 @@ struct BLev {
     const double *K;    // ring stencil table
     double slo, shi;    // ghost signs of the y sides in the prolongation
     int prob, nt;       // problem index; x-line chunks (nx / C)
+    int off;            // the fine row of the next coarser level's row 0 (semicoarsen_step)
     int sweeps;         // coarsest level of its problem: zebra sweeps there (else 0)
     // FP32 copies for the mixed-precision cycle (null otherwise): metrics,
     // stencil table, correction and right-hand side (one pair: the two

@@ -217,8 +217,11 @@ size_t BatchKMG::problem_bytes(const Problem &q, const mgp_solver_opts &o) {
         }
         ++lev;
         ++levels;
-        if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || lev >= maxlev) break;
-        ny /= 2;
+        int off = 0;
+        const int nyc = lev >= maxlev ? 0 : semicoarsen_step(ny, MIN_COARSE_NY, q.bc.type[YLO] == BC_NEUM,
+                                                              q.bc.type[YHI] == BC_NEUM, off);
+        if (nyc == 0) break;
+        ny = nyc;
     }
     return b + SC_COUNT * sizeof(double) + sizeof(BProb) + 2 * sizeof(int) + levels * sizeof(BLev);
 }
@@ -290,8 +293,11 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
                 O.r32.push_back(take(nl, sizeof(float)));
             }
             ++lev;
-            if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || lev >= maxlev) break;
-            ny /= 2;
+            int off = 0;
+            const int nyc = lev >= maxlev ? 0 : semicoarsen_step(ny, MIN_COARSE_NY, q.bc.type[YLO] == BC_NEUM,
+                                                                  q.bc.type[YHI] == BC_NEUM, off);
+            if (nyc == 0) break;
+            ny = nyc;
         }
         nlev[p] = lev;
         depth = std::max(depth, lev);
@@ -346,12 +352,14 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
                 l.f[1] = B.v[BV_S];
                 B.geo = l.geo;
             } else {
+                // from the fine level's rows off.. (off = 1: its bottom row is left out)
                 const BLev &f = L.back();
-                launch_restrict_cc(s, J, f.geo.J, l.g, f.g.pitch, 1, 2, nullptr, nullptr);
-                launch_restrict_xf(s, Jxx, f.geo.Jxx, l.g, f.g.pitch, 1, 2);
-                launch_restrict_xf(s, Jxy, f.geo.Jxy, l.g, f.g.pitch, 1, 2);
-                launch_restrict_yf(s, Jyx, f.geo.Jyx, l.g, f.g.pitch, 1, 2);
-                launch_restrict_yf(s, Jyy, f.geo.Jyy, l.g, f.g.pitch, 1, 2);
+                const size_t sh = (size_t)f.off * f.g.pitch;
+                launch_restrict_cc(s, J, f.geo.J + sh, l.g, f.g.pitch, 1, 2, nullptr, nullptr);
+                launch_restrict_xf(s, Jxx, f.geo.Jxx + sh, l.g, f.g.pitch, 1, 2);
+                launch_restrict_xf(s, Jxy, f.geo.Jxy + sh, l.g, f.g.pitch, 1, 2);
+                launch_restrict_yf(s, Jyx, f.geo.Jyx + sh, l.g, f.g.pitch, 1, 2);
+                launch_restrict_yf(s, Jyy, f.geo.Jyy + sh, l.g, f.g.pitch, 1, 2);
                 l.e[0] = l.e[1] = at(O.e[d]);
                 l.f[0] = l.f[1] = at(O.r[d]);
             }
@@ -376,8 +384,11 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             l.prob = p;
             l.nt = 0;  // set with the chunk class below
             l.sweeps = d == nlev[p] - 1 ? coarsest_sweeps(sb, ny) : 0;
+            l.off = 0;
+            const int nyc = semicoarsen_step(ny, MIN_COARSE_NY, B.bc.type[YLO] == BC_NEUM, B.bc.type[YHI] == BC_NEUM,
+                                             l.off);
             L.push_back(l);
-            ny /= 2;
+            ny = nyc;
             dy *= 2.0;
         }
     }

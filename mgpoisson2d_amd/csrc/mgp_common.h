@@ -79,11 +79,18 @@ __host__ __device__ inline size_t ix(int pitch, int i, int j) {
 // correction.
 template <typename T>
 __device__ __forceinline__ void prolong_y_cell(T *ef, const T *ec, const Grid &gf, const Grid &gc, T slo, T shi,
-                                               int i, int j) {
-    const int J = j >> 1;
+                                               int i, int j, int off = 0) {
+    const int jc = j - off;  // row within the coarse grid's fine rows
+    if (jc < 0 || jc >= 2 * gc.ny) {
+        // a row the coarsening left out (at a Neumann side): the correction
+        // of the coarse row next to it
+        ef[ix(gf.pitch, i, j)] += ec[ix(gc.pitch, i, jc < 0 ? 0 : gc.ny - 1)];
+        return;
+    }
+    const int J = jc >> 1;
     const T c = ec[ix(gc.pitch, i, J)];
     T nb;
-    if (j & 1) nb = J + 1 < gc.ny ? ec[ix(gc.pitch, i, J + 1)] : shi * c;
+    if (jc & 1) nb = J + 1 < gc.ny ? ec[ix(gc.pitch, i, J + 1)] : shi * c;
     else nb = J - 1 >= 0 ? ec[ix(gc.pitch, i, J - 1)] : slo * c;
     ef[ix(gf.pitch, i, j)] += T(0.75) * c + T(0.25) * nb;
 }

@@ -109,10 +109,10 @@ int ring_table_elems(const Grid &g);
 void launch_ring_coef(hipStream_t s, double *K, const Grid &g, const Geo &geo, const Bc &bc);
 // rc = 1x2 (y) average of the fine residual f - L e (table at ring cells); ec = 0.
 void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
-                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo);
+                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo, int off = 0);
 // FP32 form (mixed-precision preconditioner): same layout and formulas.
 void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
-                                const Grid &gf, const Grid &gc, const GeoF &geo);
+                                const Grid &gf, const Grid &gc, const GeoF &geo, int off = 0);
 // Flexible GMRES vector kernels (mgp_gmres.hip); V holds vectors i < nv at
 // V + i * stride (padded level layout, valid cells only).
 // out[i] = V_i . w (plain sum over the valid cells); partial: mdot_blocks(g) * nv doubles.
@@ -136,6 +136,23 @@ bool xline_whole(int nx, size_t elem);
 // Segments per row for elements of `elem` bytes: 1 for a whole row, 0 if nx
 // has no supported cut.
 int xline_segments(int nx, size_t elem);
+// One y-semicoarsening step from a level of ny rows (levels of at least
+// min_rows rows): the coarse row count, 0 to stop.  An even count halves.  An
+// odd count of at least 2 min_rows + 1 rows with a Neumann y side leaves that
+// side's row out (the top one when both are Neumann) and halves the rest:
+// coarse row J then covers fine rows off + 2J, off + 2J + 1 (*off = 1 when the
+// bottom row is left out, else 0); the coarse grid's side there lies one row
+// in and keeps the Neumann condition, and the left-out row takes the
+// correction of the coarse row next to it.  Without this the hierarchy stops
+// at the odd count (49 rows for ny = 784) and the coarsest level's sweeps
+// bound the convergence of the whole solve.
+inline int semicoarsen_step(int ny, int min_rows, bool neum_lo, bool neum_hi, int &off) {
+    off = 0;
+    if (ny % 2 == 0) return ny / 2 >= min_rows ? ny / 2 : 0;
+    if (ny < 2 * min_rows + 1 || !(neum_lo || neum_hi)) return 0;
+    off = neum_hi ? 0 : 1;
+    return (ny - 1) / 2;
+}
 // Sweeps on the coarsest level of the semicoarsening hierarchy: at least
 // `sb`, and half the level's rows up to 64.  The hierarchy stops at an odd
 // row count, and an x-line zebra sweep reduces errors that are smooth in y
@@ -166,11 +183,12 @@ void launch_xline(hipStream_t s, double *e, const double *f, const double *K, co
                   int row0, int rowstep, int nrows, int extra0, int extra1);
 void launch_xline(hipStream_t s, float *e, const float *f, const float *K, const Grid &g, const GeoF &geo, int row0,
                   int rowstep, int nrows, int extra0, int extra1);
-// ef += 1x2 (y) prolongation of ec (prolong_y_cell)
+// ef += 1x2 (y) prolongation of ec (prolong_y_cell); off: the fine row of
+// coarse row 0's first row (semicoarsen_step)
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi);
+                      double shi, int off = 0);
 void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi);
+                      double shi, int off = 0);
 // Precision conversions over a level array.  ghosted: also the ghost ring.
 void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted);
 void launch_convert(hipStream_t s, float *out, const double *in, const Grid &g, int ghosted);

@@ -962,11 +962,11 @@ namespace mgp {
 
 // y-direction prolongation for 1x2 semicoarsening (prolong_y_cell).
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong_y(T *ef, const T *ec, Grid gf, Grid gc, T slo, T shi) {
+__global__ void __launch_bounds__(256) k_prolong_y(T *ef, const T *ec, Grid gf, Grid gc, T slo, T shi, int off) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= gf.nx || j >= gf.ny) return;
-    prolong_y_cell(ef, ec, gf, gc, slo, shi, i, j);
+    prolong_y_cell(ef, ec, gf, gc, slo, shi, i, j, off);
 }
 
 template <typename TO, typename TI>
@@ -1217,13 +1217,12 @@ void launch_relax_block(hipStream_t s, const BottomArgs &a, double omega, int sw
 
 namespace mgp {
 void launch_prolong_y(hipStream_t s, double *ef, const double *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi) {
-    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, slo, shi);
+                      double shi, int off) {
+    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, slo, shi, off);
 }
-
 void launch_prolong_y(hipStream_t s, float *ef, const float *ec, const Grid &gf, const Grid &gc, double slo,
-                      double shi) {
-    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, (float)slo, (float)shi);
+                      double shi, int off) {
+    k_prolong_y<<<tiles(gf.nx, gf.ny), 256, 0, s>>>(ef, ec, gf, gc, (float)slo, (float)shi, off);
 }
 
 void launch_setval(hipStream_t s, float *y, float v, const Grid &g, int ghosted) {

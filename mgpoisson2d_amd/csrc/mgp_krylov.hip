@@ -65,6 +65,7 @@ constexpr int MIN_COARSE_NY = 4;
 
 struct KLevel {
     Grid g;
+    int off = 0;  // the fine row of the next coarser level's row 0 (semicoarsen_step)
     Geo geo;
     GeoF geo32;
     Bc bc;
@@ -237,12 +238,14 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
             hip_check(hipMemcpyAsync(l.Jyx.p, top.Jyx.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
             hip_check(hipMemcpyAsync(l.Jyy.p, top.Jyy.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
         } else {
+            // from the fine level's rows off.. (off = 1: its bottom row is left out)
             const KLevel &f = L[L.size() - 2];
-            launch_restrict_cc(s, l.J.p, f.J.p, l.g, f.g.pitch, 1, 2, nullptr, nullptr);
-            launch_restrict_xf(s, l.Jxx.p, f.Jxx.p, l.g, f.g.pitch, 1, 2);
-            launch_restrict_xf(s, l.Jxy.p, f.Jxy.p, l.g, f.g.pitch, 1, 2);
-            launch_restrict_yf(s, l.Jyx.p, f.Jyx.p, l.g, f.g.pitch, 1, 2);
-            launch_restrict_yf(s, l.Jyy.p, f.Jyy.p, l.g, f.g.pitch, 1, 2);
+            const size_t sh = (size_t)f.off * f.g.pitch;
+            launch_restrict_cc(s, l.J.p, f.J.p + sh, l.g, f.g.pitch, 1, 2, nullptr, nullptr);
+            launch_restrict_xf(s, l.Jxx.p, f.Jxx.p + sh, l.g, f.g.pitch, 1, 2);
+            launch_restrict_xf(s, l.Jxy.p, f.Jxy.p + sh, l.g, f.g.pitch, 1, 2);
+            launch_restrict_yf(s, l.Jyx.p, f.Jyx.p + sh, l.g, f.g.pitch, 1, 2);
+            launch_restrict_yf(s, l.Jyy.p, f.Jyy.p + sh, l.g, f.g.pitch, 1, 2);
         }
         l.geo = Geo{l.J.p, l.Jxx.p, l.Jxy.p, l.Jyx.p, l.Jyy.p};
         launch_ring_coef(s, l.K.p, l.g, l.geo, l.bc);
@@ -261,8 +264,10 @@ void KrylovMG::build(hipStream_t s, const DevGeo &top, const Bc &user_bc, const 
             l.r32.alloc(n);
             l.geo32 = GeoF{nullptr, l.Jxx32.p, l.Jxy32.p, l.Jyx32.p, l.Jyy32.p};
         }
-        if (ny % 2 != 0 || ny / 2 < MIN_COARSE_NY || (int)L.size() >= maxlev) break;
-        ny /= 2;
+        if ((int)L.size() >= maxlev) break;
+        const int nyc = semicoarsen_step(ny, MIN_COARSE_NY, bc0.type[YLO] == BC_NEUM, bc0.type[YHI] == BC_NEUM, l.off);
+        if (nyc == 0) break;
+        ny = nyc;
         dy *= 2.0;
     }
     const size_t n0 = level_elems(top.g.nx, top.g.ny);
@@ -341,9 +346,9 @@ static void vcycle_t(hipStream_t s, KrylovMG::Impl &K, size_t d, T *e, const T *
     KLevel &c = K.L[d + 1];
     T *ce = coarse_e(c, e), *cr = coarse_r(c, e);
     for (int k = 0; k < K.sd; ++k) zebra_sweep(s, l, e, f);
-    launch_residual_restrict_y(s, cr, ce, e, f, table(l, e), l.g, c.g, metrics(l, e));
+    launch_residual_restrict_y(s, cr, ce, e, f, table(l, e), l.g, c.g, metrics(l, e), l.off);
     vcycle_t(s, K, d + 1, ce, cr);
-    launch_prolong_y(s, e, ce, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI));
+    launch_prolong_y(s, e, ce, l.g, c.g, side_sign(l.bc, YLO), side_sign(l.bc, YHI), l.off);
     for (int k = 0; k < K.su; ++k) zebra_sweep(s, l, e, f);
 }
 

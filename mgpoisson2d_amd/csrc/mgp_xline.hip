@@ -707,13 +707,13 @@ __device__ __forceinline__ T interior_lap(gconst<T> *e, const G &geo_, int P, si
 // stencil from the metrics, ring cells from the table),  ec(i, J) = 0.
 template <typename T, typename G>
 __device__ __forceinline__ void restrict_cell(T *rc, T *ec, const T *e, const T *f, const T *K, const Grid &gf,
-                                              const Grid &gc, const G &geo, int i, int J) {
+                                              const Grid &gc, const G &geo, int i, int J, int off) {
     const int nring = ring_count(gf.nx, gf.ny);
     const T idx = (T)(1.0 / gf.dx), idy = (T)(1.0 / gf.dy);
     T s = 0;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-        const int j = 2 * J + b;
+        const int j = off + 2 * J + b;
         const bool ring = i == 0 || i == gf.nx - 1 || j == 0 || j == gf.ny - 1;
         const T lap = ring ? ring_lap(gptr(e), gptr(K), nring, gf, i, j)
                            : interior_lap(gptr(e), geo, gf.pitch, ix(gf.pitch, i, j), idx, idy);
@@ -726,11 +726,11 @@ __device__ __forceinline__ void restrict_cell(T *rc, T *ec, const T *e, const T 
 
 template <typename T, typename G>
 __global__ void __launch_bounds__(256) k_residual_restrict_y(T *rc, T *ec, const T *e, const T *f, const T *K,
-                                                             Grid gf, Grid gc, G geo) {
+                                                             Grid gf, Grid gc, G geo, int off) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= gc.nx || J >= gc.ny) return;
-    restrict_cell(rc, ec, e, f, K, gf, gc, geo, i, J);
+    restrict_cell(rc, ec, e, f, K, gf, gc, geo, i, J, off);
 }
 
 // ---------------------------------------------------------------------------
@@ -818,7 +818,7 @@ __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, cons
     level_fields(F, sel, fe, ff, K, geo);
     level_fields(Cl, 0, ce, cff, cK, cgeo);
     cf = const_cast<T *>(cff);
-    restrict_cell(cf, ce, (const T *)fe, ff, K, F->g, gc, geo, i, J);
+    restrict_cell(cf, ce, (const T *)fe, ff, K, F->g, gc, geo, i, J, F->off);
 }
 
 template <typename T>
@@ -835,7 +835,7 @@ __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const
     typename std::conditional<sizeof(T) == 8, Geo, GeoF>::type geo, cgeo;
     level_fields(F, sel, fe, ff, K, geo);
     level_fields(Cl, 0, ce, cf, cK, cgeo);
-    prolong_y_cell(fe, (const T *)ce, gf, gc, (T)F->slo, (T)F->shi, i, j);
+    prolong_y_cell(fe, (const T *)ce, gf, gc, (T)F->slo, (T)F->shi, i, j, F->off);
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
@@ -895,19 +895,19 @@ void xline_batch_shape(int nx, int &C, int &threads) {
 
 template <typename T, typename G>
 static void residual_restrict_y(hipStream_t s, T *rc, T *ec, const T *e, const T *f, const T *K, const Grid &gf,
-                                const Grid &gc, const G &geo) {
+                                const Grid &gc, const G &geo, int off) {
     const dim3 grid((unsigned)((gc.nx + 63) / 64), (unsigned)((gc.ny + 3) / 4));
-    k_residual_restrict_y<<<grid, 256, 0, s>>>(rc, ec, e, f, K, gf, gc, geo);
+    k_residual_restrict_y<<<grid, 256, 0, s>>>(rc, ec, e, f, K, gf, gc, geo, off);
 }
 
 void launch_residual_restrict_y(hipStream_t s, double *rc, double *ec, const double *e, const double *f,
-                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo) {
-    residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo);
+                                const double *K, const Grid &gf, const Grid &gc, const Geo &geo, int off) {
+    residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo, off);
 }
 
 void launch_residual_restrict_y(hipStream_t s, float *rc, float *ec, const float *e, const float *f, const float *K,
-                                const Grid &gf, const Grid &gc, const GeoF &geo) {
-    residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo);
+                                const Grid &gf, const Grid &gc, const GeoF &geo, int off) {
+    residual_restrict_y(s, rc, ec, e, f, K, gf, gc, geo, off);
 }
 
 // One launch of the line kernel for rows (SEG = false) or segments of seglen cells.
