@@ -748,6 +748,7 @@ def main():
     ap.add_argument("--batch-n", type=int, default=1024, help="problems in the batched mix")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 one-at-a-time comparison")
+    ap.add_argument("--no-batched-fp64", action="store_true", help="skip the batched mix with FP64 cycles")
     ap.add_argument("--no-single", action="store_true", help="skip the 2048^2 single-solve section")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving-loop measurement")
@@ -839,9 +840,15 @@ def main():
     b32.close()
 
     # the same mix with FP64 cycles
-    b64 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG, use_graph, tuning=tuning)
-    dt64, iters64, _ = timed_batch(capi, b64, args.tol, max(1, min(args.steps, 2)), 1, "batched fp64")
-    b64.close()
+    batched_fp64 = None
+    if not args.no_batched_fp64:
+        b64 = make_batch(capi, geos, bcs, maps, rhs, capi.METHOD_KRYLOV_MG, use_graph, tuning=tuning)
+        dt64, iters64, _ = timed_batch(capi, b64, args.tol, max(1, min(args.steps, 2)), 1, "batched fp64")
+        b64.close()
+        batched_fp64 = {
+            "solves_per_s": len(probs) * max(1, min(args.steps, 2)) / dt64,
+            "v_cycles_mean": 2 * float(np.mean(iters64)),
+        }
 
     plain = None
     if not args.no_plain:
@@ -894,10 +901,7 @@ def main():
         },
         "plain_fp64": plain,
         "mixed_batched_over_plain_fp64": (value / plain["solves_per_s"]) if plain else None,
-        "batched_fp64": {
-            "solves_per_s": len(probs) * max(1, min(args.steps, 2)) / dt64,
-            "v_cycles_mean": 2 * float(np.mean(iters64)),
-        },
+        "batched_fp64": batched_fp64,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_xline_batch_* FP32 (finest-level line relaxation, even rows, of all 1024 problems: "

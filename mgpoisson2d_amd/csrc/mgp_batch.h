@@ -203,13 +203,19 @@ class BatchKMG {
         std::vector<XlTable> xl;     // x-line task tables per (level, set, colour, chunk class)
         std::vector<Table> tr, tp;   // per level: restriction (coarse tiles) / prolongation (fine tiles)
         std::vector<int> bottom;     // per level: most coarsest-level sweeps of a problem ending there
-        std::vector<void *> owned;   // device allocations of the tables
         hipGraphExec_t graph = nullptr;
         int nprob = 0, depth = 0;
+        // every table in one device buffer, staged in one pinned host buffer;
+        // both are kept (and grown) across rebuilds, so that compacting the
+        // set allocates nothing once it has been built at its largest
+        void *dev = nullptr;
+        unsigned char *host = nullptr;
+        size_t dcap = 0, hcap = 0, used = 0;
     };
-    Table upload(TableSet &ts, const void *host, size_t bytes, int n);
+    Table stage(TableSet &ts, const void *host, size_t bytes, int n);
     void build_tables(TableSet &ts, const std::vector<int> &which);
-    void release(TableSet &ts);
+    void release(TableSet &ts);       // the graph and the tables' contents (waits for the stream)
+    void free_buffers(TableSet &ts);  // and the buffers
     TableSet full;               // every problem (built once)
     TableSet part;               // the problems still active, once they are at most half of `full`
     TableSet *T = &full;         // the set the launches use

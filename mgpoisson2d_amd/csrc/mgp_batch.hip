@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <string>
 #include <thread>
@@ -182,13 +183,29 @@ void tiles_of(std::vector<BTile> &out, int item, int nx, int ny) {
 
 }  // namespace
 
-BatchKMG::Table BatchKMG::upload(TableSet &ts, const void *host, size_t bytes, int n) {
+// Appends a table to the set's pinned staging at a 256-byte boundary; until
+// build_tables uploads the set, the table's p holds its offset there.
+BatchKMG::Table BatchKMG::stage(TableSet &ts, const void *host, size_t bytes, int n) {
     Table t;
     t.n = n;
     if (n == 0) return t;
-    t.p = dev_alloc_zero(bytes);
-    ts.owned.push_back(t.p);
-    hip_check(hipMemcpyAsync(t.p, host, bytes, hipMemcpyHostToDevice, s), "table upload");
+    const size_t off = round256(ts.used);
+    if (off + bytes > ts.hcap) {
+        const size_t cap = std::max(off + bytes, 2 * ts.hcap);
+        unsigned char *h = nullptr;
+        if (hipHostMalloc((void **)&h, cap, hipHostMallocDefault) != hipSuccess || !h) {
+            (void)hipGetLastError();
+            g_last_error = "batch: pinned staging of " + std::to_string(cap) + " bytes for the launch tables";
+            throw Failure(MGP_ERR_NOMEM);
+        }
+        if (ts.used) memcpy(h, ts.host, ts.used);
+        if (ts.host) (void)hipHostFree(ts.host);
+        ts.host = h;
+        ts.hcap = cap;
+    }
+    memcpy(ts.host + off, host, bytes);
+    ts.used = off + bytes;
+    t.p = reinterpret_cast<void *>(off);
     return t;
 }
 
@@ -450,8 +467,9 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
         tiles_of(t0h, p, P[p].g.nx, P[p].g.ny);
         for (int b = 0; b < P[p].nb; ++b) tredh.push_back(BTile{p, b, 0, 0});
     }
-    ts.t0 = upload(ts, t0h.data(), t0h.size() * sizeof(BTile), (int)t0h.size());
-    ts.tred = upload(ts, tredh.data(), tredh.size() * sizeof(BTile), (int)tredh.size());
+    ts.used = 0;
+    ts.t0 = stage(ts, t0h.data(), t0h.size() * sizeof(BTile), (int)t0h.size());
+    ts.tred = stage(ts, tredh.data(), tredh.size() * sizeof(BTile), (int)tredh.size());
     ts.tr.assign(ts.depth, Table());
     ts.tp.assign(ts.depth, Table());
     ts.bottom.assign(ts.depth, 0);
@@ -466,26 +484,35 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
             const int set = d == nlev[p] - 1 ? SET_BOTTOM : SET_MID;
             const int C = Cp[p], NT = NTp[p];
             if (set == SET_BOTTOM) ts.bottom[d] = std::max(ts.bottom[d], L[lev].sweeps);
-            auto add = [&](int colour, XlTask t) {
+            const size_t need = xline_lds_bytes_for(nx, C, mixed ? sizeof(float) : sizeof(double), NT);
+            auto table = [&](int colour) -> std::vector<XlTask> & {
                 const auto key = std::make_tuple(set, colour, C, NT);
-                tabs[key].push_back(t);
-                lds[key] = std::max(lds[key], xline_lds_bytes_for(nx, C, mixed ? sizeof(float) : sizeof(double), NT));
+                size_t &b = lds[key];
+                b = std::max(b, need);
+                return tabs[key];
             };
             // the order of xline_zebra: even interior rows, odd interior rows
             // (the first and last relaxing rows 0 and ny-1 after their own),
             // else rows 0 and ny-1 in a launch of their own
             const int n_even = ny >= 4 ? (ny - 4) / 2 + 1 : 0;
             const int n_odd = ny >= 3 ? (ny - 3) / 2 + 1 : 0;
-            for (int k = 0; k < n_even; ++k) add(0, XlTask{lev, 2 + 2 * k, -1, 0});
-            for (int k = 0; k < n_odd; ++k) {
-                int extra = -1;
-                if (n_odd >= 2 && k == 0) extra = 0;
-                if (n_odd >= 2 && k == n_odd - 1) extra = ny - 1;
-                add(1, XlTask{lev, 1 + 2 * k, extra, 0});
+            if (n_even) {
+                std::vector<XlTask> &v = table(0);
+                for (int k = 0; k < n_even; ++k) v.push_back(XlTask{lev, 2 + 2 * k, -1, 0});
+            }
+            if (n_odd) {
+                std::vector<XlTask> &v = table(1);
+                for (int k = 0; k < n_odd; ++k) {
+                    int extra = -1;
+                    if (n_odd >= 2 && k == 0) extra = 0;
+                    if (n_odd >= 2 && k == n_odd - 1) extra = ny - 1;
+                    v.push_back(XlTask{lev, 1 + 2 * k, extra, 0});
+                }
             }
             if (n_odd < 2) {
-                add(2, XlTask{lev, 0, -1, 0});
-                add(2, XlTask{lev, ny - 1, -1, 0});
+                std::vector<XlTask> &v = table(2);
+                v.push_back(XlTask{lev, 0, -1, 0});
+                v.push_back(XlTask{lev, ny - 1, -1, 0});
             }
             if (set == SET_MID) {
                 tiles_of(trh, lev, nx, L[lev + 1].g.ny);
@@ -497,15 +524,32 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
             x.d = d;
             x.set = std::get<0>(kv.first);
             x.colour = std::get<1>(kv.first);
-            x.t = upload(ts, kv.second.data(), kv.second.size() * sizeof(XlTask), (int)kv.second.size());
+            x.t = stage(ts, kv.second.data(), kv.second.size() * sizeof(XlTask), (int)kv.second.size());
             x.t.C = std::get<2>(kv.first);
             x.t.threads = std::get<3>(kv.first);
             x.t.lds = lds[kv.first];
             ts.xl.push_back(x);
         }
-        ts.tr[d] = upload(ts, trh.data(), trh.size() * sizeof(BTile), (int)trh.size());
-        ts.tp[d] = upload(ts, tph.data(), tph.size() * sizeof(BTile), (int)tph.size());
+        ts.tr[d] = stage(ts, trh.data(), trh.size() * sizeof(BTile), (int)trh.size());
+        ts.tp[d] = stage(ts, tph.data(), tph.size() * sizeof(BTile), (int)tph.size());
     }
+    // one upload of the staged set; the offsets become device addresses
+    if (ts.used > ts.dcap) {
+        if (ts.dev) dev_free(ts.dev);
+        ts.dev = nullptr;
+        ts.dcap = 0;
+        ts.dev = dev_alloc_zero(ts.used);
+        ts.dcap = ts.used;
+    }
+    auto place = [&](Table &t) {
+        if (t.n) t.p = static_cast<unsigned char *>(ts.dev) + reinterpret_cast<size_t>(t.p);
+    };
+    place(ts.t0);
+    place(ts.tred);
+    for (XlTable &x : ts.xl) place(x.t);
+    for (Table &t : ts.tr) place(t);
+    for (Table &t : ts.tp) place(t);
+    if (ts.used) hip_check(hipMemcpyAsync(ts.dev, ts.host, ts.used, hipMemcpyHostToDevice, s), "table upload");
     std::sort(ts.xl.begin(), ts.xl.end(), [](const XlTable &a, const XlTable &b) {
         return std::make_tuple(a.d, a.set, a.colour, a.t.C, a.t.threads) <
                std::make_tuple(b.d, b.set, b.colour, b.t.C, b.t.threads);
@@ -534,13 +578,28 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
 void BatchKMG::release(TableSet &ts) {
     (void)hipStreamSynchronize(s);
     if (ts.graph) (void)hipGraphExecDestroy(ts.graph);
-    for (void *p : ts.owned) dev_free(p);
-    ts = TableSet();
+    ts.graph = nullptr;
+    ts.t0 = ts.tred = Table();
+    ts.xl.clear();
+    ts.tr.clear();
+    ts.tp.clear();
+    ts.bottom.clear();
+    ts.nprob = ts.depth = 0;
+    ts.used = 0;
+}
+
+void BatchKMG::free_buffers(TableSet &ts) {
+    release(ts);
+    if (ts.dev) dev_free(ts.dev);
+    if (ts.host) (void)hipHostFree(ts.host);
+    ts.dev = nullptr;
+    ts.host = nullptr;
+    ts.dcap = ts.hcap = 0;
 }
 
 BatchKMG::~BatchKMG() {
-    release(part);
-    release(full);
+    free_buffers(part);
+    free_buffers(full);
     for (hipStream_t st : aux) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
@@ -1038,18 +1097,26 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
     std::vector<int> all(n);
     for (int p = 0; p < n; ++p) all[p] = p;
     start(all, std::vector<int>(n, zerophi ? 0 : 1), std::vector<double>(n, tol), std::vector<int>(n, maxiters));
+    // Finished problems' workgroups return at once, but each still costs its
+    // dispatch: about 7 us per finished problem per iteration on MI355X (a
+    // 1024-problem table set with one problem running takes 8.8 ms per
+    // iteration, the same problem in a set of its own 1.2 ms).  Rebuilding the
+    // tables over the running problems (and capturing their graph) costs
+    // about 4 ms + 60 us per problem.  The tables are rebuilt once the
+    // dispatches wasted since the last build have cost about one rebuild.
+    constexpr double DEAD_US = 7.0, REBUILD_US = 4000.0, REBUILD_PER_PROBLEM_US = 60.0;
+    double wasted = 0.0;
     while (running() > 0) {
         step();
-        // once at most half of the problems in the tables are still running,
-        // launch over those alone (the finished ones' workgroups would only
-        // return, but every level launch would still cost its latency)
         std::vector<int> live;
         for (int p = 0; p < n; ++p)
             if (act_[p]) live.push_back(p);
-        if (!live.empty() && 2 * (int)live.size() <= T->nprob) {
+        wasted += DEAD_US * (T->nprob - (int)live.size());
+        if (!live.empty() && wasted >= REBUILD_US + REBUILD_PER_PROBLEM_US * (double)live.size()) {
             release(part);
             build_tables(part, live);
             T = &part;
+            wasted = 0.0;
         }
     }
     hip_check(hipStreamSynchronize(s), "sync");

@@ -52,8 +52,8 @@ def test_krylov_mg_reaches_tolerance(case, method):
 
 @pytest.mark.parametrize("shape", [(48, 45), (40, 90), (36, 4)])
 def test_krylov_mg_odd_row_counts(shape):
-    """Row counts that stop the y-coarsening early (45: one level; 90: 45 rows
-    on the coarsest; 4: the smallest hierarchy) still reach the tolerance."""
+    """Odd row counts (45 and 90 -> 45: the Neumann y sides let the coarsening
+    drop a row and go on; 4: the smallest hierarchy) reach the tolerance."""
     nx, ny = shape
     geo, bc, _, rhs = setup(nx, ny, 0.1, 4.0, (1, 1, 0, 0), seed=17)
     s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
@@ -276,3 +276,25 @@ def test_projection_fast_solver():
     # Neumann faces keep the flux (the correction's boundary fluxes are zero)
     np.testing.assert_array_equal(gxf[:, [0, -1]], xf[:, [0, -1]])
     np.testing.assert_array_equal(gyf[[0, -1], :], yf[[0, -1], :])
+
+
+@pytest.mark.parametrize("types", [(1, 1, 0, 0), (1, 1, 1, 0), (1, 1, 0, 1), (0, 0, 0, 0)])
+@pytest.mark.parametrize("method", [mg.METHOD_KRYLOV_MG, mg.METHOD_KRYLOV_MG32])
+def test_odd_row_counts_coarsen_past_a_neumann_side(types, method):
+    """ny = 16 * 49: the semicoarsening reaches 49 rows and, with a Neumann y
+    side, leaves that side's row out (the top row when both y sides are
+    Neumann or only the top one is; the bottom row for (1, 1, 0, 1)) and goes
+    on to 24, 12 and 6 rows.  The solve reaches the reference operator's
+    residual in a bounded number of iterations at the strong stretching of
+    the batched mix's slowest problems: 21 to 53 here, where a hierarchy that
+    stops at 49 rows took 105 for (1, 1, 0, 0)."""
+    nx, ny = 800, 784
+    geo, bc, _, rhs = setup(nx, ny, 0.06, 5.9, types, seed=31)
+    s = mg.Solver(geo, bc, method=method)
+    s.set_rhs(rhs)
+    hist, iters, oc = s.solve(1e-10, 400, zerophi=True)
+    print(types, method, "iterations", iters)
+    assert oc == mg.CONVERGED, (iters, hist[-3:])
+    assert relres_oracle(geo, bc, rhs, s.get_phi()) <= 1.5e-10
+    assert iters <= 60
+    s.close()
