@@ -400,7 +400,7 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
             l.shi = B.bc.type[YHI] == BC_DIRI ? -1.0 : 1.0;
             l.prob = p;
             l.nt = 0;  // set with the chunk class below
-            l.sweeps = d == nlev[p] - 1 ? coarsest_sweeps(sb, ny) : 0;
+            l.sweeps = d == nlev[p] - 1 ? coarsest_sweeps(sb, ny, nlev[p] == 1, true) : 0;
             l.off = 0;
             const int nyc = semicoarsen_step(ny, MIN_COARSE_NY, B.bc.type[YLO] == BC_NEUM, B.bc.type[YHI] == BC_NEUM,
                                              l.off);

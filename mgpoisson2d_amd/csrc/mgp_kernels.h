@@ -154,14 +154,26 @@ inline int semicoarsen_step(int ny, int min_rows, bool neum_lo, bool neum_hi, in
     return (ny - 1) / 2;
 }
 // Sweeps on the coarsest level of the semicoarsening hierarchy: at least
-// `sb`, and half the level's rows up to 64.  The hierarchy stops at an odd
-// row count, and an x-line zebra sweep reduces errors that are smooth in y
-// like one Gauss-Seidel sweep in y, so a coarsest level of many rows needs
-// more of them (ny = 16 k with k odd leaves k rows).  The cap bounds the cost
-// of a cycle where the count stays large (an odd ny keeps every row).
-inline int coarsest_sweeps(int sb, int ny) {
+// `sb`.  An x-line zebra sweep reduces errors that are smooth in y like one
+// Gauss-Seidel sweep in y, so a coarsest level of many rows needs more of
+// them (ny = 16 k with k odd and Dirichlet y sides leaves k rows), and so
+// does a strongly stretched grid with Neumann y sides, whose lowest mode is
+// nearly constant in y (208 x 448 at beta = 6, unwarped: 30 BiCGStab
+// iterations with 3 sweeps on its 7 coarsest rows, 19 with 8, 7 with 128).
+//  * The single-problem solver: half the level's rows, up to 64.  Its
+//    sweeps are latency-bound launches: 8 on the 4 coarsest rows of the
+//    2048^2 bench problem saved 4 of 70 V-cycles but cost 0.23 ms of a
+//    2.2 ms cycle.
+//  * The batch (batched = true): 8 below a finer level.  Its coarsest
+//    sweeps are one chain of launches for all problems; the batched mix ran
+//    5 % faster with 8 than with half the rows up to 64, and 4 % slower
+//    with at least 12.
+// A hierarchy of one level (an odd row count from the start, at Dirichlet y
+// sides) solves on its only level: half its rows, up to 64, in both.
+inline int coarsest_sweeps(int sb, int ny, bool only_level, bool batched) {
     const int half = ny / 2 < 64 ? ny / 2 : 64;
-    return sb > half ? sb : half;
+    const int want = batched && !only_level ? 8 : half;
+    return sb > want ? sb : want;
 }
 // Shape of the batched line solve of an nx-cell row: threads per row (64 up
 // to 512 cells, 128 up to 1024, else 256) and chunk length C (2, 4, 8 or 16;

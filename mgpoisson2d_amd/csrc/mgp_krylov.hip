@@ -340,7 +340,7 @@ template <typename T>
 static void vcycle_t(hipStream_t s, KrylovMG::Impl &K, size_t d, T *e, const T *f) {
     KLevel &l = K.L[d];
     if (d + 1 == K.L.size()) {
-        for (int k = 0; k < coarsest_sweeps(K.sb, l.g.ny); ++k) zebra_sweep(s, l, e, f);
+        for (int k = 0; k < coarsest_sweeps(K.sb, l.g.ny, d == 0, false); ++k) zebra_sweep(s, l, e, f);
         return;
     }
     KLevel &c = K.L[d + 1];

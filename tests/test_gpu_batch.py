@@ -36,7 +36,9 @@ CASES = [
     (200, 16, 0.10, 3.0, (1, 1, 0, 0)),
     (40, 24, 0.15, 6.0, (1, 1, 1, 1)),
     (576, 32, 0.05, 1.0, (1, 1, 0, 0)),  # rows of 513-1024 cells (128 threads) in the same batch
-    (48, 90, 0.10, 5.0, (1, 1, 0, 0)),   # 45 rows on the coarsest level (its own sweep count)
+    (48, 90, 0.10, 5.0, (1, 1, 1, 1)),   # 45 rows on the coarsest level (Dirichlet y sides stop there)
+    (40, 45, 0.10, 3.0, (1, 1, 1, 1)),   # one level: its own sweep count (22)
+    (48, 90, 0.10, 5.0, (1, 1, 0, 0)),   # Neumann y sides: 90 -> 45 -> 22 -> 11 -> 5 rows
 ]
 
 
@@ -94,7 +96,7 @@ def test_batch_matches_direct_solve():
 def test_batch_alone_equals_in_mix(method):
     probs = problems(CASES)
     phis, iters, outc, _ = run_batch(probs, method=method)
-    for k in (0, 2, 5, 7):
+    for k in (0, 2, 5, 7, 9, 10):
         alone, it1, oc1, _ = run_batch([probs[k]], method=method)
         assert (int(it1[0]), int(oc1[0])) == (int(iters[k]), int(outc[k]))
         np.testing.assert_array_equal(alone[0], phis[k])
