@@ -222,8 +222,8 @@ __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
 // RG cells' loads are issued together before their stencils are evaluated,
 // so RG memory round trips overlap instead of following one another (one
 // cell at a time, a coarse-level row pass took 5.4 us of a 10.6 us row
-// solve).  The batched kernels keep RG = 1: the registers of more would cut
-// the workgroups per CU they rely on.
+// solve).  The batched kernels take the RG of batch_rg: no more than fits
+// their VGPR budget, since they rely on many workgroups per CU.
 template <int C, int NT, bool SEG = false, int RG = 1, typename T, typename GM>
 __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const T *f_, const T *K_, const Grid &g,
                                          const GM &geo_, int j, T *Ld, T *La, T *Lb, T *Lc) {
@@ -753,6 +753,16 @@ __device__ __forceinline__ void level_fields(const BLev *L, int, float *&e, cons
     geo = L->geo32;
 }
 
+// Cells whose loads the batched kernels' row pass issues together: as many
+// as fit without costing a wave per SIMD (FP32: 2 of the 3 cells of a chunk
+// of 2, 3 of 5, 2 of 9; the C = 4 kernel keeps 4 waves with 119 VGPRs, the
+// others their 87 and 154-159).  The batched mix's finest FP32 launch set
+// went from 0.70 to 0.67 ms, bit-identical.  FP64 keeps one cell at a time.
+template <int C, typename T>
+constexpr int batch_rg() {
+    return sizeof(T) == 8 ? 1 : (C == 4 ? 3 : 2);
+}
+
 template <int C, int NT, typename T>
 __device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs, const int *active, int sel, int k) {
     const XlTask t = tasks[launch_slot()];
@@ -773,7 +783,7 @@ __device__ __forceinline__ void xline_task(const XlTask *tasks, const BLev *levs
         if (r) __syncthreads();
         RowCtx<T> X = X0;
         asm volatile("" : "+s"(X.rem));
-        xline_row<C, NT>(X, e, f, K, g, geo, jr, lds);
+        xline_row<C, NT, false, batch_rg<C, T>()>(X, e, f, K, g, geo, jr, lds);
     }
 }
 
