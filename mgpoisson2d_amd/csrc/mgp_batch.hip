@@ -73,9 +73,35 @@ __global__ void __launch_bounds__(256) k_b_reduce_partial(const BTile *tiles, co
     const double *A = p->v[a], *B = p->v[b];
     const int nb = p->nb;
     Acc acc{0.0, 0.0};
-    for (int j = t.tx; j < g.ny; j += nb) {
-        const size_t row = ix(g.pitch, 0, j);
-        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) acc.add(vol * (A[row + i] * B[row + i]));
+    // The thread's cells, in the order of the sum: rows tx, tx + nb, .. and in
+    // each row cells tid, tid + 256, .. (M of them).  Their loads go out Q at
+    // a time ahead of the adds, so that Q pairs are in flight per thread; the
+    // sum keeps that order.
+    constexpr int Q = 8;
+    const int tid = threadIdx.x;
+    const int M = tid < g.nx ? (g.nx - 1 - tid) / 256 + 1 : 0;
+    const int R = t.tx < g.ny ? (g.ny - 1 - t.tx) / nb + 1 : 0;
+    const int total = M * R;
+    int k = 0, m = 0;  // row and cell index of the next load
+    for (int e0 = 0; e0 < total; e0 += Q) {
+        double va[Q], vb[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            va[q] = 0.0;
+            vb[q] = 0.0;
+            if (e0 + q < total) {
+                const size_t c = ix(g.pitch, tid + 256 * m, t.tx + nb * k);
+                va[q] = A[c];
+                vb[q] = B[c];
+                if (++m == M) {
+                    m = 0;
+                    ++k;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (e0 + q < total) acc.add(vol * (va[q] * vb[q]));
     }
     const double v = block_combine(ss, sc, acc, 0.0, 0);
     if (threadIdx.x == 0) p->partial[t.tx] = v;
