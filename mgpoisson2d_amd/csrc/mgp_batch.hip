@@ -35,10 +35,18 @@ namespace {
 
 constexpr int MIN_COARSE_NY = 4;
 
-__device__ __forceinline__ bool tile_cell(const BTile &t, const Grid &g, int &i, int &j) {
-    i = t.tx * 64 + (threadIdx.x & 63);
-    j = t.ty * 4 + (threadIdx.x >> 6);
-    return i < g.nx && j < g.ny;
+// The cells of a finest-level tile: 64 x 16, each thread taking four rows 4
+// apart, so that a thread's four cells' loads go out together.
+constexpr int T0_ROWS = 16;
+template <typename F>
+__device__ __forceinline__ void tile_cells(const BTile &t, const Grid &g, F &&f) {
+    const int i = t.tx * 64 + (threadIdx.x & 63);
+    if (i >= g.nx) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = t.ty * T0_ROWS + (threadIdx.x >> 6) + 4 * r;
+        if (j < g.ny) f(i, j);
+    }
 }
 
 // v[out] = (v[rhs] -) A v[in], A the reference operator with the problem's
@@ -49,16 +57,18 @@ __global__ void __launch_bounds__(256) k_b_apply(const BTile *tiles, const BProb
     if (!mask[t.item]) return;
     const BProb *p = P + t.item;
     const Grid g = p->g;
-    int i, j;
-    if (!tile_cell(t, g, i, j)) return;
     const Geo geo = p->geo;
     const Bc bc = p->bc;
     const double *phi = p->v[in];
-    const size_t c = ix(g.pitch, i, j);
-    const bool interior = i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
-    const double v = interior ? cell_lap_interior(CFld{phi, g.pitch}, g, geo, i, j)
-                              : cell_lap(synth(phi, g, bc), g, geo, bc, 1, i, j);
-    p->v[out][c] = rhs >= 0 ? p->v[rhs][c] - v : v;
+    double *o = p->v[out];
+    const double *r = rhs >= 0 ? p->v[rhs] : nullptr;
+    tile_cells(t, g, [&](int i, int j) {
+        const size_t c = ix(g.pitch, i, j);
+        const bool interior = i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+        const double v = interior ? cell_lap_interior(CFld{phi, g.pitch}, g, geo, i, j)
+                                  : cell_lap(synth(phi, g, bc), g, geo, bc, 1, i, j);
+        o[c] = r ? r[c] - v : v;
+    });
 }
 
 // Partial sums of vol * v[a] * v[b] over the rows b, b + nb, ... of a problem.
@@ -146,18 +156,19 @@ __global__ void __launch_bounds__(256) k_b_update(const BTile *tiles, const BPro
     if (!mask[t.item]) return;
     const BProb *p = P + t.item;
     const Grid g = p->g;
-    int i, j;
-    if (!tile_cell(t, g, i, j)) return;
-    const size_t c = ix(g.pitch, i, j);
     const double *sc = p->sc;
     double *const *v = p->v;
-    switch (op) {
-    case 0: v[BV_P][c] = v[BV_R][c] + sc[SC_BETA] * (v[BV_P][c] - sc[SC_OMEGA] * v[BV_V][c]); break;
-    case 1: v[BV_S][c] = v[BV_R][c] - sc[SC_ALPHA] * v[BV_V][c]; break;
-    case 2: v[BV_PHI][c] = v[BV_PHI][c] + sc[SC_ALPHA] * v[BV_PHAT][c]; break;
-    case 3: v[BV_PHI][c] = v[BV_PHI][c] + sc[SC_OMEGA] * v[BV_SHAT][c]; break;
-    default: v[BV_R][c] = v[BV_S][c] - sc[SC_OMEGA] * v[BV_T][c]; break;
-    }
+    const double alpha = sc[SC_ALPHA], beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    tile_cells(t, g, [&](int i, int j) {
+        const size_t c = ix(g.pitch, i, j);
+        switch (op) {
+        case 0: v[BV_P][c] = v[BV_R][c] + beta * (v[BV_P][c] - omega * v[BV_V][c]); break;
+        case 1: v[BV_S][c] = v[BV_R][c] - alpha * v[BV_V][c]; break;
+        case 2: v[BV_PHI][c] = v[BV_PHI][c] + alpha * v[BV_PHAT][c]; break;
+        case 3: v[BV_PHI][c] = v[BV_PHI][c] + omega * v[BV_SHAT][c]; break;
+        default: v[BV_R][c] = v[BV_S][c] - omega * v[BV_T][c]; break;
+        }
+    });
 }
 
 // v[dst] = v[src] (src >= 0) or 0 over the valid cells.
@@ -166,10 +177,12 @@ __global__ void __launch_bounds__(256) k_b_copy(const BTile *tiles, const BProb 
     if (!mask[t.item]) return;
     const BProb *p = P + t.item;
     const Grid g = p->g;
-    int i, j;
-    if (!tile_cell(t, g, i, j)) return;
-    const size_t c = ix(g.pitch, i, j);
-    p->v[dst][c] = src >= 0 ? p->v[src][c] : 0.0;
+    double *d = p->v[dst];
+    const double *sv = src >= 0 ? p->v[src] : nullptr;
+    tile_cells(t, g, [&](int i, int j) {
+        const size_t c = ix(g.pitch, i, j);
+        d[c] = sv ? sv[c] : 0.0;
+    });
 }
 
 // Start of an FP32 cycle: the finest level's f32 = (float) v[P or S], e32 = 0.
@@ -179,12 +192,14 @@ __global__ void __launch_bounds__(256) k_b_begin32(const BTile *tiles, const BPr
     if (!mask[t.item]) return;
     const BProb *p = P + t.item;
     const Grid g = p->g;
-    int i, j;
-    if (!tile_cell(t, g, i, j)) return;
-    const size_t c = ix(g.pitch, i, j);
     const BLev *l = L + p->lev0;
-    l->f32[c] = (float)p->v[sel ? BV_S : BV_P][c];
-    l->e32[c] = 0.0f;
+    float *f32 = l->f32, *e32 = l->e32;
+    const double *src = p->v[sel ? BV_S : BV_P];
+    tile_cells(t, g, [&](int i, int j) {
+        const size_t c = ix(g.pitch, i, j);
+        f32[c] = (float)src[c];
+        e32[c] = 0.0f;
+    });
 }
 
 // End of an FP32 cycle: v[PHAT or SHAT] = (double) e32 of the finest level.
@@ -194,16 +209,18 @@ __global__ void __launch_bounds__(256) k_b_end32(const BTile *tiles, const BProb
     if (!mask[t.item]) return;
     const BProb *p = P + t.item;
     const Grid g = p->g;
-    int i, j;
-    if (!tile_cell(t, g, i, j)) return;
-    const size_t c = ix(g.pitch, i, j);
-    p->v[sel ? BV_SHAT : BV_PHAT][c] = (double)L[p->lev0].e32[c];
+    const float *e32 = L[p->lev0].e32;
+    double *dst = p->v[sel ? BV_SHAT : BV_PHAT];
+    tile_cells(t, g, [&](int i, int j) {
+        const size_t c = ix(g.pitch, i, j);
+        dst[c] = (double)e32[c];
+    });
 }
 
 size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
-void tiles_of(std::vector<BTile> &out, int item, int nx, int ny) {
-    for (int ty = 0; ty < (ny + 3) / 4; ++ty)
+void tiles_of(std::vector<BTile> &out, int item, int nx, int ny, int rows = 4) {
+    for (int ty = 0; ty < (ny + rows - 1) / rows; ++ty)
         for (int tx = 0; tx < (nx + 63) / 64; ++tx) out.push_back(BTile{item, tx, ty, 0});
 }
 
@@ -490,7 +507,7 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
     for (int p : which) ts.depth = std::max(ts.depth, nlev[p]);
     std::vector<BTile> t0h, tredh;
     for (int p : which) {
-        tiles_of(t0h, p, P[p].g.nx, P[p].g.ny);
+        tiles_of(t0h, p, P[p].g.nx, P[p].g.ny, T0_ROWS);
         for (int b = 0; b < P[p].nb; ++b) tredh.push_back(BTile{p, b, 0, 0});
     }
     ts.used = 0;
