@@ -58,9 +58,11 @@ struct XlTask {
     int lev, row, extra, pad;
 };
 
-// Cell tile: cells (64 tx .. 64 tx + 63, 4 ty .. 4 ty + 3) of item `item`
-// (a problem or a level, as the kernel says); for reductions, tx is the row
-// block.
+// Cell tile: cells (64 tx .. 64 tx + 63, BT_ROWS ty .. BT_ROWS ty + 15) of
+// item `item` (a problem or a level, as the kernel says), each thread of a
+// 256-thread workgroup taking four rows 4 apart; for reductions, tx is the
+// row block.
+constexpr int BT_ROWS = 16;
 struct BTile {
     int item, tx, ty, pad;
 };

@@ -35,16 +35,15 @@ namespace {
 
 constexpr int MIN_COARSE_NY = 4;
 
-// The cells of a finest-level tile: 64 x 16, each thread taking four rows 4
-// apart, so that a thread's four cells' loads go out together.
-constexpr int T0_ROWS = 16;
+// The cells of a tile (64 x BT_ROWS), each thread taking four rows 4 apart,
+// so that a thread's four cells' loads go out together.
 template <typename F>
 __device__ __forceinline__ void tile_cells(const BTile &t, const Grid &g, F &&f) {
     const int i = t.tx * 64 + (threadIdx.x & 63);
     if (i >= g.nx) return;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int j = t.ty * T0_ROWS + (threadIdx.x >> 6) + 4 * r;
+        const int j = t.ty * BT_ROWS + (threadIdx.x >> 6) + 4 * r;
         if (j < g.ny) f(i, j);
     }
 }
@@ -219,8 +218,8 @@ __global__ void __launch_bounds__(256) k_b_end32(const BTile *tiles, const BProb
 
 size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
-void tiles_of(std::vector<BTile> &out, int item, int nx, int ny, int rows = 4) {
-    for (int ty = 0; ty < (ny + rows - 1) / rows; ++ty)
+void tiles_of(std::vector<BTile> &out, int item, int nx, int ny) {
+    for (int ty = 0; ty < (ny + BT_ROWS - 1) / BT_ROWS; ++ty)
         for (int tx = 0; tx < (nx + 63) / 64; ++tx) out.push_back(BTile{item, tx, ty, 0});
 }
 
@@ -507,7 +506,7 @@ void BatchKMG::build_tables(TableSet &ts, const std::vector<int> &which) {
     for (int p : which) ts.depth = std::max(ts.depth, nlev[p]);
     std::vector<BTile> t0h, tredh;
     for (int p : which) {
-        tiles_of(t0h, p, P[p].g.nx, P[p].g.ny, T0_ROWS);
+        tiles_of(t0h, p, P[p].g.nx, P[p].g.ny);
         for (int b = 0; b < P[p].nb; ++b) tredh.push_back(BTile{p, b, 0, 0});
     }
     ts.used = 0;

@@ -819,16 +819,20 @@ __global__ void __launch_bounds__(256) k_restrict_batch(const BTile *tiles, cons
     const BTile t = tiles[blockIdx.x];
     const BLev *F = levs + t.item, *Cl = levs + t.item + 1;
     if (!active[F->prob]) return;
-    const int i = t.tx * 64 + (threadIdx.x & 63), J = t.ty * 4 + (threadIdx.x >> 6);
+    const int i = t.tx * 64 + (threadIdx.x & 63);
     const Grid gc = Cl->g;
-    if (i >= gc.nx || J >= gc.ny) return;
+    if (i >= gc.nx) return;
     T *fe, *ce, *cf;
     const T *ff, *K, *cff, *cK;
     typename std::conditional<sizeof(T) == 8, Geo, GeoF>::type geo, cgeo;
     level_fields(F, sel, fe, ff, K, geo);
     level_fields(Cl, 0, ce, cff, cK, cgeo);
     cf = const_cast<T *>(cff);
-    restrict_cell(cf, ce, (const T *)fe, ff, K, F->g, gc, geo, i, J, F->off);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int J = t.ty * BT_ROWS + (threadIdx.x >> 6) + 4 * r;
+        if (J < gc.ny) restrict_cell(cf, ce, (const T *)fe, ff, K, F->g, gc, geo, i, J, F->off);
+    }
 }
 
 template <typename T>
@@ -837,15 +841,19 @@ __global__ void __launch_bounds__(256) k_prolong_batch(const BTile *tiles, const
     const BTile t = tiles[blockIdx.x];
     const BLev *F = levs + t.item, *Cl = levs + t.item + 1;
     if (!active[F->prob]) return;
-    const int i = t.tx * 64 + (threadIdx.x & 63), j = t.ty * 4 + (threadIdx.x >> 6);
+    const int i = t.tx * 64 + (threadIdx.x & 63);
     const Grid gf = F->g, gc = Cl->g;
-    if (i >= gf.nx || j >= gf.ny) return;
+    if (i >= gf.nx) return;
     T *fe, *ce;
     const T *ff, *K, *cf, *cK;
     typename std::conditional<sizeof(T) == 8, Geo, GeoF>::type geo, cgeo;
     level_fields(F, sel, fe, ff, K, geo);
     level_fields(Cl, 0, ce, cf, cK, cgeo);
-    prolong_y_cell(fe, (const T *)ce, gf, gc, (T)F->slo, (T)F->shi, i, j, F->off);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = t.ty * BT_ROWS + (threadIdx.x >> 6) + 4 * r;
+        if (j < gf.ny) prolong_y_cell(fe, (const T *)ce, gf, gc, (T)F->slo, (T)F->shi, i, j, F->off);
+    }
 }
 
 // Chunk length for a row of nx cells: the smallest C in {2, 4, 8, 16} with
