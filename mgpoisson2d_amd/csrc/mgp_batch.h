@@ -47,7 +47,7 @@ struct BProb {
     Bc bc;             // the caller's BCs (homogeneous use)
     double *v[BV_COUNT];
     double *sc;        // SC_COUNT BiCGStab scalars
-    double *partial;   // nb partial sums of the reductions
+    double *partial;   // 2 nb partial sums of the reductions (two products)
     int nb;            // reduction row blocks: block b sums rows b, b + nb, ...
     int lev0;          // index of the finest level in the level table
 };
@@ -157,7 +157,8 @@ class BatchKMG {
     void body();
     void precondition(int sel);
     void zebra(int d, int set, int sel, int k);
-    void reduce(int a, int b, int slot, const int *m);
+    // sc[slot] = sum of vol v[a] v[b] (and sc[slot2] of vol v[a] v[c], c >= 0)
+    void reduce(int a, int b, int slot, const int *m, int c = -1, int slot2 = -1);
     void restart(const int *m);
     void true_residual(const int *m);
     void upload_flags(int *dst, const std::vector<int> &h);

@@ -70,18 +70,20 @@ __global__ void __launch_bounds__(256) k_b_apply(const BTile *tiles, const BProb
     });
 }
 
-// Partial sums of vol * v[a] * v[b] over the rows b, b + nb, ... of a problem.
+// Partial sums of vol * v[a] * v[b] over the rows b, b + nb, ... of a problem
+// into partial[b]; with c >= 0 also those of vol * v[a] * v[c] into
+// partial[nb + b], in the same order (so each equals its own pass's sum).
 __global__ void __launch_bounds__(256) k_b_reduce_partial(const BTile *tiles, const BProb *P, const int *mask, int a,
-                                                          int b) {
+                                                          int b, int c2) {
     __shared__ double ss[256], sc[256];
     const BTile t = tiles[blockIdx.x];
     if (!mask[t.item]) return;  // uniform over the workgroup
     const BProb *p = P + t.item;
     const Grid g = p->g;
     const double vol = g.dx * g.dy;
-    const double *A = p->v[a], *B = p->v[b];
+    const double *A = p->v[a], *B = p->v[b], *C = c2 >= 0 ? p->v[c2] : nullptr;
     const int nb = p->nb;
-    Acc acc{0.0, 0.0};
+    Acc acc{0.0, 0.0}, acc2{0.0, 0.0};
     // The thread's cells, in the order of the sum: rows tx, tx + nb, .. and in
     // each row cells tid, tid + 256, .. (M of them).  Their loads go out Q at
     // a time ahead of the adds, so that Q pairs are in flight per thread; the
@@ -93,15 +95,17 @@ __global__ void __launch_bounds__(256) k_b_reduce_partial(const BTile *tiles, co
     const int total = M * R;
     int k = 0, m = 0;  // row and cell index of the next load
     for (int e0 = 0; e0 < total; e0 += Q) {
-        double va[Q], vb[Q];
+        double va[Q], vb[Q], vc[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             va[q] = 0.0;
             vb[q] = 0.0;
+            vc[q] = 0.0;
             if (e0 + q < total) {
                 const size_t c = ix(g.pitch, tid + 256 * m, t.tx + nb * k);
                 va[q] = A[c];
                 vb[q] = B[c];
+                if (C) vc[q] = C[c];
                 if (++m == M) {
                     m = 0;
                     ++k;
@@ -110,21 +114,31 @@ __global__ void __launch_bounds__(256) k_b_reduce_partial(const BTile *tiles, co
         }
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            if (e0 + q < total) acc.add(vol * (va[q] * vb[q]));
+            if (e0 + q < total) {
+                acc.add(vol * (va[q] * vb[q]));
+                if (C) acc2.add(vol * (va[q] * vc[q]));
+            }
     }
     const double v = block_combine(ss, sc, acc, 0.0, 0);
     if (threadIdx.x == 0) p->partial[t.tx] = v;
+    if (C) {
+        const double v2 = block_combine(ss, sc, acc2, 0.0, 0);
+        if (threadIdx.x == 0) p->partial[nb + t.tx] = v2;
+    }
 }
 
-// One workgroup per problem: sc[slot] = sum of the nb partials in block order.
-__global__ void __launch_bounds__(256) k_b_reduce_final(const BProb *P, const int *mask, int slot) {
+// One workgroup per problem: sc[slot] = sum of the nb partials in block order
+// (and sc[slot2] of the second product's, slot2 >= 0).
+__global__ void __launch_bounds__(256) k_b_reduce_final(const BProb *P, const int *mask, int slot, int slot2) {
     __shared__ double ss[256], sc[256];
     if (!mask[blockIdx.x]) return;
     const BProb *p = P + blockIdx.x;
-    Acc acc{0.0, 0.0};
-    for (int k = threadIdx.x; k < p->nb; k += blockDim.x) acc.add(p->partial[k]);
-    const double v = block_combine(ss, sc, acc, 0.0, 0);
-    if (threadIdx.x == 0) p->sc[slot] = v;
+    for (int h = 0; h < (slot2 >= 0 ? 2 : 1); ++h) {
+        Acc acc{0.0, 0.0};
+        for (int k = threadIdx.x; k < p->nb; k += blockDim.x) acc.add(p->partial[h * p->nb + k]);
+        const double v = block_combine(ss, sc, acc, 0.0, 0);
+        if (threadIdx.x == 0) p->sc[h ? slot2 : slot] = v;
+    }
 }
 
 // BiCGStab scalars of every masked problem (the ops of k_bicg_scalars).
@@ -150,7 +164,10 @@ __global__ void k_b_scalars(const BProb *P, const int *mask, int n, int op) {
 // Vector updates (the ops of k_bicg_update):
 //  0: p = r + beta (p - omega v)   1: s = r - alpha v   2: phi += alpha phat
 //  3: phi += omega shat            4: r = s - omega t
-__global__ void __launch_bounds__(256) k_b_update(const BTile *tiles, const BProb *P, const int *mask, int op) {
+// With L (FP32 cycles), ops 0 and 1 also start the cycle that preconditions
+// the vector they produce: the finest level's f32 = (float) p or s, e32 = 0.
+__global__ void __launch_bounds__(256) k_b_update(const BTile *tiles, const BProb *P, const int *mask, int op,
+                                                  const BLev *L) {
     const BTile t = tiles[blockIdx.x];
     if (!mask[t.item]) return;
     const BProb *p = P + t.item;
@@ -158,14 +175,20 @@ __global__ void __launch_bounds__(256) k_b_update(const BTile *tiles, const BPro
     const double *sc = p->sc;
     double *const *v = p->v;
     const double alpha = sc[SC_ALPHA], beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    float *f32 = L ? L[p->lev0].f32 : nullptr, *e32 = L ? L[p->lev0].e32 : nullptr;
     tile_cells(t, g, [&](int i, int j) {
         const size_t c = ix(g.pitch, i, j);
+        double out;
         switch (op) {
-        case 0: v[BV_P][c] = v[BV_R][c] + beta * (v[BV_P][c] - omega * v[BV_V][c]); break;
-        case 1: v[BV_S][c] = v[BV_R][c] - alpha * v[BV_V][c]; break;
-        case 2: v[BV_PHI][c] = v[BV_PHI][c] + alpha * v[BV_PHAT][c]; break;
-        case 3: v[BV_PHI][c] = v[BV_PHI][c] + omega * v[BV_SHAT][c]; break;
-        default: v[BV_R][c] = v[BV_S][c] - omega * v[BV_T][c]; break;
+        case 0: out = v[BV_P][c] = v[BV_R][c] + beta * (v[BV_P][c] - omega * v[BV_V][c]); break;
+        case 1: out = v[BV_S][c] = v[BV_R][c] - alpha * v[BV_V][c]; break;
+        case 2: out = v[BV_PHI][c] = v[BV_PHI][c] + alpha * v[BV_PHAT][c]; break;
+        case 3: out = v[BV_PHI][c] = v[BV_PHI][c] + omega * v[BV_SHAT][c]; break;
+        default: out = v[BV_R][c] = v[BV_S][c] - omega * v[BV_T][c]; break;
+        }
+        if (f32 && op <= 1) {
+            f32[c] = (float)out;
+            e32[c] = 0.0f;
         }
     });
 }
@@ -181,23 +204,6 @@ __global__ void __launch_bounds__(256) k_b_copy(const BTile *tiles, const BProb 
     tile_cells(t, g, [&](int i, int j) {
         const size_t c = ix(g.pitch, i, j);
         d[c] = sv ? sv[c] : 0.0;
-    });
-}
-
-// Start of an FP32 cycle: the finest level's f32 = (float) v[P or S], e32 = 0.
-__global__ void __launch_bounds__(256) k_b_begin32(const BTile *tiles, const BProb *P, const BLev *L,
-                                                   const int *mask, int sel) {
-    const BTile t = tiles[blockIdx.x];
-    if (!mask[t.item]) return;
-    const BProb *p = P + t.item;
-    const Grid g = p->g;
-    const BLev *l = L + p->lev0;
-    float *f32 = l->f32, *e32 = l->e32;
-    const double *src = p->v[sel ? BV_S : BV_P];
-    tile_cells(t, g, [&](int i, int j) {
-        const size_t c = ix(g.pitch, i, j);
-        f32[c] = (float)src[c];
-        e32[c] = 0.0f;
     });
 }
 
@@ -260,7 +266,7 @@ size_t BatchKMG::problem_bytes(const Problem &q, const mgp_solver_opts &o) {
     const size_t n0 = level_elems(q.nx, q.ny);
     b += BV_COUNT * round256(n0 * sizeof(double));
     const int nb = std::max(1, std::min(q.ny, (q.nx * q.ny + 8191) / 8192));
-    b += round256((size_t)nb * sizeof(double));
+    b += round256((size_t)2 * nb * sizeof(double));
     int ny = q.ny, lev = 0;
     size_t levels = 0;
     while (true) {
@@ -333,7 +339,7 @@ BatchKMG::BatchKMG(hipStream_t s_, const std::vector<Problem> &probs, const mgp_
         const size_t n0 = level_elems(q.nx, q.ny);
         for (int k = 0; k < BV_COUNT; ++k) O.v[k] = take(n0);
         const int nb = std::max(1, std::min(q.ny, (q.nx * q.ny + 8191) / 8192));
-        O.partial = take((size_t)nb);
+        O.partial = take((size_t)2 * nb);  // two products' partials per reduction pass
         while (true) {
             const size_t nl = level_elems(q.nx, ny);
             O.J.push_back(take(nl));
@@ -686,8 +692,9 @@ void BatchKMG::zebra(int d, int set, int sel, int k) {
 // Problems whose coarsest level is d make their bottom sweeps there while
 // the others go on down.
 void BatchKMG::precondition(int sel) {
-    if (mixed) k_b_begin32<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, dL, active, sel);
-    else k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
+    // FP32 cycles: the update that produced p or s has set the finest
+    // level's f32 and zeroed its e32 (k_b_update with the level table)
+    if (!mixed) k_b_copy<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, active, sel ? BV_SHAT : BV_PHAT, -1);
     const int depth = T->depth;
     for (int d = 0; d < depth; ++d) {
         for (int k = 0; k < sd; ++k) zebra(d, SET_MID, sel, -1);
@@ -702,9 +709,9 @@ void BatchKMG::precondition(int sel) {
     if (mixed) k_b_end32<<<T->t0.n, 256, 0, s>>>((const BTile *)T->t0.p, dP, dL, active, sel);
 }
 
-void BatchKMG::reduce(int a, int b, int slot, const int *m) {
-    k_b_reduce_partial<<<T->tred.n, 256, 0, s>>>((const BTile *)T->tred.p, dP, m, a, b);
-    k_b_reduce_final<<<count(), 256, 0, s>>>(dP, m, slot);
+void BatchKMG::reduce(int a, int b, int slot, const int *m, int c, int slot2) {
+    k_b_reduce_partial<<<T->tred.n, 256, 0, s>>>((const BTile *)T->tred.p, dP, m, a, b, c);
+    k_b_reduce_final<<<count(), 256, 0, s>>>(dP, m, slot, c >= 0 ? slot2 : -1);
 }
 
 // One BiCGStab iteration of every active problem (the body of
@@ -714,22 +721,22 @@ void BatchKMG::body() {
     const BTile *tl = (const BTile *)T->t0.p;
     const int nt = T->t0.n;
     const int sb_ = (n + 63) / 64;
+    const BLev *l32 = mixed ? dL : nullptr;
     reduce(BV_RHAT, BV_R, SC_RHO_NEW, active);
     k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 0);
-    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 0);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 0, l32);
     precondition(0);
     k_b_apply<<<nt, 256, 0, s>>>(tl, dP, active, BV_V, BV_PHAT, -1);
     reduce(BV_RHAT, BV_V, SC_RV, active);
     k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 1);
-    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 1);
-    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 2);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 1, l32);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 2, nullptr);
     precondition(1);
     k_b_apply<<<nt, 256, 0, s>>>(tl, dP, active, BV_T, BV_SHAT, -1);
-    reduce(BV_T, BV_S, SC_TS, active);
-    reduce(BV_T, BV_T, SC_TT, active);
+    reduce(BV_T, BV_S, SC_TS, active, BV_T, SC_TT);  // t.s and t.t in one pass
     k_b_scalars<<<sb_, 64, 0, s>>>(dP, active, n, 2);
-    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 3);
-    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 4);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 3, nullptr);
+    k_b_update<<<nt, 256, 0, s>>>(tl, dP, active, 4, nullptr);
     reduce(BV_R, BV_R, SC_RN, active);
 }
 
