@@ -9,7 +9,8 @@
 * Each result is the same alone and in the mix: a problem solved in a batch
   of one gives the bits (and the iteration count) it gives inside the batch.
 * Graph replay and eager launches give the same bits, and so do the launch
-  tables rebuilt over the problems still running once half have finished.
+  tables rebuilt over the problems still running once the finished ones'
+  dispatches have cost about one rebuild.
 * Metrics generated on the GPU (mgp_batch_create_mapped) give the solution
   of the single-problem solver built the same way, to the solve tolerance.
 """
@@ -164,3 +165,29 @@ def test_batch_rejects_bad_input():
     with pytest.raises(mg.MgpError):
         mg._check(mg.lib.mgp_batch_set_rhs(b._h, 5, rhs.ctypes.data_as(mg._dp)))
     b.close()
+
+
+@pytest.mark.parametrize("method", [mg.METHOD_KRYLOV_MG, mg.METHOD_KRYLOV_MG32])
+def test_rebuilt_tables_keep_the_bits(method):
+    """240 easy problems (32 x 32, unstretched) finish within a few
+    iterations while 4 stretched Neumann ones run on for 20-40.  The batch
+    then charges 7 us per finished problem and iteration, 1680 us per
+    iteration here, and rebuilds its tables over the 4 (at 4000 + 60 x 4 us)
+    within three iterations, and again as they finish.  The hard problems'
+    bits and iteration counts equal those of each solved alone (a batch of
+    one never rebuilds), and graph replay equals eager launches."""
+    hard = [(96, 368, 0.112, 5.99, (1, 1, 0, 0)), (48, 432, 0.134, 5.89, (1, 1, 0, 0)),
+            (64, 208, 0.131, 5.71, (1, 1, 0, 0)), (160, 112, 0.12, 5.5, (1, 1, 0, 0))]
+    cases = [(32, 32, 0.0, 0.0, (1, 1, 1, 1))] * 240 + hard
+    probs = problems(cases, seed0=300)
+    phis, iters, outc, _ = run_batch(probs, method=method)
+    assert np.all(outc == mg.CONVERGED)
+    assert int(np.max(iters[:240])) + 5 < int(np.min(iters[240:])), (iters[:240].max(), iters[240:])
+    e_phis, e_iters, _, _ = run_batch(probs, use_graph=False, method=method)
+    np.testing.assert_array_equal(e_iters, iters)
+    for k in range(240, 244):
+        np.testing.assert_array_equal(e_phis[k], phis[k])
+        alone, it1, oc1, _ = run_batch([probs[k]], method=method)
+        assert (int(it1[0]), int(oc1[0])) == (int(iters[k]), int(outc[k]))
+        np.testing.assert_array_equal(alone[0], phis[k])
+        assert relres_oracle(*probs[k], phis[k]) <= 1.5e-10
