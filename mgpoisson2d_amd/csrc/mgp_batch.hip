@@ -1162,6 +1162,7 @@ int BatchKMG::solve(double tol, int maxiters, int zerophi, int *iters, int *outc
             if (act_[p]) live.push_back(p);
         wasted += DEAD_US * (T->nprob - (int)live.size());
         if (!live.empty() && wasted >= REBUILD_US + REBUILD_PER_PROBLEM_US * (double)live.size()) {
+            T = &full;  // intact, whatever happens to part below
             release(part);
             build_tables(part, live);
             T = &part;
