@@ -549,7 +549,6 @@ int KrylovMG::solve(Workspace &W, hipStream_t s, const DevGeo &top, const Bc &us
     }
     Impl &K = *impl;
     const Grid &g = top.g;
-    const Geo geo = top.view();
     const double vol = g.dx * g.dy;
     double *sc = K.scal.p;
     double *partial = K.partial.p;
