@@ -224,27 +224,25 @@ __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
 // cell at a time, a coarse-level row pass took 5.4 us of a 10.6 us row
 // solve).  The batched kernels take the RG of batch_rg: no more than fits
 // their VGPR budget, since they rely on many workgroups per CU.
-// Buffer-addressed reads of an FP32 field of a level: one 32-bit byte offset
-// per cell in a vector register, the row offsets (multiples of the pitch) in
+// Buffer-addressed reads of a field of a level: one 32-bit byte offset per
+// cell in a vector register, the row offsets (multiples of the pitch) in
 // scalar registers and the column neighbours in the instruction's immediate
 // offset.  Flat loads need a 64-bit address per array and neighbour: in the
 // batched mid-row kernel 350 of its 2700 vector instructions were 64-bit
 // address arithmetic.  A buffer of 2^31 - 1 bytes: every level array is
 // smaller, so no read is clipped.
-struct BufF {
+template <typename T>
+struct Buf {
     __amdgpu_buffer_rsrc_t r;
-    __device__ __forceinline__ explicit BufF(const float *p)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0, 0x7fffffff, 0x00020000)) {}
-    __device__ __forceinline__ float operator()(int vo, int so) const {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    __device__ __forceinline__ explicit Buf(const T *p)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), 0, 0x7fffffff, 0x00020000)) {}
+    __device__ __forceinline__ T operator()(int vo, int so) const {
+        if constexpr (sizeof(T) == 4)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
     }
 };
-
-template <typename BT, typename T>
-__device__ __forceinline__ BT make_buf(const T *p) {
-    if constexpr (sizeof(T) == 4) return BT(p);
-    else return BT(0);
-}
 
 template <int C, int NT, bool SEG = false, int RG = 1, typename T, typename GM>
 __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const T *f_, const T *K_, const Grid &g,
@@ -263,10 +261,7 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
     // c = 0 at its last), as in block Gauss-Seidel over the segments.
     const bool cut_w = SEG && i0 > 0, cut_e = SEG && i0 + nx < nxg;
     if (j != 0 && j != ny - 1) {
-        // FP32: buffer reads (BufF); the FP64 fields keep flat loads
-        using BT = typename std::conditional<sizeof(T) == 4, BufF, int>::type;
-        const BT bE = make_buf<BT>(e_), bJxx = make_buf<BT>(geo_.Jxx), bJxy = make_buf<BT>(geo_.Jxy),
-                 bJyy = make_buf<BT>(geo_.Jyy), bJyx = make_buf<BT>(geo_.Jyx), bF = make_buf<BT>(f_);
+        const Buf<T> bE(e_), bJxx(geo_.Jxx), bJxy(geo_.Jxy), bJyy(geo_.Jyy), bJyx(geo_.Jyx), bF(f_);
         struct CellIn {
             T p, pe, pw, pn, ps, ene, enw, ese, esw, Jxx_e, Jxx_w, Jxy_e, Jxy_w, Jyy_n, Jyy_s, Jyx_n, Jyx_s, fv;
         };
@@ -282,49 +277,28 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
                 const size_t c = ix(P, i0 + il, j);
                 // interior stencil; for i = 0, nx-1 the reads stay inside the
                 // padded level and the values are replaced below
-                if constexpr (sizeof(T) == 4) {
-                    // bytes from cell (i-1, j-1); rows j and j+1 one and two
-                    // pitches on
-                    const int vo = (int)((c - P - 1) * 4), s1 = P * 4, s2 = P * 8;
-                    in[r].esw = bE(vo, 0);
-                    in[r].ps = bE(vo + 4, 0);
-                    in[r].ese = bE(vo + 8, 0);
-                    in[r].pw = bE(vo, s1);
-                    in[r].p = bE(vo + 4, s1);
-                    in[r].pe = bE(vo + 8, s1);
-                    in[r].enw = bE(vo, s2);
-                    in[r].pn = bE(vo + 4, s2);
-                    in[r].ene = bE(vo + 8, s2);
-                    in[r].Jxx_w = bJxx(vo + 4, s1);
-                    in[r].Jxx_e = bJxx(vo + 8, s1);
-                    in[r].Jxy_w = bJxy(vo + 4, s1);
-                    in[r].Jxy_e = bJxy(vo + 8, s1);
-                    in[r].Jyy_s = bJyy(vo + 4, s1);
-                    in[r].Jyy_n = bJyy(vo + 4, s2);
-                    in[r].Jyx_s = bJyx(vo + 4, s1);
-                    in[r].Jyx_n = bJyx(vo + 4, s2);
-                    in[r].fv = bF(vo + 4, s1);
-                } else {
-                    gconst<T> *q = e + c;
-                    in[r].p = q[0];
-                    in[r].pe = q[1];
-                    in[r].pw = q[-1];
-                    in[r].pn = q[P];
-                    in[r].ps = q[-P];
-                    in[r].ene = q[P + 1];
-                    in[r].enw = q[P - 1];
-                    in[r].ese = q[-P + 1];
-                    in[r].esw = q[-P - 1];
-                    in[r].Jxx_e = Jxx[c + 1];
-                    in[r].Jxx_w = Jxx[c];
-                    in[r].Jxy_e = Jxy[c + 1];
-                    in[r].Jxy_w = Jxy[c];
-                    in[r].Jyy_n = Jyy[c + P];
-                    in[r].Jyy_s = Jyy[c];
-                    in[r].Jyx_n = Jyx[c + P];
-                    in[r].Jyx_s = Jyx[c];
-                    in[r].fv = f[c];
-                }
+                // bytes from cell (i-1, j-1); rows j and j+1 one and two
+                // pitches on
+                constexpr int ES = (int)sizeof(T);
+                const int vo = (int)((c - P - 1) * ES), s1 = P * ES, s2 = 2 * P * ES;
+                in[r].esw = bE(vo, 0);
+                in[r].ps = bE(vo + ES, 0);
+                in[r].ese = bE(vo + 2 * ES, 0);
+                in[r].pw = bE(vo, s1);
+                in[r].p = bE(vo + ES, s1);
+                in[r].pe = bE(vo + 2 * ES, s1);
+                in[r].enw = bE(vo, s2);
+                in[r].pn = bE(vo + ES, s2);
+                in[r].ene = bE(vo + 2 * ES, s2);
+                in[r].Jxx_w = bJxx(vo + ES, s1);
+                in[r].Jxx_e = bJxx(vo + 2 * ES, s1);
+                in[r].Jxy_w = bJxy(vo + ES, s1);
+                in[r].Jxy_e = bJxy(vo + 2 * ES, s1);
+                in[r].Jyy_s = bJyy(vo + ES, s1);
+                in[r].Jyy_n = bJyy(vo + ES, s2);
+                in[r].Jyx_s = bJyx(vo + ES, s1);
+                in[r].Jyx_n = bJyx(vo + ES, s2);
+                in[r].fv = bF(vo + ES, s1);
             }
 #pragma unroll
             for (int r = 0; r < RG; ++r) {
