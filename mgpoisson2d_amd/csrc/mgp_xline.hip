@@ -249,7 +249,6 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
                                          const GM &geo_, int j, T *Ld, T *La, T *Lb, T *Lc) {
     constexpr int U = C + 1;
     gconst<T> *e = gptr(e_), *f = gptr(f_), *K = gptr(K_);
-    gconst<T> *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
     // SEG: the cells solved may be a segment of the row (xline_rows); whole
     // rows compile without the segment terms
     const int tid = threadIdx.x, nx = X.nx, ny = X.ny, P = X.P, i0 = SEG ? X.i0 : 0, nxg = SEG ? X.nxg : X.nx;
