@@ -712,20 +712,16 @@ __global__ void __launch_bounds__(XL_THREADS) k_xline_long(T *__restrict__ e, co
 // (L e)(i, j) at an interior cell from the face metrics: the stencil of
 // cell_lap_interior in the element type T (inverse spacings idx, idy).
 template <typename T, typename G>
-__device__ __forceinline__ T interior_lap(const T *e, const G &geo_, int P, size_t c, T idx, T idy) {
-    // buffer reads as in row_pass: bytes from cell (i-1, j-1), rows j and
-    // j+1 one and two pitches on
-    const Buf<T> bE(e), bJxx(geo_.Jxx), bJxy(geo_.Jxy), bJyy(geo_.Jyy), bJyx(geo_.Jyx);
-    constexpr int ES = (int)sizeof(T);
-    const int vo = (int)((c - P - 1) * ES), s1 = P * ES, s2 = 2 * P * ES;
+__device__ __forceinline__ T interior_lap(gconst<T> *e, const G &geo_, int P, size_t c, T idx, T idy) {
+    gconst<T> *Jxx = gptr(geo_.Jxx), *Jxy = gptr(geo_.Jxy), *Jyx = gptr(geo_.Jyx), *Jyy = gptr(geo_.Jyy);
     const T fourth = T(0.25);
-    const T esw = bE(vo, 0), ps = bE(vo + ES, 0), ese = bE(vo + 2 * ES, 0);
-    const T pw = bE(vo, s1), p = bE(vo + ES, s1), pe = bE(vo + 2 * ES, s1);
-    const T enw = bE(vo, s2), pn = bE(vo + ES, s2), ene = bE(vo + 2 * ES, s2);
-    const T xfe = bJxx(vo + 2 * ES, s1) * (pe - p) * idx + bJxy(vo + 2 * ES, s1) * fourth * (ene - ese + pn - ps) * idy;
-    const T xfw = bJxx(vo + ES, s1) * (p - pw) * idx + bJxy(vo + ES, s1) * fourth * (pn - ps + enw - esw) * idy;
-    const T yfn = bJyy(vo + ES, s2) * (pn - p) * idy + bJyx(vo + ES, s2) * fourth * (ene - enw + pe - pw) * idx;
-    const T yfs = bJyy(vo + ES, s1) * (p - ps) * idy + bJyx(vo + ES, s1) * fourth * (pe - pw + ese - esw) * idx;
+    gconst<T> *q = e + c;
+    const T p = q[0], pe = q[1], pw = q[-1], pn = q[P], ps = q[-P];
+    const T ene = q[P + 1], enw = q[P - 1], ese = q[-P + 1], esw = q[-P - 1];
+    const T xfe = Jxx[c + 1] * (pe - p) * idx + Jxy[c + 1] * fourth * (ene - ese + pn - ps) * idy;
+    const T xfw = Jxx[c] * (p - pw) * idx + Jxy[c] * fourth * (pn - ps + enw - esw) * idy;
+    const T yfn = Jyy[c + P] * (pn - p) * idy + Jyx[c + P] * fourth * (ene - enw + pe - pw) * idx;
+    const T yfs = Jyy[c] * (p - ps) * idy + Jyx[c] * fourth * (pe - pw + ese - esw) * idx;
     return (xfe - xfw) * idx + (yfn - yfs) * idy;
 }
 
@@ -743,7 +739,7 @@ __device__ __forceinline__ void restrict_cell(T *rc, T *ec, const T *e, const T 
         const int j = off + 2 * J + b;
         const bool ring = i == 0 || i == gf.nx - 1 || j == 0 || j == gf.ny - 1;
         const T lap = ring ? ring_lap(gptr(e), gptr(K), nring, gf, i, j)
-                           : interior_lap(e, geo, gf.pitch, ix(gf.pitch, i, j), idx, idy);
+                           : interior_lap(gptr(e), geo, gf.pitch, ix(gf.pitch, i, j), idx, idy);
         s = s + T(0.5) * (f[ix(gf.pitch, i, j)] - lap);
     }
     const size_t c = ix(gc.pitch, i, J);
