@@ -229,8 +229,8 @@ __device__ __forceinline__ RowCtx<T> row_ctx(const Grid &g, int nt, int C) {
 // scalar registers and the column neighbours in the instruction's immediate
 // offset.  Flat loads need a 64-bit address per array and neighbour: in the
 // batched mid-row kernel 350 of its 2700 vector instructions were 64-bit
-// address arithmetic.  A buffer of 2^31 - 1 bytes: every level array is
-// smaller, so no read is clipped.
+// address arithmetic.  A buffer of 2^31 - 1 bytes from the start of the
+// row above the one solved: the reads span three rows, so none is clipped.
 template <typename T>
 struct Buf {
     __amdgpu_buffer_rsrc_t r;
@@ -260,7 +260,12 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
     // c = 0 at its last), as in block Gauss-Seidel over the segments.
     const bool cut_w = SEG && i0 > 0, cut_e = SEG && i0 + nx < nxg;
     if (j != 0 && j != ny - 1) {
-        const Buf<T> bE(e_), bJxx(geo_.Jxx), bJxy(geo_.Jxy), bJyy(geo_.Jyy), bJyx(geo_.Jyx), bF(f_);
+        // the buffers start at cell (-1, j-1) of each array (j is the same
+        // for the whole workgroup), so the offsets are small and never
+        // negative whatever the array's size
+        const size_t r0 = ix(P, 0, j - 1) - 1;
+        const Buf<T> bE(e_ + r0), bJxx(geo_.Jxx + r0), bJxy(geo_.Jxy + r0), bJyy(geo_.Jyy + r0),
+            bJyx(geo_.Jyx + r0), bF(f_ + r0);
         struct CellIn {
             T p, pe, pw, pn, ps, ene, enw, ese, esw, Jxx_e, Jxx_w, Jxy_e, Jxy_w, Jyy_n, Jyy_s, Jyx_n, Jyx_s, fv;
         };
@@ -276,10 +281,10 @@ __device__ __forceinline__ void row_pass(const RowCtx<T> &X, const T *e_, const 
                 const size_t c = ix(P, i0 + il, j);
                 // interior stencil; for i = 0, nx-1 the reads stay inside the
                 // padded level and the values are replaced below
-                // bytes from cell (i-1, j-1); rows j and j+1 one and two
-                // pitches on
+                // bytes from cell (i-1, j-1) (i elements past the buffer's
+                // start); rows j and j+1 one and two pitches on
                 constexpr int ES = (int)sizeof(T);
-                const int vo = (int)((c - P - 1) * ES), s1 = P * ES, s2 = 2 * P * ES;
+                const int vo = (int)((c - P - 1 - r0) * ES), s1 = P * ES, s2 = 2 * P * ES;
                 in[r].esw = bE(vo, 0);
                 in[r].ps = bE(vo + ES, 0);
                 in[r].ese = bE(vo + 2 * ES, 0);
