@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -60,17 +62,111 @@ Grid make_grid(int nx, int ny, double dx, double dy) {
 
 // Host <-> padded device copies.  (i0, j0) is the first element of the host
 // block, w x h its extent.
+//
+// The copies go through a pinned staging buffer.  The runtime moves a 2D
+// region of pageable host memory one row at a time (setting the rhs of the
+// 1024 problems of the batched mix took 2.8 s, 2.8 ms for a problem of 400
+// rows); from pinned memory the region is one DMA.  One buffer per device,
+// grown as needed, under a mutex; an upload returns once its DMA is queued,
+// and the next use of the buffer first waits for that DMA (an event on its
+// stream).  A download waits for its DMA, then copies out.  The direct copy
+// remains for a stream of another device than the current one, and for a
+// copy on one stream while the buffer's last DMA is still queued on another
+// (so that streams do not wait on each other's work through the buffer).
+namespace {
+struct Staging {
+    std::mutex mu;
+    void *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;  // the stream of the pending DMA
+    bool pending = false;
+};
+Staging *staging_for(hipStream_t s) {
+    int cur = 0, sd = 0;
+    if (hipGetDevice(&cur) != hipSuccess || hipStreamGetDevice(s, &sd) != hipSuccess || sd != cur) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    static std::mutex m;
+    static std::map<int, std::unique_ptr<Staging>> all;
+    std::lock_guard<std::mutex> lk(m);
+    std::unique_ptr<Staging> &x = all[cur];
+    if (!x) x = std::make_unique<Staging>();
+    return x.get();
+}
+// The buffer, at least `bytes` long, once its previous copy has completed;
+// nullptr if that copy is still queued on another stream.
+void *stage_buffer(Staging &st, size_t bytes, hipStream_t s) {
+    if (st.pending) {
+        if (st.last != s) {
+            const hipError_t q = hipEventQuery(st.ev);
+            if (q == hipErrorNotReady) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+            hip_check(q, "staging query");
+        }
+        hip_check(hipEventSynchronize(st.ev), "staging wait");
+        st.pending = false;
+    }
+    if (!st.ev) hip_check(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming), "hipEventCreate");
+    if (bytes > st.cap) {
+        if (st.p) (void)hipHostFree(st.p);
+        st.p = nullptr;
+        st.cap = 0;
+        if (hipHostMalloc(&st.p, bytes, hipHostMallocPortable) != hipSuccess || !st.p) {
+            (void)hipGetLastError();
+            st.p = nullptr;
+            g_last_error = "pinned staging of " + std::to_string(bytes) + " bytes";
+            throw Failure(MGP_ERR_NOMEM);
+        }
+        st.cap = bytes;
+    }
+    return st.p;
+}
+}  // namespace
+
 void upload_block(hipStream_t s, double *dev, const Grid &g, const double *host, int i0, int j0, int w, int h) {
-    hip_check(hipMemcpy2DAsync(dev + ix(g.pitch, i0, j0), (size_t)g.pitch * sizeof(double), host,
-                               (size_t)w * sizeof(double), (size_t)w * sizeof(double), (size_t)h,
-                               hipMemcpyHostToDevice, s),
-              "upload");
+    const size_t row = (size_t)w * sizeof(double), bytes = row * (size_t)h;
+    double *d = dev + ix(g.pitch, i0, j0);
+    const size_t dpitch = (size_t)g.pitch * sizeof(double);
+    Staging *st = bytes ? staging_for(s) : nullptr;
+    std::unique_lock<std::mutex> lk;
+    void *p = nullptr;
+    if (st) {
+        lk = std::unique_lock<std::mutex>(st->mu);
+        p = stage_buffer(*st, bytes, s);
+    }
+    if (!p) {
+        hip_check(hipMemcpy2DAsync(d, dpitch, host, row, row, (size_t)h, hipMemcpyHostToDevice, s), "upload");
+        return;
+    }
+    memcpy(p, host, bytes);
+    hip_check(hipMemcpy2DAsync(d, dpitch, p, row, row, (size_t)h, hipMemcpyHostToDevice, s), "upload");
+    hip_check(hipEventRecord(st->ev, s), "staging event");
+    st->last = s;
+    st->pending = true;
 }
 void download_block(hipStream_t s, double *host, const double *dev, const Grid &g, int i0, int j0, int w, int h) {
-    hip_check(hipMemcpy2DAsync(host, (size_t)w * sizeof(double), dev + ix(g.pitch, i0, j0),
-                               (size_t)g.pitch * sizeof(double), (size_t)w * sizeof(double), (size_t)h,
-                               hipMemcpyDeviceToHost, s),
-              "download");
+    const size_t row = (size_t)w * sizeof(double), bytes = row * (size_t)h;
+    const double *d = dev + ix(g.pitch, i0, j0);
+    const size_t dpitch = (size_t)g.pitch * sizeof(double);
+    Staging *st = bytes ? staging_for(s) : nullptr;
+    std::unique_lock<std::mutex> lk;
+    void *p = nullptr;
+    if (st) {
+        lk = std::unique_lock<std::mutex>(st->mu);
+        p = stage_buffer(*st, bytes, s);
+    }
+    if (!p) {
+        hip_check(hipMemcpy2DAsync(host, row, d, dpitch, row, (size_t)h, hipMemcpyDeviceToHost, s), "download");
+        return;
+    }
+    hip_check(hipMemcpy2DAsync(p, row, d, dpitch, row, (size_t)h, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipEventRecord(st->ev, s), "staging event");
+    hip_check(hipEventSynchronize(st->ev), "download wait");
+    memcpy(host, p, bytes);
 }
 
 // --------------------------------------------------------------------------
