@@ -785,10 +785,11 @@ __device__ __forceinline__ void level_fields(const BLev *L, int, float *&e, cons
 // as fit without costing a wave per SIMD (FP32: 2 of the 3 cells of a chunk
 // of 2, 3 of 5, 2 of 9; the C = 4 kernel keeps 4 waves with 119 VGPRs, the
 // others their 87 and 154-159).  The batched mix's finest FP32 launch set
-// went from 0.70 to 0.67 ms, bit-identical.  FP64 keeps one cell at a time.
+// went from 0.70 to 0.67 ms, bit-identical.  FP64: 2 cells for every chunk
+// length (126, 152, 228 and 240 VGPRs: 4, 3, 2 and 2 waves, as with one).
 template <int C, typename T>
 constexpr int batch_rg() {
-    return sizeof(T) == 8 ? 1 : (C == 4 ? 3 : 2);
+    return sizeof(T) == 8 ? 2 : (C == 4 ? 3 : 2);
 }
 
 template <int C, int NT, typename T>
