@@ -3,6 +3,7 @@
 // on resident HBM data, and the extern "C" entry points of include/mgp2d.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -69,10 +70,12 @@ Grid make_grid(int nx, int ny, double dx, double dy) {
 // rows); from pinned memory the region is one DMA.  One buffer per device,
 // grown as needed, under a mutex; an upload returns once its DMA is queued,
 // and the next use of the buffer first waits for that DMA (an event on its
-// stream).  A download waits for its DMA, then copies out.  The direct copy
-// remains for a stream of another device than the current one, and for a
-// copy on one stream while the buffer's last DMA is still queued on another
-// (so that streams do not wait on each other's work through the buffer).
+// stream).  A download waits for its DMA, then copies out.  A region larger
+// than the buffer's 64 MiB moves in blocks of rows, one after the other.  The
+// direct copy remains for a stream of another device than the current one,
+// for a copy on one stream while the buffer's last DMA is still queued on
+// another (so that streams do not wait on each other's work through the
+// buffer), and where no pinned memory can be had.
 namespace {
 struct Staging {
     std::mutex mu;
@@ -95,8 +98,11 @@ Staging *staging_for(hipStream_t s) {
     if (!x) x = std::make_unique<Staging>();
     return x.get();
 }
+constexpr size_t STAGE_MAX = (size_t)64 << 20;
+
 // The buffer, at least `bytes` long, once its previous copy has completed;
-// nullptr if that copy is still queued on another stream.
+// nullptr if that copy is still queued on another stream, or if the pinned
+// allocation fails.
 void *stage_buffer(Staging &st, size_t bytes, hipStream_t s) {
     if (st.pending) {
         if (st.last != s) {
@@ -118,8 +124,7 @@ void *stage_buffer(Staging &st, size_t bytes, hipStream_t s) {
         if (hipHostMalloc(&st.p, bytes, hipHostMallocPortable) != hipSuccess || !st.p) {
             (void)hipGetLastError();
             st.p = nullptr;
-            g_last_error = "pinned staging of " + std::to_string(bytes) + " bytes";
-            throw Failure(MGP_ERR_NOMEM);
+            return nullptr;
         }
         st.cap = bytes;
     }
@@ -131,42 +136,62 @@ void upload_block(hipStream_t s, double *dev, const Grid &g, const double *host,
     const size_t row = (size_t)w * sizeof(double), bytes = row * (size_t)h;
     double *d = dev + ix(g.pitch, i0, j0);
     const size_t dpitch = (size_t)g.pitch * sizeof(double);
-    Staging *st = bytes ? staging_for(s) : nullptr;
+    Staging *st = bytes && row <= STAGE_MAX ? staging_for(s) : nullptr;
     std::unique_lock<std::mutex> lk;
+    const size_t rows = row ? STAGE_MAX / row : 0;  // rows per block
     void *p = nullptr;
     if (st) {
         lk = std::unique_lock<std::mutex>(st->mu);
-        p = stage_buffer(*st, bytes, s);
+        p = stage_buffer(*st, std::min(bytes, rows * row), s);
     }
     if (!p) {
         hip_check(hipMemcpy2DAsync(d, dpitch, host, row, row, (size_t)h, hipMemcpyHostToDevice, s), "upload");
         return;
     }
-    memcpy(p, host, bytes);
-    hip_check(hipMemcpy2DAsync(d, dpitch, p, row, row, (size_t)h, hipMemcpyHostToDevice, s), "upload");
-    hip_check(hipEventRecord(st->ev, s), "staging event");
-    st->last = s;
-    st->pending = true;
+    for (size_t j = 0; j < (size_t)h; j += rows) {
+        const size_t nr = std::min(rows, (size_t)h - j);
+        if (j) p = stage_buffer(*st, nr * row, s);  // the previous block's DMA done (same stream)
+        if (!p) {
+            hip_check(hipMemcpy2DAsync(reinterpret_cast<char *>(d) + j * dpitch, dpitch,
+                                       reinterpret_cast<const char *>(host) + j * row, row, row, (size_t)h - j,
+                                       hipMemcpyHostToDevice, s),
+                      "upload");
+            return;
+        }
+        memcpy(p, reinterpret_cast<const char *>(host) + j * row, nr * row);
+        hip_check(hipMemcpy2DAsync(reinterpret_cast<char *>(d) + j * dpitch, dpitch, p, row, row, nr,
+                                   hipMemcpyHostToDevice, s),
+                  "upload");
+        hip_check(hipEventRecord(st->ev, s), "staging event");
+        st->last = s;
+        st->pending = true;
+    }
 }
 void download_block(hipStream_t s, double *host, const double *dev, const Grid &g, int i0, int j0, int w, int h) {
     const size_t row = (size_t)w * sizeof(double), bytes = row * (size_t)h;
     const double *d = dev + ix(g.pitch, i0, j0);
     const size_t dpitch = (size_t)g.pitch * sizeof(double);
-    Staging *st = bytes ? staging_for(s) : nullptr;
+    Staging *st = bytes && row <= STAGE_MAX ? staging_for(s) : nullptr;
     std::unique_lock<std::mutex> lk;
+    const size_t rows = row ? STAGE_MAX / row : 0;
     void *p = nullptr;
     if (st) {
         lk = std::unique_lock<std::mutex>(st->mu);
-        p = stage_buffer(*st, bytes, s);
+        p = stage_buffer(*st, std::min(bytes, rows * row), s);
     }
     if (!p) {
         hip_check(hipMemcpy2DAsync(host, row, d, dpitch, row, (size_t)h, hipMemcpyDeviceToHost, s), "download");
         return;
     }
-    hip_check(hipMemcpy2DAsync(p, row, d, dpitch, row, (size_t)h, hipMemcpyDeviceToHost, s), "download");
-    hip_check(hipEventRecord(st->ev, s), "staging event");
-    hip_check(hipEventSynchronize(st->ev), "download wait");
-    memcpy(host, p, bytes);
+    for (size_t j = 0; j < (size_t)h; j += rows) {
+        const size_t nr = std::min(rows, (size_t)h - j);
+        hip_check(hipMemcpy2DAsync(p, row, reinterpret_cast<const char *>(d) + j * dpitch, dpitch, row, nr,
+                                   hipMemcpyDeviceToHost, s),
+                  "download");
+        hip_check(hipEventRecord(st->ev, s), "staging event");
+        hip_check(hipEventSynchronize(st->ev), "download wait");
+        memcpy(reinterpret_cast<char *>(host) + j * row, p, nr * row);
+    }
 }
 
 // --------------------------------------------------------------------------

@@ -298,3 +298,19 @@ def test_odd_row_counts_coarsen_past_a_neumann_side(types, method):
     assert relres_oracle(geo, bc, rhs, s.get_phi()) <= 1.5e-10
     assert iters <= 60
     s.close()
+
+
+def test_host_copies_larger_than_the_staging_buffer():
+    """phi of a 3072 x 3072 grid with ghosts is 3074^2 doubles (75.6 MB): the
+    host copies move it through the 64 MiB pinned staging buffer in two
+    blocks of rows.  A round trip returns the same bits, and so does the
+    one of a grid whose copy fits in one block."""
+    from mgpoisson2d_amd import geometry as G
+    for n in (3072, 512):
+        geo = G.mapped_geometry(n, n, 0.0, 0.0)
+        bc = G.BoundaryConditions.uniform((1, 1, 1, 1))
+        s = mg.Solver(geo, bc, method=mg.METHOD_KRYLOV_MG)
+        phi = np.random.default_rng(n).standard_normal((n + 2, n + 2))
+        s.set_phi(phi)
+        np.testing.assert_array_equal(s.get_phi(), phi)
+        s.close()
