@@ -263,7 +263,10 @@ typedef struct {
     int ordering;       /* smoother ordering for methods 0/1 */
     int numcycles;      /* methods 0-3: 1 = V, 2 = W; KRYLOV_MG: V-cycles per preconditioner application
                            (the cycle iterated from e = 0), 1 by default */
-    int smooth_down, smooth_up, smooth_bottom;
+    int smooth_down, smooth_up, smooth_bottom;  /* KRYLOV_MG: x-line sweeps before and after the
+                           coarse correction, and at least smooth_bottom on the coarsest level (the
+                           single-problem solver makes half that level's rows, up to 64; mgp_batch 8
+                           below a finer level) */
     int maxdepth;       /* -1: as deep as the schedule allows */
     int homog;          /* BCs of the top-level residual */
     int use_graph;      /* KRYLOV_MG: replay each iteration as a captured hipGraph */
